@@ -1,0 +1,171 @@
+"""Benchmark: 30-qubit depth-40 seeded random circuit on one MI355X.
+
+One step = one full run of the depth-40 circuit (30 single-qubit gates from
+{H, X, RX, RY, RZ} + 10 CNOT/CZ on disjoint pairs per layer, SmallRng seed
+2024) applied in place to a resident complex128 state (16 GiB) through the
+fused tile-pass plan.  The state is set to |0..0> once before the warmup.
+
+Prints one JSON line: gate-layers/s, the roofline of the dominant kernel
+(tile pass; algorithmic bytes = 2 * 2^n * 16 per launch, its average launch
+duration from HIP events on the engine's stream) and the CPU baseline (the
+oracle's restatement of the reference CPU path, timed on a bounded sample of
+the same circuit on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(circuit, qubits: int, sample_ops: int):
+    """Times the restated reference CPU path (clone+collect 1q apply with
+    OpenMP, sequential CNOT/CZ loops; oracle/qf_oracle.cpp) on the first
+    `sample_ops` ops of layer 0 in the 3:1 single:two-qubit mix of a layer,
+    and scales to layers/s."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    ops = circuit.to_ops()
+    one = [o for o in ops[: qubits + 10] if o["kind"] == 0]
+    two = [o for o in ops[: qubits + 10] if o["kind"] != 0]
+    n1 = max(1, (3 * sample_ops) // 4)
+    n2 = max(1, sample_ops - n1)
+    sample = np.array(one[:n1] + two[:n2], dtype=ops.dtype)
+    t = O.time_run(qubits, sample)
+    per_1q = None
+    # Per-layer time: 30 one-qubit + 10 two-qubit ops, scaled from the sample mix.
+    layer_time = t * (qubits + 10) / len(sample)
+    return {
+        "value": 1.0 / layer_time,
+        "unit": "layers/s",
+        "cores": O.max_threads(),
+        "kind": "port",
+        "sample": f"{n1} single-qubit + {n2} two-qubit ops of layer 0 at {qubits} qubits ({t:.1f} s), "
+                  f"scaled to a {qubits}+10-op layer",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--precision", default="complex128", choices=["complex128", "complex64"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--low", type=int, default=0)
+    ap.add_argument("--cpu-sample-ops", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the state-vector workload runs on a single MI355X; "
+              "multi-GPU sharding is not part of this build (run with --gpus 1).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from quantumforge_amd import StateVector
+    from quantumforge_amd.backend import HipBackend, Plan
+    from quantumforge_amd.workloads import random_layer_circuit
+
+    n = args.qubits
+    circuit = random_layer_circuit(n, args.depth, seed=args.seed)
+    plan = Plan(circuit, n, fuse=True, tile_qubits=args.tile, low_qubits=args.low)
+    stats = plan.stats()
+    desc = plan.describe()
+    tile_bits = desc["passes"][0]["tile_bits"]
+    log(f"plan: {stats['passes']} passes for {stats['ops_in']} ops (tile {tile_bits} qubits)")
+
+    be = HipBackend(0)
+    state = be.zero_state(n, precision=args.precision)
+    amp_bytes = 16 if args.precision == "complex128" else 8
+    stream = torch.cuda.ExternalStream(be.stream())
+
+    for w in range(args.warmup):
+        state.run_circuit(circuit, plan)
+        be.synchronize()
+        log(f"warmup {w + 1}/{args.warmup} done")
+
+    torch.cuda.synchronize()
+    be.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        state.run_circuit(circuit, plan)
+    end.record(stream)
+    be.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    log(f"timed {args.steps} steps: {gpu_ms:.1f} ms GPU, {wall * 1e3:.1f} ms wall")
+
+    norm = state.norm_sqr()
+    log(f"norm after {args.warmup + args.steps} runs: {norm:.15f}")
+
+    ms_per_step = gpu_ms / args.steps
+    layers_per_s = args.depth / (ms_per_step / 1e3)
+    launches = stats["passes"] * args.steps
+    avg_launch_s = (gpu_ms / 1e3) / launches
+    bytes_per_launch = 2 * (1 << n) * amp_bytes
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    out = {
+        "metric": f"{n}q gate-layers/s",
+        "value": round(layers_per_s, 3),
+        "unit": "layers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.precision == "complex128" else "c64",
+        "data": f"synthetic: seeded random circuit (SmallRng seed {args.seed}), {args.depth} layers",
+        "config": {
+            "workload": f"random layered circuit, {n} qubits, depth {args.depth}, "
+                        f"{'complex128' if amp_bytes == 16 else 'complex64'}, fused tile passes",
+            "qubits": n, "depth": args.depth, "ops": stats["ops_in"], "passes_per_step": stats["passes"],
+            "tile_qubits": tile_bits,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_tile",
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        },
+        "cpu_baseline": None,
+    }
+    del state
+    if not args.no_cpu:
+        log("timing the CPU baseline (oracle restatement of the reference CPU path)")
+        out["cpu_baseline"] = cpu_baseline(circuit, n, args.cpu_sample_ops)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

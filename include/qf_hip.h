@@ -1,0 +1,278 @@
+/*
+ * qf_hip.h - C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int32 status codes.  No
+ * C++ or torch types cross this boundary.  Every entry point states which
+ * interface of the reference crate (quantumforge 0.2.10) it replaces, as
+ * file:line in that crate.
+ *
+ * Conventions (identical to the reference):
+ *   - qubit 0 is the least-significant bit of the basis index
+ *     (src/lib.rs:674-676);
+ *   - amplitudes are interleaved {re, im} complex128, the layout of the
+ *     reference's repr(C) Complex64 (src/lib.rs:437-445); complex64 states
+ *     use {float re, float im};
+ *   - gate matrices are computed on the host with the C library's cos/sin,
+ *     exactly as Gate::matrix does (src/lib.rs:2267-2333).
+ *
+ * Threading: a backend owns one HIP stream.  Calls on one backend (and on
+ * states created from it) are stream-ordered; a handle must not be used from
+ * two threads at once without external locking.  Gate calls validate their
+ * arguments on the host before anything is enqueued (the validate-then-launch
+ * order of src/cuda.rs:1297-1311), so a returned error leaves the device state
+ * untouched.  Device faults surface at the next synchronizing call
+ * (download, reductions, qf_backend_synchronize), as the reference's batch
+ * path defers them to download (src/cuda.rs:1817-1818).
+ *
+ * Ownership: the caller destroys every handle it receives.  A state or batch
+ * holds a reference on its backend, so it stays valid after
+ * qf_backend_destroy (the Arc of src/cuda.rs:519-527, tests/quantum_core.rs:436).
+ */
+#ifndef QF_HIP_H
+#define QF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes: one per QuantumError / CudaError variant on the path        */
+/* (QuantumError src/lib.rs:509-582, CudaError src/cuda.rs:3051-3080).       */
+/* ------------------------------------------------------------------------ */
+typedef int32_t qf_status;
+enum {
+    QF_OK = 0,
+    QF_ZERO_QUBITS = 1,                 /* QuantumError::ZeroQubits */
+    QF_QUBIT_OUT_OF_RANGE = 2,          /* QuantumError::QubitOutOfRange */
+    QF_INVALID_QUBIT_PAIR = 3,          /* QuantumError::InvalidQubitPair */
+    QF_NON_FINITE_ANGLE = 4,            /* CudaError::NonFiniteAngle */
+    QF_STATE_TOO_LARGE = 5,             /* CudaError::StateTooLarge */
+    QF_EMPTY_BATCH = 6,                 /* CudaError::EmptyBatch */
+    QF_INCOMPATIBLE_BATCH = 7,          /* CudaError::IncompatibleBatch */
+    QF_INVALID_PARAMETER_COUNT = 8,     /* QuantumError::InvalidParameterCount */
+    QF_INVALID_INPUT_COUNT = 9,         /* QuantumError::InvalidInputCount */
+    QF_NON_FINITE_PARAMETER = 10,       /* QuantumError::NonFiniteParameter */
+    QF_OUT_OF_MEMORY = 11,              /* QuantumError::MemoryAllocationFailed */
+    QF_DEVICE_ERROR = 12,               /* CudaError::Driver */
+    QF_REQUIRES_DENSITY_MATRIX = 13,    /* QuantumError::OperationRequiresDensityMatrix */
+    QF_REQUIRES_SHOT_EXECUTION = 14,    /* QuantumError::OperationRequiresShotExecution */
+    QF_INVALID_ARGUMENT = 15,           /* null handle / bad enum value */
+    QF_INCOMPATIBLE_DEVICE_STATE = 16,  /* CudaError::IncompatibleDeviceState */
+    QF_STATE_SIZE_NOT_POWER_OF_TWO = 17,/* QuantumError::StateSizeNotPowerOfTwo */
+    QF_INVALID_BATCH_SAMPLES = 18,      /* CudaError::InvalidBatchSamples */
+    QF_MEMORY_BUDGET_EXCEEDED = 19,     /* QuantumError::MemoryBudgetExceeded */
+    QF_INVALID_FD_STEP = 20,            /* QuantumError::InvalidFiniteDifferenceStep */
+    QF_SIMULATION_SIZE_OVERFLOW = 21    /* QuantumError::SimulationSizeOverflow */
+};
+
+/* Gate ids, in the order of `enum Gate` (src/lib.rs:2239-2265). */
+enum {
+    QF_GATE_I = 0, QF_GATE_X = 1, QF_GATE_Y = 2, QF_GATE_Z = 3, QF_GATE_H = 4,
+    QF_GATE_S = 5, QF_GATE_SDG = 6, QF_GATE_T = 7, QF_GATE_TDG = 8,
+    QF_GATE_RX = 9, QF_GATE_RY = 10, QF_GATE_RZ = 11
+};
+
+/* Two-qubit op ids, numbered as the reference kernel's `op` argument
+ * (0 CNOT, 1 CZ, 2 SWAP; src/cuda.rs:99-113). */
+enum { QF_2Q_CNOT = 0, QF_2Q_CZ = 1, QF_2Q_SWAP = 2 };
+
+/* Single-qubit observables (SingleQubitObservable, src/lib.rs:2931-2938). */
+enum { QF_OBS_X = 0, QF_OBS_Y = 1, QF_OBS_Z = 2 };
+
+/* Amplitude precision of a device state. */
+enum { QF_C128 = 0, QF_C64 = 1 };
+
+/* Gradient strategies (GradientStrategy, src/gradients.rs:48-71). */
+enum {
+    QF_GRAD_PARAMETER_SHIFT = 0, QF_GRAD_ADJOINT = 1, QF_GRAD_BACKPROP = 2,
+    QF_GRAD_FINITE_DIFFERENCE = 3, QF_GRAD_AUTO = 4
+};
+
+typedef struct { double re, im; } qf_c128;
+typedef struct { float re, im; } qf_c64;
+
+/* One operation of a Circuit (CircuitOp, src/lib.rs:2386-2440).
+ * kind: 0 single-qubit {gate, a=target, theta for Rx/Ry/Rz},
+ *       1 CNOT {a=control, b=target}, 2 CZ {a, b}, 3 SWAP {a, b},
+ *       4 channel, 5 measure, 6 classically conditioned gate
+ *       (4-6 are rejected with the reference's errors, src/lib.rs:2793-2798). */
+typedef struct {
+    int32_t kind;
+    int32_t gate;
+    uint32_t a, b;
+    double theta;
+} qf_op;
+
+/* One operation of a ParameterizedCircuit (ParameterizedOp, src/lib.rs:2942-2996).
+ * kind: 0 fixed gate {gate, a, theta}, 1 rotation {gate=axis 0 X / 1 Y / 2 Z,
+ *       a=target, index=parameter}, 2 encoding rotation {axis, a, index=input},
+ *       3 CNOT {a=control, b=target}, 4 CZ, 5 SWAP, 6 channel (rejected). */
+typedef struct {
+    int32_t kind;
+    int32_t gate;
+    uint32_t a, b;
+    uint32_t index;
+    uint32_t reserved;
+    double theta;
+} qf_pop;
+
+/* One weighted Pauli word of a PauliSum (src/analytic.rs:2549-2778): qubit q
+ * carries X if only bit q of x_mask is set, Z if only bit q of z_mask is set,
+ * Y if both are set.  Y follows apply_basis_index (src/analytic.rs:2629-2652):
+ * +i on a 0 bit, -i on a 1 bit. */
+typedef struct {
+    double re, im;
+    uint64_t x_mask, z_mask;
+} qf_pauli_term;
+
+/* One rotation of apply_rotation_chain (src/cuda.rs:1325-1347). */
+typedef struct {
+    int32_t axis;     /* 0 X, 1 Y, 2 Z */
+    uint32_t qubit;
+    double theta;
+} qf_rotation;
+
+/* Fusion / tiling options of the circuit planner (no reference counterpart:
+ * the reference dispatches one kernel per gate, src/cuda.rs:1050-1083). */
+typedef struct {
+    int32_t fuse;          /* 0: one sweep per gate (reference order, bit-exact arithmetic); 1: tile passes */
+    uint32_t tile_qubits;  /* qubits held per workgroup tile (0 = autotuned default) */
+    uint32_t low_qubits;   /* contiguous low qubits always in a tile (0 = default) */
+    uint32_t max_block;    /* widest dense fused block k (2..5; 0 = default) */
+} qf_plan_options;
+
+/* Summary of a plan (counts the HBM sweeps the plan streams). */
+typedef struct {
+    uint64_t passes;       /* full-state sweeps (kernel launches that stream the state once) */
+    uint64_t ops_in;       /* operations in the circuit */
+    uint64_t ops_applied;  /* operations after 1q merging */
+    uint64_t dense_blocks; /* fused dense k-qubit blocks */
+    uint64_t max_ops_per_pass;
+} qf_plan_stats;
+
+typedef struct qf_backend qf_backend;
+typedef struct qf_state qf_state;
+typedef struct qf_batch qf_batch;
+typedef struct qf_plan qf_plan;
+
+/* ------------------------------------------------------------------------ */
+/* Errors                                                                    */
+/* ------------------------------------------------------------------------ */
+/* Message of the last failed call on this thread (Display of CudaError /
+ * QuantumError, src/cuda.rs:3081-3104, src/lib.rs:584-658). */
+const char* qf_last_error_message(void);
+/* Stable name of a status code ("QubitOutOfRange", ...). */
+const char* qf_status_name(qf_status status);
+
+/* ------------------------------------------------------------------------ */
+/* Backend (CudaBackend, src/cuda.rs:513-790)                                */
+/* ------------------------------------------------------------------------ */
+/* CudaBackend::new(ordinal), src/cuda.rs:556. */
+qf_status qf_backend_create(int32_t ordinal, qf_backend** out);
+/* Drop for CudaBackend; live states keep the device alive. */
+qf_status qf_backend_destroy(qf_backend* backend);
+/* cuCtxSynchronize, src/cuda.rs:1983; reports deferred device errors. */
+qf_status qf_backend_synchronize(qf_backend* backend);
+/* CudaBackend::device_ordinal. */
+qf_status qf_backend_device_ordinal(const qf_backend* backend, int32_t* out);
+/* Planner options used by qf_state_run_circuit when no plan is passed. */
+qf_status qf_backend_set_plan_options(qf_backend* backend, const qf_plan_options* options);
+/* The backend's HIP stream (hipStream_t as void*), for interop with callers
+ * that enqueue their own work (e.g. a torch stream wait). */
+qf_status qf_backend_stream(const qf_backend* backend, void** out);
+
+/* ------------------------------------------------------------------------ */
+/* Single resident state (CudaDeviceState, src/cuda.rs:1277-1484)            */
+/* ------------------------------------------------------------------------ */
+/* StateVector::zero (src/lib.rs:930) allocated directly on the device. */
+qf_status qf_state_zero(qf_backend* backend, uint32_t qubits, int32_t precision, qf_state** out);
+/* CudaBackend::upload, src/cuda.rs:844-867. */
+qf_status qf_state_upload(qf_backend* backend, const qf_c128* amps, uint32_t qubits, qf_state** out);
+/* complex64 upload (no reference counterpart: the reference is complex128 only). */
+qf_status qf_state_upload_c64(qf_backend* backend, const qf_c64* amps, uint32_t qubits, qf_state** out);
+/* CudaDeviceState::download, src/cuda.rs:1459-1470 (complex64 states are widened). */
+qf_status qf_state_download(const qf_state* state, qf_c128* out);
+/* Narrow download of a complex64 state. */
+qf_status qf_state_download_c64(const qf_state* state, qf_c64* out);
+/* Drop for CudaDeviceState, src/cuda.rs:1473-1484. */
+qf_status qf_state_destroy(qf_state* state);
+/* CudaDeviceState::qubits, src/cuda.rs:1279. */
+qf_status qf_state_qubits(const qf_state* state, uint32_t* out);
+qf_status qf_state_precision(const qf_state* state, int32_t* out);
+/* CudaDeviceState::copy_from_device, src/cuda.rs:1444-1456. */
+qf_status qf_state_copy_from(qf_state* dst, const qf_state* src);
+/* Device address of the amplitudes (interop, e.g. wrapping as a torch tensor). */
+qf_status qf_state_device_pointer(const qf_state* state, void** out);
+
+/* CudaDeviceState::apply_gate, src/cuda.rs:1296-1323 (StateVector::apply, src/lib.rs:1017). */
+qf_status qf_state_apply_gate(qf_state* state, int32_t gate, uint32_t target, double theta);
+/* apply_matrix_kernel with a caller matrix m = {m00, m01, m10, m11} as re,im pairs
+ * (src/cuda.rs:114-128). */
+qf_status qf_state_apply_matrix(qf_state* state, uint32_t target, const double* m8);
+/* CudaDeviceState::apply_rotation_chain, src/cuda.rs:1325-1347. */
+qf_status qf_state_apply_rotation_chain(qf_state* state, const qf_rotation* rotations, size_t count);
+/* apply_cnot / apply_cz / apply_swap, src/cuda.rs:1349-1382 (src/lib.rs:1062-1122). */
+qf_status qf_state_apply_2q(qf_state* state, int32_t op, uint32_t a, uint32_t b);
+/* Exact controlled phase diag(1,1,1,e^{i phi}) (the gate algorithms.rs:139-145
+ * lowers to Rz/CNOT up to the global phase e^{-i phi/4}). */
+qf_status qf_state_apply_cp(qf_state* state, uint32_t control, uint32_t target, double phi);
+/* CudaBackend::run_circuit / Circuit::run on a resident state
+ * (src/cuda.rs:1050-1083, src/lib.rs:2786-2802).  plan may be NULL. */
+qf_status qf_state_run_circuit(qf_state* state, const qf_op* ops, size_t count, const qf_plan* plan);
+
+/* StateVector::norm_sqr, src/lib.rs:1000-1002 (FP64, compensated). */
+qf_status qf_state_norm_sqr(const qf_state* state, double* out);
+/* StateVector::probabilities, src/lib.rs:1144-1146. */
+qf_status qf_state_probabilities(const qf_state* state, double* out);
+/* StateVector::expectation{,_x,_y,_z}, src/lib.rs:1149-1318. */
+qf_status qf_state_expectation(const qf_state* state, int32_t observable, uint32_t qubit, double* out);
+/* PauliSum::expectation_ket, src/analytic.rs:2775-2778 (FP64, compensated). */
+qf_status qf_state_expectation_pauli_sum(const qf_state* state, const qf_pauli_term* terms, size_t count,
+                                         double* out_re, double* out_im);
+/* <a|b> (Bra::inner, src/quantum.rs:327-337), FP64 compensated. */
+qf_status qf_state_inner(const qf_state* a, const qf_state* b, double* out_re, double* out_im);
+
+/* ------------------------------------------------------------------------ */
+/* Batches (CudaDeviceStateBatch, src/cuda.rs:1486-2005)                     */
+/* ------------------------------------------------------------------------ */
+/* CudaBackend::upload_batch, src/cuda.rs:871-954: equal widths, keeps the
+ * reference's EmptyBatch / IncompatibleBatch errors.  amps is [count][2^qubits]. */
+qf_status qf_batch_upload(qf_backend* backend, const qf_c128* amps, const uint32_t* qubits, uint32_t count,
+                          qf_batch** out);
+/* |0..0> batch of `count` equal-width states. */
+qf_status qf_batch_zero(qf_backend* backend, uint32_t qubits, uint32_t count, int32_t precision, qf_batch** out);
+qf_status qf_batch_destroy(qf_batch* batch);
+/* CudaDeviceStateBatch::qubits / batch_size, src/cuda.rs:1488-1496. */
+qf_status qf_batch_shape(const qf_batch* batch, uint32_t* qubits, uint32_t* count);
+/* apply_gate / apply_cnot / apply_cz / apply_swap on every state, src/cuda.rs:1819-1936. */
+qf_status qf_batch_apply_gate(qf_batch* batch, int32_t gate, uint32_t target, double theta);
+qf_status qf_batch_apply_2q(qf_batch* batch, int32_t op, uint32_t a, uint32_t b);
+/* CudaDeviceStateBatch::run_circuit, src/cuda.rs:1949-1970. */
+qf_status qf_batch_run_circuit(qf_batch* batch, const qf_op* ops, size_t count, const qf_plan* plan);
+/* CudaDeviceStateBatch::download, src/cuda.rs:1973-1992; out is [count][2^qubits]. */
+qf_status qf_batch_download(const qf_batch* batch, qf_c128* out);
+/* Per-state Pauli-sum energies (FP64 compensated). out is [count]. */
+qf_status qf_batch_expectation_pauli_sum(const qf_batch* batch, const qf_pauli_term* terms, size_t nterms,
+                                         double* out);
+
+/* ------------------------------------------------------------------------ */
+/* Planning (no reference counterpart; see qf_plan_options)                  */
+/* ------------------------------------------------------------------------ */
+/* Builds the fused tile-pass schedule of a circuit (host only, no device use).
+ * options may be NULL (defaults). */
+qf_status qf_plan_build(const qf_op* ops, size_t count, uint32_t qubits, const qf_plan_options* options,
+                        qf_plan** out);
+qf_status qf_plan_stats_get(const qf_plan* plan, qf_plan_stats* out);
+/* JSON text of the plan's passes (inspection and planner tests).  Writes at
+ * most cap bytes including the terminating NUL; *needed gets the full length + 1. */
+qf_status qf_plan_describe(const qf_plan* plan, char* buf, size_t cap, size_t* needed);
+qf_status qf_plan_destroy(qf_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QF_HIP_H */

@@ -1,0 +1,142 @@
+"""Brute-force NumPy oracle: dense 2^n x 2^n unitaries built from textbook
+definitions (n <= 10).
+
+TEST INFRASTRUCTURE ONLY.  Independent of qf_oracle.cpp: gate matrices come
+from their mathematical definitions (rotations as matrix exponentials
+exp(-i theta P / 2) via scipy), two-qubit gates as explicit permutation /
+diagonal matrices, the state as U |0..0>.  Qubit 0 is the least-significant
+index bit.  Used to pin the C++ restatement.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.linalg import expm
+
+I2 = np.eye(2, dtype=complex)
+PX = np.array([[0, 1], [1, 0]], dtype=complex)
+PY = np.array([[0, -1j], [1j, 0]], dtype=complex)
+PZ = np.array([[1, 0], [0, -1]], dtype=complex)
+PAULI = {"I": I2, "X": PX, "Y": PY, "Z": PZ}
+
+
+def gate(gid: int, theta: float = 0.0) -> np.ndarray:
+    """Gate id of include/qf_hip.h -> 2x2 matrix from the definitions."""
+    s = 1 / np.sqrt(2)
+    fixed = {
+        0: I2, 1: PX, 2: PY, 3: PZ,
+        4: s * np.array([[1, 1], [1, -1]], dtype=complex),
+        5: np.diag([1, 1j]), 6: np.diag([1, -1j]),
+        7: np.diag([1, np.exp(1j * np.pi / 4)]), 8: np.diag([1, np.exp(-1j * np.pi / 4)]),
+    }
+    if gid in fixed:
+        return np.asarray(fixed[gid], dtype=complex)
+    P = {9: PX, 10: PY, 11: PZ}[gid]
+    return expm(-0.5j * theta * P)
+
+
+def embed_1q(m: np.ndarray, q: int, n: int) -> np.ndarray:
+    return np.kron(np.kron(np.eye(1 << (n - 1 - q)), m), np.eye(1 << q))
+
+
+def perm_matrix(n: int, f) -> np.ndarray:
+    d = 1 << n
+    P = np.zeros((d, d), dtype=complex)
+    for i in range(d):
+        P[f(i), i] = 1.0
+    return P
+
+
+def cnot(c: int, t: int, n: int) -> np.ndarray:
+    return perm_matrix(n, lambda i: i ^ (((i >> c) & 1) << t))
+
+
+def swap(a: int, b: int, n: int) -> np.ndarray:
+    def f(i):
+        ba, bb = (i >> a) & 1, (i >> b) & 1
+        return i if ba == bb else i ^ (1 << a) ^ (1 << b)
+    return perm_matrix(n, f)
+
+
+def cz(a: int, b: int, n: int) -> np.ndarray:
+    return np.diag([-1.0 if ((i >> a) & 1) and ((i >> b) & 1) else 1.0 for i in range(1 << n)]).astype(complex)
+
+
+def cp(c: int, t: int, phi: float, n: int) -> np.ndarray:
+    return np.diag([np.exp(1j * phi) if ((i >> c) & 1) and ((i >> t) & 1) else 1.0 for i in range(1 << n)])
+
+
+def op_unitary(op, n: int) -> np.ndarray:
+    kind, g, a, b, th = int(op["kind"]), int(op["gate"]), int(op["a"]), int(op["b"]), float(op["theta"])
+    if kind == 0:
+        return embed_1q(gate(g, th), a, n)
+    if kind == 1:
+        return cnot(a, b, n)
+    if kind == 2:
+        return cz(a, b, n)
+    if kind == 3:
+        return swap(a, b, n)
+    raise ValueError("non-unitary op")
+
+
+def run(ops, n: int, psi0: np.ndarray | None = None) -> np.ndarray:
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1.0
+    if psi0 is not None:
+        psi = psi0.astype(complex).copy()
+    for op in ops:
+        psi = op_unitary(op, n) @ psi
+    return psi
+
+
+def pauli_word_matrix(word: str) -> np.ndarray:
+    """Character k acts on qubit k (qubit 0 = least significant)."""
+    m = np.eye(1)
+    for ch in reversed(word):
+        m = np.kron(m, PAULI[ch])
+    return m
+
+
+def pauli_sum_matrix(terms) -> np.ndarray:
+    """terms: iterable of (coefficient, word)."""
+    H = None
+    for c, w in terms:
+        M = c * pauli_word_matrix(w)
+        H = M if H is None else H + M
+    return H
+
+
+def expectation(psi: np.ndarray, H: np.ndarray) -> complex:
+    return complex(np.vdot(psi, H @ psi))
+
+
+def pc_ops(pops, params, inputs=(), shift=None):
+    """Binds a parameterized op list (qf_pop records) to fixed op dicts."""
+    axis_gate = {0: 9, 1: 10, 2: 11}
+    out = []
+    for i, p in enumerate(pops):
+        k = int(p["kind"])
+        if k == 0:
+            out.append(dict(kind=0, gate=int(p["gate"]), a=int(p["a"]), b=0, theta=float(p["theta"])))
+        elif k in (1, 2):
+            th = float(params[int(p["index"])] if k == 1 else inputs[int(p["index"])])
+            if shift is not None and shift[0] == i:
+                th += shift[1]
+            out.append(dict(kind=0, gate=axis_gate[int(p["gate"])], a=int(p["a"]), b=0, theta=th))
+        else:
+            out.append(dict(kind=k - 2, gate=0, a=int(p["a"]), b=int(p["b"]), theta=0.0))
+    return out
+
+
+def pc_energy(pops, params, n, H, inputs=(), shift=None) -> float:
+    return expectation(run(pc_ops(pops, params, inputs, shift), n), H).real
+
+
+def pc_shift_gradient(pops, nparams, params, n, H, inputs=()):
+    """Exact derivative of <H> by the two-term shift rule per occurrence."""
+    g = np.zeros(nparams)
+    for i, p in enumerate(pops):
+        if int(p["kind"]) == 1:
+            plus = pc_energy(pops, params, n, H, inputs, (i, np.pi / 2))
+            minus = pc_energy(pops, params, n, H, inputs, (i, -np.pi / 2))
+            g[int(p["index"])] += 0.5 * (plus - minus)
+    return g

@@ -1,0 +1,699 @@
+// FP64 CPU oracle for the state-vector apply path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates the reference crate's CPU
+// algorithm (crate `quantumforge` 0.2.10, Rust) so that the HIP engine can be
+// checked against it.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker / the timed CPU
+// baseline; the product library never links or calls it.
+//
+// Compiled with -ffp-contract=off: Rust never contracts a*b+c into an FMA,
+// so neither may the restatement (the complex product below is the exact
+// operation sequence of `impl Mul for Complex64`, src/lib.rs:486-494).
+//
+// Every function names the reference lines it follows.  Pinning of each
+// function (golden vectors, closed forms, brute force) lives in
+// tests/test_oracle_*.py.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct C {  // src/lib.rs:437-445  repr(C) {re, im}
+    double re, im;
+};
+inline C cadd(C a, C b) { return {a.re + b.re, a.im + b.im}; }           // lib.rs:472-477
+inline C csub(C a, C b) { return {a.re - b.re, a.im - b.im}; }           // lib.rs:478-483
+inline C cmul(C a, C b) {                                                // lib.rs:486-494
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+inline C cmulf(C a, double s) { return {a.re * s, a.im * s}; }           // lib.rs:496-500
+inline C cconj(C a) { return {a.re, -a.im}; }                            // lib.rs:467-469
+inline double cnorm(C a) { return a.re * a.re + a.im * a.im; }           // lib.rs:459-461
+
+// Status codes shared with include/qf_hip.h.
+enum {
+    OK = 0,
+    ZERO_QUBITS = 1,
+    QUBIT_OUT_OF_RANGE = 2,
+    INVALID_QUBIT_PAIR = 3,
+    NON_FINITE_ANGLE = 4,
+    STATE_TOO_LARGE = 5,
+    INVALID_PARAMETER_COUNT = 8,
+    INVALID_INPUT_COUNT = 9,
+    NON_FINITE_PARAMETER = 10,
+    REQUIRES_DENSITY_MATRIX = 13,
+    REQUIRES_SHOT_EXECUTION = 14,
+    INVALID_ARGUMENT = 15,
+    INVALID_FD_STEP = 20,
+};
+
+// Gate ids (qf_gate in include/qf_hip.h) in the order of `enum Gate`, src/lib.rs:2239-2265.
+enum { G_I, G_X, G_Y, G_Z, G_H, G_S, G_SDG, G_T, G_TDG, G_RX, G_RY, G_RZ };
+
+// Gate::matrix, src/lib.rs:2267-2333.  (m00, m01, m10, m11) row major.
+void gate_matrix(int gate, double theta, C m[4]) {
+    const C z{0, 0}, o{1, 0}, i{0, 1};
+    const double s = 0.70710678118654752440;  // std::f64::consts::FRAC_1_SQRT_2
+    switch (gate) {
+        case G_I: m[0] = o; m[1] = z; m[2] = z; m[3] = o; break;
+        case G_X: m[0] = z; m[1] = o; m[2] = o; m[3] = z; break;
+        case G_Y: m[0] = z; m[1] = C{0, -1}; m[2] = i; m[3] = z; break;
+        case G_Z: m[0] = o; m[1] = z; m[2] = z; m[3] = C{-1, 0}; break;
+        case G_H: m[0] = C{s, 0}; m[1] = C{s, 0}; m[2] = C{s, 0}; m[3] = C{-s, 0}; break;
+        case G_S: m[0] = o; m[1] = z; m[2] = z; m[3] = i; break;
+        case G_SDG: m[0] = o; m[1] = z; m[2] = z; m[3] = C{0, -1}; break;
+        case G_T: m[0] = o; m[1] = z; m[2] = z; m[3] = C{s, s}; break;
+        case G_TDG: m[0] = o; m[1] = z; m[2] = z; m[3] = C{s, -s}; break;
+        case G_RX: {
+            double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            m[0] = C{c, 0}; m[1] = C{0, -si}; m[2] = C{0, -si}; m[3] = C{c, 0};
+            break;
+        }
+        case G_RY: {
+            double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            m[0] = C{c, 0}; m[1] = C{-si, 0}; m[2] = C{si, 0}; m[3] = C{c, 0};
+            break;
+        }
+        default: {  // G_RZ
+            double half = theta * 0.5;
+            m[0] = C{std::cos(half), -std::sin(half)}; m[1] = z; m[2] = z;
+            m[3] = C{std::cos(half), std::sin(half)};
+            break;
+        }
+    }
+}
+
+// adjoint_gate, src/lib.rs:2370-2382.
+void adjoint_gate(int gate, double theta, int* out_gate, double* out_theta) {
+    *out_theta = theta;
+    switch (gate) {
+        case G_RX: case G_RY: case G_RZ: *out_gate = gate; *out_theta = -theta; break;
+        case G_S: *out_gate = G_SDG; break;
+        case G_SDG: *out_gate = G_S; break;
+        case G_T: *out_gate = G_TDG; break;
+        case G_TDG: *out_gate = G_T; break;
+        default: *out_gate = gate; break;
+    }
+}
+
+struct State {
+    std::vector<C> amps;
+    uint32_t qubits;
+};
+
+// StateVector::apply, rayon branch, src/lib.rs:1017-1044: clone, parallel map, collect.
+int apply_1q_m(State& st, const C m[4], uint32_t q) {
+    if (q >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    const size_t mask = size_t(1) << q;
+    const size_t n = st.amps.size();
+    std::vector<C> old(st.amps);          // self.amplitudes.clone()
+    std::vector<C> next(n);               // collect() target
+    const C* o = old.data();
+    C* w = next.data();
+#pragma omp parallel for schedule(static)
+    for (int64_t sidx = 0; sidx < int64_t(n); ++sidx) {
+        size_t idx = size_t(sidx);
+        if ((idx & mask) == 0) {
+            w[idx] = cadd(cmul(m[0], o[idx]), cmul(m[1], o[idx | mask]));
+        } else {
+            w[idx] = cadd(cmul(m[2], o[idx & ~mask]), cmul(m[3], o[idx]));
+        }
+    }
+    st.amps.swap(next);                   // self.amplitudes = new
+    return OK;
+}
+
+int apply_1q(State& st, int gate, double theta, uint32_t q) {
+    if (q >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    C m[4];
+    gate_matrix(gate, theta, m);
+    return apply_1q_m(st, m, q);
+}
+
+// check_pair, src/lib.rs:1124-1141.
+int check_pair(const State& st, uint32_t a, uint32_t b) {
+    if (a >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    if (b >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    if (a == b) return INVALID_QUBIT_PAIR;
+    return OK;
+}
+
+// Shared free-qubit enumeration of apply_cnot / apply_cz / apply_swap
+// (src/lib.rs:1062-1122): sequential, index rebuilt bit by bit.
+template <class F>
+void for_free(const State& st, uint32_t a, uint32_t b, size_t seed, F f) {
+    std::vector<uint32_t> free_q;
+    for (uint32_t q = 0; q < st.qubits; ++q)
+        if (q != a && q != b) free_q.push_back(q);
+    const size_t n_free = free_q.size();
+    for (size_t fv = 0; fv < (size_t(1) << n_free); ++fv) {
+        size_t base = seed;
+        for (size_t i = 0; i < n_free; ++i)
+            if (fv & (size_t(1) << i)) base |= size_t(1) << free_q[i];
+        f(base);
+    }
+}
+
+// StateVector::apply_cnot, src/lib.rs:1062-1080.
+int apply_cnot(State& st, uint32_t c, uint32_t t) {
+    int e = check_pair(st, c, t);
+    if (e) return e;
+    const size_t tm = size_t(1) << t;
+    for_free(st, c, t, size_t(1) << c, [&](size_t base) { std::swap(st.amps[base], st.amps[base | tm]); });
+    return OK;
+}
+
+// StateVector::apply_cz, src/lib.rs:1083-1101.
+int apply_cz(State& st, uint32_t a, uint32_t b) {
+    int e = check_pair(st, a, b);
+    if (e) return e;
+    for_free(st, a, b, (size_t(1) << a) | (size_t(1) << b),
+             [&](size_t idx) { st.amps[idx] = cmulf(st.amps[idx], -1.0); });
+    return OK;
+}
+
+// StateVector::apply_swap, src/lib.rs:1104-1122.
+int apply_swap(State& st, uint32_t a, uint32_t b) {
+    int e = check_pair(st, a, b);
+    if (e) return e;
+    const size_t flip = (size_t(1) << a) ^ (size_t(1) << b);
+    for_free(st, a, b, size_t(1) << b, [&](size_t idx) { std::swap(st.amps[idx], st.amps[idx ^ flip]); });
+    return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- op records (layout identical to qf_op / qf_pop / qf_pauli_term in include/qf_hip.h)
+struct or_op {
+    int32_t kind;   // 0 single-qubit, 1 CNOT, 2 CZ, 3 SWAP, 4 channel, 5 measure, 6 conditional
+    int32_t gate;
+    uint32_t a, b;
+    double theta;
+};
+struct or_pop {
+    int32_t kind;   // 0 fixed, 1 rotation, 2 encoding rotation, 3 CNOT, 4 CZ, 5 SWAP, 6 channel
+    int32_t gate;   // gate id (fixed) or axis 0=X 1=Y 2=Z (rotations)
+    uint32_t a, b;
+    uint32_t index; // parameter / input index
+    uint32_t pad;
+    double theta;   // angle of a fixed rotation gate
+};
+struct or_pauli_term {
+    double re, im;
+    uint64_t x_mask, z_mask;  // Y on q <=> both bits set
+};
+
+// ---- SmallRng, src/lib.rs:4881-4912
+uint64_t or_rng_seeded(uint64_t seed) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z == 0 ? 0x9E3779B97F4A7C15ull : z;
+}
+uint64_t or_rng_next_u64(uint64_t* s) {
+    uint64_t x = *s;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    *s = x;
+    return x;
+}
+double or_rng_next_f64(uint64_t* s) {
+    return double(or_rng_next_u64(s) >> 11) * (1.0 / double(uint64_t(1) << 53));
+}
+
+void or_gate_matrix(int gate, double theta, double* out8) {
+    C m[4];
+    gate_matrix(gate, theta, m);
+    std::memcpy(out8, m, sizeof(m));
+}
+
+// ---- state handle
+void* or_state_zero(uint32_t qubits) {  // StateVector::zero, src/lib.rs:930-966
+    if (qubits == 0 || qubits > 40) return nullptr;
+    State* st = new State;
+    st->qubits = qubits;
+    st->amps.assign(size_t(1) << qubits, C{0, 0});
+    st->amps[0] = C{1, 0};
+    return st;
+}
+void* or_state_from(const double* amps, uint32_t qubits) {  // from_amplitudes, src/lib.rs:969-978
+    State* st = new State;
+    st->qubits = qubits;
+    st->amps.resize(size_t(1) << qubits);
+    std::memcpy(st->amps.data(), amps, st->amps.size() * sizeof(C));
+    return st;
+}
+void or_state_free(void* s) { delete static_cast<State*>(s); }
+void or_state_get(const void* s, double* out) {
+    const State* st = static_cast<const State*>(s);
+    std::memcpy(out, st->amps.data(), st->amps.size() * sizeof(C));
+}
+int or_apply_gate(void* s, int gate, double theta, uint32_t q) {
+    return apply_1q(*static_cast<State*>(s), gate, theta, q);
+}
+int or_apply_matrix(void* s, const double* m8, uint32_t q) {
+    C m[4];
+    std::memcpy(m, m8, sizeof(m));
+    return apply_1q_m(*static_cast<State*>(s), m, q);
+}
+int or_apply_cnot(void* s, uint32_t c, uint32_t t) { return apply_cnot(*static_cast<State*>(s), c, t); }
+int or_apply_cz(void* s, uint32_t a, uint32_t b) { return apply_cz(*static_cast<State*>(s), a, b); }
+int or_apply_swap(void* s, uint32_t a, uint32_t b) { return apply_swap(*static_cast<State*>(s), a, b); }
+
+// Circuit::run, src/lib.rs:2786-2802 (ops validated one at a time, as they run).
+int or_run(void* s, const or_op* ops, size_t nops) {
+    State& st = *static_cast<State*>(s);
+    for (size_t i = 0; i < nops; ++i) {
+        const or_op& op = ops[i];
+        int e = OK;
+        switch (op.kind) {
+            case 0: e = apply_1q(st, op.gate, op.theta, op.a); break;
+            case 1: e = apply_cnot(st, op.a, op.b); break;
+            case 2: e = apply_cz(st, op.a, op.b); break;
+            case 3: e = apply_swap(st, op.a, op.b); break;
+            case 4: return REQUIRES_DENSITY_MATRIX;
+            case 5: case 6: return REQUIRES_SHOT_EXECUTION;
+            default: return INVALID_ARGUMENT;
+        }
+        if (e) return e;
+    }
+    return OK;
+}
+
+// Wall-clock of or_run over `reps` fresh |0..0> states (bench CPU leg).
+double or_time_run(uint32_t qubits, const or_op* ops, size_t nops, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    State* st = static_cast<State*>(or_state_zero(qubits));
+    auto t0 = std::chrono::steady_clock::now();
+    int e = or_run(st, ops, nops);
+    auto t1 = std::chrono::steady_clock::now();
+    or_state_free(st);
+    if (e) return -double(e);
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+int or_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// ---- reductions
+double or_norm_sqr(const void* s) {  // src/lib.rs:1000-1002, sequential left-to-right
+    const State& st = *static_cast<const State*>(s);
+    double acc = 0.0;
+    for (const C& a : st.amps) acc += cnorm(a);
+    return acc;
+}
+void or_probabilities(const void* s, double* out) {  // src/lib.rs:1144-1146
+    const State& st = *static_cast<const State*>(s);
+    for (size_t i = 0; i < st.amps.size(); ++i) out[i] = cnorm(st.amps[i]);
+}
+// expectation_z / _x / _y, src/lib.rs:1149-1318.  obs: 0=X 1=Y 2=Z.
+int or_expectation(const void* s, int obs, uint32_t q, double* out) {
+    const State& st = *static_cast<const State*>(s);
+    if (q >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    const size_t mask = size_t(1) << q;
+    double acc = 0.0;
+    if (obs == 2) {
+        for (size_t i = 0; i < st.amps.size(); ++i) {
+            double sign = (i & mask) == 0 ? 1.0 : -1.0;
+            acc += sign * cnorm(st.amps[i]);
+        }
+    } else {
+        for (size_t i = 0; i < st.amps.size(); ++i) {
+            if (i & mask) continue;
+            C p = cmul(cconj(st.amps[i]), st.amps[i | mask]);
+            acc += (obs == 0 ? p.re : p.im) * 2.0;
+        }
+    }
+    *out = acc;
+    return OK;
+}
+
+// PauliWord::apply_basis_index, src/analytic.rs:2629-2652.
+static void pauli_basis(const or_pauli_term& t, uint32_t qubits, size_t index, size_t* dest, C* phase) {
+    C ph{1, 0};
+    for (uint32_t q = 0; q < qubits; ++q) {
+        const uint64_t m = uint64_t(1) << q;
+        const bool x = t.x_mask & m, z = t.z_mask & m;
+        const bool one = index & m;
+        if (x && z) {  // Y
+            ph = cmul(ph, one ? C{0, -1} : C{0, 1});
+            index ^= m;
+        } else if (x) {
+            index ^= m;
+        } else if (z && one) {
+            ph = cmulf(ph, -1.0);
+        }
+    }
+    *dest = index;
+    *phase = ph;
+}
+
+// PauliSum::expectation_ket, src/analytic.rs:2751-2778 -> SparseOperator::new
+// (entries merged per (row, col), src/quantum.rs:797-837) -> apply_amplitudes
+// (src/quantum.rs:1340-1354) -> Bra::inner (src/quantum.rs:327-337).
+int or_pauli_expectation(const void* s, const or_pauli_term* terms, size_t nterms, double* out_re,
+                         double* out_im) {
+    const State& st = *static_cast<const State*>(s);
+    const size_t dim = st.amps.size();
+    // Group terms by x_mask: every term of one group maps column -> row = column ^ x.
+    std::map<uint64_t, std::vector<size_t>> groups;
+    for (size_t t = 0; t < nterms; ++t) groups[terms[t].x_mask].push_back(t);
+    C acc{0, 0};
+    std::vector<std::pair<size_t, C>> row_entries;
+    for (size_t row = 0; row < dim; ++row) {
+        row_entries.clear();
+        for (auto& g : groups) {
+            const size_t col = row ^ size_t(g.first);
+            C v{0, 0};
+            bool any = false;
+            for (size_t t : g.second) {
+                size_t dest;
+                C ph;
+                pauli_basis(terms[t], st.qubits, col, &dest, &ph);
+                C coeff{terms[t].re, terms[t].im};
+                v = any ? cadd(v, cmul(coeff, ph)) : cmul(coeff, ph);
+                any = true;
+            }
+            row_entries.emplace_back(col, v);
+        }
+        std::sort(row_entries.begin(), row_entries.end(),
+                  [](const std::pair<size_t, C>& x, const std::pair<size_t, C>& y) { return x.first < y.first; });
+        C hv{0, 0};
+        for (auto& e : row_entries) hv = cadd(hv, cmul(e.second, st.amps[e.first]));
+        acc = cadd(acc, cmul(cconj(st.amps[row]), hv));
+    }
+    *out_re = acc.re;
+    *out_im = acc.im;
+    return OK;
+}
+
+// H|psi> (sparse-operator product, same merge as above) written into out.
+static void pauli_apply(const State& st, const or_pauli_term* terms, size_t nterms, std::vector<C>& out) {
+    const size_t dim = st.amps.size();
+    out.assign(dim, C{0, 0});
+    for (size_t t = 0; t < nterms; ++t) {
+        C coeff{terms[t].re, terms[t].im};
+        for (size_t col = 0; col < dim; ++col) {
+            size_t row;
+            C ph;
+            pauli_basis(terms[t], st.qubits, col, &row, &ph);
+            out[row] = cadd(out[row], cmul(cmul(coeff, ph), st.amps[col]));
+        }
+    }
+}
+
+// ---- parameterized circuits
+struct PCirc {
+    const or_pop* ops;
+    size_t nops;
+    uint32_t nparams, ninputs;
+};
+
+// validate_parameters / validate_inputs, src/lib.rs:4651-4675.
+static int validate(const PCirc& pc, const double* params, size_t np, const double* inputs, size_t ni) {
+    if (ni != pc.ninputs) return INVALID_INPUT_COUNT;
+    for (size_t i = 0; i < ni; ++i)
+        if (!std::isfinite(inputs[i])) return NON_FINITE_PARAMETER;
+    if (np != pc.nparams) return INVALID_PARAMETER_COUNT;
+    for (size_t i = 0; i < np; ++i)
+        if (!std::isfinite(params[i])) return NON_FINITE_PARAMETER;
+    return OK;
+}
+
+static int axis_gate(int axis) { return axis == 0 ? G_RX : (axis == 1 ? G_RY : G_RZ); }
+
+// run_with_shift, src/lib.rs:4677-4738.
+static int run_with_shift(const PCirc& pc, const double* params, const double* inputs, State& st,
+                          int64_t shift_op, double delta) {
+    for (size_t i = 0; i < pc.nops; ++i) {
+        const or_pop& op = pc.ops[i];
+        int e = OK;
+        switch (op.kind) {
+            case 0: e = apply_1q(st, op.gate, op.theta, op.a); break;
+            case 1: {
+                if (op.index >= pc.nparams) return INVALID_PARAMETER_COUNT;
+                double th = params[op.index] + (int64_t(i) == shift_op ? delta : 0.0);
+                e = apply_1q(st, axis_gate(op.gate), th, op.a);
+                break;
+            }
+            case 2:
+                if (op.index >= pc.ninputs) return INVALID_INPUT_COUNT;
+                e = apply_1q(st, axis_gate(op.gate), inputs[op.index], op.a);
+                break;
+            case 3: e = apply_cnot(st, op.a, op.b); break;
+            case 4: e = apply_cz(st, op.a, op.b); break;
+            case 5: e = apply_swap(st, op.a, op.b); break;
+            case 6: return REQUIRES_DENSITY_MATRIX;
+            default: return INVALID_ARGUMENT;
+        }
+        if (e) return e;
+    }
+    return OK;
+}
+
+// Observable = Pauli sum; a single-qubit observable is one term with coefficient 1.
+static double energy(const State& st, const or_pauli_term* terms, size_t nterms) {
+    double re, im;
+    or_pauli_expectation(&st, terms, nterms, &re, &im);
+    return re;
+}
+static bool single_qubit_obs(const or_pauli_term* terms, size_t nterms, int* obs, uint32_t* q) {
+    if (nterms != 1 || terms[0].re != 1.0 || terms[0].im != 0.0) return false;
+    uint64_t x = terms[0].x_mask, z = terms[0].z_mask, all = x | z;
+    if (all == 0 || (all & (all - 1))) return false;
+    *q = uint32_t(__builtin_ctzll(all));
+    *obs = (x && z) ? 1 : (x ? 0 : 2);
+    return true;
+}
+static double observe(const State& st, const or_pauli_term* terms, size_t nterms) {
+    int obs;
+    uint32_t q;
+    if (single_qubit_obs(terms, nterms, &obs, &q)) {  // State::expectation, src/lib.rs:1306-1318
+        double v;
+        or_expectation(&st, obs, q, &v);
+        return v;
+    }
+    return energy(st, terms, nterms);
+}
+static int check_obs(const or_pauli_term* terms, size_t nterms, uint32_t qubits) {
+    for (size_t t = 0; t < nterms; ++t)
+        if (qubits < 64 && ((terms[t].x_mask | terms[t].z_mask) >> qubits)) return QUBIT_OUT_OF_RANGE;
+    return OK;
+}
+
+int or_pc_run(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs, const double* params,
+              size_t np, const double* inputs, size_t ni, uint32_t qubits, int64_t shift_op, double delta,
+              double* out_amps) {
+    PCirc pc{ops, nops, nparams, ninputs};
+    int e = validate(pc, params, np, inputs, ni);
+    if (e) return e;
+    if (qubits == 0) return ZERO_QUBITS;
+    State* st = static_cast<State*>(or_state_zero(qubits));
+    e = run_with_shift(pc, params, inputs, *st, shift_op, delta);
+    if (!e && out_amps) std::memcpy(out_amps, st->amps.data(), st->amps.size() * sizeof(C));
+    or_state_free(st);
+    return e;
+}
+
+int or_pc_expectation(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs, const double* params,
+                      const double* inputs, uint32_t qubits, const or_pauli_term* terms, size_t nterms,
+                      double* out) {
+    PCirc pc{ops, nops, nparams, ninputs};
+    int e = validate(pc, params, nparams, inputs, ninputs);
+    if (e) return e;
+    if ((e = check_obs(terms, nterms, qubits))) return e;
+    State* st = static_cast<State*>(or_state_zero(qubits));
+    e = run_with_shift(pc, params, inputs, *st, -1, 0.0);
+    if (!e) *out = observe(*st, terms, nterms);
+    or_state_free(st);
+    return e;
+}
+
+// parameter_shift_gradient_with_inputs, src/lib.rs:3516-3546 (shift ±pi/2 per
+// rotation occurrence), and finite_difference_gradient_with_inputs,
+// src/gradients.rs:344-375 (step h, divided by 2h).  fd_step <= 0 selects the shift rule.
+static int shift_like(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs, const double* params,
+                      const double* inputs, uint32_t qubits, const or_pauli_term* terms, size_t nterms,
+                      bool fd, double step, double* grad) {
+    PCirc pc{ops, nops, nparams, ninputs};
+    if (fd && (!std::isfinite(step) || step <= 0.0)) return INVALID_FD_STEP;
+    int e = validate(pc, params, nparams, inputs, ninputs);
+    if (e) return e;
+    if ((e = check_obs(terms, nterms, qubits))) return e;
+    for (uint32_t p = 0; p < nparams; ++p) grad[p] = 0.0;
+    const double shift = fd ? step : 1.57079632679489661923;  // FRAC_PI_2
+    for (size_t i = 0; i < nops; ++i) {
+        if (ops[i].kind != 1) continue;
+        State* a = static_cast<State*>(or_state_zero(qubits));
+        State* b = static_cast<State*>(or_state_zero(qubits));
+        e = run_with_shift(pc, params, inputs, *a, int64_t(i), shift);
+        if (!e) e = run_with_shift(pc, params, inputs, *b, int64_t(i), -shift);
+        if (!e) {
+            double plus = observe(*a, terms, nterms), minus = observe(*b, terms, nterms);
+            if (fd)
+                grad[ops[i].index] += (plus - minus) / (2.0 * step);
+            else
+                grad[ops[i].index] += 0.5 * (plus - minus);
+        }
+        or_state_free(a);
+        or_state_free(b);
+        if (e) return e;
+    }
+    return OK;
+}
+int or_pc_shift_gradient(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs, const double* params,
+                         const double* inputs, uint32_t qubits, const or_pauli_term* terms, size_t nterms,
+                         double* grad) {
+    return shift_like(ops, nops, nparams, ninputs, params, inputs, qubits, terms, nterms, false, 0.0, grad);
+}
+int or_pc_fd_gradient(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs, const double* params,
+                      const double* inputs, uint32_t qubits, const or_pauli_term* terms, size_t nterms, double step,
+                      double* grad) {
+    return shift_like(ops, nops, nparams, ninputs, params, inputs, qubits, terms, nterms, true, step, grad);
+}
+
+// adjoint_gradient_with_inputs, src/lib.rs:3564-3704: forward sweep storing every
+// state, costate = O psi_final, reverse sweep accumulating
+// 2 Re sum conj(lambda) * ((P psi_after) * (-i/2)) then lambda <- U^dagger lambda.
+// With a Pauli-sum observable the costate is H psi_final (the sum's sparse product).
+int or_pc_adjoint_gradient(const or_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                           const double* params, const double* inputs, uint32_t qubits,
+                           const or_pauli_term* terms, size_t nterms, double* grad) {
+    PCirc pc{ops, nops, nparams, ninputs};
+    int e = validate(pc, params, nparams, inputs, ninputs);
+    if (e) return e;
+    if ((e = check_obs(terms, nterms, qubits))) return e;
+    std::vector<State> states;
+    states.reserve(nops + 1);
+    State* z = static_cast<State*>(or_state_zero(qubits));
+    states.push_back(*z);
+    or_state_free(z);
+    for (size_t i = 0; i < nops; ++i) {
+        State next = states[i];
+        const or_pop& op = ops[i];
+        switch (op.kind) {
+            case 0: e = apply_1q(next, op.gate, op.theta, op.a); break;
+            case 1:
+                if (op.index >= nparams) return INVALID_PARAMETER_COUNT;
+                e = apply_1q(next, axis_gate(op.gate), params[op.index], op.a);
+                break;
+            case 2:
+                if (op.index >= ninputs) return INVALID_INPUT_COUNT;
+                e = apply_1q(next, axis_gate(op.gate), inputs[op.index], op.a);
+                break;
+            case 3: e = apply_cnot(next, op.a, op.b); break;
+            case 4: e = apply_cz(next, op.a, op.b); break;
+            case 5: e = apply_swap(next, op.a, op.b); break;
+            case 6: return REQUIRES_DENSITY_MATRIX;
+            default: return INVALID_ARGUMENT;
+        }
+        if (e) return e;
+        states.push_back(std::move(next));
+    }
+    State costate = states.back();
+    int obs;
+    uint32_t oq;
+    if (single_qubit_obs(terms, nterms, &obs, &oq)) {
+        e = apply_1q(costate, obs == 0 ? G_X : (obs == 1 ? G_Y : G_Z), 0.0, oq);
+        if (e) return e;
+    } else {
+        pauli_apply(states.back(), terms, nterms, costate.amps);
+    }
+    for (uint32_t p = 0; p < nparams; ++p) grad[p] = 0.0;
+    const C minus_i_half{0.0, -0.5};
+    for (size_t ii = nops; ii-- > 0;) {
+        const or_pop& op = ops[ii];
+        const State& after = states[ii + 1];
+        if (op.kind == 1) {
+            State deriv = after;
+            e = apply_1q(deriv, op.gate == 0 ? G_X : (op.gate == 1 ? G_Y : G_Z), 0.0, op.a);
+            if (e) return e;
+            double overlap = 0.0;
+            for (size_t k = 0; k < deriv.amps.size(); ++k)
+                overlap += cmul(cconj(costate.amps[k]), cmul(deriv.amps[k], minus_i_half)).re;
+            grad[op.index] += 2.0 * overlap;
+        }
+        switch (op.kind) {
+            case 0: {
+                int g;
+                double th;
+                adjoint_gate(op.gate, op.theta, &g, &th);
+                e = apply_1q(costate, g, th, op.a);
+                break;
+            }
+            case 1: e = apply_1q(costate, axis_gate(op.gate), -params[op.index], op.a); break;
+            case 2: e = apply_1q(costate, axis_gate(op.gate), -inputs[op.index], op.a); break;
+            case 3: e = apply_cnot(costate, op.a, op.b); break;
+            case 4: e = apply_cz(costate, op.a, op.b); break;
+            case 5: e = apply_swap(costate, op.a, op.b); break;
+            default: break;
+        }
+        if (e) return e;
+    }
+    return OK;
+}
+
+// ---- in-place CPU variant (reported separately from the restatement in the bench):
+// same arithmetic per pair, no clone, OpenMP over pairs and over permutation quads.
+int or_fast_run(double* amps_d, uint32_t qubits, const or_op* ops, size_t nops) {
+    C* amps = reinterpret_cast<C*>(amps_d);
+    const size_t half = size_t(1) << (qubits - 1), quarter = size_t(1) << (qubits >= 2 ? qubits - 2 : 0);
+    for (size_t i = 0; i < nops; ++i) {
+        const or_op& op = ops[i];
+        if (op.kind == 0) {
+            if (op.a >= qubits) return QUBIT_OUT_OF_RANGE;
+            C m[4];
+            gate_matrix(op.gate, op.theta, m);
+            const size_t mask = size_t(1) << op.a, lo = mask - 1;
+#pragma omp parallel for schedule(static)
+            for (int64_t sp = 0; sp < int64_t(half); ++sp) {
+                size_t p = size_t(sp);
+                size_t i0 = ((p & ~lo) << 1) | (p & lo), i1 = i0 | mask;
+                C a0 = amps[i0], a1 = amps[i1];
+                amps[i0] = cadd(cmul(m[0], a0), cmul(m[1], a1));
+                amps[i1] = cadd(cmul(m[2], a0), cmul(m[3], a1));
+            }
+        } else if (op.kind >= 1 && op.kind <= 3) {
+            if (op.a >= qubits || op.b >= qubits) return QUBIT_OUT_OF_RANGE;
+            if (op.a == op.b) return INVALID_QUBIT_PAIR;
+            const uint32_t lo_q = std::min(op.a, op.b), hi_q = std::max(op.a, op.b);
+            const size_t ma = size_t(1) << op.a, mb = size_t(1) << op.b;
+#pragma omp parallel for schedule(static)
+            for (int64_t sp = 0; sp < int64_t(quarter); ++sp) {
+                size_t p = size_t(sp);
+                size_t lm = (size_t(1) << lo_q) - 1;
+                size_t x = ((p & ~lm) << 1) | (p & lm);
+                size_t hm = (size_t(1) << hi_q) - 1;
+                size_t base = ((x & ~hm) << 1) | (x & hm);
+                if (op.kind == 1) std::swap(amps[base | ma], amps[base | ma | mb]);
+                else if (op.kind == 2) amps[base | ma | mb] = cmulf(amps[base | ma | mb], -1.0);
+                else std::swap(amps[base | ma], amps[base | mb]);
+            }
+        } else if (op.kind == 4) {
+            return REQUIRES_DENSITY_MATRIX;
+        } else {
+            return REQUIRES_SHOT_EXECUTION;
+        }
+    }
+    return OK;
+}
+
+}  // extern "C"

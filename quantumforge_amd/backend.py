@@ -1,0 +1,292 @@
+"""HIP backend: the drop-in for the reference's `cuda.rs` (CudaBackend,
+CudaDeviceState, CudaDeviceStateBatch, src/cuda.rs:484-2005), over the C ABI
+of include/qf_hip.h."""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from ._native import QuantumError, check, lib, ptr
+from .core import Circuit, Gate, PauliSum, RotationAxis, SingleQubitObservable, StateVector
+
+C128, C64 = 0, 1
+_PRECISION = {"complex128": C128, "c128": C128, C128: C128, "complex64": C64, "c64": C64, C64: C64}
+
+
+def _precision(p) -> int:
+    try:
+        return _PRECISION[p]
+    except KeyError:
+        raise ValueError(f"unknown precision {p!r}; use 'complex128' or 'complex64'") from None
+
+
+class Plan:
+    """Fused execution plan of a circuit (qf_plan_build)."""
+
+    def __init__(self, circuit: Circuit, qubits: int, fuse: bool = True, tile_qubits: int = 0,
+                 low_qubits: int = 0) -> None:
+        self._ops = circuit.to_ops()
+        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0)
+        h = C.c_void_p()
+        check(lib().qf_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.qubits = qubits
+
+    def stats(self) -> dict:
+        s = N.PlanStats()
+        check(lib().qf_plan_stats_get(self._h, C.byref(s)))
+        return {f: int(getattr(s, f)) for f, _ in N.PlanStats._fields_}
+
+    def describe(self) -> dict:
+        need = C.c_size_t()
+        check(lib().qf_plan_describe(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        check(lib().qf_plan_describe(self._h, buf, need.value, C.byref(need)))
+        return json.loads(buf.value.decode())
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            lib().qf_plan_destroy(h)
+            self._h = None
+
+
+class HipBackend:
+    """CudaBackend::new(ordinal), src/cuda.rs:556."""
+
+    def __init__(self, ordinal: int = 0) -> None:
+        h = C.c_void_p()
+        check(lib().qf_backend_create(int(ordinal), C.byref(h)))
+        self._h = h
+        self.ordinal = int(ordinal)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            lib().qf_backend_destroy(h)
+            self._h = None
+
+    def device_ordinal(self) -> int:
+        return self.ordinal
+
+    def synchronize(self) -> None:
+        check(lib().qf_backend_synchronize(self._h))
+
+    def set_plan_options(self, fuse: bool = True, tile_qubits: int = 0, low_qubits: int = 0) -> None:
+        o = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0)
+        check(lib().qf_backend_set_plan_options(self._h, C.byref(o)))
+
+    def stream(self) -> int:
+        out = C.c_void_p()
+        check(lib().qf_backend_stream(self._h, C.byref(out)))
+        return out.value or 0
+
+    # --- states -----------------------------------------------------------
+    def zero_state(self, qubits: int, precision="complex128") -> "HipDeviceState":
+        h = C.c_void_p()
+        check(lib().qf_state_zero(self._h, qubits, _precision(precision), C.byref(h)))
+        return HipDeviceState(h, self)
+
+    def upload(self, state: StateVector, precision="complex128") -> "HipDeviceState":
+        """CudaBackend::upload, src/cuda.rs:844-867."""
+        h = C.c_void_p()
+        if _precision(precision) == C128:
+            a = np.ascontiguousarray(state.amplitudes, dtype=np.complex128)
+            check(lib().qf_state_upload(self._h, ptr(a), state.qubits, C.byref(h)))
+        else:
+            a = np.ascontiguousarray(state.amplitudes, dtype=np.complex64)
+            check(lib().qf_state_upload_c64(self._h, ptr(a), state.qubits, C.byref(h)))
+        return HipDeviceState(h, self)
+
+    def upload_batch(self, states: Sequence[StateVector]) -> "HipDeviceStateBatch":
+        """CudaBackend::upload_batch, src/cuda.rs:871-954 (EmptyBatch, IncompatibleBatch)."""
+        if len(states) == 0:
+            raise QuantumError(6, "a HIP state batch cannot be empty")
+        qubits = np.array([s.qubits for s in states], dtype=np.uint32)
+        if np.any(qubits != qubits[0]):
+            raise QuantumError(7, "all states in a HIP batch must have equal dimensions")
+        amps = np.ascontiguousarray(np.stack([s.amplitudes for s in states]), dtype=np.complex128)
+        h = C.c_void_p()
+        check(lib().qf_batch_upload(self._h, ptr(amps), ptr(qubits), len(states), C.byref(h)))
+        return HipDeviceStateBatch(h, self)
+
+    def zero_batch(self, qubits: int, count: int, precision="complex128") -> "HipDeviceStateBatch":
+        h = C.c_void_p()
+        check(lib().qf_batch_zero(self._h, qubits, count, _precision(precision), C.byref(h)))
+        return HipDeviceStateBatch(h, self)
+
+    # --- convenience (src/cuda.rs:958-1083) --------------------------------
+    def run_circuit(self, state: StateVector, circuit: Circuit, plan: Optional[Plan] = None) -> StateVector:
+        dev = self.upload(state)
+        dev.run_circuit(circuit, plan)
+        return dev.download()
+
+    def run_circuit_batch(self, states: Sequence[StateVector], circuit: Circuit) -> List[StateVector]:
+        batch = self.upload_batch(states)
+        batch.run_circuit(circuit)
+        return batch.download()
+
+    def apply_rotation_chain(self, state: StateVector, gates: Sequence[Tuple[RotationAxis, int, float]]) -> None:
+        """Updates `state` in place (CudaBackend::apply_rotation_chain)."""
+        dev = self.upload(state)
+        dev.apply_rotation_chain(gates)
+        state.amplitudes[:] = dev.download().amplitudes
+
+    def apply_ry(self, state: StateVector, qubit: int, theta: float) -> None:
+        self.apply_rotation_chain(state, [(RotationAxis.Y, qubit, theta)])
+
+
+class HipDeviceState:
+    """CudaDeviceState, src/cuda.rs:1277-1484: a resident device state."""
+
+    def __init__(self, handle: C.c_void_p, backend: HipBackend) -> None:
+        self._h = handle
+        self._backend = backend  # keeps the Python object alive; the C side refcounts too
+        q = C.c_uint32()
+        check(lib().qf_state_qubits(handle, C.byref(q)))
+        self.qubits_ = int(q.value)
+        p = C.c_int32()
+        check(lib().qf_state_precision(handle, C.byref(p)))
+        self.precision = int(p.value)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            lib().qf_state_destroy(h)
+            self._h = None
+
+    def qubits(self) -> int:
+        return self.qubits_
+
+    def device_pointer(self) -> int:
+        out = C.c_void_p()
+        check(lib().qf_state_device_pointer(self._h, C.byref(out)))
+        return out.value or 0
+
+    def apply_gate(self, gate: Gate, qubit: int) -> None:
+        check(lib().qf_state_apply_gate(self._h, gate.id, int(qubit), float(gate.theta)))
+
+    def apply_matrix(self, m: np.ndarray, qubit: int) -> None:
+        mm = np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(4))
+        check(lib().qf_state_apply_matrix(self._h, int(qubit), ptr(mm)))
+
+    def apply_rotation(self, axis: RotationAxis, qubit: int, theta: float) -> None:
+        self.apply_rotation_chain([(axis, qubit, theta)])
+
+    def apply_rotation_chain(self, gates: Sequence[Tuple[RotationAxis, int, float]]) -> None:
+        arr = np.zeros(len(gates), dtype=N.ROT_DTYPE)
+        for i, (ax, q, t) in enumerate(gates):
+            arr[i] = (int(ax), int(q), float(t))
+        check(lib().qf_state_apply_rotation_chain(self._h, ptr(arr), len(arr)))
+
+    def apply_cnot(self, control: int, target: int) -> None:
+        check(lib().qf_state_apply_2q(self._h, 0, int(control), int(target)))
+
+    def apply_cz(self, first: int, second: int) -> None:
+        check(lib().qf_state_apply_2q(self._h, 1, int(first), int(second)))
+
+    def apply_swap(self, first: int, second: int) -> None:
+        check(lib().qf_state_apply_2q(self._h, 2, int(first), int(second)))
+
+    def apply_cp(self, control: int, target: int, phi: float) -> None:
+        check(lib().qf_state_apply_cp(self._h, int(control), int(target), float(phi)))
+
+    def run_circuit(self, circuit: Circuit, plan: Optional[Plan] = None) -> None:
+        ops = circuit.to_ops()
+        check(lib().qf_state_run_circuit(self._h, ptr(ops), len(ops), plan._h if plan is not None else None))
+
+    def copy_from_device(self, source: "HipDeviceState") -> None:
+        check(lib().qf_state_copy_from(self._h, source._h))
+
+    def download(self) -> StateVector:
+        out = np.empty(1 << self.qubits_, dtype=np.complex128)
+        check(lib().qf_state_download(self._h, ptr(out)))
+        return StateVector(out, self.qubits_)
+
+    def download_c64(self) -> np.ndarray:
+        out = np.empty(1 << self.qubits_, dtype=np.complex64)
+        check(lib().qf_state_download_c64(self._h, ptr(out)))
+        return out
+
+    def norm_sqr(self) -> float:
+        v = C.c_double()
+        check(lib().qf_state_norm_sqr(self._h, C.byref(v)))
+        return v.value
+
+    def probabilities(self) -> np.ndarray:
+        out = np.empty(1 << self.qubits_, dtype=np.float64)
+        check(lib().qf_state_probabilities(self._h, ptr(out)))
+        return out
+
+    def expectation(self, observable: SingleQubitObservable, qubit: int) -> float:
+        v = C.c_double()
+        check(lib().qf_state_expectation(self._h, int(observable), int(qubit), C.byref(v)))
+        return v.value
+
+    def expectation_pauli_sum(self, h: PauliSum) -> complex:
+        t = h.to_terms()
+        re, im = C.c_double(), C.c_double()
+        check(lib().qf_state_expectation_pauli_sum(self._h, ptr(t), len(t), C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def inner(self, other: "HipDeviceState") -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().qf_state_inner(self._h, other._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+
+class HipDeviceStateBatch:
+    """CudaDeviceStateBatch, src/cuda.rs:1486-2005: equal-width resident batch."""
+
+    def __init__(self, handle: C.c_void_p, backend: HipBackend) -> None:
+        self._h = handle
+        self._backend = backend
+        q, n = C.c_uint32(), C.c_uint32()
+        check(lib().qf_batch_shape(handle, C.byref(q), C.byref(n)))
+        self.qubits_, self.count = int(q.value), int(n.value)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            lib().qf_batch_destroy(h)
+            self._h = None
+
+    def qubits(self) -> int:
+        return self.qubits_
+
+    def batch_size(self) -> int:
+        return self.count
+
+    def apply_gate(self, gate: Gate, qubit: int) -> None:
+        check(lib().qf_batch_apply_gate(self._h, gate.id, int(qubit), float(gate.theta)))
+
+    def apply_cnot(self, control: int, target: int) -> None:
+        check(lib().qf_batch_apply_2q(self._h, 0, int(control), int(target)))
+
+    def apply_cz(self, first: int, second: int) -> None:
+        check(lib().qf_batch_apply_2q(self._h, 1, int(first), int(second)))
+
+    def apply_swap(self, first: int, second: int) -> None:
+        check(lib().qf_batch_apply_2q(self._h, 2, int(first), int(second)))
+
+    def run_circuit(self, circuit: Circuit, plan: Optional[Plan] = None) -> None:
+        ops = circuit.to_ops()
+        check(lib().qf_batch_run_circuit(self._h, ptr(ops), len(ops), plan._h if plan is not None else None))
+
+    def synchronize(self) -> None:
+        self._backend.synchronize()
+
+    def download(self) -> List[StateVector]:
+        out = np.empty((self.count, 1 << self.qubits_), dtype=np.complex128)
+        check(lib().qf_batch_download(self._h, ptr(out)))
+        return [StateVector(out[i].copy(), self.qubits_) for i in range(self.count)]
+
+    def expectation_pauli_sum(self, h: PauliSum) -> np.ndarray:
+        t = h.to_terms()
+        out = np.empty(self.count, dtype=np.float64)
+        check(lib().qf_batch_expectation_pauli_sum(self._h, ptr(t), len(t), ptr(out)))
+        return out

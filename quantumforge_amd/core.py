@@ -1,0 +1,301 @@
+"""Circuit IR mirroring the reference crate's API (src/lib.rs).
+
+`Gate`, `Circuit`, `ParameterizedCircuit`, `RotationAxis`,
+`SingleQubitObservable`, `PauliSum` and `StateVector` keep the reference's names,
+argument meaning and error variants.  These are plain host-side descriptions;
+all amplitude arithmetic runs in the HIP engine (quantumforge_amd.backend).
+"""
+from __future__ import annotations
+
+import enum
+import math
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._native import OP_DTYPE, PAULI_DTYPE, POP_DTYPE, QuantumError
+
+# Gate ids of include/qf_hip.h, in the order of `enum Gate` (src/lib.rs:2239-2265).
+_GATE_IDS = {"I": 0, "X": 1, "Y": 2, "Z": 3, "H": 4, "S": 5, "Sdg": 6, "T": 7, "Tdg": 8,
+             "Rx": 9, "Ry": 10, "Rz": 11}
+_GATE_NAMES = {v: k for k, v in _GATE_IDS.items()}
+
+
+@dataclass(frozen=True)
+class Gate:
+    """One-qubit gate (`enum Gate`, src/lib.rs:2239-2265).  Rotations carry an angle."""
+
+    name: str
+    theta: float = 0.0
+
+    @property
+    def id(self) -> int:
+        return _GATE_IDS[self.name]
+
+    def is_rotation(self) -> bool:
+        return self.name in ("Rx", "Ry", "Rz")
+
+    def params(self) -> List[float]:
+        """Gate::params, src/lib.rs:2335-2341."""
+        return [self.theta] if self.is_rotation() else []
+
+    def adjoint(self) -> "Gate":
+        """adjoint_gate, src/lib.rs:2370-2382."""
+        if self.is_rotation():
+            return Gate(self.name, -self.theta)
+        swap = {"S": "Sdg", "Sdg": "S", "T": "Tdg", "Tdg": "T"}
+        return Gate(swap.get(self.name, self.name))
+
+    def __str__(self) -> str:  # Display, src/lib.rs:2344-2361
+        return f"{self.name}({self.theta})" if self.is_rotation() else self.name
+
+    @staticmethod
+    def Rx(theta: float) -> "Gate":
+        return Gate("Rx", float(theta))
+
+    @staticmethod
+    def Ry(theta: float) -> "Gate":
+        return Gate("Ry", float(theta))
+
+    @staticmethod
+    def Rz(theta: float) -> "Gate":
+        return Gate("Rz", float(theta))
+
+
+for _n in ("I", "X", "Y", "Z", "H", "S", "Sdg", "T", "Tdg"):
+    setattr(Gate, _n, Gate(_n))
+
+
+class RotationAxis(enum.IntEnum):
+    """src/lib.rs:2921-2928."""
+    X = 0
+    Y = 1
+    Z = 2
+
+
+class SingleQubitObservable(enum.IntEnum):
+    """src/lib.rs:2931-2938."""
+    X = 0
+    Y = 1
+    Z = 2
+
+
+def rotation_gate(axis: RotationAxis, theta: float) -> Gate:
+    """rotation_gate, src/lib.rs:2363-2368."""
+    return Gate(("Rx", "Ry", "Rz")[int(axis)], float(theta))
+
+
+# ---------------------------------------------------------------------------
+# Circuit (src/lib.rs:2386-2802)
+# ---------------------------------------------------------------------------
+class Circuit:
+    """Ordered list of operations (CircuitOp, src/lib.rs:2386-2440)."""
+
+    def __init__(self) -> None:
+        self._ops: List[tuple] = []
+
+    def push(self, gate: Gate, target: int) -> None:
+        self._ops.append((0, gate.id, int(target), 0, float(gate.theta)))
+
+    def cnot(self, control: int, target: int) -> None:
+        self._ops.append((1, 0, int(control), int(target), 0.0))
+
+    def cz(self, first: int, second: int) -> None:
+        self._ops.append((2, 0, int(first), int(second), 0.0))
+
+    def swap(self, first: int, second: int) -> None:
+        self._ops.append((3, 0, int(first), int(second), 0.0))
+
+    def channel(self, target: int, channel: object = None) -> None:
+        """A noise channel: kept in the list; running it reports OperationRequiresDensityMatrix."""
+        self._ops.append((4, 0, int(target), 0, 0.0))
+
+    def measure(self, target: int, classical_bit: int) -> None:
+        """A measurement: running it reports OperationRequiresShotExecution."""
+        self._ops.append((5, 0, int(target), int(classical_bit), 0.0))
+
+    def extend(self, other: "Circuit") -> None:
+        self._ops.extend(other._ops)
+
+    def __len__(self) -> int:
+        return len(self._ops)
+
+    @property
+    def operations(self) -> List[tuple]:
+        return list(self._ops)
+
+    def num_qubits(self) -> int:
+        """Highest qubit touched + 1 (Circuit::num_qubits, src/lib.rs:2760-2784)."""
+        top = -1
+        for kind, _g, a, b, _t in self._ops:
+            top = max(top, a, b if kind in (1, 2, 3) else -1)
+        return top + 1
+
+    def to_ops(self) -> np.ndarray:
+        """The op records of include/qf_hip.h (qf_op)."""
+        arr = np.zeros(len(self._ops), dtype=OP_DTYPE)
+        for i, (k, g, a, b, t) in enumerate(self._ops):
+            arr[i] = (k, g, a, b, t)
+        return arr
+
+    @staticmethod
+    def from_ops(ops: np.ndarray) -> "Circuit":
+        c = Circuit()
+        for r in ops:
+            c._ops.append((int(r["kind"]), int(r["gate"]), int(r["a"]), int(r["b"]), float(r["theta"])))
+        return c
+
+
+# ---------------------------------------------------------------------------
+# ParameterizedCircuit (src/lib.rs:2942-3121)
+# ---------------------------------------------------------------------------
+class ParameterizedCircuit:
+    """Circuit with runtime-bound rotation parameters (src/lib.rs:2998-3104)."""
+
+    def __init__(self, parameter_count: int, input_count: int = 0) -> None:
+        self.parameter_count = int(parameter_count)
+        self.input_count = int(input_count)
+        self._ops: List[tuple] = []
+
+    @staticmethod
+    def with_inputs(parameter_count: int, input_count: int) -> "ParameterizedCircuit":
+        return ParameterizedCircuit(parameter_count, input_count)
+
+    def push(self, gate: Gate, target: int) -> None:
+        self._ops.append((0, gate.id, int(target), 0, 0, float(gate.theta)))
+
+    def rotation(self, axis: RotationAxis, target: int, parameter: int) -> None:
+        self._ops.append((1, int(axis), int(target), 0, int(parameter), 0.0))
+
+    def encode_rotation(self, axis: RotationAxis, target: int, input: int) -> None:
+        self._ops.append((2, int(axis), int(target), 0, int(input), 0.0))
+
+    def cnot(self, control: int, target: int) -> None:
+        self._ops.append((3, 0, int(control), int(target), 0, 0.0))
+
+    def cz(self, first: int, second: int) -> None:
+        self._ops.append((4, 0, int(first), int(second), 0, 0.0))
+
+    def swap(self, first: int, second: int) -> None:
+        self._ops.append((5, 0, int(first), int(second), 0, 0.0))
+
+    def channel(self, target: int, channel: object = None) -> None:
+        self._ops.append((6, 0, int(target), 0, 0, 0.0))
+
+    def operations(self) -> List[tuple]:
+        return list(self._ops)
+
+    def __len__(self) -> int:
+        return len(self._ops)
+
+    def rotation_count(self) -> int:
+        """Parameterized occurrences (CircuitProfile::parameterized_operations)."""
+        return sum(1 for o in self._ops if o[0] == 1)
+
+    def has_channel(self) -> bool:
+        return any(o[0] == 6 for o in self._ops)
+
+    def to_pops(self) -> np.ndarray:
+        """The op records of include/qf_hip.h (qf_pop)."""
+        arr = np.zeros(len(self._ops), dtype=POP_DTYPE)
+        for i, (k, g, a, b, idx, t) in enumerate(self._ops):
+            arr[i] = (k, g, a, b, idx, 0, t)
+        return arr
+
+    def bind(self, parameters: Sequence[float], inputs: Sequence[float] = (),
+             shift: Optional[Tuple[int, float]] = None) -> Circuit:
+        """Fixed circuit for one parameter binding (run_with_shift, src/lib.rs:4677-4738)."""
+        c = Circuit()
+        for i, (k, g, a, b, idx, t) in enumerate(self._ops):
+            if k == 0:
+                c._ops.append((0, g, a, 0, t))
+            elif k == 1:
+                d = shift[1] if shift is not None and shift[0] == i else 0.0
+                c.push(rotation_gate(RotationAxis(g), float(parameters[idx]) + d), a)
+            elif k == 2:
+                c.push(rotation_gate(RotationAxis(g), float(inputs[idx])), a)
+            elif k == 3:
+                c.cnot(a, b)
+            elif k == 4:
+                c.cz(a, b)
+            elif k == 5:
+                c.swap(a, b)
+            else:
+                c.channel(a)
+        return c
+
+
+# ---------------------------------------------------------------------------
+# Pauli sums (src/analytic.rs:2549-2778)
+# ---------------------------------------------------------------------------
+class PauliSum:
+    """Weighted sum of Pauli words.  A word is a string over I/X/Y/Z whose
+    character k acts on qubit k (PauliWord(Vec<Pauli>), src/analytic.rs:2555)."""
+
+    def __init__(self, qubits: int, terms: Iterable[Tuple[complex, str]] = ()) -> None:
+        self.qubits = int(qubits)
+        self.terms: List[Tuple[complex, str]] = []
+        for c, w in terms:
+            self.add(c, w)
+
+    def add(self, coefficient: complex, word: str) -> None:
+        if len(word) != self.qubits or any(ch not in "IXYZ" for ch in word):
+            raise ValueError(f"Pauli word {word!r} must have {self.qubits} characters from IXYZ")
+        self.terms.append((complex(coefficient), word))
+
+    @staticmethod
+    def single(observable: SingleQubitObservable, qubit: int, qubits: int) -> "PauliSum":
+        w = ["I"] * qubits
+        w[qubit] = "XYZ"[int(observable)]
+        return PauliSum(qubits, [(1.0, "".join(w))])
+
+    def to_terms(self) -> np.ndarray:
+        """qf_pauli_term records: Y on q <=> bit q set in both masks."""
+        arr = np.zeros(len(self.terms), dtype=PAULI_DTYPE)
+        for i, (c, w) in enumerate(self.terms):
+            x = z = 0
+            for q, ch in enumerate(w):
+                if ch in "XY":
+                    x |= 1 << q
+                if ch in "ZY":
+                    z |= 1 << q
+            arr[i] = (c.real, c.imag, x, z)
+        return arr
+
+
+# ---------------------------------------------------------------------------
+# Host-side state container (StateVector, src/lib.rs:671-978)
+# ---------------------------------------------------------------------------
+class StateVector:
+    """Host amplitudes, qubit 0 = least-significant index bit.  A plain
+    container: gates and reductions run on a HIP backend (upload / download)."""
+
+    def __init__(self, amplitudes: np.ndarray, qubits: int) -> None:
+        self.amplitudes = amplitudes
+        self.qubits = qubits
+
+    @staticmethod
+    def zero(qubits: int) -> "StateVector":
+        """StateVector::zero, src/lib.rs:930-966."""
+        if qubits == 0:
+            raise QuantumError(1, "a state vector must contain at least one qubit")
+        if qubits > 40:
+            raise QuantumError(21, "dense simulation size exceeds the supported addressable range")
+        a = np.zeros(1 << qubits, dtype=np.complex128)
+        a[0] = 1.0
+        return StateVector(a, qubits)
+
+    @staticmethod
+    def from_amplitudes(amplitudes: Sequence[complex]) -> "StateVector":
+        """StateVector::from_amplitudes, src/lib.rs:969-978."""
+        a = np.ascontiguousarray(np.asarray(amplitudes, dtype=np.complex128))
+        n = a.shape[0]
+        if n < 2:
+            raise QuantumError(1, "a state vector must contain at least one qubit")
+        if n & (n - 1):
+            raise QuantumError(17, f"state vector length {n} is not a power of two")
+        return StateVector(a, n.bit_length() - 1)
+
+    def copy(self) -> "StateVector":
+        return StateVector(self.amplitudes.copy(), self.qubits)

@@ -1,0 +1,424 @@
+// Circuit planner: exact per-gate sweeps, or fused LDS tile passes.
+//
+// The fused lowering works in three steps:
+//   1. peephole merging on the gate list: runs of single-qubit gates on one
+//      qubit become one 2x2 matrix; the CNOT(c,t) . diag(t) . CNOT(c,t)
+//      sandwich that the reference's QAOA builder emits for every cost term
+//      (src/algorithms.rs:122-126) becomes one two-qubit diagonal; CZ becomes
+//      a diagonal;
+//   2. greedy tile selection: starting from the contiguous low qubits, add the
+//      qubit that lets the most pending gates run, until the tile holds M
+//      qubits (gates run in per-qubit dependency order, so any order the
+//      schedule picks is equivalent to the circuit order);
+//   3. emission: each tile becomes one pass, i.e. one HBM sweep.
+#include "qf_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+
+namespace qf {
+
+namespace {
+
+inline cd mul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline cd add(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+
+// Product r = x * y of 2x2 matrices stored (m00, m01, m10, m11).
+void mat2_mul(const cd x[4], const cd y[4], cd r[4]) {
+    r[0] = add(mul(x[0], y[0]), mul(x[1], y[2]));
+    r[1] = add(mul(x[0], y[1]), mul(x[1], y[3]));
+    r[2] = add(mul(x[2], y[0]), mul(x[3], y[2]));
+    r[3] = add(mul(x[2], y[1]), mul(x[3], y[3]));
+}
+
+bool is_zero(cd a) { return a.re == 0.0 && a.im == 0.0; }
+
+enum NodeKind { N_MAT1, N_DIAG1, N_DIAG2, N_CNOT, N_SWAP };
+
+struct Node {
+    NodeKind kind;
+    uint32_t q[2];
+    int nq;
+    cd m[4];  // MAT1 matrix, DIAG1 (d0, d1), DIAG2 d[bit(q1)*2 + bit(q0)]
+    bool dead = false;
+    int prev[2] = {-1, -1};  // previous live node on q[0], q[1]
+    bool diag() const { return kind == N_DIAG1 || kind == N_DIAG2; }
+};
+
+int slot_of(const Node& n, uint32_t q) { return n.q[0] == q ? 0 : 1; }
+
+// Step 1: peephole merging.
+std::vector<Node> lower(const std::vector<GateIn>& gates, uint32_t qubits) {
+    std::vector<Node> nodes;
+    nodes.reserve(gates.size());
+    std::vector<int> last(qubits, -1);
+    auto push = [&](Node n) {
+        for (int k = 0; k < n.nq; ++k) n.prev[k] = last[n.q[k]];
+        nodes.push_back(n);
+        const int id = int(nodes.size()) - 1;
+        for (int k = 0; k < n.nq; ++k) last[n.q[k]] = id;
+    };
+    for (const GateIn& g : gates) {
+        if (g.kind == 0) {
+            Node n{};
+            n.nq = 1;
+            n.q[0] = g.a;
+            std::memcpy(n.m, g.m, sizeof(n.m));
+            n.kind = (is_zero(g.m[1]) && is_zero(g.m[2])) ? N_DIAG1 : N_MAT1;
+            const int L = last[g.a];
+            if (L >= 0 && (nodes[L].kind == N_MAT1 || nodes[L].kind == N_DIAG1)) {
+                cd r[4];
+                mat2_mul(n.m, nodes[L].m, r);  // new gate after the old one
+                std::memcpy(nodes[L].m, r, sizeof(r));
+                nodes[L].kind = (is_zero(r[1]) && is_zero(r[2])) ? N_DIAG1 : N_MAT1;
+                continue;
+            }
+            if (L >= 0 && n.kind == N_DIAG1 && nodes[L].kind == N_DIAG2) {
+                Node& D = nodes[L];
+                const int s = slot_of(D, g.a);
+                for (int k = 0; k < 4; ++k) D.m[k] = mul(((k >> s) & 1) ? n.m[3] : n.m[0], D.m[k]);
+                continue;
+            }
+            push(n);
+        } else if (g.kind == 1) {  // CNOT(a control, b target)
+            const uint32_t c = g.a, t = g.b;
+            const int Lt = last[t];
+            if (Lt >= 0 && nodes[Lt].kind == N_DIAG1) {
+                const int K = nodes[Lt].prev[0];
+                if (K >= 0 && nodes[K].kind == N_CNOT && nodes[K].q[0] == c && nodes[K].q[1] == t && last[c] == K) {
+                    // CNOT(c,t) diag(d0,d1)_t CNOT(c,t) = diag over (c,t): c=0 -> d(t), c=1 -> d(!t).
+                    const cd d0 = nodes[Lt].m[0], d1 = nodes[Lt].m[3];
+                    Node& D = nodes[K];
+                    D.kind = N_DIAG2;
+                    // slots: q[0]=c, q[1]=t ; index = bit(t)*2 + bit(c)
+                    D.m[0] = d0;  // c0 t0
+                    D.m[1] = d1;  // c1 t0 -> X d X on t: d1
+                    D.m[2] = d1;  // c0 t1
+                    D.m[3] = d0;  // c1 t1
+                    nodes[Lt].dead = true;
+                    last[t] = K;
+                    continue;
+                }
+            }
+            Node n{};
+            n.kind = N_CNOT;
+            n.nq = 2;
+            n.q[0] = c;
+            n.q[1] = t;
+            push(n);
+        } else if (g.kind == 2 || g.kind == 4) {  // CZ or general two-qubit diagonal
+            Node n{};
+            n.kind = N_DIAG2;
+            n.nq = 2;
+            n.q[0] = g.a;
+            n.q[1] = g.b;
+            if (g.kind == 2) {
+                n.m[0] = n.m[1] = n.m[2] = cd{1, 0};
+                n.m[3] = cd{-1, 0};
+            } else {
+                std::memcpy(n.m, g.m, sizeof(n.m));
+            }
+            const int La = last[g.a], Lb = last[g.b];
+            if (La >= 0 && La == Lb && nodes[La].kind == N_DIAG2) {  // diag2 . diag2 on the same pair
+                Node& D = nodes[La];
+                const bool same = D.q[0] == g.a;
+                for (int k = 0; k < 4; ++k) {
+                    const int kk = same ? k : (((k & 1) << 1) | (k >> 1));
+                    D.m[kk] = mul(n.m[k], D.m[kk]);
+                }
+                continue;
+            }
+            push(n);
+        } else {  // SWAP
+            Node n{};
+            n.kind = N_SWAP;
+            n.nq = 2;
+            n.q[0] = g.a;
+            n.q[1] = g.b;
+            push(n);
+        }
+    }
+    std::vector<Node> live;
+    for (auto& n : nodes)
+        if (!n.dead) live.push_back(n);
+    return live;
+}
+
+struct Scheduler {
+    const std::vector<Node>& nodes;
+    uint32_t qubits;
+    std::vector<std::vector<int>> queue;  // node ids per qubit, circuit order
+    std::vector<size_t> head;
+    size_t remaining;
+
+    Scheduler(const std::vector<Node>& n, uint32_t nq) : nodes(n), qubits(nq), queue(nq), head(nq, 0) {
+        for (size_t i = 0; i < nodes.size(); ++i)
+            for (int k = 0; k < nodes[i].nq; ++k) queue[nodes[i].q[k]].push_back(int(i));
+        remaining = nodes.size();
+    }
+
+    bool at_head(const std::vector<size_t>& h, int id) const {
+        const Node& n = nodes[id];
+        for (int k = 0; k < n.nq; ++k) {
+            const auto& qq = queue[n.q[k]];
+            if (h[n.q[k]] >= qq.size() || qq[h[n.q[k]]] != id) return false;
+        }
+        return true;
+    }
+    bool fits(const Node& n, const std::vector<char>& in) const {
+        if (n.diag()) return true;
+        for (int k = 0; k < n.nq; ++k)
+            if (!in[n.q[k]]) return false;
+        return true;
+    }
+
+    // Runs every node that can run with tile set `in`, from heads `h`
+    // (updated).  Records the order in `out` when given.  Returns the weighted
+    // count (non-diagonal nodes weigh 2, diagonal 1).
+    int simulate(std::vector<size_t>& h, const std::vector<char>& in, std::vector<int>* out) const {
+        int score = 0;
+        std::deque<int> work;
+        for (uint32_t q = 0; q < qubits; ++q)
+            if (h[q] < queue[q].size()) work.push_back(queue[q][h[q]]);
+        while (!work.empty()) {
+            const int id = work.front();
+            work.pop_front();
+            if (!at_head(h, id)) continue;
+            const Node& n = nodes[id];
+            if (!fits(n, in)) continue;
+            for (int k = 0; k < n.nq; ++k) ++h[n.q[k]];
+            score += n.diag() ? 1 : 2;
+            if (out) out->push_back(id);
+            for (int k = 0; k < n.nq; ++k) {
+                const uint32_t q = n.q[k];
+                if (h[q] < queue[q].size()) work.push_back(queue[q][h[q]]);
+            }
+        }
+        return score;
+    }
+};
+
+TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
+    TileOp op;
+    std::memset(&op, 0, sizeof(op));
+    switch (n.kind) {
+        case N_MAT1:
+            op.kind = TOP_MAT1;
+            op.q0 = tile_bit[n.q[0]];
+            op.q1 = -1;
+            std::memcpy(op.m, n.m, 4 * sizeof(cd));
+            break;
+        case N_DIAG1:
+            op.kind = TOP_DIAG;
+            op.q0 = int(n.q[0]);
+            op.q1 = -1;
+            op.m[0] = n.m[0].re; op.m[1] = n.m[0].im;
+            op.m[2] = n.m[3].re; op.m[3] = n.m[3].im;
+            op.m[4] = op.m[6] = 0.0; op.m[5] = op.m[7] = 0.0;
+            break;
+        case N_DIAG2:
+            op.kind = TOP_DIAG;
+            op.q0 = int(n.q[0]);
+            op.q1 = int(n.q[1]);
+            std::memcpy(op.m, n.m, 4 * sizeof(cd));
+            break;
+        case N_CNOT:
+            op.kind = TOP_CNOT;
+            op.q0 = tile_bit[n.q[0]];
+            op.q1 = tile_bit[n.q[1]];
+            break;
+        case N_SWAP:
+            op.kind = TOP_SWAP;
+            op.q0 = tile_bit[n.q[0]];
+            op.q1 = tile_bit[n.q[1]];
+            break;
+    }
+    return op;
+}
+
+}  // namespace
+
+std::string describe_plan(const Plan& plan) {
+    std::string s;
+    char buf[64];
+    auto num = [&](double v) {
+        std::snprintf(buf, sizeof(buf), "%.17g", v);
+        s += buf;
+    };
+    auto cplx = [&](const double* m, int count) {
+        s += "[";
+        for (int k = 0; k < 2 * count; ++k) {
+            if (k) s += ",";
+            num(m[k]);
+        }
+        s += "]";
+    };
+    s += "{\"qubits\":" + std::to_string(plan.qubits) + ",\"passes\":[";
+    for (size_t i = 0; i < plan.passes.size(); ++i) {
+        const Pass& p = plan.passes[i];
+        if (i) s += ",";
+        if (p.kind == Pass::TILE) {
+            s += "{\"kind\":\"tile\",\"tile_bits\":" + std::to_string(p.tile_bits) + ",\"low\":" +
+                 std::to_string(p.low) + ",\"high\":[";
+            for (size_t k = 0; k < p.high.size(); ++k) s += (k ? "," : "") + std::to_string(p.high[k]);
+            s += "],\"ops\":[";
+            for (size_t k = 0; k < p.ops.size(); ++k) {
+                const TileOp& o = p.ops[k];
+                if (k) s += ",";
+                s += "{\"kind\":" + std::to_string(o.kind) + ",\"q0\":" + std::to_string(o.q0) + ",\"q1\":" +
+                     std::to_string(o.q1) + ",\"m\":";
+                cplx(o.m, o.kind == TOP_MAT2 ? 16 : 4);
+                s += "}";
+            }
+            s += "]}";
+        } else {
+            static const char* names[] = {"sweep_1q", "sweep_2q", "tile", "sweep_diag"};
+            s += std::string("{\"kind\":\"") + names[p.kind] + "\",\"op2\":" + std::to_string(p.op2) +
+                 ",\"q0\":" + std::to_string(p.q0) + ",\"q1\":" + std::to_string(p.q1) + ",\"m\":";
+            cplx(&p.m[0].re, 4);
+            s += "}";
+        }
+    }
+    s += "],\"stats\":{\"passes\":" + std::to_string(plan.stats.passes) +
+         ",\"ops_in\":" + std::to_string(plan.stats.ops_in) +
+         ",\"ops_applied\":" + std::to_string(plan.stats.ops_applied) +
+         ",\"max_ops_per_pass\":" + std::to_string(plan.stats.max_ops_per_pass) + "}}";
+    return s;
+}
+
+uint32_t default_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
+uint32_t default_low_bits(int precision) { return precision == 0 ? 5 : 6; }
+
+std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt,
+                       Plan& out) {
+    out = Plan{};
+    out.qubits = qubits;
+    out.precision = precision;
+    out.stats.ops_in = gates.size();
+    if (qubits == 0) return "a state vector must contain at least one qubit";
+    for (const GateIn& g : gates) {
+        if (g.a >= qubits || (g.kind != 0 && g.b >= qubits)) return "qubit out of range";
+        if (g.kind != 0 && g.a == g.b) return "two-qubit operation requires distinct qubits";
+    }
+    if (!opt.fuse) {
+        for (const GateIn& g : gates) {
+            Pass p{};
+            if (g.kind == 0) {
+                p.kind = Pass::SWEEP_1Q;
+                p.q0 = g.a;
+                std::memcpy(p.m, g.m, sizeof(p.m));
+            } else if (g.kind == 4) {
+                p.kind = Pass::SWEEP_DIAG;
+                p.q0 = g.a;
+                p.q1 = g.b;
+                std::memcpy(p.m, g.m, sizeof(p.m));
+            } else {
+                p.kind = Pass::SWEEP_2Q;
+                p.op2 = g.kind == 1 ? 0 : (g.kind == 2 ? 1 : 2);
+                p.q0 = g.a;
+                p.q1 = g.b;
+            }
+            out.passes.push_back(p);
+        }
+        out.stats.passes = out.passes.size();
+        out.stats.ops_applied = gates.size();
+        out.stats.max_ops_per_pass = gates.empty() ? 0 : 1;
+        return "";
+    }
+
+    const std::vector<Node> nodes = lower(gates, qubits);
+    out.stats.ops_applied = nodes.size();
+    uint32_t M = opt.tile_qubits ? opt.tile_qubits : default_tile_bits(precision);
+    uint32_t low = opt.low_qubits ? opt.low_qubits : default_low_bits(precision);
+    if (M > qubits) M = qubits;
+    if (low > M) low = M;
+    if (M > 14) return "tile_qubits must be at most 14";
+
+    Scheduler sch(nodes, qubits);
+    size_t done = 0;
+    while (done < nodes.size()) {
+        std::vector<char> in(qubits, 0);
+        for (uint32_t q = 0; q < low; ++q) in[q] = 1;
+        uint32_t size = low;
+        std::vector<size_t> h_base = sch.head;
+        int base_score = sch.simulate(h_base, in, nullptr);
+        while (size < M) {
+            int best = -1, best_score = -1, best_tie = -1;
+            for (uint32_t q = low; q < qubits; ++q) {
+                if (in[q]) continue;
+                in[q] = 1;
+                std::vector<size_t> h = sch.head;
+                const int sc = sch.simulate(h, in, nullptr);
+                in[q] = 0;
+                // Tie break: pending work on q (queue length left), then the lowest index.
+                const int tie = int(sch.queue[q].size() - sch.head[q]);
+                if (sc > best_score || (sc == best_score && tie > best_tie)) {
+                    best = int(q);
+                    best_score = sc;
+                    best_tie = tie;
+                }
+            }
+            if (best < 0) break;
+            if (best_score <= base_score) {
+                // No single qubit unlocks anything: every runnable gate needs two
+                // new qubits.  Take the earliest runnable gate that still fits.
+                int pick = -1;
+                for (uint32_t q = 0; q < qubits; ++q) {
+                    if (h_base[q] >= sch.queue[q].size()) continue;
+                    const int id = sch.queue[q][h_base[q]];
+                    if (!sch.at_head(h_base, id)) continue;
+                    const Node& nd = nodes[id];
+                    uint32_t missing = 0;
+                    for (int k = 0; k < nd.nq; ++k) missing += in[nd.q[k]] ? 0 : 1;
+                    if (missing == 0 || missing > M - size) continue;
+                    if (pick < 0 || id < pick) pick = id;
+                }
+                if (pick >= 0) {
+                    const Node& nd = nodes[pick];
+                    for (int k = 0; k < nd.nq; ++k)
+                        if (!in[nd.q[k]]) {
+                            in[nd.q[k]] = 1;
+                            ++size;
+                        }
+                    h_base = sch.head;
+                    base_score = sch.simulate(h_base, in, nullptr);
+                    continue;
+                }
+            }
+            in[best] = 1;
+            ++size;
+            h_base = sch.head;
+            base_score = sch.simulate(h_base, in, nullptr);
+        }
+        std::vector<int> order;
+        sch.simulate(sch.head, in, &order);
+        if (order.empty()) {
+            // Cannot happen for gates of at most two qubits once M >= low + 2,
+            // but guard against a planner bug turning into an endless loop.
+            return "planner made no progress";
+        }
+        done += order.size();
+        Pass p{};
+        p.kind = Pass::TILE;
+        p.tile_bits = M;
+        p.low = low;
+        std::vector<int> tile_bit(qubits, -1);
+        for (uint32_t q = 0; q < low; ++q) tile_bit[q] = int(q);
+        uint32_t k = low;
+        for (uint32_t q = low; q < qubits; ++q)
+            if (in[q]) {
+                p.high.push_back(q);
+                tile_bit[q] = int(k++);
+            }
+        for (int id : order) p.ops.push_back(make_tile_op(nodes[id], tile_bit));
+        out.stats.max_ops_per_pass = std::max<uint64_t>(out.stats.max_ops_per_pass, p.ops.size());
+        out.passes.push_back(std::move(p));
+    }
+    out.stats.passes = out.passes.size();
+    return "";
+}
+
+}  // namespace qf

@@ -1,0 +1,92 @@
+// Host-side circuit planner: lowers a gate list to device passes.
+//
+// Two lowering modes:
+//   * exact (fuse = 0): one full-state sweep per gate, in circuit order, with
+//     the reference's unfused multiply/add sequence (src/lib.rs:1037-1039), so
+//     complex128 results match the reference CPU path bit for bit;
+//   * fused (fuse = 1): tile passes.  A tile is the set of 2^M amplitudes whose
+//     indices differ only in M chosen qubits (the `low` contiguous qubits
+//     0..low-1 plus up to M-low further qubits).  One workgroup loads a tile
+//     into LDS, applies every scheduled gate whose qubits lie in the tile, and
+//     writes it back: one HBM sweep for many gates.  Diagonal gates (Rz, Z, S,
+//     T, CZ, controlled phase, ZZ rotations) are applied from the element's
+//     global index, so they never need their qubits inside the tile.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace qf {
+
+struct cd {
+    double re, im;
+};
+
+// Device op kinds (shared with the tile kernel).
+enum TileOpKind : int32_t {
+    TOP_MAT1 = 0,   // dense 2x2 on tile bit q0
+    TOP_DIAG = 1,   // diagonal on up to 2 global qubits (g0, g1): d[bits(g1,g0)]
+    TOP_CNOT = 2,   // permutation: control tile bit q0, target tile bit q1
+    TOP_SWAP = 3,   // permutation on tile bits q0, q1
+    TOP_MAT2 = 4,   // dense 4x4 on tile bits (q0 = low bit of the 2-bit index, q1 = high bit)
+};
+
+// One op as the tile kernel reads it (uniform across the workgroup).
+struct TileOp {
+    int32_t kind;
+    int32_t q0, q1;   // tile bit positions (MAT1/CNOT/SWAP/MAT2) or global qubits (DIAG; q1 < 0 if 1-qubit)
+    int32_t pad;
+    double m[32];     // MAT1: 4 complex; DIAG: 4 complex (index g1bit*2+g0bit); MAT2: 16 complex row major
+};
+
+// One full-state pass.
+struct Pass {
+    enum Kind : int32_t { SWEEP_1Q = 0, SWEEP_2Q = 1, TILE = 2, SWEEP_DIAG = 3 } kind;
+    // SWEEP_1Q: q0 target, m; SWEEP_2Q: op2 (0 CNOT, 1 CZ, 2 SWAP), q0, q1; SWEEP_DIAG: q0, q1, m as diag
+    int32_t op2 = 0;
+    uint32_t q0 = 0, q1 = 0;
+    cd m[4]{};
+    // TILE
+    uint32_t tile_bits = 0;            // M
+    uint32_t low = 0;                  // contiguous low qubits
+    std::vector<uint32_t> high;        // global qubit of tile bit low+k (ascending)
+    std::vector<TileOp> ops;
+};
+
+struct PlanOptions {
+    int fuse = 1;
+    uint32_t tile_qubits = 0;  // 0 = default for the precision
+    uint32_t low_qubits = 0;   // 0 = default
+    uint32_t max_block = 0;
+};
+
+struct PlanStats {
+    uint64_t passes = 0, ops_in = 0, ops_applied = 0, dense_blocks = 0, max_ops_per_pass = 0;
+};
+
+// One circuit gate in planner form.
+struct GateIn {
+    int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.)
+    uint32_t a, b;
+    cd m[4];           // 1q: matrix; diag2: d[bits(b,a)]
+};
+
+struct Plan {
+    uint32_t qubits = 0;
+    int precision = 0;
+    std::vector<Pass> passes;
+    PlanStats stats;
+};
+
+// Builds a plan; returns an empty string on success or an error message.
+std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt,
+                       Plan& out);
+
+// JSON text of a plan (passes, tile sets, ops with exact matrices).
+std::string describe_plan(const Plan& plan);
+
+// Default tile geometry for a precision (amplitude bytes 16 or 8).
+uint32_t default_tile_bits(int precision);
+uint32_t default_low_bits(int precision);
+
+}  // namespace qf

@@ -1,0 +1,34 @@
+// Internal helpers shared by the runtime translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qf_hip.h"
+#include "qf_kernels.h"
+#include "qf_plan.h"
+
+namespace qfrt {
+
+qf_status fail(qf_status st, const std::string& msg);
+qf_status hip_fail(hipError_t e, const char* what);
+void gate_matrix(int gate, double theta, qf::cd m[4]);
+bool is_rotation(int gate);
+qf_status check_pair(uint32_t qubits, uint32_t a, uint32_t b);
+qf_status check_qubits(uint32_t qubits);
+qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector<qf::GateIn>& out);
+qf_status activate(qf_backend* be);
+qf_status device_alloc(void** out, size_t bytes);
+qf_status ensure(qf_backend* be, void** buf, size_t* cap, size_t bytes);
+qf_status launch_check(const char* what);
+size_t amp_bytes(int precision);
+unsigned grid_for(uint64_t work);
+qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan);
+qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count,
+                      const std::vector<qf::GateIn>& gates, const qf::PlanOptions& opt);
+qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, uint64_t dim, uint32_t count, int kind,
+                 uint32_t q, const std::vector<qf::PauliDev>& terms, std::vector<double>& out);
+qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits, std::vector<qf::PauliDev>& out);
+
+}  // namespace qfrt

@@ -1,0 +1,1051 @@
+// Host runtime of the state-vector engine: implements the C ABI of
+// include/qf_hip.h over HIP.  One HIP stream per backend (default priority),
+// 64-bit indexing throughout, validation on the host before any launch.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qf_hip.h"
+#include "qf_kernels.h"
+#include "qf_plan.h"
+#include "qf_runtime.h"
+
+using qf::cd;
+using qf::cx;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace qfrt {
+
+qf_status fail(qf_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+qf_status hip_fail(hipError_t e, const char* what) {
+    return fail(QF_DEVICE_ERROR, std::string("HIP error ") + hipGetErrorName(e) + " (" + std::to_string(int(e)) +
+                                     "): " + hipGetErrorString(e) + " in " + what);
+}
+
+}  // namespace qfrt
+
+using namespace qfrt;
+
+extern "C" const char* qf_last_error_message(void) { return g_last_error.c_str(); }
+
+extern "C" const char* qf_status_name(qf_status s) {
+    switch (s) {
+        case QF_OK: return "Ok";
+        case QF_ZERO_QUBITS: return "ZeroQubits";
+        case QF_QUBIT_OUT_OF_RANGE: return "QubitOutOfRange";
+        case QF_INVALID_QUBIT_PAIR: return "InvalidQubitPair";
+        case QF_NON_FINITE_ANGLE: return "NonFiniteAngle";
+        case QF_STATE_TOO_LARGE: return "StateTooLarge";
+        case QF_EMPTY_BATCH: return "EmptyBatch";
+        case QF_INCOMPATIBLE_BATCH: return "IncompatibleBatch";
+        case QF_INVALID_PARAMETER_COUNT: return "InvalidParameterCount";
+        case QF_INVALID_INPUT_COUNT: return "InvalidInputCount";
+        case QF_NON_FINITE_PARAMETER: return "NonFiniteParameter";
+        case QF_OUT_OF_MEMORY: return "MemoryAllocationFailed";
+        case QF_DEVICE_ERROR: return "Driver";
+        case QF_REQUIRES_DENSITY_MATRIX: return "OperationRequiresDensityMatrix";
+        case QF_REQUIRES_SHOT_EXECUTION: return "OperationRequiresShotExecution";
+        case QF_INVALID_ARGUMENT: return "InvalidArgument";
+        case QF_INCOMPATIBLE_DEVICE_STATE: return "IncompatibleDeviceState";
+        case QF_STATE_SIZE_NOT_POWER_OF_TWO: return "StateSizeNotPowerOfTwo";
+        case QF_INVALID_BATCH_SAMPLES: return "InvalidBatchSamples";
+        case QF_MEMORY_BUDGET_EXCEEDED: return "MemoryBudgetExceeded";
+        case QF_INVALID_FD_STEP: return "InvalidFiniteDifferenceStep";
+        case QF_SIMULATION_SIZE_OVERFLOW: return "SimulationSizeOverflow";
+        default: return "Unknown";
+    }
+}
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Gate matrices on the host: Gate::matrix, src/lib.rs:2267-2333.
+// ---------------------------------------------------------------------------
+namespace qfrt {
+
+void gate_matrix(int gate, double theta, cd m[4]) {
+    const cd z{0, 0}, o{1, 0}, i{0, 1};
+    const double s = 0.70710678118654752440;  // FRAC_1_SQRT_2
+    switch (gate) {
+        case QF_GATE_I: m[0] = o; m[1] = z; m[2] = z; m[3] = o; return;
+        case QF_GATE_X: m[0] = z; m[1] = o; m[2] = o; m[3] = z; return;
+        case QF_GATE_Y: m[0] = z; m[1] = cd{0, -1}; m[2] = i; m[3] = z; return;
+        case QF_GATE_Z: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{-1, 0}; return;
+        case QF_GATE_H: m[0] = cd{s, 0}; m[1] = cd{s, 0}; m[2] = cd{s, 0}; m[3] = cd{-s, 0}; return;
+        case QF_GATE_S: m[0] = o; m[1] = z; m[2] = z; m[3] = i; return;
+        case QF_GATE_SDG: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{0, -1}; return;
+        case QF_GATE_T: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{s, s}; return;
+        case QF_GATE_TDG: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{s, -s}; return;
+        case QF_GATE_RX: {
+            const double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            m[0] = cd{c, 0}; m[1] = cd{0, -si}; m[2] = cd{0, -si}; m[3] = cd{c, 0};
+            return;
+        }
+        case QF_GATE_RY: {
+            const double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            m[0] = cd{c, 0}; m[1] = cd{-si, 0}; m[2] = cd{si, 0}; m[3] = cd{c, 0};
+            return;
+        }
+        default: {  // RZ
+            const double half = theta * 0.5;
+            m[0] = cd{std::cos(half), -std::sin(half)}; m[1] = z; m[2] = z;
+            m[3] = cd{std::cos(half), std::sin(half)};
+            return;
+        }
+    }
+}
+
+bool is_rotation(int gate) { return gate == QF_GATE_RX || gate == QF_GATE_RY || gate == QF_GATE_RZ; }
+
+}  // namespace qfrt
+
+// ---------------------------------------------------------------------------
+// Handles
+// ---------------------------------------------------------------------------
+struct qf_backend {
+    int32_t ordinal = 0;
+    hipStream_t stream = nullptr;
+    std::atomic<int> refs{1};
+    qf::PlanOptions opts;
+    // Stream-ordered scratch; reused after a stream synchronize.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double* d_out = nullptr;
+    size_t d_out_count = 0;
+    void* d_ops = nullptr;
+    size_t d_ops_bytes = 0;
+};
+
+struct qf_state {
+    qf_backend* be = nullptr;
+    void* d = nullptr;
+    uint32_t qubits = 0;
+    int32_t precision = QF_C128;
+    uint64_t dim = 0;
+};
+
+struct qf_batch {
+    qf_backend* be = nullptr;
+    void* d = nullptr;
+    uint32_t qubits = 0;
+    uint32_t count = 0;
+    int32_t precision = QF_C128;
+    uint64_t dim = 0;
+};
+
+struct qf_plan {
+    qf::Plan plan;
+};
+
+namespace qfrt {
+
+void backend_release(qf_backend* be) {
+    if (!be) return;
+    if (be->refs.fetch_sub(1) == 1) {
+        (void)hipSetDevice(be->ordinal);
+        (void)hipStreamSynchronize(be->stream);
+        if (be->scratch) (void)hipFree(be->scratch);
+        if (be->d_out) (void)hipFree(be->d_out);
+        if (be->d_ops) (void)hipFree(be->d_ops);
+        (void)hipStreamDestroy(be->stream);
+        delete be;
+    }
+}
+
+qf_status activate(qf_backend* be) {
+    HIP_TRY(hipSetDevice(be->ordinal));
+    return QF_OK;
+}
+
+// Grows a backend-owned device buffer; waits for the stream first so no
+// in-flight kernel still reads the old contents.
+qf_status ensure(qf_backend* be, void** buf, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *buf) return QF_OK;
+    HIP_TRY(hipStreamSynchronize(be->stream));
+    if (*buf) {
+        HIP_TRY(hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    hipError_t e = hipMalloc(buf, want);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        (void)hipGetLastError();
+        return fail(QF_OUT_OF_MEMORY, "could not allocate " + std::to_string(want) + " bytes of device scratch");
+    }
+    *cap = want;
+    return QF_OK;
+}
+
+size_t amp_bytes(int precision) { return precision == QF_C128 ? 16 : 8; }
+
+// Size checks of StateVector::zero_with_memory_budget, src/lib.rs:940-966,
+// with 64-bit device indexing (states up to 2^40 amplitudes).
+qf_status check_qubits(uint32_t qubits) {
+    if (qubits == 0) return fail(QF_ZERO_QUBITS, "a state vector must contain at least one qubit");
+    if (qubits > 40) return fail(QF_STATE_TOO_LARGE, "state vector is too large for the HIP kernels (more than 40 qubits)");
+    return QF_OK;
+}
+
+qf_status device_alloc(void** out, size_t bytes) {
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(QF_OUT_OF_MEMORY, "could not reserve " + std::to_string(bytes) +
+                                          " bytes of device memory for the dense quantum state");
+    }
+    return QF_OK;
+}
+
+unsigned grid_for(uint64_t work) {
+    uint64_t g = (work + qf::kThreads - 1) / qf::kThreads;
+    if (g > (uint64_t(1) << 20)) g = uint64_t(1) << 20;  // grid-stride beyond 2^28 threads
+    return unsigned(g ? g : 1);
+}
+
+qf_status launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return QF_OK;
+}
+
+// Validates a circuit op list; converts unitary ops to planner gates.
+// Errors follow the first offending op in circuit order, with the variants of
+// Circuit::run / CudaBackend::run_circuit (src/lib.rs:2786-2802, src/cuda.rs:1050-1083).
+qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector<qf::GateIn>& out) {
+    out.clear();
+    out.reserve(count);
+    for (size_t i = 0; i < count; ++i) {
+        const qf_op& op = ops[i];
+        qf::GateIn g{};
+        switch (op.kind) {
+            case 0:
+                if (op.gate < QF_GATE_I || op.gate > QF_GATE_RZ)
+                    return fail(QF_INVALID_ARGUMENT, "unknown gate id " + std::to_string(op.gate));
+                if (op.a >= qubits)
+                    return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(op.a) +
+                                                           " is out of range for " + std::to_string(qubits) + " qubits");
+                if (is_rotation(op.gate) && !std::isfinite(op.theta))
+                    return fail(QF_NON_FINITE_ANGLE, "rotation angle must be finite");
+                g.kind = 0;
+                g.a = op.a;
+                gate_matrix(op.gate, op.theta, g.m);
+                break;
+            case 1: case 2: case 3: {
+                qf_status st = check_pair(qubits, op.a, op.b);
+                if (st) return st;
+                g.kind = op.kind;
+                g.a = op.a;
+                g.b = op.b;
+                break;
+            }
+            case 4: return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+            case 5: case 6:
+                return fail(QF_REQUIRES_SHOT_EXECUTION,
+                            "measurement and classically conditioned circuit operations require shot execution");
+            default: return fail(QF_INVALID_ARGUMENT, "unknown op kind " + std::to_string(op.kind));
+        }
+        out.push_back(g);
+    }
+    return QF_OK;
+}
+
+// check_pair, src/lib.rs:1124-1141 (the CPU's variants; see the header).
+qf_status check_pair(uint32_t qubits, uint32_t a, uint32_t b) {
+    if (a >= qubits)
+        return fail(QF_QUBIT_OUT_OF_RANGE,
+                    "qubit index " + std::to_string(a) + " is out of range for " + std::to_string(qubits) + " qubits");
+    if (b >= qubits)
+        return fail(QF_QUBIT_OUT_OF_RANGE,
+                    "qubit index " + std::to_string(b) + " is out of range for " + std::to_string(qubits) + " qubits");
+    if (a == b)
+        return fail(QF_INVALID_QUBIT_PAIR, "two-qubit operation requires distinct qubits, got " + std::to_string(a) +
+                                               " and " + std::to_string(b));
+    return QF_OK;
+}
+
+template <class R>
+qf_status launch_pass(qf_backend* be, cx<R>* d, uint32_t qubits, uint32_t count, const qf::Pass& p,
+                      const qf::TileOp* d_ops) {
+    const uint64_t dim = uint64_t(1) << qubits;
+    const uint64_t total = dim * count;
+    hipStream_t st = be->stream;
+    switch (p.kind) {
+        case qf::Pass::SWEEP_1Q:
+            hipLaunchKernelGGL(qf::k_sweep_1q<R>, dim3(grid_for(total / 2)), dim3(qf::kThreads), 0, st, d, total / 2,
+                               p.q0, p.m[0], p.m[1], p.m[2], p.m[3]);
+            return launch_check("k_sweep_1q");
+        case qf::Pass::SWEEP_2Q:
+            hipLaunchKernelGGL(qf::k_sweep_2q<R>, dim3(grid_for(total / 4)), dim3(qf::kThreads), 0, st, d, total / 4,
+                               p.q0, p.q1, p.op2);
+            return launch_check("k_sweep_2q");
+        case qf::Pass::SWEEP_DIAG:
+            hipLaunchKernelGGL(qf::k_sweep_diag<R>, dim3(grid_for(total)), dim3(qf::kThreads), 0, st, d, total,
+                               int(p.q0), int(p.q1), p.m[0], p.m[1], p.m[2], p.m[3]);
+            return launch_check("k_sweep_diag");
+        case qf::Pass::TILE: {
+            qf::TileArgs a{};
+            a.n = qubits;
+            a.M = p.tile_bits;
+            a.low = p.low;
+            a.log_blocks = qubits - p.tile_bits;
+            a.nops = uint32_t(p.ops.size());
+            for (size_t k = 0; k < p.high.size(); ++k) a.high[k] = p.high[k];
+            const uint32_t nh = p.tile_bits - p.low;
+            const size_t lds = (size_t(1) << p.tile_bits) * sizeof(cx<R>) + (size_t(1) << nh) * sizeof(uint64_t);
+            if (lds > 64 * 1024) {
+                HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&qf::k_tile<R>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            }
+            uint32_t gx_log = a.log_blocks < 16 ? a.log_blocks : 16;
+            dim3 grid(1u << gx_log, count << (a.log_blocks - gx_log));
+            hipLaunchKernelGGL(qf::k_tile<R>, grid, dim3(qf::kThreads), lds, st, d, a, d_ops);
+            return launch_check("k_tile");
+        }
+    }
+    return fail(QF_INVALID_ARGUMENT, "unknown pass kind");
+}
+
+// Runs a plan on `count` contiguous states of `qubits` qubits.
+qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
+    if (plan.qubits != qubits) return fail(QF_INCOMPATIBLE_DEVICE_STATE, "plan was built for a different qubit count");
+    // Gather all tile ops into one device buffer.
+    std::vector<size_t> offset(plan.passes.size(), 0);
+    size_t nops = 0;
+    for (size_t i = 0; i < plan.passes.size(); ++i) {
+        offset[i] = nops;
+        nops += plan.passes[i].ops.size();
+    }
+    qf::TileOp* d_ops = nullptr;
+    if (nops) {
+        std::vector<qf::TileOp> host;
+        host.reserve(nops);
+        for (const auto& p : plan.passes) host.insert(host.end(), p.ops.begin(), p.ops.end());
+        // The buffer may still be read by earlier launches: wait before reuse.
+        HIP_TRY(hipStreamSynchronize(be->stream));
+        qf_status s = ensure(be, &be->d_ops, &be->d_ops_bytes, nops * sizeof(qf::TileOp));
+        if (s) return s;
+        HIP_TRY(hipMemcpy(be->d_ops, host.data(), nops * sizeof(qf::TileOp), hipMemcpyHostToDevice));
+        d_ops = static_cast<qf::TileOp*>(be->d_ops);
+    }
+    for (size_t i = 0; i < plan.passes.size(); ++i) {
+        qf_status s = precision == QF_C128
+                          ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, plan.passes[i],
+                                                d_ops ? d_ops + offset[i] : nullptr)
+                          : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, plan.passes[i],
+                                               d_ops ? d_ops + offset[i] : nullptr);
+        if (s) return s;
+    }
+    return QF_OK;
+}
+
+qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count,
+                      const std::vector<qf::GateIn>& gates, const qf::PlanOptions& opt) {
+    qf::Plan plan;
+    std::string err = qf::build_plan(gates, qubits, precision, opt, plan);
+    if (!err.empty()) return fail(QF_INVALID_ARGUMENT, err);
+    return run_plan(be, d, precision, qubits, count, plan);
+}
+
+// Compensated reduction over `count` contiguous states; out gets count (re, im) pairs.
+qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, uint64_t dim, uint32_t count, int kind,
+                 uint32_t q, const std::vector<qf::PauliDev>& terms, std::vector<double>& out) {
+    if (count > 65535) return fail(QF_INVALID_ARGUMENT, "reductions take at most 65535 states per call");
+    uint64_t blocks = (dim + qf::kThreads * 4 - 1) / (qf::kThreads * 4);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    const size_t partial_bytes = size_t(count) * blocks * 2 * sizeof(qf::dd);
+    const size_t terms_bytes = terms.size() * sizeof(qf::PauliDev);
+    qf_status s = ensure(be, &be->scratch, &be->scratch_bytes, partial_bytes + terms_bytes + 256);
+    if (s) return s;
+    size_t cap = be->d_out_count * sizeof(double);
+    void* dout = be->d_out;
+    s = ensure(be, &dout, &cap, size_t(count) * 2 * sizeof(double));
+    if (s) return s;
+    be->d_out = static_cast<double*>(dout);
+    be->d_out_count = cap / sizeof(double);
+    qf::dd* partial = static_cast<qf::dd*>(be->scratch);
+    qf::PauliDev* d_terms = nullptr;
+    if (!terms.empty()) {
+        d_terms = reinterpret_cast<qf::PauliDev*>(static_cast<char*>(be->scratch) + ((partial_bytes + 255) / 256) * 256);
+        HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), terms_bytes, hipMemcpyHostToDevice, be->stream));
+    }
+    dim3 grid(unsigned(blocks), count);
+    if (precision == QF_C128)
+        hipLaunchKernelGGL(qf::k_reduce<double>, grid, dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<double>*>(d), static_cast<const cx<double>*>(d2), dim, kind, q,
+                           d_terms, uint32_t(terms.size()), partial);
+    else
+        hipLaunchKernelGGL(qf::k_reduce<float>, grid, dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<float>*>(d), static_cast<const cx<float>*>(d2), dim, kind, q, d_terms,
+                           uint32_t(terms.size()), partial);
+    s = launch_check("k_reduce");
+    if (s) return s;
+    hipLaunchKernelGGL(qf::k_reduce_finish, dim3(count), dim3(qf::kThreads), 0, be->stream, partial,
+                       uint32_t(blocks), be->d_out);
+    s = launch_check("k_reduce_finish");
+    if (s) return s;
+    out.resize(size_t(count) * 2);
+    HIP_TRY(hipMemcpyAsync(out.data(), be->d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, be->stream));
+    HIP_TRY(hipStreamSynchronize(be->stream));
+    return QF_OK;
+}
+
+// Device form of a Pauli sum: c' = c * i^(#Y) (see qf::PauliDev).
+qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits, std::vector<qf::PauliDev>& out) {
+    out.clear();
+    for (size_t t = 0; t < count; ++t) {
+        const uint64_t all = terms[t].x_mask | terms[t].z_mask;
+        if (qubits < 64 && (all >> qubits))
+            return fail(QF_QUBIT_OUT_OF_RANGE, "Pauli term acts on a qubit out of range for " +
+                                                   std::to_string(qubits) + " qubits");
+        const int ny = __builtin_popcountll(terms[t].x_mask & terms[t].z_mask) & 3;
+        double cr = terms[t].re, ci = terms[t].im;
+        for (int k = 0; k < ny; ++k) {  // multiply by i
+            const double r = -ci, i = cr;
+            cr = r;
+            ci = i;
+        }
+        out.push_back(qf::PauliDev{cr, ci, terms[t].x_mask, terms[t].z_mask});
+    }
+    return QF_OK;
+}
+
+}  // namespace qfrt
+
+// ---------------------------------------------------------------------------
+// Backend
+// ---------------------------------------------------------------------------
+extern "C" qf_status qf_backend_create(int32_t ordinal, qf_backend** out) {
+    if (!out) return fail(QF_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (ordinal < 0 || ordinal >= n)
+        return fail(QF_DEVICE_ERROR, "HIP device ordinal " + std::to_string(ordinal) + " not present (" +
+                                         std::to_string(n) + " devices)");
+    HIP_TRY(hipSetDevice(ordinal));
+    qf_backend* be = new qf_backend;
+    be->ordinal = ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&be->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete be;
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    *out = be;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_destroy(qf_backend* be) {
+    if (!be) return fail(QF_INVALID_ARGUMENT, "null backend");
+    backend_release(be);
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_synchronize(qf_backend* be) {
+    if (!be) return fail(QF_INVALID_ARGUMENT, "null backend");
+    qf_status s = activate(be);
+    if (s) return s;
+    HIP_TRY(hipStreamSynchronize(be->stream));
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_device_ordinal(const qf_backend* be, int32_t* out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = be->ordinal;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_set_plan_options(qf_backend* be, const qf_plan_options* o) {
+    if (!be || !o) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (o->tile_qubits > 14) return fail(QF_INVALID_ARGUMENT, "tile_qubits must be at most 14");
+    be->opts.fuse = o->fuse;
+    be->opts.tile_qubits = o->tile_qubits;
+    be->opts.low_qubits = o->low_qubits;
+    be->opts.max_block = o->max_block;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_stream(const qf_backend* be, void** out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = be->stream;
+    return QF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// States
+// ---------------------------------------------------------------------------
+namespace {
+
+qf_status new_state(qf_backend* be, uint32_t qubits, int32_t precision, qf_state** out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    qf_status s = check_qubits(qubits);
+    if (s) return s;
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    s = activate(be);
+    if (s) return s;
+    const uint64_t dim = uint64_t(1) << qubits;
+    void* d = nullptr;
+    s = device_alloc(&d, dim * amp_bytes(precision));
+    if (s) return s;
+    qf_state* st = new qf_state;
+    st->be = be;
+    be->refs.fetch_add(1);
+    st->d = d;
+    st->qubits = qubits;
+    st->precision = precision;
+    st->dim = dim;
+    *out = st;
+    return QF_OK;
+}
+
+template <class R>
+void fill_zero(qf_backend* be, void* d, uint64_t total, uint32_t qubits) {
+    hipLaunchKernelGGL(qf::k_fill_zero_state<R>, dim3(grid_for(total)), dim3(qf::kThreads), 0, be->stream,
+                       static_cast<cx<R>*>(d), total, qubits);
+}
+
+}  // namespace
+
+extern "C" qf_status qf_state_zero(qf_backend* be, uint32_t qubits, int32_t precision, qf_state** out) {
+    qf_status s = new_state(be, qubits, precision, out);
+    if (s) return s;
+    qf_state* st = *out;
+    if (precision == QF_C128)
+        fill_zero<double>(be, st->d, st->dim, qubits);
+    else
+        fill_zero<float>(be, st->d, st->dim, qubits);
+    s = launch_check("k_fill_zero_state");
+    if (s) {
+        qf_state_destroy(st);
+        *out = nullptr;
+    }
+    return s;
+}
+
+extern "C" qf_status qf_state_upload(qf_backend* be, const qf_c128* amps, uint32_t qubits, qf_state** out) {
+    if (!amps) return fail(QF_INVALID_ARGUMENT, "null amplitudes");
+    qf_status s = new_state(be, qubits, QF_C128, out);
+    if (s) return s;
+    hipError_t e = hipMemcpy((*out)->d, amps, (*out)->dim * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        qf_state_destroy(*out);
+        *out = nullptr;
+        return hip_fail(e, "hipMemcpy upload");
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_upload_c64(qf_backend* be, const qf_c64* amps, uint32_t qubits, qf_state** out) {
+    if (!amps) return fail(QF_INVALID_ARGUMENT, "null amplitudes");
+    qf_status s = new_state(be, qubits, QF_C64, out);
+    if (s) return s;
+    hipError_t e = hipMemcpy((*out)->d, amps, (*out)->dim * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        qf_state_destroy(*out);
+        *out = nullptr;
+        return hip_fail(e, "hipMemcpy upload");
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_download(const qf_state* st, qf_c128* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_backend* be = st->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    if (st->precision == QF_C128) {
+        HIP_TRY(hipMemcpyAsync(out, st->d, st->dim * 16, hipMemcpyDeviceToHost, be->stream));
+        HIP_TRY(hipStreamSynchronize(be->stream));
+        return QF_OK;
+    }
+    void* tmp = nullptr;
+    s = device_alloc(&tmp, st->dim * 16);
+    if (s) return s;
+    hipLaunchKernelGGL((qf::k_convert<float, double>), dim3(grid_for(st->dim)), dim3(qf::kThreads), 0, be->stream,
+                       static_cast<const cx<float>*>(st->d), static_cast<cx<double>*>(tmp), st->dim);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, st->dim * 16, hipMemcpyDeviceToHost, be->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(be->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "download complex64");
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_download_c64(const qf_state* st, qf_c64* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (st->precision != QF_C64) return fail(QF_INCOMPATIBLE_DEVICE_STATE, "state is complex128");
+    qf_status s = activate(st->be);
+    if (s) return s;
+    HIP_TRY(hipMemcpyAsync(out, st->d, st->dim * 8, hipMemcpyDeviceToHost, st->be->stream));
+    HIP_TRY(hipStreamSynchronize(st->be->stream));
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_destroy(qf_state* st) {
+    if (!st) return fail(QF_INVALID_ARGUMENT, "null state");
+    qf_backend* be = st->be;
+    (void)hipSetDevice(be->ordinal);
+    (void)hipStreamSynchronize(be->stream);
+    (void)hipFree(st->d);
+    delete st;
+    backend_release(be);
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_qubits(const qf_state* st, uint32_t* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = st->qubits;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_precision(const qf_state* st, int32_t* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = st->precision;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_copy_from(qf_state* dst, const qf_state* src) {
+    if (!dst || !src) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (dst->dim != src->dim || dst->precision != src->precision || dst->be != src->be)
+        return fail(QF_INCOMPATIBLE_DEVICE_STATE, "device states must share a backend and have equal dimensions");
+    qf_status s = activate(dst->be);
+    if (s) return s;
+    HIP_TRY(hipMemcpyAsync(dst->d, src->d, dst->dim * amp_bytes(dst->precision), hipMemcpyDeviceToDevice,
+                           dst->be->stream));
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_device_pointer(const qf_state* st, void** out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = st->d;
+    return QF_OK;
+}
+
+namespace {
+qf_status exact_opts(qf::PlanOptions& o) {
+    o = qf::PlanOptions{};
+    o.fuse = 0;
+    return QF_OK;
+}
+}  // namespace
+
+extern "C" qf_status qf_state_apply_gate(qf_state* st, int32_t gate, uint32_t target, double theta) {
+    if (!st) return fail(QF_INVALID_ARGUMENT, "null state");
+    qf_op op{0, gate, target, 0, theta};
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(&op, 1, st->qubits, g);
+    if (s) return s;
+    if ((s = activate(st->be))) return s;
+    qf::PlanOptions o;
+    exact_opts(o);
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, g, o);
+}
+
+extern "C" qf_status qf_state_apply_matrix(qf_state* st, uint32_t target, const double* m8) {
+    if (!st || !m8) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (target >= st->qubits)
+        return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(target) + " is out of range for " +
+                                               std::to_string(st->qubits) + " qubits");
+    for (int k = 0; k < 8; ++k)
+        if (!std::isfinite(m8[k])) return fail(QF_INVALID_ARGUMENT, "matrix entries must be finite");
+    qf::GateIn g{};
+    g.kind = 0;
+    g.a = target;
+    std::memcpy(g.m, m8, sizeof(g.m));
+    qf_status s = activate(st->be);
+    if (s) return s;
+    qf::PlanOptions o;
+    exact_opts(o);
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, {g}, o);
+}
+
+extern "C" qf_status qf_state_apply_rotation_chain(qf_state* st, const qf_rotation* r, size_t count) {
+    if (!st || (!r && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    // validate_gates, src/cuda.rs:2256-2272: every gate before any launch.
+    std::vector<qf_op> ops(count);
+    for (size_t i = 0; i < count; ++i) {
+        if (r[i].axis < 0 || r[i].axis > 2) return fail(QF_INVALID_ARGUMENT, "unknown rotation axis");
+        ops[i] = qf_op{0, QF_GATE_RX + r[i].axis, r[i].qubit, 0, r[i].theta};
+    }
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(ops.data(), count, st->qubits, g);
+    if (s) return s;
+    if ((s = activate(st->be))) return s;
+    qf::PlanOptions o;
+    exact_opts(o);
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, g, o);
+}
+
+extern "C" qf_status qf_state_apply_2q(qf_state* st, int32_t op, uint32_t a, uint32_t b) {
+    if (!st) return fail(QF_INVALID_ARGUMENT, "null state");
+    if (op < 0 || op > 2) return fail(QF_INVALID_ARGUMENT, "unknown two-qubit op");
+    qf_op o{op + 1, 0, a, b, 0.0};
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(&o, 1, st->qubits, g);
+    if (s) return s;
+    if ((s = activate(st->be))) return s;
+    qf::PlanOptions po;
+    exact_opts(po);
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, g, po);
+}
+
+extern "C" qf_status qf_state_apply_cp(qf_state* st, uint32_t control, uint32_t target, double phi) {
+    if (!st) return fail(QF_INVALID_ARGUMENT, "null state");
+    qf_status s = check_pair(st->qubits, control, target);
+    if (s) return s;
+    if (!std::isfinite(phi)) return fail(QF_NON_FINITE_ANGLE, "rotation angle must be finite");
+    qf::GateIn g{};
+    g.kind = 4;
+    g.a = control;
+    g.b = target;
+    g.m[0] = g.m[1] = g.m[2] = cd{1, 0};
+    g.m[3] = cd{std::cos(phi), std::sin(phi)};
+    if ((s = activate(st->be))) return s;
+    qf::PlanOptions po;
+    exact_opts(po);
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, {g}, po);
+}
+
+extern "C" qf_status qf_state_run_circuit(qf_state* st, const qf_op* ops, size_t count, const qf_plan* plan) {
+    if (!st || (!ops && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(ops, count, st->qubits, g);
+    if (s) return s;
+    if ((s = activate(st->be))) return s;
+    if (plan) {
+        if (plan->plan.stats.ops_in != count)
+            return fail(QF_INVALID_ARGUMENT, "plan was built for a different circuit");
+        return run_plan(st->be, st->d, st->precision, st->qubits, 1, plan->plan);
+    }
+    return apply_gates(st->be, st->d, st->precision, st->qubits, 1, g, st->be->opts);
+}
+
+extern "C" qf_status qf_state_norm_sqr(const qf_state* st, double* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_status s = activate(st->be);
+    if (s) return s;
+    std::vector<double> r;
+    s = reduce(st->be, st->d, nullptr, st->precision, st->dim, 1, qf::RED_NORM, 0, {}, r);
+    if (s) return s;
+    *out = r[0];
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_probabilities(const qf_state* st, double* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_backend* be = st->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    void* tmp = nullptr;
+    s = device_alloc(&tmp, st->dim * sizeof(double));
+    if (s) return s;
+    if (st->precision == QF_C128)
+        hipLaunchKernelGGL(qf::k_probabilities<double>, dim3(grid_for(st->dim)), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<double>*>(st->d), static_cast<double*>(tmp), st->dim);
+    else
+        hipLaunchKernelGGL(qf::k_probabilities<float>, dim3(grid_for(st->dim)), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<float>*>(st->d), static_cast<double*>(tmp), st->dim);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, st->dim * sizeof(double), hipMemcpyDeviceToHost, be->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(be->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "probabilities");
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_expectation(const qf_state* st, int32_t obs, uint32_t qubit, double* out) {
+    if (!st || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (obs < 0 || obs > 2) return fail(QF_INVALID_ARGUMENT, "unknown observable");
+    if (qubit >= st->qubits)
+        return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(qubit) + " is out of range for " +
+                                               std::to_string(st->qubits) + " qubits");
+    qf_status s = activate(st->be);
+    if (s) return s;
+    std::vector<double> r;
+    s = reduce(st->be, st->d, nullptr, st->precision, st->dim, 1, obs == QF_OBS_Z ? qf::RED_Z : qf::RED_XY, qubit, {},
+               r);
+    if (s) return s;
+    // expectation_x = sum 2 Re(conj(a0) a1), expectation_y = sum 2 Im(conj(a0) a1) (src/lib.rs:1272-1303).
+    *out = obs == QF_OBS_Z ? r[0] : (obs == QF_OBS_X ? 2.0 * r[0] : 2.0 * r[1]);
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_expectation_pauli_sum(const qf_state* st, const qf_pauli_term* terms, size_t count,
+                                                    double* out_re, double* out_im) {
+    if (!st || (!terms && count) || !out_re || !out_im) return fail(QF_INVALID_ARGUMENT, "null argument");
+    std::vector<qf::PauliDev> t;
+    qf_status s = pauli_terms(terms, count, st->qubits, t);
+    if (s) return s;
+    if (count == 0) {
+        *out_re = *out_im = 0.0;
+        return QF_OK;
+    }
+    if ((s = activate(st->be))) return s;
+    std::vector<double> r;
+    s = reduce(st->be, st->d, nullptr, st->precision, st->dim, 1, qf::RED_PAULI, 0, t, r);
+    if (s) return s;
+    *out_re = r[0];
+    *out_im = r[1];
+    return QF_OK;
+}
+
+extern "C" qf_status qf_state_inner(const qf_state* a, const qf_state* b, double* out_re, double* out_im) {
+    if (!a || !b || !out_re || !out_im) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (a->dim != b->dim || a->precision != b->precision || a->be != b->be)
+        return fail(QF_INCOMPATIBLE_DEVICE_STATE, "device states must share a backend and have equal dimensions");
+    qf_status s = activate(a->be);
+    if (s) return s;
+    std::vector<double> r;
+    s = reduce(a->be, a->d, b->d, a->precision, a->dim, 1, qf::RED_INNER, 0, {}, r);
+    if (s) return s;
+    *out_re = r[0];
+    *out_im = r[1];
+    return QF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+namespace {
+qf_status new_batch(qf_backend* be, uint32_t qubits, uint32_t count, int32_t precision, qf_batch** out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (count == 0) return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
+    qf_status s = check_qubits(qubits);
+    if (s) return s;
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    const uint64_t dim = uint64_t(1) << qubits;
+    if (qubits + 32 > 63 && (dim * count) / count != dim) return fail(QF_STATE_TOO_LARGE, "batch too large");
+    s = activate(be);
+    if (s) return s;
+    void* d = nullptr;
+    s = device_alloc(&d, dim * count * amp_bytes(precision));
+    if (s) return s;
+    qf_batch* b = new qf_batch;
+    b->be = be;
+    be->refs.fetch_add(1);
+    b->d = d;
+    b->qubits = qubits;
+    b->count = count;
+    b->precision = precision;
+    b->dim = dim;
+    *out = b;
+    return QF_OK;
+}
+}  // namespace
+
+extern "C" qf_status qf_batch_upload(qf_backend* be, const qf_c128* amps, const uint32_t* qubits, uint32_t count,
+                                     qf_batch** out) {
+    if (!out) return fail(QF_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (count == 0) return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
+    if (!amps || !qubits) return fail(QF_INVALID_ARGUMENT, "null argument");
+    for (uint32_t i = 1; i < count; ++i)
+        if (qubits[i] != qubits[0])
+            return fail(QF_INCOMPATIBLE_BATCH, "all states in a HIP batch must have equal dimensions");
+    qf_status s = new_batch(be, qubits[0], count, QF_C128, out);
+    if (s) return s;
+    hipError_t e = hipMemcpy((*out)->d, amps, (*out)->dim * count * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        qf_batch_destroy(*out);
+        *out = nullptr;
+        return hip_fail(e, "hipMemcpy batch upload");
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_batch_zero(qf_backend* be, uint32_t qubits, uint32_t count, int32_t precision,
+                                   qf_batch** out) {
+    qf_status s = new_batch(be, qubits, count, precision, out);
+    if (s) return s;
+    qf_batch* b = *out;
+    if (precision == QF_C128)
+        fill_zero<double>(be, b->d, b->dim * count, qubits);
+    else
+        fill_zero<float>(be, b->d, b->dim * count, qubits);
+    s = launch_check("k_fill_zero_state");
+    if (s) {
+        qf_batch_destroy(b);
+        *out = nullptr;
+    }
+    return s;
+}
+
+extern "C" qf_status qf_batch_destroy(qf_batch* b) {
+    if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
+    qf_backend* be = b->be;
+    (void)hipSetDevice(be->ordinal);
+    (void)hipStreamSynchronize(be->stream);
+    (void)hipFree(b->d);
+    delete b;
+    backend_release(be);
+    return QF_OK;
+}
+
+extern "C" qf_status qf_batch_shape(const qf_batch* b, uint32_t* qubits, uint32_t* count) {
+    if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
+    if (qubits) *qubits = b->qubits;
+    if (count) *count = b->count;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_batch_apply_gate(qf_batch* b, int32_t gate, uint32_t target, double theta) {
+    if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
+    qf_op op{0, gate, target, 0, theta};
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(&op, 1, b->qubits, g);
+    if (s) return s;
+    if ((s = activate(b->be))) return s;
+    qf::PlanOptions o;
+    exact_opts(o);
+    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, o);
+}
+
+extern "C" qf_status qf_batch_apply_2q(qf_batch* b, int32_t op, uint32_t a, uint32_t c) {
+    if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
+    if (op < 0 || op > 2) return fail(QF_INVALID_ARGUMENT, "unknown two-qubit op");
+    qf_op o{op + 1, 0, a, c, 0.0};
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(&o, 1, b->qubits, g);
+    if (s) return s;
+    if ((s = activate(b->be))) return s;
+    qf::PlanOptions po;
+    exact_opts(po);
+    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, po);
+}
+
+extern "C" qf_status qf_batch_run_circuit(qf_batch* b, const qf_op* ops, size_t count, const qf_plan* plan) {
+    if (!b || (!ops && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    std::vector<qf::GateIn> g;
+    qf_status s = lower_ops(ops, count, b->qubits, g);
+    if (s) return s;
+    if ((s = activate(b->be))) return s;
+    if (plan) {
+        if (plan->plan.stats.ops_in != count)
+            return fail(QF_INVALID_ARGUMENT, "plan was built for a different circuit");
+        return run_plan(b->be, b->d, b->precision, b->qubits, b->count, plan->plan);
+    }
+    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, b->be->opts);
+}
+
+extern "C" qf_status qf_batch_download(const qf_batch* b, qf_c128* out) {
+    if (!b || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_backend* be = b->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    const uint64_t total = b->dim * b->count;
+    if (b->precision == QF_C128) {
+        HIP_TRY(hipMemcpyAsync(out, b->d, total * 16, hipMemcpyDeviceToHost, be->stream));
+        HIP_TRY(hipStreamSynchronize(be->stream));
+        return QF_OK;
+    }
+    void* tmp = nullptr;
+    s = device_alloc(&tmp, total * 16);
+    if (s) return s;
+    hipLaunchKernelGGL((qf::k_convert<float, double>), dim3(grid_for(total)), dim3(qf::kThreads), 0, be->stream,
+                       static_cast<const cx<float>*>(b->d), static_cast<cx<double>*>(tmp), total);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, total * 16, hipMemcpyDeviceToHost, be->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(be->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "batch download complex64");
+    return QF_OK;
+}
+
+extern "C" qf_status qf_batch_expectation_pauli_sum(const qf_batch* b, const qf_pauli_term* terms, size_t nterms,
+                                                    double* out) {
+    if (!b || (!terms && nterms) || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    std::vector<qf::PauliDev> t;
+    qf_status s = pauli_terms(terms, nterms, b->qubits, t);
+    if (s) return s;
+    if ((s = activate(b->be))) return s;
+    std::vector<double> r;
+    for (uint32_t off = 0; off < b->count; off += 65535) {
+        const uint32_t n = std::min<uint32_t>(65535, b->count - off);
+        const char* base = static_cast<const char*>(b->d) + size_t(off) * b->dim * amp_bytes(b->precision);
+        s = reduce(b->be, base, nullptr, b->precision, b->dim, n, qf::RED_PAULI, 0, t, r);
+        if (s) return s;
+        for (uint32_t i = 0; i < n; ++i) out[off + i] = r[2 * i];
+    }
+    return QF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+extern "C" qf_status qf_plan_build(const qf_op* ops, size_t count, uint32_t qubits, const qf_plan_options* o,
+                                   qf_plan** out) {
+    if (!out || (!ops && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    qf_status s = check_qubits(qubits);
+    if (s) return s;
+    std::vector<qf::GateIn> g;
+    s = lower_ops(ops, count, qubits, g);
+    if (s) return s;
+    qf::PlanOptions po;
+    if (o) {
+        po.fuse = o->fuse;
+        po.tile_qubits = o->tile_qubits;
+        po.low_qubits = o->low_qubits;
+        po.max_block = o->max_block;
+    }
+    qf_plan* p = new qf_plan;
+    std::string err = qf::build_plan(g, qubits, QF_C128, po, p->plan);
+    if (!err.empty()) {
+        delete p;
+        return fail(QF_INVALID_ARGUMENT, err);
+    }
+    *out = p;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_plan_stats_get(const qf_plan* p, qf_plan_stats* out) {
+    if (!p || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    out->passes = p->plan.stats.passes;
+    out->ops_in = p->plan.stats.ops_in;
+    out->ops_applied = p->plan.stats.ops_applied;
+    out->dense_blocks = p->plan.stats.dense_blocks;
+    out->max_ops_per_pass = p->plan.stats.max_ops_per_pass;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_plan_describe(const qf_plan* p, char* buf, size_t cap, size_t* needed) {
+    if (!p) return fail(QF_INVALID_ARGUMENT, "null plan");
+    const std::string s = qf::describe_plan(p->plan);
+    if (needed) *needed = s.size() + 1;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_plan_destroy(qf_plan* p) {
+    if (!p) return fail(QF_INVALID_ARGUMENT, "null plan");
+    delete p;
+    return QF_OK;
+}
