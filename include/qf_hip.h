@@ -259,6 +259,57 @@ qf_status qf_batch_expectation_pauli_sum(const qf_batch* batch, const qf_pauli_t
                                          double* out);
 
 /* ------------------------------------------------------------------------ */
+/* Gradients (src/gradients.rs, src/lib.rs:3484-3704)                        */
+/* ------------------------------------------------------------------------ */
+/* Options of a gradient call.  Defaults (all zero except fd_step) follow
+ * GradientOptions::default, src/gradients.rs:155-165 (Auto, step 1e-3, 0 shots). */
+typedef struct {
+    int32_t strategy;      /* QF_GRAD_* */
+    int32_t precision;     /* QF_C128 or QF_C64 amplitudes (reductions stay FP64) */
+    double fd_step;        /* central-difference step (QF_GRAD_FINITE_DIFFERENCE) */
+    uint64_t shots;        /* only informs QF_GRAD_AUTO (GradientStrategy::select, src/gradients.rs:81-89) */
+    uint32_t tile_qubits;  /* forward tile (0 = default); the reverse sweep uses one qubit less */
+    uint32_t chunk;        /* parameter sets per device wave (0 = as many as fit) */
+} qf_gradient_options;
+
+/* ParameterizedCircuit::gradient_with_inputs, src/gradients.rs:232-312, with
+ * the observable as a Pauli sum (SingleQubitObservable = one term, weight 1).
+ * value_out gets <obs>; strategy_used the resolved rule; evaluations the
+ * GradientEstimate::evaluations count (2 for adjoint/backprop, 2 x rotation
+ * occurrences for shift / finite differences).  Errors as the reference:
+ * InvalidInputCount / NonFiniteParameter (inputs), InvalidParameterCount /
+ * NonFiniteParameter (parameters), QubitOutOfRange (observable, gates),
+ * OperationRequiresDensityMatrix (channels), InvalidFiniteDifferenceStep. */
+qf_status qf_gradient(qf_backend* backend, const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                      const double* inputs, size_t inputs_len, const double* params, size_t params_len,
+                      uint32_t qubits, const qf_pauli_term* obs, size_t nterms, const qf_gradient_options* options,
+                      double* grad_out, double* value_out, int32_t* strategy_used, uint64_t* evaluations);
+
+/* Batched gradients of one circuit at `sets` parameter sets (params is
+ * [sets][nparams], grads [sets][nparams], energies [sets]) with shared inputs.
+ * QF_GRAD_ADJOINT runs the two-state reverse sweep (psi is uncomputed, lambda
+ * carried back; replaces adjoint_gradient_with_inputs, src/lib.rs:3564-3704);
+ * QF_GRAD_PARAMETER_SHIFT runs 2 circuits per rotation occurrence
+ * (src/lib.rs:3516-3546); QF_GRAD_FINITE_DIFFERENCE runs the central
+ * difference (src/gradients.rs:344-375); QF_GRAD_BACKPROP the store-every-state
+ * reverse mode (src/gradients.rs:392-506). */
+qf_status qf_batch_gradient(qf_backend* backend, const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                            const double* inputs, uint32_t qubits, const double* params, uint32_t sets,
+                            const qf_pauli_term* obs, size_t nterms, const qf_gradient_options* options,
+                            double* energies, double* grads);
+
+/* <obs> at `sets` parameter sets (ParameterizedCircuit::run + expectation). */
+qf_status qf_batch_expectation(qf_backend* backend, const qf_pop* ops, size_t nops, uint32_t nparams,
+                               uint32_t ninputs, const double* inputs, uint32_t qubits, const double* params,
+                               uint32_t sets, const qf_pauli_term* obs, size_t nterms,
+                               const qf_gradient_options* options, double* energies);
+
+/* ParameterizedCircuit::run_with_inputs, src/lib.rs:3108-3121, into a resident state. */
+qf_status qf_state_run_parameterized(qf_state* state, const qf_pop* ops, size_t nops, uint32_t nparams,
+                                     uint32_t ninputs, const double* inputs, size_t inputs_len, const double* params,
+                                     size_t params_len);
+
+/* ------------------------------------------------------------------------ */
 /* Planning (no reference counterpart; see qf_plan_options)                  */
 /* ------------------------------------------------------------------------ */
 /* Builds the fused tile-pass schedule of a circuit (host only, no device use).

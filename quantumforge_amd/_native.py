@@ -43,7 +43,7 @@ class PlanStats(C.Structure):
 
 class GradOptions(C.Structure):
     _fields_ = [("strategy", C.c_int32), ("precision", C.c_int32), ("fd_step", C.c_double),
-                ("shots", C.c_uint64)]
+                ("shots", C.c_uint64), ("tile_qubits", C.c_uint32), ("chunk", C.c_uint32)]
 
 
 P = C.c_void_p
@@ -96,8 +96,9 @@ SIGNATURES = {
     "qf_plan_destroy": (None, [P]),
     "qf_gradient": (None, [P, P, sz, u32, u32, P, sz, P, sz, u32, P, sz, C.POINTER(GradOptions), P,
                            C.POINTER(dbl), C.POINTER(i32), C.POINTER(u64)]),
-    "qf_batch_adjoint_gradient": (None, [P, P, sz, u32, u32, P, u32, P, sz, i32, P, P]),
-    "qf_batch_shift_gradient": (None, [P, P, sz, u32, u32, P, u32, P, sz, i32, P, P]),
+    "qf_batch_gradient": (None, [P, P, sz, u32, u32, P, u32, P, u32, P, sz, C.POINTER(GradOptions), P, P]),
+    "qf_batch_expectation": (None, [P, P, sz, u32, u32, P, u32, P, u32, P, sz, C.POINTER(GradOptions), P]),
+    "qf_state_run_parameterized": (None, [P, P, sz, u32, u32, P, sz, P, sz]),
 }
 
 _lib = None

@@ -144,8 +144,9 @@ class HipDeviceState:
     """CudaDeviceState, src/cuda.rs:1277-1484: a resident device state."""
 
     def __init__(self, handle: C.c_void_p, backend: HipBackend) -> None:
+        # The C handle holds a reference on its backend (freed with the last
+        # state), so the Python backend object may be dropped first.
         self._h = handle
-        self._backend = backend  # keeps the Python object alive; the C side refcounts too
         q = C.c_uint32()
         check(lib().qf_state_qubits(handle, C.byref(q)))
         self.qubits_ = int(q.value)

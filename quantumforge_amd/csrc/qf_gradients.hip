@@ -1,0 +1,831 @@
+// Gradients of parameterized circuits on the device.
+//
+// Adjoint (two-state reverse sweep):
+//   forward:  psi_b = U(theta_b)|0..0> for every parameter set b (tile passes;
+//             rotation angles read per instance from an angle table);
+//   costate:  lambda_b = O psi_b and E_b = <psi_b|lambda_b> in one kernel;
+//   reverse:  over the inverted ops in reverse order, each rotation
+//             occurrence first records Im <lambda|G|psi> (G its generator)
+//             per tile, then psi and lambda are both multiplied by U_k^dagger.
+// This replaces the reference's store-every-state adjoint
+// (src/lib.rs:3564-3704, O(ops * 2^n) memory) with O(2 * 2^n) per set; the
+// accumulated quantity is the same: dE/dtheta = 2 Re <lambda|(-i/2) G|psi_after>
+// = Im <lambda|G|psi_after> (src/lib.rs:3649-3664).
+//
+// Parameter shift / finite differences: the 2 x (rotation occurrences)
+// shifted circuits of every set run as one batch of instances whose angle
+// table rows differ in the shifted occurrence (src/lib.rs:3516-3546,
+// src/gradients.rs:344-375).
+//
+// Backprop: the reference's own store-every-state reverse mode
+// (src/gradients.rs:392-506) with exact per-gate sweeps, kept as an
+// independent implementation to cross-check the adjoint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qf_runtime.h"
+
+using qf::cd;
+using qf::cx;
+using namespace qfrt;
+
+struct qf_backend;
+struct qf_state;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// lambda = O psi (Pauli sum, terms sorted by x mask) and the energy partials
+// conj(psi_i) lambda_i, FP64 compensated, one row per instance (grid.y).
+template <class R>
+__global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                              uint32_t n, const qf::PauliDev* __restrict__ terms,
+                                                              uint32_t nterms, qf::dd* __restrict__ partial) {
+    const uint64_t dim = uint64_t(1) << n;
+    const uint64_t inst = blockIdx.y;
+    const cx<R>* __restrict__ p = psi + inst * dim;
+    cx<R>* __restrict__ l = lam ? lam + inst * dim : nullptr;
+    qf::dd er{0, 0}, ei{0, 0};
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        double ar = 0.0, ai = 0.0;
+        uint64_t cur = ~uint64_t(0);
+        double pr = 0.0, pi = 0.0;
+        for (uint32_t t = 0; t < nterms; ++t) {
+            const qf::PauliDev& T = terms[t];
+            if (T.x != cur) {
+                cur = T.x;
+                const cx<R> v = p[i ^ cur];
+                pr = v.re;
+                pi = v.im;
+            }
+            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
+            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
+            const double cr = sg * T.cr, ci = sg * T.ci;
+            ar += cr * pr - ci * pi;
+            ai += cr * pi + ci * pr;
+        }
+        if (l) l[i] = cx<R>{R(ar), R(ai)};
+        const cx<R> v = p[i];
+        const double vr = v.re, vi = v.im;
+        qf::neumaier(er, vr * ar + vi * ai);  // Re(conj(psi) lambda)
+        qf::neumaier(ei, vr * ai - vi * ar);  // Im(conj(psi) lambda)
+    }
+    qf::block_reduce(er, ei);
+    if (threadIdx.x == 0) {
+        partial[2 * (inst * gridDim.x + blockIdx.x)] = er;
+        partial[2 * (inst * gridDim.x + blockIdx.x) + 1] = ei;
+    }
+}
+
+// grads[inst][p] = sum over the occurrences of p, in reverse circuit order as
+// the reference's reverse loop adds them, of the per-block overlaps summed
+// in block order (Neumaier).
+__global__ __launch_bounds__(qf::kThreads) void k_grad_finish(const double* __restrict__ partial, uint32_t nblk,
+                                                              uint32_t nslots, const uint32_t* __restrict__ pstart,
+                                                              const uint32_t* __restrict__ pslots, uint32_t nparams,
+                                                              uint32_t count, double* __restrict__ grads) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= uint64_t(count) * nparams) return;
+    const uint64_t inst = t / nparams;
+    const uint32_t p = uint32_t(t % nparams);
+    double g = 0.0;
+    for (uint32_t k = pstart[p]; k < pstart[p + 1]; ++k) {
+        const uint32_t slot = pslots[k];
+        qf::dd acc{0, 0};
+        for (uint32_t b = 0; b < nblk; ++b) qf::neumaier(acc, partial[(inst * nblk + b) * nslots + slot]);
+        g += acc.s + acc.c;
+    }
+    grads[t] = g;
+}
+
+// Im <lambda| G |psi> for G = X/Y/Z on qubit q (backprop's per-gate overlap).
+template <class R>
+__global__ __launch_bounds__(qf::kThreads) void k_gen_overlap(const cx<R>* __restrict__ lam,
+                                                              const cx<R>* __restrict__ psi, uint64_t dim, int axis,
+                                                              uint32_t q, qf::dd* __restrict__ partial) {
+    qf::dd acc{0, 0}, unused{0, 0};
+    const uint64_t m = uint64_t(1) << q;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const cx<R> l = lam[i];
+        const cx<R> p = axis == 2 ? psi[i] : psi[i ^ m];
+        const double lr = l.re, li = l.im, pr = p.re, pi = p.im;
+        const double re = lr * pr + li * pi, im = lr * pi - li * pr;
+        double v;
+        if (axis == 0)
+            v = im;
+        else if (axis == 1)
+            v = (i & m) ? re : -re;
+        else
+            v = (i & m) ? -im : im;
+        qf::neumaier(acc, v);
+    }
+    qf::block_reduce(acc, unused);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = acc;
+        partial[2 * blockIdx.x + 1] = qf::dd{0, 0};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct PCirc {
+    const qf_pop* ops;
+    size_t nops;
+    uint32_t nparams, ninputs;
+    std::vector<int32_t> slot_param;  // rotation occurrence -> parameter
+    std::vector<size_t> slot_op;      // rotation occurrence -> op index
+};
+
+qf_status qubit_error(uint32_t q, uint32_t qubits) {
+    return fail(QF_QUBIT_OUT_OF_RANGE,
+                "qubit index " + std::to_string(q) + " is out of range for " + std::to_string(qubits) + " qubits");
+}
+
+// validate_inputs / validate_parameters (src/lib.rs:4651-4675) for one set.
+qf_status validate_values(const PCirc& pc, const double* inputs, size_t inputs_len, const double* params,
+                          size_t params_len) {
+    if (inputs_len != pc.ninputs)
+        return fail(QF_INVALID_INPUT_COUNT, "expected " + std::to_string(pc.ninputs) + " encoded inputs, got " +
+                                                std::to_string(inputs_len));
+    for (size_t i = 0; i < inputs_len; ++i)
+        if (!std::isfinite(inputs[i])) return fail(QF_NON_FINITE_PARAMETER, "circuit parameters must be finite");
+    if (params_len != pc.nparams)
+        return fail(QF_INVALID_PARAMETER_COUNT, "expected " + std::to_string(pc.nparams) + " circuit parameters, got " +
+                                                    std::to_string(params_len));
+    for (size_t i = 0; i < params_len; ++i)
+        if (!std::isfinite(params[i])) return fail(QF_NON_FINITE_PARAMETER, "circuit parameters must be finite");
+    return QF_OK;
+}
+
+// Structural checks in op order (run_with_shift, src/lib.rs:4686-4736) and the
+// rotation-occurrence table.
+qf_status validate_ops(PCirc& pc, uint32_t qubits) {
+    pc.slot_param.clear();
+    pc.slot_op.clear();
+    for (size_t i = 0; i < pc.nops; ++i) {
+        const qf_pop& op = pc.ops[i];
+        switch (op.kind) {
+            case 0:
+                if (op.gate < QF_GATE_I || op.gate > QF_GATE_RZ) return fail(QF_INVALID_ARGUMENT, "unknown gate id");
+                if (op.a >= qubits) return qubit_error(op.a, qubits);
+                if (is_rotation(op.gate) && !std::isfinite(op.theta))
+                    return fail(QF_NON_FINITE_ANGLE, "rotation angle must be finite");
+                break;
+            case 1:
+            case 2:
+                if (op.gate < 0 || op.gate > 2) return fail(QF_INVALID_ARGUMENT, "unknown rotation axis");
+                if (op.kind == 1 && op.index >= pc.nparams)
+                    return fail(QF_INVALID_PARAMETER_COUNT, "expected " + std::to_string(pc.nparams) +
+                                                                " circuit parameters, got " +
+                                                                std::to_string(op.index + 1));
+                if (op.kind == 2 && op.index >= pc.ninputs)
+                    return fail(QF_INVALID_INPUT_COUNT, "expected " + std::to_string(pc.ninputs) +
+                                                            " encoded inputs, got " + std::to_string(op.index + 1));
+                if (op.a >= qubits) return qubit_error(op.a, qubits);
+                if (op.kind == 1) {
+                    pc.slot_param.push_back(int32_t(op.index));
+                    pc.slot_op.push_back(i);
+                }
+                break;
+            case 3:
+            case 4:
+            case 5: {
+                qf_status s = check_pair(qubits, op.a, op.b);
+                if (s) return s;
+                break;
+            }
+            case 6: return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+            default: return fail(QF_INVALID_ARGUMENT, "unknown op kind " + std::to_string(op.kind));
+        }
+    }
+    return QF_OK;
+}
+
+void adjoint_gate_id(int gate, double theta, int* g, double* th) {  // adjoint_gate, src/lib.rs:2370-2382
+    *th = theta;
+    switch (gate) {
+        case QF_GATE_RX: case QF_GATE_RY: case QF_GATE_RZ: *g = gate; *th = -theta; break;
+        case QF_GATE_S: *g = QF_GATE_SDG; break;
+        case QF_GATE_SDG: *g = QF_GATE_S; break;
+        case QF_GATE_T: *g = QF_GATE_TDG; break;
+        case QF_GATE_TDG: *g = QF_GATE_T; break;
+        default: *g = gate; break;
+    }
+}
+
+// Planner gates of the circuit (forward) or of its inverse in reverse order
+// with every rotation flagged TF_NEGATE | TF_RECORD (reverse).
+std::vector<qf::GateIn> param_gates(const PCirc& pc, const double* inputs, bool reverse) {
+    std::vector<qf::GateIn> out;
+    out.reserve(pc.nops);
+    std::vector<int32_t> slot_of(pc.nops, -1);
+    for (size_t s = 0; s < pc.slot_op.size(); ++s) slot_of[pc.slot_op[s]] = int32_t(s);
+    for (size_t k = 0; k < pc.nops; ++k) {
+        const size_t i = reverse ? pc.nops - 1 - k : k;
+        const qf_pop& op = pc.ops[i];
+        qf::GateIn g{};
+        g.a = op.a;
+        g.b = op.b;
+        switch (op.kind) {
+            case 0: {
+                int gid = op.gate;
+                double th = op.theta;
+                if (reverse) adjoint_gate_id(op.gate, op.theta, &gid, &th);
+                g.kind = 0;
+                gate_matrix(gid, th, g.m);
+                break;
+            }
+            case 1:
+                g.kind = 5;
+                g.axis = op.gate;
+                g.param = int32_t(op.index);
+                g.slot = slot_of[i];
+                g.flags = reverse ? (qf::TF_NEGATE | qf::TF_RECORD) : 0;
+                break;
+            case 2: {
+                const double th = reverse ? -inputs[op.index] : inputs[op.index];
+                g.kind = 0;
+                gate_matrix(QF_GATE_RX + op.gate, th, g.m);
+                break;
+            }
+            case 3: g.kind = 1; break;
+            case 4: g.kind = 2; break;
+            default: g.kind = 3; break;
+        }
+        out.push_back(g);
+    }
+    return out;
+}
+
+// Angle table row: (cos(theta/2), sin(theta/2)) per rotation occurrence with
+// theta = params[param] (+ delta on occurrence `shift_slot`), computed on the
+// host as Gate::matrix does (src/lib.rs:2303-2331).
+void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row) {
+    for (size_t s = 0; s < pc.slot_param.size(); ++s) {
+        double th = params[pc.slot_param[s]];
+        if (int64_t(s) == shift_slot) th = th + delta;
+        row[2 * s] = std::cos(th * 0.5);
+        row[2 * s + 1] = std::sin(th * 0.5);
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    qf_status alloc(size_t bytes) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        return device_alloc(&p, bytes ? bytes : 16);
+    }
+};
+
+template <class R>
+void fill_zero_states(hipStream_t st, void* d, uint64_t total, uint32_t qubits) {
+    hipLaunchKernelGGL(qf::k_fill_zero_state<R>, dim3(grid_for(total)), dim3(qf::kThreads), 0, st,
+                       static_cast<cx<R>*>(d), total, qubits);
+}
+
+// Observable terms sorted by x mask (k_pauli_apply reloads psi[i ^ x] only when x changes).
+std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t) {
+    std::vector<qf::PauliDev> s(t);
+    std::stable_sort(s.begin(), s.end(), [](const qf::PauliDev& a, const qf::PauliDev& b) { return a.x < b.x; });
+    return s;
+}
+
+struct GradRun {
+    qf_backend* be;
+    PCirc pc;
+    uint32_t qubits;
+    int precision;
+    std::vector<qf::PauliDev> terms;
+    qf::Plan fwd, rev;
+    uint32_t fwd_tile = 0, rev_tile = 0;
+};
+
+qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_rev) {
+    qf::PlanOptions po;
+    po.fuse = 1;
+    po.tile_qubits = tile ? tile : qf::default_tile_bits(R.precision);
+    std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
+    if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
+    if (need_rev) {
+        po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
+        e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
+        if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
+    }
+    return QF_OK;
+}
+
+uint32_t reduce_blocks(uint64_t dim) {
+    uint64_t b = (dim + qf::kThreads * 4 - 1) / (qf::kThreads * 4);
+    if (b > 1024) b = 1024;
+    return uint32_t(b ? b : 1);
+}
+
+// Largest wave of instances that fits `per_inst` bytes each in the free device memory.
+uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_t log_blocks_max) {
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = size_t(8) << 30;
+    const double budget = double(freeb) * 0.8;
+    uint64_t n = per_inst ? uint64_t(budget / double(per_inst)) : want;
+    uint32_t ylim = 65535u >> (log_blocks_max > 16 ? std::min<uint32_t>(log_blocks_max - 16, 16) : 0);
+    n = std::min<uint64_t>({n, want, ylim ? ylim : 1, 65535});
+    if (cap_chunk) n = std::min<uint64_t>(n, cap_chunk);
+    return uint32_t(n ? n : 0);
+}
+
+// Core: runs `cnt` instances whose angle-table rows are given (table is
+// [cnt][nslots][2] host doubles): forward, energy, and with grads != nullptr
+// the reverse sweep (grads [cnt][nparams]).  Waves of instances sized to the
+// free device memory.
+qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_t chunk_opt, double* energies,
+                    double* grads) {
+    qf_backend* be = R.be;
+    hipStream_t st = nullptr;
+    {
+        void* sp = nullptr;
+        qf_backend_stream(be, &sp);
+        st = static_cast<hipStream_t>(sp);
+    }
+    const bool want_grad = grads != nullptr;
+    const uint32_t n = R.qubits;
+    const uint64_t dim = uint64_t(1) << n;
+    const size_t ab = amp_bytes(R.precision);
+    const uint32_t nslots = uint32_t(R.pc.slot_param.size());
+    const uint32_t rev_M = want_grad && !R.rev.passes.empty() ? R.rev.passes[0].tile_bits : n;
+    const uint32_t nblk = uint32_t(1) << (n - std::min(rev_M, n));
+    const uint32_t eblk = reduce_blocks(dim);
+    const uint64_t per_inst = dim * ab * (want_grad ? 2 : 1) + uint64_t(nslots) * 16 +
+                              (want_grad ? uint64_t(nblk) * nslots * 8 + uint64_t(R.pc.nparams) * 8 : 0) +
+                              eblk * 32 + 64;
+    const uint32_t fwd_M = R.fwd.passes.empty() ? n : R.fwd.passes[0].tile_bits;
+    const uint32_t logmax = n - std::min(std::min(fwd_M, rev_M), n);
+    const uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax);
+    if (wave == 0) return fail(QF_OUT_OF_MEMORY, "device memory cannot hold one parameter set's states");
+
+    DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms;
+    qf_status s;
+    if ((s = psi.alloc(size_t(wave) * dim * ab))) return s;
+    if ((s = lam.alloc(want_grad ? size_t(wave) * dim * ab : 16))) return s;
+    if ((s = cs.alloc(size_t(wave) * nslots * 16))) return s;
+    if (want_grad && (s = part.alloc(size_t(wave) * nblk * nslots * 8))) return s;
+    if ((s = epart.alloc(size_t(wave) * eblk * 2 * sizeof(qf::dd)))) return s;
+    if ((s = eout.alloc(size_t(wave) * 2 * sizeof(double)))) return s;
+    if (want_grad && (s = gout.alloc(size_t(wave) * R.pc.nparams * sizeof(double)))) return s;
+    // parameter -> occurrences in reverse circuit order (CSR)
+    std::vector<uint32_t> ps(R.pc.nparams + 1, 0), pl(nslots);
+    for (uint32_t k = 0; k < nslots; ++k) ps[R.pc.slot_param[k] + 1]++;
+    for (uint32_t p = 0; p < R.pc.nparams; ++p) ps[p + 1] += ps[p];
+    {
+        std::vector<uint32_t> fill(ps.begin(), ps.end() - 1);
+        for (uint32_t k = nslots; k-- > 0;) pl[fill[R.pc.slot_param[k]]++] = k;
+    }
+    if ((s = pstart.alloc(ps.size() * 4))) return s;
+    if ((s = pslots.alloc(std::max<size_t>(pl.size(), 1) * 4))) return s;
+    if ((s = terms.alloc(std::max<size_t>(R.terms.size(), 1) * sizeof(qf::PauliDev)))) return s;
+    QF_HIP_TRY(hipMemcpyAsync(pstart.p, ps.data(), ps.size() * 4, hipMemcpyHostToDevice, st));
+    if (!pl.empty()) QF_HIP_TRY(hipMemcpyAsync(pslots.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, st));
+    const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
+    if (!tsorted.empty())
+        QF_HIP_TRY(hipMemcpyAsync(terms.p, tsorted.data(), tsorted.size() * sizeof(qf::PauliDev),
+                                  hipMemcpyHostToDevice, st));
+    QF_HIP_TRY(hipStreamSynchronize(st));  // host vectors above go out of scope with this function only
+
+    std::vector<const qf::TileOp*> bases;
+    std::vector<const qf::Plan*> plans{&R.fwd};
+    if (want_grad) plans.push_back(&R.rev);
+    if ((s = upload_plans(be, plans, bases))) return s;
+
+    std::vector<double> eo(size_t(wave) * 2);
+    for (uint32_t off = 0; off < cnt_total; off += wave) {
+        const uint32_t cnt = std::min(wave, cnt_total - off);
+        // Synchronous copy: the previous wave finished at its download below.
+        if (nslots)
+            QF_HIP_TRY(hipMemcpy(cs.p, table + size_t(off) * nslots * 2, size_t(cnt) * nslots * 16,
+                                 hipMemcpyHostToDevice));
+        if (R.precision == QF_C128)
+            fill_zero_states<double>(st, psi.p, dim * cnt, n);
+        else
+            fill_zero_states<float>(st, psi.p, dim * cnt, n);
+        if ((s = launch_check("k_fill_zero_state"))) return s;
+        LaunchCtx fctx;
+        fctx.cs = static_cast<const double*>(cs.p);
+        fctx.nslots = nslots;
+        if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.fwd, bases[0], fctx))) return s;
+        dim3 eg(eblk, cnt);
+        void* lam_out = want_grad ? lam.p : nullptr;
+        if (R.precision == QF_C128)
+            hipLaunchKernelGGL(k_pauli_apply<double>, eg, dim3(qf::kThreads), 0, st,
+                               static_cast<const cx<double>*>(psi.p), static_cast<cx<double>*>(lam_out), n,
+                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                               static_cast<qf::dd*>(epart.p));
+        else
+            hipLaunchKernelGGL(k_pauli_apply<float>, eg, dim3(qf::kThreads), 0, st,
+                               static_cast<const cx<float>*>(psi.p), static_cast<cx<float>*>(lam_out), n,
+                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                               static_cast<qf::dd*>(epart.p));
+        if ((s = launch_check("k_pauli_apply"))) return s;
+        hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(cnt), dim3(qf::kThreads), 0, st,
+                           static_cast<const qf::dd*>(epart.p), eblk, static_cast<double*>(eout.p));
+        if ((s = launch_check("k_reduce_finish"))) return s;
+        if (want_grad) {
+            LaunchCtx rctx;
+            rctx.cs = static_cast<const double*>(cs.p);
+            rctx.nslots = nslots;
+            rctx.lam = lam.p;
+            rctx.partial = static_cast<double*>(part.p);
+            if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.rev, bases[1], rctx))) return s;
+            const uint64_t work = uint64_t(cnt) * R.pc.nparams;
+            if (work) {
+                hipLaunchKernelGGL(k_grad_finish, dim3(unsigned((work + 255) / 256)), dim3(256), 0, st,
+                                   static_cast<const double*>(part.p), nblk, nslots,
+                                   static_cast<const uint32_t*>(pstart.p), static_cast<const uint32_t*>(pslots.p),
+                                   R.pc.nparams, cnt, static_cast<double*>(gout.p));
+                if ((s = launch_check("k_grad_finish"))) return s;
+                QF_HIP_TRY(hipMemcpyAsync(grads + size_t(off) * R.pc.nparams, gout.p, work * sizeof(double),
+                                          hipMemcpyDeviceToHost, st));
+            }
+        }
+        QF_HIP_TRY(hipMemcpyAsync(eo.data(), eout.p, size_t(cnt) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        QF_HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t b = 0; b < cnt; ++b) energies[off + b] = eo[2 * b];
+    }
+    return QF_OK;
+}
+
+// Energies (and with grads, adjoint gradients) at `sets` parameter rows.
+qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t chunk_opt, double* energies,
+                       double* grads) {
+    const uint32_t nslots = uint32_t(R.pc.slot_param.size());
+    std::vector<double> table(std::max<size_t>(size_t(sets) * nslots * 2, 1));
+    for (uint32_t b = 0; b < sets; ++b)
+        fill_row(R.pc, params + size_t(b) * R.pc.nparams, -1, 0.0, table.data() + size_t(b) * nslots * 2);
+    return table_run(R, table.data(), sets, chunk_opt, energies, grads);
+}
+
+// Shift rule (fd = false, delta = pi/2) or central differences (fd = true,
+// delta = step): 2 x occurrences shifted circuits per set, run as instances
+// whose angle rows differ in the shifted occurrence only.
+qf_status shift_sets(GradRun& R, const double* params, uint32_t sets, uint32_t chunk_opt, bool fd, double step,
+                     double* energies, double* grads) {
+    const uint32_t nslots = uint32_t(R.pc.slot_param.size());
+    qf_status s = adjoint_sets(R, params, sets, chunk_opt, energies, nullptr);
+    if (s) return s;
+    std::memset(grads, 0, size_t(sets) * R.pc.nparams * sizeof(double));
+    if (nslots == 0) return QF_OK;
+    const double delta = fd ? step : 1.57079632679489661923;  // FRAC_PI_2
+    // Groups of sets keep the host tables bounded (about 64 MiB).
+    const uint64_t row_bytes = uint64_t(nslots) * 16;
+    const uint64_t per_set = row_bytes * nslots * 2;
+    const uint32_t per_group = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(sets, (uint64_t(64) << 20) / per_set)));
+    std::vector<double> table;
+    std::vector<double> eg;
+    for (uint32_t g0 = 0; g0 < sets; g0 += per_group) {
+        const uint32_t gs = std::min(per_group, sets - g0);
+        const uint32_t cnt = gs * nslots * 2;
+        table.assign(size_t(cnt) * nslots * 2, 0.0);
+        for (uint32_t b = 0; b < gs; ++b)
+            for (uint32_t k = 0; k < nslots; ++k)
+                for (int sg = 0; sg < 2; ++sg) {
+                    const size_t row = (size_t(b) * nslots + k) * 2 + sg;
+                    fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), sg == 0 ? delta : -delta,
+                             table.data() + row * nslots * 2);
+                }
+        eg.assign(cnt, 0.0);
+        if ((s = table_run(R, table.data(), cnt, chunk_opt, eg.data(), nullptr))) return s;
+        for (uint32_t b = 0; b < gs; ++b) {
+            double* gr = grads + size_t(g0 + b) * R.pc.nparams;
+            for (uint32_t k = 0; k < nslots; ++k) {  // occurrence order, as src/lib.rs:3534-3543
+                const double plus = eg[(size_t(b) * nslots + k) * 2], minus = eg[(size_t(b) * nslots + k) * 2 + 1];
+                if (fd)
+                    gr[R.pc.slot_param[k]] += (plus - minus) / (2.0 * step);
+                else
+                    gr[R.pc.slot_param[k]] += 0.5 * (plus - minus);
+            }
+        }
+    }
+    return QF_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// The reference's backprop (src/gradients.rs:392-506): forward storing every
+// intermediate state, costate O psi_final, reverse accumulation of
+// Im <lambda|G|psi_after> with exact per-gate sweeps.  One set at a time.
+qf_status backprop_set(GradRun& R, const double* inputs, const double* params, double* energy, double* grad) {
+    qf_backend* be = R.be;
+    void* sp = nullptr;
+    qf_backend_stream(be, &sp);
+    hipStream_t st = static_cast<hipStream_t>(sp);
+    const uint32_t n = R.qubits;
+    const uint64_t dim = uint64_t(1) << n;
+    const size_t ab = amp_bytes(R.precision);
+    const size_t nstates = R.pc.nops + 1;
+    size_t freeb = 0, total = 0;
+    QF_HIP_TRY(hipMemGetInfo(&freeb, &total));
+    const double need = double(nstates + 1) * double(dim) * double(ab);
+    if (need > 0.5 * double(freeb))
+        return fail(QF_MEMORY_BUDGET_EXCEEDED, "backprop stores every intermediate state: needs " +
+                                                   std::to_string(uint64_t(need)) + " bytes, " +
+                                                   std::to_string(uint64_t(freeb / 2)) + " allowed");
+    DevBuf states, lam, part, eout, terms;
+    qf_status s;
+    if ((s = states.alloc(nstates * dim * ab))) return s;
+    if ((s = lam.alloc(dim * ab))) return s;
+    const uint32_t eblk = reduce_blocks(dim);
+    if ((s = part.alloc(size_t(eblk) * 2 * sizeof(qf::dd)))) return s;
+    if ((s = eout.alloc(2 * sizeof(double)))) return s;
+    const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
+    if ((s = terms.alloc(std::max<size_t>(tsorted.size(), 1) * sizeof(qf::PauliDev)))) return s;
+    if (!tsorted.empty())
+        QF_HIP_TRY(hipMemcpy(terms.p, tsorted.data(), tsorted.size() * sizeof(qf::PauliDev), hipMemcpyHostToDevice));
+    char* base = static_cast<char*>(states.p);
+    auto sp_at = [&](size_t k) { return static_cast<void*>(base + k * dim * ab); };
+    if (R.precision == QF_C128)
+        fill_zero_states<double>(st, sp_at(0), dim, n);
+    else
+        fill_zero_states<float>(st, sp_at(0), dim, n);
+    if ((s = launch_check("k_fill_zero_state"))) return s;
+    qf::PlanOptions exact;
+    exact.fuse = 0;
+    auto gate_of = [&](size_t i, bool inverse) {
+        const qf_pop& op = R.pc.ops[i];
+        qf::GateIn g{};
+        g.a = op.a;
+        g.b = op.b;
+        if (op.kind == 0) {
+            int gid = op.gate;
+            double th = op.theta;
+            if (inverse) adjoint_gate_id(op.gate, op.theta, &gid, &th);
+            g.kind = 0;
+            gate_matrix(gid, th, g.m);
+        } else if (op.kind == 1 || op.kind == 2) {
+            const double v = op.kind == 1 ? params[op.index] : inputs[op.index];
+            g.kind = 0;
+            gate_matrix(QF_GATE_RX + op.gate, inverse ? -v : v, g.m);
+        } else {
+            g.kind = op.kind - 2;  // 3 CNOT -> 1, 4 CZ -> 2, 5 SWAP -> 3
+        }
+        return g;
+    };
+    for (size_t i = 0; i < R.pc.nops; ++i) {
+        QF_HIP_TRY(hipMemcpyAsync(sp_at(i + 1), sp_at(i), dim * ab, hipMemcpyDeviceToDevice, st));
+        if ((s = apply_gates(be, sp_at(i + 1), R.precision, n, 1, {gate_of(i, false)}, exact))) return s;
+    }
+    if (R.precision == QF_C128)
+        hipLaunchKernelGGL(k_pauli_apply<double>, dim3(eblk, 1), dim3(qf::kThreads), 0, st,
+                           static_cast<const cx<double>*>(sp_at(R.pc.nops)), static_cast<cx<double>*>(lam.p), n,
+                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                           static_cast<qf::dd*>(part.p));
+    else
+        hipLaunchKernelGGL(k_pauli_apply<float>, dim3(eblk, 1), dim3(qf::kThreads), 0, st,
+                           static_cast<const cx<float>*>(sp_at(R.pc.nops)), static_cast<cx<float>*>(lam.p), n,
+                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                           static_cast<qf::dd*>(part.p));
+    if ((s = launch_check("k_pauli_apply"))) return s;
+    double e2[2];
+    hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(1), dim3(qf::kThreads), 0, st, static_cast<const qf::dd*>(part.p),
+                       eblk, static_cast<double*>(eout.p));
+    QF_HIP_TRY(hipMemcpyAsync(e2, eout.p, sizeof(e2), hipMemcpyDeviceToHost, st));
+    QF_HIP_TRY(hipStreamSynchronize(st));
+    *energy = e2[0];
+    for (uint32_t p = 0; p < R.pc.nparams; ++p) grad[p] = 0.0;
+    for (size_t i = R.pc.nops; i-- > 0;) {
+        const qf_pop& op = R.pc.ops[i];
+        if (op.kind == 1) {
+            if (R.precision == QF_C128)
+                hipLaunchKernelGGL(k_gen_overlap<double>, dim3(eblk), dim3(qf::kThreads), 0, st,
+                                   static_cast<const cx<double>*>(lam.p), static_cast<const cx<double>*>(sp_at(i + 1)),
+                                   dim, op.gate, op.a, static_cast<qf::dd*>(part.p));
+            else
+                hipLaunchKernelGGL(k_gen_overlap<float>, dim3(eblk), dim3(qf::kThreads), 0, st,
+                                   static_cast<const cx<float>*>(lam.p), static_cast<const cx<float>*>(sp_at(i + 1)),
+                                   dim, op.gate, op.a, static_cast<qf::dd*>(part.p));
+            if ((s = launch_check("k_gen_overlap"))) return s;
+            hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(1), dim3(qf::kThreads), 0, st,
+                               static_cast<const qf::dd*>(part.p), eblk, static_cast<double*>(eout.p));
+            QF_HIP_TRY(hipMemcpyAsync(e2, eout.p, sizeof(e2), hipMemcpyDeviceToHost, st));
+            QF_HIP_TRY(hipStreamSynchronize(st));
+            grad[op.index] += e2[0];
+        }
+        if ((s = apply_gates(be, lam.p, R.precision, n, 1, {gate_of(i, true)}, exact))) return s;
+    }
+    QF_HIP_TRY(hipStreamSynchronize(st));
+    return QF_OK;
+}
+
+qf_gradient_options default_options() {
+    qf_gradient_options o{};
+    o.strategy = QF_GRAD_AUTO;
+    o.precision = QF_C128;
+    o.fd_step = 1e-3;  // GradientOptions::default, src/gradients.rs:161
+    return o;
+}
+
+// Shared validation of the gradient entry points, in the reference's order.
+qf_status prepare(qf_backend* be, const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                  const double* inputs, size_t inputs_len, const double* params, uint32_t sets, uint32_t qubits,
+                  const qf_pauli_term* obs, size_t nterms, const qf_gradient_options& o, int32_t strategy,
+                  GradRun& R) {
+    if (!be || (!ops && nops) || (!params && nparams && sets) || (!inputs && inputs_len) || (!obs && nterms))
+        return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_status s = check_qubits(qubits);
+    if (s) return s;
+    if (o.precision != QF_C128 && o.precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    R.be = be;
+    R.pc.ops = ops;
+    R.pc.nops = nops;
+    R.pc.nparams = nparams;
+    R.pc.ninputs = ninputs;
+    R.qubits = qubits;
+    R.precision = o.precision;
+    if (strategy == QF_GRAD_FINITE_DIFFERENCE && (!std::isfinite(o.fd_step) || o.fd_step <= 0.0))
+        return fail(QF_INVALID_FD_STEP, "finite-difference step must be positive and finite, got " +
+                                            std::to_string(o.fd_step));
+    for (uint32_t b = 0; b < sets; ++b)
+        if ((s = validate_values(R.pc, inputs, inputs_len, params + size_t(b) * nparams, nparams))) return s;
+    if ((s = pauli_terms(obs, nterms, qubits, R.terms))) return s;
+    if ((s = validate_ops(R.pc, qubits))) return s;
+    return activate(be);
+}
+
+int32_t resolve(int32_t strategy, const qf_pop* ops, size_t nops, uint64_t shots, bool* channel) {
+    *channel = false;
+    for (size_t i = 0; i < nops; ++i)
+        if (ops[i].kind == 6) *channel = true;
+    if (strategy != QF_GRAD_AUTO) return strategy;
+    // GradientStrategy::select, src/gradients.rs:81-89.
+    if (*channel) return -1;
+    return shots > 0 ? QF_GRAD_PARAMETER_SHIFT : QF_GRAD_ADJOINT;
+}
+
+qf_status run_strategy(GradRun& R, int32_t strategy, const double* inputs, const double* params, uint32_t sets,
+                       const qf_gradient_options& o, double* energies, double* grads) {
+    qf_status s;
+    switch (strategy) {
+        case QF_GRAD_ADJOINT:
+            if ((s = plan_for(R, inputs, o.tile_qubits, true))) return s;
+            return adjoint_sets(R, params, sets, o.chunk, energies, grads);
+        case QF_GRAD_PARAMETER_SHIFT:
+        case QF_GRAD_FINITE_DIFFERENCE:
+            if ((s = plan_for(R, inputs, o.tile_qubits, false))) return s;
+            return shift_sets(R, params, sets, o.chunk, strategy == QF_GRAD_FINITE_DIFFERENCE, o.fd_step, energies,
+                              grads);
+        case QF_GRAD_BACKPROP:
+            for (uint32_t b = 0; b < sets; ++b)
+                if ((s = backprop_set(R, inputs, params + size_t(b) * R.pc.nparams, energies + b,
+                                      grads + size_t(b) * R.pc.nparams)))
+                    return s;
+            return QF_OK;
+        default: return fail(QF_INVALID_ARGUMENT, "unknown gradient strategy " + std::to_string(strategy));
+    }
+}
+
+}  // namespace
+
+extern "C" qf_status qf_batch_gradient(qf_backend* be, const qf_pop* ops, size_t nops, uint32_t nparams,
+                                       uint32_t ninputs, const double* inputs, uint32_t qubits, const double* params,
+                                       uint32_t sets, const qf_pauli_term* obs, size_t nterms,
+                                       const qf_gradient_options* options, double* energies, double* grads) {
+    if (sets == 0) return fail(QF_EMPTY_BATCH, "a gradient batch cannot be empty");
+    if (!energies || (!grads && nparams)) return fail(QF_INVALID_ARGUMENT, "null output");
+    const qf_gradient_options o = options ? *options : default_options();
+    bool channel = false;
+    const int32_t strategy = resolve(o.strategy, ops, nops, o.shots, &channel);
+    if (strategy < 0) return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+    GradRun R;
+    qf_status s = prepare(be, ops, nops, nparams, ninputs, inputs, ninputs, params, sets, qubits, obs, nterms, o,
+                          strategy, R);
+    if (s) return s;
+    std::vector<double> gtmp;
+    if (!grads) {
+        gtmp.resize(1);
+        grads = gtmp.data();
+    }
+    return run_strategy(R, strategy, inputs, params, sets, o, energies, grads);
+}
+
+extern "C" qf_status qf_gradient(qf_backend* be, const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                                 const double* inputs, size_t inputs_len, const double* params, size_t params_len,
+                                 uint32_t qubits, const qf_pauli_term* obs, size_t nterms,
+                                 const qf_gradient_options* options, double* grad_out, double* value_out,
+                                 int32_t* strategy_used, uint64_t* evaluations) {
+    const qf_gradient_options o = options ? *options : default_options();
+    bool channel = false;
+    const int32_t strategy = resolve(o.strategy, ops, nops, o.shots, &channel);
+    if (strategy < 0) return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+    if (!grad_out && nparams) return fail(QF_INVALID_ARGUMENT, "null output");
+    GradRun R;
+    // validate_parameters checks the supplied length before anything else on it.
+    if (strategy == QF_GRAD_FINITE_DIFFERENCE && (!std::isfinite(o.fd_step) || o.fd_step <= 0.0))
+        return fail(QF_INVALID_FD_STEP, "finite-difference step must be positive and finite, got " +
+                                            std::to_string(o.fd_step));
+    if (inputs_len != ninputs)
+        return fail(QF_INVALID_INPUT_COUNT, "expected " + std::to_string(ninputs) + " encoded inputs, got " +
+                                                std::to_string(inputs_len));
+    for (size_t i = 0; i < inputs_len; ++i)
+        if (!std::isfinite(inputs[i])) return fail(QF_NON_FINITE_PARAMETER, "circuit parameters must be finite");
+    if (params_len != nparams)
+        return fail(QF_INVALID_PARAMETER_COUNT, "expected " + std::to_string(nparams) + " circuit parameters, got " +
+                                                    std::to_string(params_len));
+    qf_status s = prepare(be, ops, nops, nparams, ninputs, inputs, inputs_len, params, 1, qubits, obs, nterms, o,
+                          strategy, R);
+    if (s) return s;
+    double energy = 0.0;
+    std::vector<double> g(std::max<size_t>(nparams, 1));
+    if ((s = run_strategy(R, strategy, inputs, params, 1, o, &energy, g.data()))) return s;
+    if (nparams) std::memcpy(grad_out, g.data(), nparams * sizeof(double));
+    if (value_out) *value_out = energy;
+    if (strategy_used) *strategy_used = strategy;
+    if (evaluations) {  // GradientEstimate::evaluations, src/gradients.rs:256-306
+        const uint64_t occ = R.pc.slot_param.size();
+        *evaluations = (strategy == QF_GRAD_ADJOINT || strategy == QF_GRAD_BACKPROP) ? 2 : 2 * occ;
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_batch_expectation(qf_backend* be, const qf_pop* ops, size_t nops, uint32_t nparams,
+                                          uint32_t ninputs, const double* inputs, uint32_t qubits,
+                                          const double* params, uint32_t sets, const qf_pauli_term* obs,
+                                          size_t nterms, const qf_gradient_options* options, double* energies) {
+    if (sets == 0) return fail(QF_EMPTY_BATCH, "a batch cannot be empty");
+    if (!energies) return fail(QF_INVALID_ARGUMENT, "null output");
+    const qf_gradient_options o = options ? *options : default_options();
+    GradRun R;
+    qf_status s = prepare(be, ops, nops, nparams, ninputs, inputs, ninputs, params, sets, qubits, obs, nterms, o,
+                          QF_GRAD_ADJOINT, R);
+    if (s) return s;
+    for (size_t i = 0; i < nops; ++i)
+        if (ops[i].kind == 6)
+            return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+    if ((s = plan_for(R, inputs, o.tile_qubits, false))) return s;
+    return adjoint_sets(R, params, sets, o.chunk, energies, nullptr);
+}
+
+extern "C" qf_status qf_state_run_parameterized(qf_state* st, const qf_pop* ops, size_t nops, uint32_t nparams,
+                                                uint32_t ninputs, const double* inputs, size_t inputs_len,
+                                                const double* params, size_t params_len) {
+    if (!st) return fail(QF_INVALID_ARGUMENT, "null state");
+    uint32_t qubits = 0;
+    int32_t precision = 0;
+    qf_state_qubits(st, &qubits);
+    qf_state_precision(st, &precision);
+    void* d = nullptr;
+    qf_state_device_pointer(st, &d);
+    qf_backend* be = state_backend(st);
+    GradRun R;
+    R.be = be;
+    R.pc.ops = ops;
+    R.pc.nops = nops;
+    R.pc.nparams = nparams;
+    R.pc.ninputs = ninputs;
+    R.qubits = qubits;
+    R.precision = precision;
+    if ((!ops && nops) || (!params && params_len) || (!inputs && inputs_len))
+        return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_status s = validate_values(R.pc, inputs, inputs_len, params, params_len);
+    if (s) return s;
+    if ((s = validate_ops(R.pc, qubits))) return s;
+    if ((s = activate(be))) return s;
+    if ((s = plan_for(R, inputs, 0, false))) return s;
+    const uint32_t nslots = uint32_t(R.pc.slot_param.size());
+    std::vector<double> row(std::max<size_t>(size_t(nslots) * 2, 2));
+    fill_row(R.pc, params, -1, 0.0, row.data());
+    void* sp = nullptr;
+    qf_backend_stream(be, &sp);
+    hipStream_t strm = static_cast<hipStream_t>(sp);
+    DevBuf cs;
+    if ((s = cs.alloc(row.size() * sizeof(double)))) return s;
+    QF_HIP_TRY(hipMemcpy(cs.p, row.data(), row.size() * sizeof(double), hipMemcpyHostToDevice));
+    const uint64_t dim = uint64_t(1) << qubits;
+    if (precision == QF_C128)
+        fill_zero_states<double>(strm, d, dim, qubits);
+    else
+        fill_zero_states<float>(strm, d, dim, qubits);
+    if ((s = launch_check("k_fill_zero_state"))) return s;
+    std::vector<const qf::TileOp*> bases;
+    if ((s = upload_plans(be, {&R.fwd}, bases))) return s;
+    LaunchCtx ctx;
+    ctx.cs = static_cast<const double*>(cs.p);
+    ctx.nslots = nslots;
+    if ((s = launch_plan(be, d, precision, qubits, 1, R.fwd, bases[0], ctx))) return s;
+    QF_HIP_TRY(hipStreamSynchronize(strm));  // cs is freed on return
+    return QF_OK;
+}

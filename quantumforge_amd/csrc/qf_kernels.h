@@ -134,6 +134,7 @@ struct TileArgs {
     uint32_t low;        // contiguous low qubits
     uint32_t log_blocks; // n - M
     uint32_t nops;
+    uint32_t nslots;     // columns of the per-instance angle table (rotation occurrences)
     uint32_t high[40];   // high[k] for k < M - low
 };
 
@@ -144,110 +145,253 @@ __device__ __forceinline__ cx<R> cmul_d(const double* m, int k, cx<R> a) {
 }
 
 template <class R>
-__global__ __launch_bounds__(kThreads) void k_tile(cx<R>* __restrict__ s, TileArgs args,
-                                                   const TileOp* __restrict__ ops) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const uint32_t M = args.M, low = args.low, nh = M - low;
-    const uint32_t T = 1u << M;
-    cx<R>* tile = reinterpret_cast<cx<R>*>(smem_raw);
-    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem_raw + sizeof(cx<R>) * T);
+__device__ __forceinline__ cx<R> cmul_rr(R mr, R mi, cx<R> a) {
+    return cx<R>{mr * a.re - mi * a.im, mr * a.im + mi * a.re};
+}
 
+// Workgroup geometry of a tile: global index of tile element j is
+// base + hoff[j >> low] + (j & lowmask).
+struct TileGeo {
+    uint64_t inst, bl, base;
+    uint32_t T, low, lowmask, nh;
+};
+
+__device__ __forceinline__ TileGeo tile_geometry(const TileArgs& args, uint64_t* hoff) {
+    TileGeo t;
+    t.nh = args.M - args.low;
+    t.T = 1u << args.M;
+    t.low = args.low;
+    t.lowmask = (1u << args.low) - 1;
     const uint64_t b = uint64_t(blockIdx.x) + uint64_t(blockIdx.y) * gridDim.x;
-    const uint64_t inst = b >> args.log_blocks;
-    const uint64_t bl = b & ((uint64_t(1) << args.log_blocks) - 1);
+    t.inst = b >> args.log_blocks;
+    t.bl = b & ((uint64_t(1) << args.log_blocks) - 1);
     // Deposit the workgroup bits into the qubits outside the tile.
     uint64_t base = 0;
-    {
-        uint32_t k = 0, bit = 0;
-        for (uint32_t g = low; g < args.n; ++g) {
-            if (k < nh && args.high[k] == g) {
-                ++k;
-                continue;
-            }
-            base |= ((bl >> bit) & 1) << g;
-            ++bit;
+    uint32_t k = 0, bit = 0;
+    for (uint32_t g = args.low; g < args.n; ++g) {
+        if (k < t.nh && args.high[k] == g) {
+            ++k;
+            continue;
         }
+        base |= ((t.bl >> bit) & 1) << g;
+        ++bit;
     }
-    base += inst << args.n;
-    for (uint32_t h = threadIdx.x; h < (1u << nh); h += blockDim.x) {
+    t.base = base + (t.inst << args.n);
+    for (uint32_t h = threadIdx.x; h < (1u << t.nh); h += blockDim.x) {
         uint64_t off = 0;
-        for (uint32_t k = 0; k < nh; ++k) off |= uint64_t((h >> k) & 1) << args.high[k];
+        for (uint32_t kk = 0; kk < t.nh; ++kk) off |= uint64_t((h >> kk) & 1) << args.high[kk];
         hoff[h] = off;
     }
-    __syncthreads();
-    const uint32_t lowmask = (1u << low) - 1;
-    cx<R>* __restrict__ g = s + base;
-    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) tile[j] = g[hoff[j >> low] + (j & lowmask)];
-    __syncthreads();
+    return t;
+}
 
-    for (uint32_t o = 0; o < args.nops; ++o) {
-        const TileOp& op = ops[o];
-        const int kind = op.kind;
+// Rotation exp(-i theta P/2) from the instance's (cos(theta/2), sin(theta/2));
+// the inverse (TF_NEGATE) uses sin -> -sin.  Matrices as Gate::matrix,
+// src/lib.rs:2303-2331.
+template <class R>
+__device__ __forceinline__ void rot_cs(const TileOp& op, const double* __restrict__ cs_row, R& c, R& s) {
+    const double cc = cs_row[2 * op.slot], ss = cs_row[2 * op.slot + 1];
+    c = R(cc);
+    s = R((op.flags & TF_NEGATE) ? -ss : ss);
+}
+
+// Applies one op to a tile held in LDS (all threads participate; caller syncs).
+template <class R>
+__device__ __forceinline__ void tile_apply(cx<R>* __restrict__ tile, const TileOp& op, const TileGeo& g,
+                                           const uint64_t* __restrict__ hoff, const double* __restrict__ cs_row) {
+    const uint32_t T = g.T;
+    const int kind = op.kind;
+    if (kind == TOP_MAT1 || (kind == TOP_ROT && op.q1 != 2)) {
+        const uint32_t q = uint32_t(op.q0);
+        R m0r, m0i, m1r, m1i, m2r, m2i, m3r, m3i;
         if (kind == TOP_MAT1) {
-            const uint32_t q = uint32_t(op.q0);
-            for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
-                const uint32_t lo = p & ((1u << q) - 1);
-                const uint32_t i0 = ((p ^ lo) << 1) | lo, i1 = i0 | (1u << q);
-                const cx<R> a0 = tile[i0], a1 = tile[i1];
-                const cx<R> x0 = cmul_d(op.m, 0, a0), x1 = cmul_d(op.m, 1, a1);
-                const cx<R> y0 = cmul_d(op.m, 2, a0), y1 = cmul_d(op.m, 3, a1);
-                tile[i0] = cx<R>{x0.re + x1.re, x0.im + x1.im};
-                tile[i1] = cx<R>{y0.re + y1.re, y0.im + y1.im};
-            }
-        } else if (kind == TOP_DIAG) {
-            const int q0 = op.q0, q1 = op.q1;
-            for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
-                const uint64_t gi = base + hoff[j >> low] + (j & lowmask);
-                const int k = int((gi >> q0) & 1) | (q1 >= 0 ? int((gi >> q1) & 1) << 1 : 0);
-                tile[j] = cmul_d(op.m, k, tile[j]);
-            }
-        } else if (kind == TOP_CNOT || kind == TOP_SWAP) {
-            const uint32_t a = uint32_t(op.q0), c = uint32_t(op.q1);
-            const uint32_t lo = a < c ? a : c, hi = a < c ? c : a;
-            for (uint32_t p = threadIdx.x; p < T / 4; p += blockDim.x) {
-                uint32_t x = p;
-                uint32_t l = x & ((1u << lo) - 1);
-                x = ((x ^ l) << 1) | l;
-                l = x & ((1u << hi) - 1);
-                x = ((x ^ l) << 1) | l;
-                const uint32_t ia = x | (1u << a), ic = x | (1u << c);
-                if (kind == TOP_CNOT) {
-                    const cx<R> u = tile[ia], v = tile[ia | (1u << c)];
-                    tile[ia] = v;
-                    tile[ia | (1u << c)] = u;
-                } else {
-                    const cx<R> u = tile[ia], v = tile[ic];
-                    tile[ia] = v;
-                    tile[ic] = u;
-                }
-            }
-        } else {  // TOP_MAT2
-            const uint32_t a = uint32_t(op.q0), c = uint32_t(op.q1);
-            const uint32_t lo = a < c ? a : c, hi = a < c ? c : a;
-            for (uint32_t p = threadIdx.x; p < T / 4; p += blockDim.x) {
-                uint32_t x = p;
-                uint32_t l = x & ((1u << lo) - 1);
-                x = ((x ^ l) << 1) | l;
-                l = x & ((1u << hi) - 1);
-                x = ((x ^ l) << 1) | l;
-                const uint32_t idx[4] = {x, x | (1u << a), x | (1u << c), x | (1u << a) | (1u << c)};
-                cx<R> v[4];
-                for (int r = 0; r < 4; ++r) v[r] = tile[idx[r]];
-                for (int r = 0; r < 4; ++r) {
-                    cx<R> acc{R(0), R(0)};
-                    for (int k = 0; k < 4; ++k) {
-                        const cx<R> t = cmul_d(op.m, r * 4 + k, v[k]);
-                        acc.re += t.re;
-                        acc.im += t.im;
-                    }
-                    tile[idx[r]] = acc;
-                }
+            m0r = R(op.m[0]); m0i = R(op.m[1]); m1r = R(op.m[2]); m1i = R(op.m[3]);
+            m2r = R(op.m[4]); m2i = R(op.m[5]); m3r = R(op.m[6]); m3i = R(op.m[7]);
+        } else {
+            R c, s;
+            rot_cs<R>(op, cs_row, c, s);
+            if (op.q1 == 0) {  // Rx = [[c, -is], [-is, c]]
+                m0r = c; m0i = 0; m1r = 0; m1i = -s; m2r = 0; m2i = -s; m3r = c; m3i = 0;
+            } else {           // Ry = [[c, -s], [s, c]]
+                m0r = c; m0i = 0; m1r = -s; m1i = 0; m2r = s; m2i = 0; m3r = c; m3i = 0;
             }
         }
+        for (uint32_t p = threadIdx.x; p < T / 2; p += blockDim.x) {
+            const uint32_t lo = p & ((1u << q) - 1);
+            const uint32_t i0 = ((p ^ lo) << 1) | lo, i1 = i0 | (1u << q);
+            const cx<R> a0 = tile[i0], a1 = tile[i1];
+            const cx<R> x0 = cmul_rr(m0r, m0i, a0), x1 = cmul_rr(m1r, m1i, a1);
+            const cx<R> y0 = cmul_rr(m2r, m2i, a0), y1 = cmul_rr(m3r, m3i, a1);
+            tile[i0] = cx<R>{x0.re + x1.re, x0.im + x1.im};
+            tile[i1] = cx<R>{y0.re + y1.re, y0.im + y1.im};
+        }
+    } else if (kind == TOP_DIAG) {
+        const int q0 = op.q0, q1 = op.q1;
+        for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+            const uint64_t gi = g.base + hoff[j >> g.low] + (j & g.lowmask);
+            const int k = int((gi >> q0) & 1) | (q1 >= 0 ? int((gi >> q1) & 1) << 1 : 0);
+            tile[j] = cmul_d(op.m, k, tile[j]);
+        }
+    } else if (kind == TOP_ROT || kind == TOP_ROTZZ) {  // Rz (diagonal) or ZZ rotation
+        R c, s;
+        rot_cs<R>(op, cs_row, c, s);
+        const int q0 = op.q0, q1 = kind == TOP_ROTZZ ? op.q1 : -1;
+        for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+            const uint64_t gi = g.base + hoff[j >> g.low] + (j & g.lowmask);
+            const uint32_t odd = uint32_t((gi >> q0) & 1) ^ (q1 >= 0 ? uint32_t((gi >> q1) & 1) : 0u);
+            tile[j] = cmul_rr(c, odd ? s : -s, tile[j]);  // e^{-i theta/2} on +1, e^{+i theta/2} on -1
+        }
+    } else if (kind == TOP_CNOT || kind == TOP_SWAP) {
+        const uint32_t a = uint32_t(op.q0), c = uint32_t(op.q1);
+        const uint32_t lo = a < c ? a : c, hi = a < c ? c : a;
+        for (uint32_t p = threadIdx.x; p < T / 4; p += blockDim.x) {
+            uint32_t x = p;
+            uint32_t l = x & ((1u << lo) - 1);
+            x = ((x ^ l) << 1) | l;
+            l = x & ((1u << hi) - 1);
+            x = ((x ^ l) << 1) | l;
+            const uint32_t ia = x | (1u << a), ic = x | (1u << c);
+            if (kind == TOP_CNOT) {
+                const cx<R> u = tile[ia], v = tile[ia | (1u << c)];
+                tile[ia] = v;
+                tile[ia | (1u << c)] = u;
+            } else {
+                const cx<R> u = tile[ia], v = tile[ic];
+                tile[ia] = v;
+                tile[ic] = u;
+            }
+        }
+    } else {  // TOP_MAT2
+        const uint32_t a = uint32_t(op.q0), c = uint32_t(op.q1);
+        const uint32_t lo = a < c ? a : c, hi = a < c ? c : a;
+        for (uint32_t p = threadIdx.x; p < T / 4; p += blockDim.x) {
+            uint32_t x = p;
+            uint32_t l = x & ((1u << lo) - 1);
+            x = ((x ^ l) << 1) | l;
+            l = x & ((1u << hi) - 1);
+            x = ((x ^ l) << 1) | l;
+            const uint32_t idx[4] = {x, x | (1u << a), x | (1u << c), x | (1u << a) | (1u << c)};
+            cx<R> v[4];
+            for (int r = 0; r < 4; ++r) v[r] = tile[idx[r]];
+            for (int r = 0; r < 4; ++r) {
+                cx<R> acc{R(0), R(0)};
+                for (int k = 0; k < 4; ++k) {
+                    const cx<R> t = cmul_d(op.m, r * 4 + k, v[k]);
+                    acc.re += t.re;
+                    acc.im += t.im;
+                }
+                tile[idx[r]] = acc;
+            }
+        }
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_tile(cx<R>* __restrict__ s, TileArgs args,
+                                                   const TileOp* __restrict__ ops, const double* __restrict__ cs) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const uint32_t T = 1u << args.M;
+    cx<R>* tile = reinterpret_cast<cx<R>*>(smem_raw);
+    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem_raw + sizeof(cx<R>) * T);
+    const TileGeo g = tile_geometry(args, hoff);
+    const double* cs_row = cs ? cs + 2 * g.inst * args.nslots : nullptr;
+    __syncthreads();
+    cx<R>* __restrict__ gp = s + g.base;
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) tile[j] = gp[hoff[j >> g.low] + (j & g.lowmask)];
+    __syncthreads();
+    for (uint32_t o = 0; o < args.nops; ++o) {
+        tile_apply<R>(tile, ops[o], g, hoff, cs_row);
         __syncthreads();
     }
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) gp[hoff[j >> g.low] + (j & g.lowmask)] = tile[j];
+}
 
-    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) g[hoff[j >> low] + (j & lowmask)] = tile[j];
+// Sum of one double over the workgroup; every thread gets the result.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < int(blockDim.x / 64); ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// Im <lambda| G |psi> over one tile for the generator G of a rotation op
+// (X / Y on a tile bit, Z on a global qubit, Z_u Z_v for TOP_ROTZZ).  FP64.
+template <class R>
+__device__ __forceinline__ double tile_overlap(const cx<R>* __restrict__ psi, const cx<R>* __restrict__ lam,
+                                               const TileOp& op, const TileGeo& g, const uint64_t* __restrict__ hoff) {
+    double acc = 0.0;
+    const uint32_t T = g.T;
+    if (op.kind == TOP_ROT && op.q1 != 2) {
+        const uint32_t m = 1u << op.q0;
+        for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+            const cx<R> l = lam[j], p = psi[j ^ m];
+            const double lr = l.re, li = l.im, pr = p.re, pi = p.im;
+            const double re = lr * pr + li * pi;      // Re(conj(l) p)
+            const double im = lr * pi - li * pr;      // Im(conj(l) p)
+            if (op.q1 == 0)
+                acc += im;                            // X: (X psi)_j = psi_{j^m}
+            else
+                acc += (j & m) ? re : -re;            // Y: (Y psi)_j = -i psi_{j|m} (bit 0), +i psi_{j&~m} (bit 1)
+        }
+    } else {
+        const int q0 = op.q0, q1 = op.kind == TOP_ROTZZ ? op.q1 : -1;
+        for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+            const uint64_t gi = g.base + hoff[j >> g.low] + (j & g.lowmask);
+            const uint32_t odd = uint32_t((gi >> q0) & 1) ^ (q1 >= 0 ? uint32_t((gi >> q1) & 1) : 0u);
+            const cx<R> l = lam[j], p = psi[j];
+            const double im = double(l.re) * double(p.im) - double(l.im) * double(p.re);
+            acc += odd ? -im : im;
+        }
+    }
+    return acc;
+}
+
+// Reverse sweep over (psi, lambda): for each op (already inverted and in
+// reverse order), record Im <lambda|G|psi> for rotations flagged TF_RECORD
+// into partial[(inst * 2^(n-M) + block) * nslots + slot], then apply the op
+// to both states (psi is uncomputed, lambda is carried back).
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_tile_adjoint(cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                           TileArgs args, const TileOp* __restrict__ ops,
+                                                           const double* __restrict__ cs,
+                                                           double* __restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[kThreads / 64];
+    const uint32_t T = 1u << args.M;
+    cx<R>* tp = reinterpret_cast<cx<R>*>(smem_raw);
+    cx<R>* tl = tp + T;
+    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem_raw + 2 * sizeof(cx<R>) * T);
+    const TileGeo g = tile_geometry(args, hoff);
+    const double* cs_row = cs + 2 * g.inst * args.nslots;
+    double* prow = partial + (g.inst * (uint64_t(1) << args.log_blocks) + g.bl) * args.nslots;
+    __syncthreads();
+    cx<R>* __restrict__ gp = psi + g.base;
+    cx<R>* __restrict__ gl = lam + g.base;
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+        const uint64_t off = hoff[j >> g.low] + (j & g.lowmask);
+        tp[j] = gp[off];
+        tl[j] = gl[off];
+    }
+    __syncthreads();
+    for (uint32_t o = 0; o < args.nops; ++o) {
+        const TileOp& op = ops[o];
+        if ((op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD)) {
+            const double v = block_sum(tile_overlap<R>(tp, tl, op, g, hoff), red);
+            if (threadIdx.x == 0) prow[op.slot] = v;
+        }
+        tile_apply<R>(tp, op, g, hoff, cs_row);
+        tile_apply<R>(tl, op, g, hoff, cs_row);
+        __syncthreads();
+    }
+    for (uint32_t j = threadIdx.x; j < T; j += blockDim.x) {
+        const uint64_t off = hoff[j >> g.low] + (j & g.lowmask);
+        gp[off] = tp[j];
+        gl[off] = tl[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -362,6 +506,8 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const cx<R>* __restrict__ s
 }
 
 // One workgroup per instance folds its `nblocks` partials; out[inst] = (re, im).
+// (A template so the header can be included by several translation units.)
+template <int Unused = 0>
 __global__ __launch_bounds__(kThreads) void k_reduce_finish(const dd* __restrict__ partial, uint32_t nblocks,
                                                             double* __restrict__ out) {
     const uint64_t inst = blockIdx.x;

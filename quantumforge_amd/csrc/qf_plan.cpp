@@ -36,16 +36,19 @@ void mat2_mul(const cd x[4], const cd y[4], cd r[4]) {
 
 bool is_zero(cd a) { return a.re == 0.0 && a.im == 0.0; }
 
-enum NodeKind { N_MAT1, N_DIAG1, N_DIAG2, N_CNOT, N_SWAP };
+enum NodeKind { N_MAT1, N_DIAG1, N_DIAG2, N_CNOT, N_SWAP, N_ROT, N_ROTZZ };
 
 struct Node {
     NodeKind kind;
     uint32_t q[2];
     int nq;
     cd m[4];  // MAT1 matrix, DIAG1 (d0, d1), DIAG2 d[bit(q1)*2 + bit(q0)]
+    int axis = 0, param = 0, slot = 0, flags = 0;  // N_ROT / N_ROTZZ
     bool dead = false;
     int prev[2] = {-1, -1};  // previous live node on q[0], q[1]
-    bool diag() const { return kind == N_DIAG1 || kind == N_DIAG2; }
+    bool diag() const {
+        return kind == N_DIAG1 || kind == N_DIAG2 || kind == N_ROTZZ || (kind == N_ROT && axis == 2);
+    }
 };
 
 int slot_of(const Node& n, uint32_t q) { return n.q[0] == q ? 0 : 1; }
@@ -83,21 +86,42 @@ std::vector<Node> lower(const std::vector<GateIn>& gates, uint32_t qubits) {
                 continue;
             }
             push(n);
+        } else if (g.kind == 5) {  // rotation with a per-instance angle
+            Node n{};
+            n.kind = N_ROT;
+            n.nq = 1;
+            n.q[0] = g.a;
+            n.axis = g.axis;
+            n.param = g.param;
+            n.slot = g.slot;
+            n.flags = g.flags;
+            push(n);
         } else if (g.kind == 1) {  // CNOT(a control, b target)
             const uint32_t c = g.a, t = g.b;
             const int Lt = last[t];
-            if (Lt >= 0 && nodes[Lt].kind == N_DIAG1) {
+            const bool mid_diag = Lt >= 0 && (nodes[Lt].kind == N_DIAG1 || (nodes[Lt].kind == N_ROT && nodes[Lt].axis == 2));
+            if (mid_diag) {
                 const int K = nodes[Lt].prev[0];
                 if (K >= 0 && nodes[K].kind == N_CNOT && nodes[K].q[0] == c && nodes[K].q[1] == t && last[c] == K) {
-                    // CNOT(c,t) diag(d0,d1)_t CNOT(c,t) = diag over (c,t): c=0 -> d(t), c=1 -> d(!t).
-                    const cd d0 = nodes[Lt].m[0], d1 = nodes[Lt].m[3];
                     Node& D = nodes[K];
-                    D.kind = N_DIAG2;
-                    // slots: q[0]=c, q[1]=t ; index = bit(t)*2 + bit(c)
-                    D.m[0] = d0;  // c0 t0
-                    D.m[1] = d1;  // c1 t0 -> X d X on t: d1
-                    D.m[2] = d1;  // c0 t1
-                    D.m[3] = d0;  // c1 t1
+                    if (nodes[Lt].kind == N_DIAG1) {
+                        // CNOT(c,t) diag(d0,d1)_t CNOT(c,t) = diag over (c,t): c=0 -> d(t), c=1 -> d(!t).
+                        const cd d0 = nodes[Lt].m[0], d1 = nodes[Lt].m[3];
+                        D.kind = N_DIAG2;
+                        // slots: q[0]=c, q[1]=t ; index = bit(t)*2 + bit(c)
+                        D.m[0] = d0;  // c0 t0
+                        D.m[1] = d1;  // c1 t0 -> X d X on t: d1
+                        D.m[2] = d1;  // c0 t1
+                        D.m[3] = d0;  // c1 t1
+                    } else {
+                        // CNOT(c,t) Rz(theta)_t CNOT(c,t) = exp(-i theta Z_c Z_t / 2); its
+                        // generator at this point is Z_c Z_t (the QAOA cost term).
+                        D.kind = N_ROTZZ;
+                        D.axis = 2;
+                        D.param = nodes[Lt].param;
+                        D.slot = nodes[Lt].slot;
+                        D.flags = nodes[Lt].flags;
+                    }
                     nodes[Lt].dead = true;
                     last[t] = K;
                     continue;
@@ -235,6 +259,22 @@ TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
             op.q0 = tile_bit[n.q[0]];
             op.q1 = tile_bit[n.q[1]];
             break;
+        case N_ROT:
+            op.kind = TOP_ROT;
+            op.q0 = n.axis == 2 ? int(n.q[0]) : tile_bit[n.q[0]];
+            op.q1 = n.axis;
+            op.flags = n.flags;
+            op.param = n.param;
+            op.slot = n.slot;
+            break;
+        case N_ROTZZ:
+            op.kind = TOP_ROTZZ;
+            op.q0 = int(n.q[0]);
+            op.q1 = int(n.q[1]);
+            op.flags = n.flags;
+            op.param = n.param;
+            op.slot = n.slot;
+            break;
     }
     return op;
 }
@@ -300,10 +340,13 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     out.stats.ops_in = gates.size();
     if (qubits == 0) return "a state vector must contain at least one qubit";
     for (const GateIn& g : gates) {
-        if (g.a >= qubits || (g.kind != 0 && g.b >= qubits)) return "qubit out of range";
-        if (g.kind != 0 && g.a == g.b) return "two-qubit operation requires distinct qubits";
+        const bool two = g.kind != 0 && g.kind != 5;
+        if (g.a >= qubits || (two && g.b >= qubits)) return "qubit out of range";
+        if (two && g.a == g.b) return "two-qubit operation requires distinct qubits";
     }
     if (!opt.fuse) {
+        for (const GateIn& g : gates)
+            if (g.kind == 5) return "per-instance rotations need the tile lowering";
         for (const GateIn& g : gates) {
             Pass p{};
             if (g.kind == 0) {

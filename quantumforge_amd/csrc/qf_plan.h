@@ -29,13 +29,26 @@ enum TileOpKind : int32_t {
     TOP_CNOT = 2,   // permutation: control tile bit q0, target tile bit q1
     TOP_SWAP = 3,   // permutation on tile bits q0, q1
     TOP_MAT2 = 4,   // dense 4x4 on tile bits (q0 = low bit of the 2-bit index, q1 = high bit)
+    TOP_ROT = 5,    // exp(-i theta P / 2), P = X/Y on tile bit q0, or Z on global qubit q0; axis in q1
+    TOP_ROTZZ = 6,  // exp(-i theta Z_q0 Z_q1 / 2) on global qubits q0, q1
+};
+
+// TileOp flags (rotations).
+enum : int32_t {
+    TF_NEGATE = 1,   // apply the inverse rotation (angle -theta)
+    TF_RECORD = 2,   // before applying, record Im <lambda| G |psi> for this occurrence (reverse sweep)
 };
 
 // One op as the tile kernel reads it (uniform across the workgroup).
+// Rotation angles are per instance: the kernel reads (cos(theta/2), sin(theta/2))
+// of parameter `param` from the instance's row of the angle table.
 struct TileOp {
     int32_t kind;
-    int32_t q0, q1;   // tile bit positions (MAT1/CNOT/SWAP/MAT2) or global qubits (DIAG; q1 < 0 if 1-qubit)
-    int32_t pad;
+    int32_t q0, q1;   // tile bit positions (MAT1/CNOT/SWAP/MAT2, ROT X/Y) or global qubits (DIAG, ROT Z, ROTZZ)
+    int32_t flags;
+    int32_t param;    // rotation: column of the per-instance angle table
+    int32_t slot;     // rotation: occurrence index (gradient partial column)
+    int32_t pad[2];
     double m[32];     // MAT1: 4 complex; DIAG: 4 complex (index g1bit*2+g0bit); MAT2: 16 complex row major
 };
 
@@ -66,9 +79,13 @@ struct PlanStats {
 
 // One circuit gate in planner form.
 struct GateIn {
-    int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.)
+    int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.), 5 rotation
     uint32_t a, b;
     cd m[4];           // 1q: matrix; diag2: d[bits(b,a)]
+    int32_t axis = 0;  // rotation: 0 X, 1 Y, 2 Z
+    int32_t param = 0; // rotation: angle-table column
+    int32_t slot = 0;  // rotation: occurrence index
+    int32_t flags = 0; // rotation: TF_* flags
 };
 
 struct Plan {

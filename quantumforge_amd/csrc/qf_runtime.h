@@ -9,7 +9,23 @@
 #include "qf_kernels.h"
 #include "qf_plan.h"
 
+#define QF_HIP_TRY(expr)                                            \
+    do {                                                            \
+        hipError_t e_ = (expr);                                     \
+        if (e_ != hipSuccess) return ::qfrt::hip_fail(e_, #expr);   \
+    } while (0)
+
 namespace qfrt {
+
+// Extra inputs of a plan launch: the per-instance angle table (cos, sin of
+// half angles, [count][nslots]) and, for the reverse sweep, the costate and
+// the gradient partial buffer.
+struct LaunchCtx {
+    const double* cs = nullptr;
+    uint32_t nslots = 0;
+    void* lam = nullptr;
+    double* partial = nullptr;
+};
 
 qf_status fail(qf_status st, const std::string& msg);
 qf_status hip_fail(hipError_t e, const char* what);
@@ -25,10 +41,15 @@ qf_status launch_check(const char* what);
 size_t amp_bytes(int precision);
 unsigned grid_for(uint64_t work);
 qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan);
+qf_status upload_plans(qf_backend* be, const std::vector<const qf::Plan*>& plans,
+                       std::vector<const qf::TileOp*>& bases);
+qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan,
+                      const qf::TileOp* d_ops, const LaunchCtx& ctx);
 qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count,
                       const std::vector<qf::GateIn>& gates, const qf::PlanOptions& opt);
 qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, uint64_t dim, uint32_t count, int kind,
                  uint32_t q, const std::vector<qf::PauliDev>& terms, std::vector<double>& out);
+qf_backend* state_backend(const qf_state* st);
 qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits, std::vector<qf::PauliDev>& out);
 
 }  // namespace qfrt

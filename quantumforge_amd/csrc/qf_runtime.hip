@@ -158,6 +158,8 @@ struct qf_plan {
 
 namespace qfrt {
 
+qf_backend* state_backend(const qf_state* st) { return st->be; }
+
 void backend_release(qf_backend* be) {
     if (!be) return;
     if (be->refs.fetch_sub(1) == 1) {
@@ -287,7 +289,7 @@ qf_status check_pair(uint32_t qubits, uint32_t a, uint32_t b) {
 
 template <class R>
 qf_status launch_pass(qf_backend* be, cx<R>* d, uint32_t qubits, uint32_t count, const qf::Pass& p,
-                      const qf::TileOp* d_ops) {
+                      const qf::TileOp* d_ops, const LaunchCtx& ctx) {
     const uint64_t dim = uint64_t(1) << qubits;
     const uint64_t total = dim * count;
     hipStream_t st = be->stream;
@@ -311,53 +313,72 @@ qf_status launch_pass(qf_backend* be, cx<R>* d, uint32_t qubits, uint32_t count,
             a.low = p.low;
             a.log_blocks = qubits - p.tile_bits;
             a.nops = uint32_t(p.ops.size());
+            a.nslots = ctx.nslots;
             for (size_t k = 0; k < p.high.size(); ++k) a.high[k] = p.high[k];
             const uint32_t nh = p.tile_bits - p.low;
-            const size_t lds = (size_t(1) << p.tile_bits) * sizeof(cx<R>) + (size_t(1) << nh) * sizeof(uint64_t);
-            if (lds > 64 * 1024) {
-                HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&qf::k_tile<R>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            const int tiles = ctx.lam ? 2 : 1;
+            const size_t lds =
+                tiles * (size_t(1) << p.tile_bits) * sizeof(cx<R>) + (size_t(1) << nh) * sizeof(uint64_t);
+            const void* fn = ctx.lam ? reinterpret_cast<const void*>(&qf::k_tile_adjoint<R>)
+                                     : reinterpret_cast<const void*>(&qf::k_tile<R>);
+            if (lds > 64 * 1024)
+                HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            const uint32_t gx_log = a.log_blocks < 16 ? a.log_blocks : 16;
+            const dim3 grid(1u << gx_log, count << (a.log_blocks - gx_log));
+            if (ctx.lam) {
+                hipLaunchKernelGGL(qf::k_tile_adjoint<R>, grid, dim3(qf::kThreads), lds, st, d,
+                                   static_cast<cx<R>*>(ctx.lam), a, d_ops, ctx.cs, ctx.partial);
+                return launch_check("k_tile_adjoint");
             }
-            uint32_t gx_log = a.log_blocks < 16 ? a.log_blocks : 16;
-            dim3 grid(1u << gx_log, count << (a.log_blocks - gx_log));
-            hipLaunchKernelGGL(qf::k_tile<R>, grid, dim3(qf::kThreads), lds, st, d, a, d_ops);
+            hipLaunchKernelGGL(qf::k_tile<R>, grid, dim3(qf::kThreads), lds, st, d, a, d_ops, ctx.cs);
             return launch_check("k_tile");
         }
     }
     return fail(QF_INVALID_ARGUMENT, "unknown pass kind");
 }
 
-// Runs a plan on `count` contiguous states of `qubits` qubits.
-qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
+// Copies the tile ops of several plans into the backend's op buffer (one
+// upload); bases[i] is the device address of plan i's first op.
+qf_status upload_plans(qf_backend* be, const std::vector<const qf::Plan*>& plans,
+                       std::vector<const qf::TileOp*>& bases) {
+    std::vector<qf::TileOp> host;
+    std::vector<size_t> start;
+    for (const qf::Plan* pl : plans) {
+        start.push_back(host.size());
+        for (const auto& p : pl->passes) host.insert(host.end(), p.ops.begin(), p.ops.end());
+    }
+    bases.assign(plans.size(), nullptr);
+    if (host.empty()) return QF_OK;
+    // The buffer may still be read by earlier launches: wait before reuse.
+    HIP_TRY(hipStreamSynchronize(be->stream));
+    qf_status s = ensure(be, &be->d_ops, &be->d_ops_bytes, host.size() * sizeof(qf::TileOp));
+    if (s) return s;
+    HIP_TRY(hipMemcpy(be->d_ops, host.data(), host.size() * sizeof(qf::TileOp), hipMemcpyHostToDevice));
+    for (size_t i = 0; i < plans.size(); ++i) bases[i] = static_cast<const qf::TileOp*>(be->d_ops) + start[i];
+    return QF_OK;
+}
+
+// Launches every pass of a plan whose ops were uploaded at d_ops.
+qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan,
+                      const qf::TileOp* d_ops, const LaunchCtx& ctx) {
     if (plan.qubits != qubits) return fail(QF_INCOMPATIBLE_DEVICE_STATE, "plan was built for a different qubit count");
-    // Gather all tile ops into one device buffer.
-    std::vector<size_t> offset(plan.passes.size(), 0);
-    size_t nops = 0;
-    for (size_t i = 0; i < plan.passes.size(); ++i) {
-        offset[i] = nops;
-        nops += plan.passes[i].ops.size();
-    }
-    qf::TileOp* d_ops = nullptr;
-    if (nops) {
-        std::vector<qf::TileOp> host;
-        host.reserve(nops);
-        for (const auto& p : plan.passes) host.insert(host.end(), p.ops.begin(), p.ops.end());
-        // The buffer may still be read by earlier launches: wait before reuse.
-        HIP_TRY(hipStreamSynchronize(be->stream));
-        qf_status s = ensure(be, &be->d_ops, &be->d_ops_bytes, nops * sizeof(qf::TileOp));
-        if (s) return s;
-        HIP_TRY(hipMemcpy(be->d_ops, host.data(), nops * sizeof(qf::TileOp), hipMemcpyHostToDevice));
-        d_ops = static_cast<qf::TileOp*>(be->d_ops);
-    }
-    for (size_t i = 0; i < plan.passes.size(); ++i) {
+    const qf::TileOp* cur = d_ops;
+    for (const qf::Pass& p : plan.passes) {
         qf_status s = precision == QF_C128
-                          ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, plan.passes[i],
-                                                d_ops ? d_ops + offset[i] : nullptr)
-                          : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, plan.passes[i],
-                                               d_ops ? d_ops + offset[i] : nullptr);
+                          ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, p, cur, ctx)
+                          : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, p, cur, ctx);
         if (s) return s;
+        if (cur) cur += p.ops.size();
     }
     return QF_OK;
+}
+
+// Runs a plan on `count` contiguous states of `qubits` qubits.
+qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
+    std::vector<const qf::TileOp*> bases;
+    qf_status s = upload_plans(be, {&plan}, bases);
+    if (s) return s;
+    return launch_plan(be, d, precision, qubits, count, plan, bases[0], LaunchCtx{});
 }
 
 qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count,
@@ -402,7 +423,7 @@ qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, u
                            uint32_t(terms.size()), partial);
     s = launch_check("k_reduce");
     if (s) return s;
-    hipLaunchKernelGGL(qf::k_reduce_finish, dim3(count), dim3(qf::kThreads), 0, be->stream, partial,
+    hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(count), dim3(qf::kThreads), 0, be->stream, partial,
                        uint32_t(blocks), be->d_out);
     s = launch_check("k_reduce_finish");
     if (s) return s;

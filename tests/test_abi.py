@@ -1,0 +1,188 @@
+"""CPU checks of the C ABI and the host planner (no device calls):
+every entry point declared in include/qf_hip.h is exported, the record
+layouts match the C structs, and fused plans replayed in NumPy reproduce the
+oracle's result (so the schedule, the peephole merges and the tile/bit
+mapping are right before any kernel runs)."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quantumforge_amd import Circuit, Gate, QuantumError
+from quantumforge_amd import _native as N
+from quantumforge_amd.backend import Plan
+from quantumforge_amd.workloads import qft_circuit, qft_like_circuit, qaoa_maxcut, random_layer_circuit, MaxCutProblem
+
+from .conftest import ROOT, close
+
+HEADER = os.path.join(ROOT, "include", "qf_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(qf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (qf_[a-z0-9_]+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    lib = N.lib()
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+def test_record_layouts():
+    assert N.OP_DTYPE.itemsize == 24
+    assert N.POP_DTYPE.itemsize == 32
+    assert N.PAULI_DTYPE.itemsize == 32
+    assert N.ROT_DTYPE.itemsize == 16
+    src = open(HEADER).read()
+    assert "int32_t kind;\n    int32_t gate;\n    uint32_t a, b;\n    double theta;\n} qf_op;" in src
+
+
+def test_status_names_match_reference_variants():
+    lib = N.lib()
+    for code, name in N.STATUS.items():
+        assert lib.qf_status_name(code).decode() == name
+
+
+def test_plan_errors_are_typed():
+    c = Circuit()
+    c.push(Gate.X, 4)
+    with pytest.raises(QuantumError) as e:
+        Plan(c, 3)
+    assert e.value.kind == "QubitOutOfRange"
+    c = Circuit()
+    c.cnot(1, 1)
+    with pytest.raises(QuantumError) as e:
+        Plan(c, 3)
+    assert e.value.kind == "InvalidQubitPair"
+    c = Circuit()
+    c.push(Gate.Rx(float("inf")), 0)
+    with pytest.raises(QuantumError) as e:
+        Plan(c, 3)
+    assert e.value.kind == "NonFiniteAngle"
+    with pytest.raises(QuantumError) as e:
+        Plan(Circuit(), 0)
+    assert e.value.kind == "ZeroQubits"
+
+
+# ---------------------------------------------------------------------------
+# NumPy replay of a plan (test-side executor)
+# ---------------------------------------------------------------------------
+def _c(m, k):
+    return complex(m[2 * k], m[2 * k + 1])
+
+
+def replay(desc, psi):
+    n = desc["qubits"]
+    dim = 1 << n
+    idx = np.arange(dim, dtype=np.int64)
+    psi = psi.astype(complex).copy()
+    for p in desc["passes"]:
+        if p["kind"] != "tile":
+            raise AssertionError("fused plans contain tile passes only")
+        low, high = p["low"], p["high"]
+        bits = list(range(low)) + list(high)  # tile bit -> global qubit
+        for op in p["ops"]:
+            k, m = op["kind"], op["m"]
+            if k == 0:  # MAT1 on tile bit
+                q = bits[op["q0"]]
+                mask = 1 << q
+                i0 = idx[(idx & mask) == 0]
+                a0, a1 = psi[i0].copy(), psi[i0 | mask].copy()
+                psi[i0] = _c(m, 0) * a0 + _c(m, 1) * a1
+                psi[i0 | mask] = _c(m, 2) * a0 + _c(m, 3) * a1
+            elif k == 1:  # DIAG on global qubits
+                q0, q1 = op["q0"], op["q1"]
+                kk = (idx >> q0) & 1
+                if q1 >= 0:
+                    kk = kk | (((idx >> q1) & 1) << 1)
+                d = np.array([_c(m, j) for j in range(4)])
+                psi = psi * d[kk]
+            elif k in (2, 3):
+                a, b = bits[op["q0"]], bits[op["q1"]]
+                if k == 2:
+                    sel = idx[((idx >> a) & 1) == 1]
+                    sel = sel[((sel >> b) & 1) == 0]
+                    tmp = psi[sel].copy()
+                    psi[sel] = psi[sel | (1 << b)]
+                    psi[sel | (1 << b)] = tmp
+                else:
+                    sel = idx[(((idx >> a) & 1) == 1) & (((idx >> b) & 1) == 0)]
+                    other = sel ^ (1 << a) ^ (1 << b)
+                    tmp = psi[sel].copy()
+                    psi[sel] = psi[other]
+                    psi[other] = tmp
+            else:
+                raise AssertionError(f"unexpected op kind {k}")
+    return psi
+
+
+def check_tile_invariants(desc):
+    n = desc["qubits"]
+    for p in desc["passes"]:
+        M, low, high = p["tile_bits"], p["low"], p["high"]
+        assert len(high) == M - low and high == sorted(high) and all(low <= h < n for h in high)
+        for op in p["ops"]:
+            if op["kind"] in (0, 2, 3):
+                assert 0 <= op["q0"] < M and (op["kind"] == 0 or 0 <= op["q1"] < M)
+
+
+@pytest.mark.parametrize("n,depth,tile,low", [(8, 10, 4, 2), (12, 12, 6, 3), (14, 20, 8, 4), (16, 10, 11, 5),
+                                              (10, 6, 10, 5), (5, 4, 11, 5)])
+def test_fused_plan_replay_equals_oracle(n, depth, tile, low):
+    c = random_layer_circuit(n, depth, seed=n * 31 + depth, pairs_per_layer=n // 2)
+    c.swap(0, n - 1)
+    c.push(Gate.T, 2)
+    c.push(Gate.Sdg, 2)
+    plan = Plan(c, n, True, tile, low)
+    desc = plan.describe()
+    check_tile_invariants(desc)
+    st = plan.stats()
+    assert st["ops_in"] == len(c) and st["passes"] == len(desc["passes"])
+    psi0 = np.zeros(1 << n, complex)
+    psi0[0] = 1
+    close(replay(desc, psi0), O.run_circuit(n, c.to_ops()), 1e-12)
+
+
+def test_qaoa_and_qft_peepholes_replay():
+    prob = MaxCutProblem.random_3_regular(10, 2).with_random_weights(3, 0.5, 1.0)
+    c = qaoa_maxcut(10, prob.edges, 3, [0.2, 0.5, 0.9], [0.4, 0.1, 0.7])
+    plan = Plan(c, 10, True, 6, 3)
+    desc = plan.describe()
+    # every CNOT Rz CNOT cost term became one two-qubit diagonal: no CNOT remains
+    assert all(op["kind"] != 2 for p in desc["passes"] for op in p["ops"])
+    psi0 = np.zeros(1 << 10, complex)
+    psi0[0] = 1
+    close(replay(desc, psi0), O.run_circuit(10, c.to_ops()), 1e-12)
+    c2 = qft_circuit(9)
+    d2 = Plan(c2, 9, True, 5, 2).describe()
+    basis5 = np.eye(1 << 9)[5]
+    close(replay(d2, basis5), _oracle_from(9, c2, basis5), 1e-12)
+    c3, _ = qft_like_circuit(10, 1, seed=1, max_distance=4)
+    d3 = Plan(c3, 10, True, 7, 3).describe()
+    close(replay(d3, psi0), O.run_circuit(10, c3.to_ops()), 1e-12)
+
+
+def _oracle_from(n, c, psi):
+    s = O.State(n, psi)
+    s.run(c.to_ops())
+    return s.amplitudes()
+
+
+def test_30q_plan_is_fused():
+    c = random_layer_circuit(30, 40, seed=2024)
+    st = Plan(c, 30, True).stats()
+    assert st["ops_in"] == 1600
+    assert st["passes"] <= 60, st
+    exact = Plan(c, 30, False).stats()
+    assert exact["passes"] == 1600

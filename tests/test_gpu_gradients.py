@@ -1,0 +1,169 @@
+"""GPU gradients against the oracle: the reference's gradient tests
+(tests/gradients.rs, tests/quantum_core.rs:79-123) run through the HIP engine,
+the batched adjoint against the oracle's shift rule and finite differences,
+and the error variants."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quantumforge_amd import ParameterizedCircuit, PauliSum, QuantumError, RotationAxis, SingleQubitObservable
+from quantumforge_amd import gradients as G
+from quantumforge_amd.workloads import (MaxCutProblem, hea_circuit, heisenberg_hamiltonian, maxcut_cost_hamiltonian,
+                                        qaoa_occurrence_angles, qaoa_parameterized, uniform_parameters)
+
+from .test_oracle_gradients import seven_qubit_circuit
+
+pytestmark = pytest.mark.gpu
+EXACT = (G.GradientStrategy.ParameterShift, G.GradientStrategy.Adjoint, G.GradientStrategy.Backprop)
+
+
+def test_ry_gradient_matches_analytic(backend):
+    pc = ParameterizedCircuit(1)
+    pc.rotation(RotationAxis.Y, 0, 0)
+    for st in EXACT:
+        est = G.gradient(backend, pc, G.GradientOptions(st), [0.7], 1, (SingleQubitObservable.Z, 0))
+        assert abs(est.gradient[0] + math.sin(0.7)) < 1e-10, st
+        assert est.strategy == st
+        assert abs(est.value - math.cos(0.7)) < 1e-14
+
+
+def test_all_exact_strategies_agree_7q(backend):
+    pc, params = seven_qubit_circuit()
+    t = PauliSum.single(SingleQubitObservable.Z, 0, 7).to_terms()
+    want = O.pc_shift_gradient(pc.to_pops(), 42, 0, params, [], 7, t)
+    for st in EXACT:
+        got = G.gradient(backend, pc, G.GradientOptions(st), params, 7, (SingleQubitObservable.Z, 0)).gradient
+        assert np.max(np.abs(got - want)) < 1e-9, st
+    # evaluation counts, tests/gradients.rs:324-361
+    est = G.gradient(backend, pc, G.GradientOptions(G.GradientStrategy.ParameterShift), params, 7,
+                     (SingleQubitObservable.Z, 0))
+    assert est.evaluations == 2 * 42
+    est = G.gradient(backend, pc, G.GradientOptions(), params, 7, (SingleQubitObservable.Z, 0))
+    assert est.strategy == G.GradientStrategy.Adjoint and est.evaluations == 2
+    est = G.gradient(backend, pc, G.GradientOptions(shots=100), params, 7, (SingleQubitObservable.Z, 0))
+    assert est.strategy == G.GradientStrategy.ParameterShift
+
+
+def test_adjoint_equals_shift_with_inputs_and_shared_parameters(backend):
+    pc = ParameterizedCircuit.with_inputs(2, 1)
+    pc.encode_rotation(RotationAxis.Y, 0, 0)
+    pc.rotation(RotationAxis.X, 1, 0)
+    pc.cnot(0, 1)
+    pc.rotation(RotationAxis.Z, 0, 1)
+    pc.rotation(RotationAxis.Y, 1, 0)
+    for obs in (0, 1, 2):
+        t = PauliSum.single(SingleQubitObservable(obs), 1, 2).to_terms()
+        want = O.pc_adjoint_gradient(pc.to_pops(), 2, 1, [0.31, -0.82], [0.43], 2, t)
+        for st in EXACT:
+            got = G.gradient(backend, pc, G.GradientOptions(st), [0.31, -0.82], 2,
+                             (SingleQubitObservable(obs), 1), inputs=[0.43]).gradient
+            assert np.max(np.abs(got - want)) < 1e-12, (obs, st)
+    pc.channel(0)
+    with pytest.raises(QuantumError) as e:
+        G.gradient(backend, pc, G.GradientOptions(), [0.31, -0.82], 2, (SingleQubitObservable.Z, 0), inputs=[0.43])
+    assert e.value.kind == "OperationRequiresDensityMatrix"
+
+
+def test_finite_difference_second_order(backend):
+    pc = ParameterizedCircuit(2)
+    pc.rotation(RotationAxis.Y, 0, 0)
+    pc.rotation(RotationAxis.X, 1, 1)
+    pc.cnot(0, 1)
+    pc.rotation(RotationAxis.Y, 2, 0)
+    exact = G.parameter_shift_gradient(backend, pc, [0.5, -0.4], 3, (SingleQubitObservable.Z, 0))
+    prev = math.inf
+    for h in (1e-2, 1e-3, 1e-4):
+        fd = G.finite_difference_gradient_with_inputs(backend, pc, [], [0.5, -0.4], 3, (SingleQubitObservable.Z, 0), h)
+        err = np.max(np.abs(fd - exact))
+        assert err * 50 < prev
+        prev = err
+    assert prev < 1e-8
+    for h in (0.0, -1.0, math.nan, math.inf):
+        with pytest.raises(QuantumError) as e:
+            G.finite_difference_gradient_with_inputs(backend, pc, [], [0.5, -0.4], 3, (SingleQubitObservable.Z, 0), h)
+        assert e.value.kind == "InvalidFiniteDifferenceStep"
+
+
+def test_validation_errors(backend):
+    pc = ParameterizedCircuit.with_inputs(2, 1)
+    pc.rotation(RotationAxis.Y, 0, 1)
+    obs = (SingleQubitObservable.Z, 0)
+    cases = [(dict(parameters=[0.1, 0.2], inputs=[]), "InvalidInputCount"),
+             (dict(parameters=[0.1], inputs=[0.3]), "InvalidParameterCount"),
+             (dict(parameters=[0.1, math.inf], inputs=[0.3]), "NonFiniteParameter"),
+             (dict(parameters=[0.1, 0.2], inputs=[math.nan]), "NonFiniteParameter")]
+    for kw, kind in cases:
+        with pytest.raises(QuantumError) as e:
+            G.gradient(backend, pc, G.GradientOptions(G.GradientStrategy.Adjoint), kw["parameters"], 1, obs,
+                       inputs=kw["inputs"])
+        assert e.value.kind == kind
+    with pytest.raises(QuantumError) as e:
+        G.gradient(backend, pc, G.GradientOptions(), [0.1, 0.2], 1, (SingleQubitObservable.Z, 3), inputs=[0.3])
+    assert e.value.kind == "QubitOutOfRange"
+    bad = ParameterizedCircuit(1)
+    bad.rotation(RotationAxis.Y, 0, 5)
+    with pytest.raises(QuantumError) as e:
+        G.gradient(backend, bad, G.GradientOptions(), [0.1], 1, obs)
+    assert e.value.kind == "InvalidParameterCount"
+
+
+@pytest.mark.parametrize("n,layers,sets", [(6, 2, 8), (9, 3, 5), (12, 2, 3)])
+def test_batched_adjoint_heisenberg_vs_oracle(backend, n, layers, sets):
+    pc = hea_circuit(n, layers)
+    H = heisenberg_hamiltonian(n, seed=n)
+    params = uniform_parameters(pc.parameter_count, sets, seed=100 + n)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    t = H.to_terms()
+    for s in range(sets):
+        want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        want_g = O.pc_shift_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        assert abs(e[s] - want_e) < 1e-12
+        assert np.max(np.abs(g[s] - want_g)) < 1e-10
+    es, gs = G.batch_shift_gradient(backend, pc, n, params, H)
+    assert np.max(np.abs(es - e)) < 1e-13
+    assert np.max(np.abs(gs - g)) < 1e-10
+    assert np.max(np.abs(G.batch_expectation(backend, pc, n, params, H) - e)) < 1e-13
+
+
+def test_batched_adjoint_tile_shapes(backend):
+    n = 10
+    pc = hea_circuit(n, 2)
+    H = heisenberg_hamiltonian(n, seed=1)
+    params = uniform_parameters(pc.parameter_count, 4, seed=2)
+    e0, g0 = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    for tile in (4, 6, 8, 10):
+        for chunk in (0, 1, 3):
+            e, g = G.batch_adjoint_gradient(backend, pc, n, params, H, tile_qubits=tile, chunk=chunk)
+            assert np.max(np.abs(e - e0)) < 1e-13 and np.max(np.abs(g - g0)) < 1e-12, (tile, chunk)
+
+
+def test_complex64_adjoint_error_bound(backend):
+    n = 10
+    pc = hea_circuit(n, 3)
+    H = heisenberg_hamiltonian(n, seed=4)
+    params = uniform_parameters(pc.parameter_count, 4, seed=5)
+    e64, g64 = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    e32, g32 = G.batch_adjoint_gradient(backend, pc, n, params, H, precision="complex64")
+    assert np.max(np.abs(e32 - e64)) < 1e-4
+    assert np.max(np.abs(g32 - g64)) < 1e-4
+
+
+def test_qaoa_occurrence_gradients(backend):
+    prob = MaxCutProblem.random_3_regular(10, 3).with_random_weights(9, 0.5, 1.0)
+    p = 3
+    pc = qaoa_parameterized(10, prob.edges, p)
+    rng = np.random.default_rng(0)
+    ang = qaoa_occurrence_angles(prob.edges, 10, p, rng.uniform(0, math.pi, p), rng.uniform(0, math.pi, p))
+    H = maxcut_cost_hamiltonian(prob)
+    want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, ang, [], 10, H.to_terms())
+    e, g = G.batch_adjoint_gradient(backend, pc, 10, ang[None, :], H)
+    assert np.max(np.abs(g[0] - want)) < 1e-11
+
+
+def test_run_parameterized_matches_oracle(backend):
+    pc, params = seven_qubit_circuit()
+    st = G.run_parameterized(backend, pc, params, 7)
+    want = O.pc_run(pc.to_pops(), 42, 0, params, [], 7)
+    assert np.max(np.abs(st.download().amplitudes - want)) < 1e-12
