@@ -139,9 +139,11 @@ typedef struct {
  * the reference dispatches one kernel per gate, src/cuda.rs:1050-1083). */
 typedef struct {
     int32_t fuse;          /* 0: one sweep per gate (reference order, bit-exact arithmetic); 1: tile passes */
-    uint32_t tile_qubits;  /* qubits held per workgroup tile (0 = autotuned default) */
+    uint32_t tile_qubits;  /* qubits held per workgroup tile (0 = default) */
     uint32_t low_qubits;   /* contiguous low qubits always in a tile (0 = default) */
     uint32_t max_block;    /* widest dense fused block k (2..5; 0 = default) */
+    int32_t kernel;        /* 0 default, 1 LDS op-by-op tile kernel, 2 register-resident tile kernel */
+    uint32_t reg_bits;     /* register kernel: 2^reg_bits amplitudes per thread (0 = default) */
 } qf_plan_options;
 
 /* Summary of a plan (counts the HBM sweeps the plan streams). */
@@ -151,6 +153,7 @@ typedef struct {
     uint64_t ops_applied;  /* operations after 1q merging */
     uint64_t dense_blocks; /* fused dense k-qubit blocks */
     uint64_t max_ops_per_pass;
+    uint64_t phases;       /* register-kernel phases (LDS re-distributions + 1 per pass) */
 } qf_plan_stats;
 
 typedef struct qf_backend qf_backend;

@@ -33,12 +33,12 @@ ROT_DTYPE = np.dtype([("axis", "<i4"), ("qubit", "<u4"), ("theta", "<f8")])
 
 class PlanOptions(C.Structure):
     _fields_ = [("fuse", C.c_int32), ("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32),
-                ("max_block", C.c_uint32)]
+                ("max_block", C.c_uint32), ("kernel", C.c_int32), ("reg_bits", C.c_uint32)]
 
 
 class PlanStats(C.Structure):
     _fields_ = [("passes", C.c_uint64), ("ops_in", C.c_uint64), ("ops_applied", C.c_uint64),
-                ("dense_blocks", C.c_uint64), ("max_ops_per_pass", C.c_uint64)]
+                ("dense_blocks", C.c_uint64), ("max_ops_per_pass", C.c_uint64), ("phases", C.c_uint64)]
 
 
 class GradOptions(C.Structure):

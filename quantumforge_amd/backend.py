@@ -28,9 +28,9 @@ class Plan:
     """Fused execution plan of a circuit (qf_plan_build)."""
 
     def __init__(self, circuit: Circuit, qubits: int, fuse: bool = True, tile_qubits: int = 0,
-                 low_qubits: int = 0) -> None:
+                 low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0) -> None:
         self._ops = circuit.to_ops()
-        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0)
+        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits)
         h = C.c_void_p()
         check(lib().qf_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
         self._h = h
@@ -76,8 +76,9 @@ class HipBackend:
     def synchronize(self) -> None:
         check(lib().qf_backend_synchronize(self._h))
 
-    def set_plan_options(self, fuse: bool = True, tile_qubits: int = 0, low_qubits: int = 0) -> None:
-        o = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0)
+    def set_plan_options(self, fuse: bool = True, tile_qubits: int = 0, low_qubits: int = 0, kernel: int = 0,
+                         reg_bits: int = 0) -> None:
+        o = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits)
         check(lib().qf_backend_set_plan_options(self._h, C.byref(o)))
 
     def stream(self) -> int:

@@ -177,8 +177,12 @@ struct Scheduler {
     std::vector<std::vector<int>> queue;  // node ids per qubit, circuit order
     std::vector<size_t> head;
     size_t remaining;
+    // 0: every qubit of a non-diagonal gate must be in the set;
+    // 1: a CNOT needs only its target (the control bit is read from the index).
+    int mode = 0;
 
-    Scheduler(const std::vector<Node>& n, uint32_t nq) : nodes(n), qubits(nq), queue(nq), head(nq, 0) {
+    Scheduler(const std::vector<Node>& n, uint32_t nq, int m = 0)
+        : nodes(n), qubits(nq), queue(nq), head(nq, 0), mode(m) {
         for (size_t i = 0; i < nodes.size(); ++i)
             for (int k = 0; k < nodes[i].nq; ++k) queue[nodes[i].q[k]].push_back(int(i));
         remaining = nodes.size();
@@ -194,6 +198,7 @@ struct Scheduler {
     }
     bool fits(const Node& n, const std::vector<char>& in) const {
         if (n.diag()) return true;
+        if (mode == 1 && n.kind == N_CNOT) return in[n.q[1]] != 0;
         for (int k = 0; k < n.nq; ++k)
             if (!in[n.q[k]]) return false;
         return true;
@@ -224,6 +229,132 @@ struct Scheduler {
         return score;
     }
 };
+
+// Greedy set selection: starting from `in` (size `size`), add candidates
+// until the set holds `target` qubits, each time taking the qubit that lets
+// the most pending gates run; when no single qubit helps, seed the set with
+// the qubits of the earliest runnable gate.  Returns false if nothing can run.
+bool choose_set(const Scheduler& sch, const std::vector<Node>& nodes, std::vector<char>& in, uint32_t size,
+                uint32_t target, const std::vector<uint32_t>& cand) {
+    std::vector<size_t> h_base = sch.head;
+    int base_score = sch.simulate(h_base, in, nullptr);
+    while (size < target) {
+        int best = -1, best_score = -1, best_tie = -1;
+        for (uint32_t q : cand) {
+            if (in[q]) continue;
+            in[q] = 1;
+            std::vector<size_t> h = sch.head;
+            const int sc = sch.simulate(h, in, nullptr);
+            in[q] = 0;
+            // Tie break: pending work on q, then the lowest index.
+            const int tie = int(sch.queue[q].size() - sch.head[q]);
+            if (sc > best_score || (sc == best_score && tie > best_tie)) {
+                best = int(q);
+                best_score = sc;
+                best_tie = tie;
+            }
+        }
+        if (best < 0) break;
+        if (best_score <= base_score) {
+            int pick = -1;
+            for (uint32_t q : cand) {
+                if (h_base[q] >= sch.queue[q].size()) continue;
+                const int id = sch.queue[q][h_base[q]];
+                if (!sch.at_head(h_base, id)) continue;
+                const Node& nd = nodes[id];
+                if (nd.diag()) continue;
+                uint32_t missing = 0;
+                bool ok = true;
+                for (int k = 0; k < nd.nq; ++k) {
+                    const bool needed = !(sch.mode == 1 && nd.kind == N_CNOT && k == 0);
+                    if (!needed || in[nd.q[k]]) continue;
+                    if (std::find(cand.begin(), cand.end(), nd.q[k]) == cand.end()) ok = false;
+                    ++missing;
+                }
+                if (!ok || missing == 0 || missing > target - size) continue;
+                if (pick < 0 || id < pick) pick = id;
+            }
+            if (pick >= 0) {
+                const Node& nd = nodes[pick];
+                for (int k = 0; k < nd.nq; ++k) {
+                    const bool needed = !(sch.mode == 1 && nd.kind == N_CNOT && k == 0);
+                    if (needed && !in[nd.q[k]]) {
+                        in[nd.q[k]] = 1;
+                        ++size;
+                    }
+                }
+                h_base = sch.head;
+                base_score = sch.simulate(h_base, in, nullptr);
+                continue;
+            }
+        }
+        in[best] = 1;
+        ++size;
+        h_base = sch.head;
+        base_score = sch.simulate(h_base, in, nullptr);
+    }
+    std::vector<size_t> h = sch.head;
+    return sch.simulate(h, in, nullptr) > 0;
+}
+
+TileOp make_reg_op(const Node& n, const std::vector<int>& reg_index);
+
+// Splits one pass's ops into register phases of RB tile bits each.  Emits a
+// TOP_PHASE header (packed tile bits of the register set, register index i
+// <-> 4 bits at 4*i) before each phase's ops.
+std::string build_phases(const std::vector<Node>& nodes, const std::vector<int>& order, uint32_t qubits,
+                         const std::vector<int>& tile_bit, uint32_t RB, std::vector<TileOp>& ops_out,
+                         uint64_t& phase_count) {
+    std::vector<Node> sub;
+    sub.reserve(order.size());
+    for (int id : order) sub.push_back(nodes[id]);
+    Scheduler ps(sub, qubits, 1);
+    std::vector<uint32_t> tq;
+    for (uint32_t q = 0; q < qubits; ++q)
+        if (tile_bit[q] >= 0) tq.push_back(q);
+    size_t done = 0;
+    while (done < sub.size()) {
+        std::vector<char> in(qubits, 0);
+        if (!choose_set(ps, sub, in, 0, RB, tq)) {
+            // Only diagonal gates can be pending here with an empty useful set;
+            // fill the set with any tile qubits so they still run.
+            uint32_t size = 0;
+            for (uint32_t q : tq)
+                if (size < RB && !in[q]) {
+                    in[q] = 1;
+                    ++size;
+                }
+        }
+        std::vector<int> ord;
+        ps.simulate(ps.head, in, &ord);
+        if (ord.empty()) return "phase planner made no progress";
+        done += ord.size();
+        std::vector<int> reg_index(qubits, -1);
+        TileOp hdr;
+        std::memset(&hdr, 0, sizeof(hdr));
+        hdr.kind = TOP_PHASE;
+        uint32_t i = 0, packed = 0;
+        for (uint32_t q : tq)
+            if (in[q] && i < RB) {
+                reg_index[q] = int(i);
+                packed |= uint32_t(tile_bit[q]) << (4 * i);
+                ++i;
+            }
+        // Fill unused register slots with further tile bits (the set must have RB distinct bits).
+        for (uint32_t q : tq)
+            if (i < RB && reg_index[q] < 0) {
+                reg_index[q] = int(i);
+                packed |= uint32_t(tile_bit[q]) << (4 * i);
+                ++i;
+            }
+        hdr.q0 = int32_t(packed);
+        hdr.q1 = int32_t(RB);
+        ops_out.push_back(hdr);
+        ++phase_count;
+        for (int id : ord) ops_out.push_back(make_reg_op(sub[id], reg_index));
+    }
+    return "";
+}
 
 TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
     TileOp op;
@@ -279,6 +410,59 @@ TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
     return op;
 }
 
+TileOp make_reg_op(const Node& n, const std::vector<int>& reg) {
+    TileOp op;
+    std::memset(&op, 0, sizeof(op));
+    switch (n.kind) {
+        case N_MAT1:
+            op.kind = TOP_MAT1;
+            op.q0 = reg[n.q[0]];
+            op.q1 = -1;
+            std::memcpy(op.m, n.m, 4 * sizeof(cd));
+            break;
+        case N_DIAG1:
+            op.kind = TOP_DIAG;
+            op.q0 = int(n.q[0]);
+            op.q1 = -1;
+            op.m[0] = n.m[0].re; op.m[1] = n.m[0].im;
+            op.m[2] = n.m[3].re; op.m[3] = n.m[3].im;
+            break;
+        case N_DIAG2:
+            op.kind = TOP_DIAG;
+            op.q0 = int(n.q[0]);
+            op.q1 = int(n.q[1]);
+            std::memcpy(op.m, n.m, 4 * sizeof(cd));
+            break;
+        case N_CNOT:
+            op.kind = TOP_CNOT;
+            op.q0 = int(n.q[0]);      // control: global qubit
+            op.q1 = reg[n.q[1]];      // target: register index
+            break;
+        case N_SWAP:
+            op.kind = TOP_SWAP;
+            op.q0 = reg[n.q[0]];
+            op.q1 = reg[n.q[1]];
+            break;
+        case N_ROT:
+            op.kind = TOP_ROT;
+            op.q0 = n.axis == 2 ? int(n.q[0]) : reg[n.q[0]];
+            op.q1 = n.axis;
+            op.flags = n.flags;
+            op.param = n.param;
+            op.slot = n.slot;
+            break;
+        case N_ROTZZ:
+            op.kind = TOP_ROTZZ;
+            op.q0 = int(n.q[0]);
+            op.q1 = int(n.q[1]);
+            op.flags = n.flags;
+            op.param = n.param;
+            op.slot = n.slot;
+            break;
+    }
+    return op;
+}
+
 }  // namespace
 
 std::string describe_plan(const Plan& plan) {
@@ -301,8 +485,8 @@ std::string describe_plan(const Plan& plan) {
         const Pass& p = plan.passes[i];
         if (i) s += ",";
         if (p.kind == Pass::TILE) {
-            s += "{\"kind\":\"tile\",\"tile_bits\":" + std::to_string(p.tile_bits) + ",\"low\":" +
-                 std::to_string(p.low) + ",\"high\":[";
+            s += "{\"kind\":\"tile\",\"tile_bits\":" + std::to_string(p.tile_bits) + ",\"reg_bits\":" +
+                 std::to_string(p.reg_bits) + ",\"low\":" + std::to_string(p.low) + ",\"high\":[";
             for (size_t k = 0; k < p.high.size(); ++k) s += (k ? "," : "") + std::to_string(p.high[k]);
             s += "],\"ops\":[";
             for (size_t k = 0; k < p.ops.size(); ++k) {
@@ -325,7 +509,8 @@ std::string describe_plan(const Plan& plan) {
     s += "],\"stats\":{\"passes\":" + std::to_string(plan.stats.passes) +
          ",\"ops_in\":" + std::to_string(plan.stats.ops_in) +
          ",\"ops_applied\":" + std::to_string(plan.stats.ops_applied) +
-         ",\"max_ops_per_pass\":" + std::to_string(plan.stats.max_ops_per_pass) + "}}";
+         ",\"max_ops_per_pass\":" + std::to_string(plan.stats.max_ops_per_pass) +
+         ",\"phases\":" + std::to_string(plan.stats.phases) + "}}";
     return s;
 }
 
@@ -376,73 +561,25 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     out.stats.ops_applied = nodes.size();
     uint32_t M = opt.tile_qubits ? opt.tile_qubits : default_tile_bits(precision);
     uint32_t low = opt.low_qubits ? opt.low_qubits : default_low_bits(precision);
+    if (M > 14) return "tile_qubits must be at most 14";
+    // Register kernel: 2^RB amplitudes per thread, 2^(M-RB) <= 256 threads.
+    uint32_t RB = opt.reg_bits ? opt.reg_bits : (precision == 0 ? 3 : 4);
+    bool regk = opt.kernel != 1 && M <= qubits && RB >= 3 && RB <= 4 && M >= RB + 6 && M - RB <= 8;
+    if (opt.kernel == 2 && !regk) return "register kernel needs tile_qubits <= qubits and 6 <= tile - reg_bits <= 8";
     if (M > qubits) M = qubits;
     if (low > M) low = M;
-    if (M > 14) return "tile_qubits must be at most 14";
 
-    Scheduler sch(nodes, qubits);
+    Scheduler sch(nodes, qubits, regk ? 1 : 0);
+    std::vector<uint32_t> all_q(qubits);
+    for (uint32_t q = 0; q < qubits; ++q) all_q[q] = q;
     size_t done = 0;
     while (done < nodes.size()) {
         std::vector<char> in(qubits, 0);
         for (uint32_t q = 0; q < low; ++q) in[q] = 1;
-        uint32_t size = low;
-        std::vector<size_t> h_base = sch.head;
-        int base_score = sch.simulate(h_base, in, nullptr);
-        while (size < M) {
-            int best = -1, best_score = -1, best_tie = -1;
-            for (uint32_t q = low; q < qubits; ++q) {
-                if (in[q]) continue;
-                in[q] = 1;
-                std::vector<size_t> h = sch.head;
-                const int sc = sch.simulate(h, in, nullptr);
-                in[q] = 0;
-                // Tie break: pending work on q (queue length left), then the lowest index.
-                const int tie = int(sch.queue[q].size() - sch.head[q]);
-                if (sc > best_score || (sc == best_score && tie > best_tie)) {
-                    best = int(q);
-                    best_score = sc;
-                    best_tie = tie;
-                }
-            }
-            if (best < 0) break;
-            if (best_score <= base_score) {
-                // No single qubit unlocks anything: every runnable gate needs two
-                // new qubits.  Take the earliest runnable gate that still fits.
-                int pick = -1;
-                for (uint32_t q = 0; q < qubits; ++q) {
-                    if (h_base[q] >= sch.queue[q].size()) continue;
-                    const int id = sch.queue[q][h_base[q]];
-                    if (!sch.at_head(h_base, id)) continue;
-                    const Node& nd = nodes[id];
-                    uint32_t missing = 0;
-                    for (int k = 0; k < nd.nq; ++k) missing += in[nd.q[k]] ? 0 : 1;
-                    if (missing == 0 || missing > M - size) continue;
-                    if (pick < 0 || id < pick) pick = id;
-                }
-                if (pick >= 0) {
-                    const Node& nd = nodes[pick];
-                    for (int k = 0; k < nd.nq; ++k)
-                        if (!in[nd.q[k]]) {
-                            in[nd.q[k]] = 1;
-                            ++size;
-                        }
-                    h_base = sch.head;
-                    base_score = sch.simulate(h_base, in, nullptr);
-                    continue;
-                }
-            }
-            in[best] = 1;
-            ++size;
-            h_base = sch.head;
-            base_score = sch.simulate(h_base, in, nullptr);
-        }
+        if (!choose_set(sch, nodes, in, low, M, all_q)) return "planner made no progress";
         std::vector<int> order;
         sch.simulate(sch.head, in, &order);
-        if (order.empty()) {
-            // Cannot happen for gates of at most two qubits once M >= low + 2,
-            // but guard against a planner bug turning into an endless loop.
-            return "planner made no progress";
-        }
+        if (order.empty()) return "planner made no progress";
         done += order.size();
         Pass p{};
         p.kind = Pass::TILE;
@@ -456,8 +593,14 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
                 p.high.push_back(q);
                 tile_bit[q] = int(k++);
             }
-        for (int id : order) p.ops.push_back(make_tile_op(nodes[id], tile_bit));
-        out.stats.max_ops_per_pass = std::max<uint64_t>(out.stats.max_ops_per_pass, p.ops.size());
+        if (!regk) {
+            for (int id : order) p.ops.push_back(make_tile_op(nodes[id], tile_bit));
+        } else {
+            p.reg_bits = RB;
+            std::string e = build_phases(nodes, order, qubits, tile_bit, RB, p.ops, out.stats.phases);
+            if (!e.empty()) return e;
+        }
+        out.stats.max_ops_per_pass = std::max<uint64_t>(out.stats.max_ops_per_pass, order.size());
         out.passes.push_back(std::move(p));
     }
     out.stats.passes = out.passes.size();

@@ -31,7 +31,15 @@ enum TileOpKind : int32_t {
     TOP_MAT2 = 4,   // dense 4x4 on tile bits (q0 = low bit of the 2-bit index, q1 = high bit)
     TOP_ROT = 5,    // exp(-i theta P / 2), P = X/Y on tile bit q0, or Z on global qubit q0; axis in q1
     TOP_ROTZZ = 6,  // exp(-i theta Z_q0 Z_q1 / 2) on global qubits q0, q1
+    TOP_PHASE = 7,  // register kernel: switch the register-bit set (q0 = packed tile bits, 4 bits each)
 };
+
+// Register-resident tile kernel ("phases"): each thread keeps 2^RB amplitudes
+// in registers; the ops of a phase address register bits:
+//   TOP_MAT1, TOP_ROT (X/Y), TOP_SWAP: q0 (and q1) are register indices;
+//   TOP_CNOT: q0 = control as a global qubit, q1 = target register index;
+//   TOP_DIAG, TOP_ROT (Z), TOP_ROTZZ: global qubits (bits taken from the
+//   element's global index).
 
 // TileOp flags (rotations).
 enum : int32_t {
@@ -61,6 +69,7 @@ struct Pass {
     cd m[4]{};
     // TILE
     uint32_t tile_bits = 0;            // M
+    uint32_t reg_bits = 0;             // > 0: register kernel with 2^reg_bits amplitudes per thread
     uint32_t low = 0;                  // contiguous low qubits
     std::vector<uint32_t> high;        // global qubit of tile bit low+k (ascending)
     std::vector<TileOp> ops;
@@ -71,10 +80,12 @@ struct PlanOptions {
     uint32_t tile_qubits = 0;  // 0 = default for the precision
     uint32_t low_qubits = 0;   // 0 = default
     uint32_t max_block = 0;
+    int kernel = 0;            // 0 = default, 1 = LDS op-by-op tile kernel, 2 = register kernel
+    uint32_t reg_bits = 0;     // register kernel: amplitudes per thread = 2^reg_bits (0 = default)
 };
 
 struct PlanStats {
-    uint64_t passes = 0, ops_in = 0, ops_applied = 0, dense_blocks = 0, max_ops_per_pass = 0;
+    uint64_t passes = 0, ops_in = 0, ops_applied = 0, dense_blocks = 0, max_ops_per_pass = 0, phases = 0;
 };
 
 // One circuit gate in planner form.

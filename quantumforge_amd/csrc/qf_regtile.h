@@ -1,0 +1,333 @@
+// Register-resident tile kernel for gfx950.
+//
+// A workgroup of 2^(M-RB) threads owns one 2^M-amplitude tile (same tile
+// geometry as k_tile).  Each thread keeps 2^RB amplitudes of the tile in
+// registers: the RB "register bits" of the current phase index them, the
+// remaining M-RB tile bits index the thread.  Gates on register bits are
+// pure register arithmetic; diagonal gates and CNOT controls read their bits
+// from the element's global index, so they run on any qubit.  Between phases
+// the tile is re-distributed through LDS (one write + one read), instead of
+// one LDS round trip per gate.  Loads and stores go straight between HBM and
+// registers with the first / last phase's mapping.
+//
+// NS = 2 carries a second state (the adjoint costate) through the same ops and
+// records Im <lambda|G|psi> for TF_RECORD rotations before applying them.
+#pragma once
+#include "qf_kernels.h"
+
+namespace qf {
+
+// Global offset (without the workgroup base) of tile index j.
+__device__ __forceinline__ uint64_t tile_to_global(uint32_t j, const TileArgs& a) {
+    uint64_t g = j & ((1u << a.low) - 1);
+    const uint32_t nh = a.M - a.low;
+    for (uint32_t k = 0; k < nh; ++k) g |= uint64_t((j >> (a.low + k)) & 1) << a.high[k];
+    return g;
+}
+
+// Tile index of the thread's register-0 element: thread bits go, in order,
+// into the tile bits that are not register bits.
+__device__ __forceinline__ uint32_t thread_tile_index(uint32_t t, uint32_t rbmask, uint32_t M) {
+    uint32_t j = 0, bit = 0;
+    for (uint32_t b = 0; b < M; ++b) {
+        if ((rbmask >> b) & 1) continue;
+        j |= ((t >> bit) & 1) << b;
+        ++bit;
+    }
+    return j;
+}
+
+template <int RB>
+struct RegMap {
+    uint32_t jk[1 << RB];  // tile index offset of register k
+    uint64_t gk[1 << RB];  // global offset of register k
+    uint32_t jt;           // tile index of the thread's register 0
+    uint64_t gt;           // global index of the thread's register 0 (with base)
+};
+
+template <int RB>
+__device__ __forceinline__ void decode_map(uint32_t packed, const TileArgs& a, uint64_t base, RegMap<RB>& m) {
+    uint32_t rb[RB];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+        rb[i] = (packed >> (4 * i)) & 15u;
+        mask |= 1u << rb[i];
+    }
+#pragma unroll
+    for (int k = 0; k < (1 << RB); ++k) {
+        uint32_t j = 0;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) j |= uint32_t((k >> i) & 1) << rb[i];
+        m.jk[k] = j;
+        m.gk[k] = tile_to_global(j, a);
+    }
+    m.jt = thread_tile_index(threadIdx.x, mask, a.M);
+    m.gt = base + tile_to_global(m.jt, a);
+}
+
+template <class R>
+__device__ __forceinline__ cx<R> cm(R mr, R mi, cx<R> v) {
+    return cx<R>{mr * v.re - mi * v.im, mr * v.im + mi * v.re};
+}
+
+// 2x2 on register bit B.
+template <class R, int RB, int B>
+__device__ __forceinline__ void reg_mat1(cx<R> (&a)[1 << RB], const R (&m)[8]) {
+    if constexpr (B < RB) {
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) {
+            if (k & (1 << B)) continue;
+            const int k1 = k | (1 << B);
+            const cx<R> a0 = a[k], a1 = a[k1];
+            const cx<R> x0 = cm(m[0], m[1], a0), x1 = cm(m[2], m[3], a1);
+            const cx<R> y0 = cm(m[4], m[5], a0), y1 = cm(m[6], m[7], a1);
+            a[k] = cx<R>{x0.re + x1.re, x0.im + x1.im};
+            a[k1] = cx<R>{y0.re + y1.re, y0.im + y1.im};
+        }
+    }
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_mat1_any(cx<R> (&a)[1 << RB], int bit, const R (&m)[8]) {
+    switch (bit) {
+        case 0: reg_mat1<R, RB, 0>(a, m); break;
+        case 1: reg_mat1<R, RB, 1>(a, m); break;
+        case 2: reg_mat1<R, RB, 2>(a, m); break;
+        case 3: reg_mat1<R, RB, 3>(a, m); break;
+        default: reg_mat1<R, RB, 4>(a, m); break;
+    }
+}
+
+// CNOT with target register bit B and a control bit read from the global index.
+template <class R, int RB, int B>
+__device__ __forceinline__ void reg_cnot(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, int ctrl) {
+    if constexpr (B < RB) {
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) {
+            if (k & (1 << B)) continue;
+            const int k1 = k | (1 << B);
+            const bool c = ((mp.gt | mp.gk[k]) >> ctrl) & 1;
+            const cx<R> u = a[k], v = a[k1];
+            a[k] = c ? v : u;
+            a[k1] = c ? u : v;
+        }
+    }
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_cnot_any(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, int ctrl, int bit) {
+    switch (bit) {
+        case 0: reg_cnot<R, RB, 0>(a, mp, ctrl); break;
+        case 1: reg_cnot<R, RB, 1>(a, mp, ctrl); break;
+        case 2: reg_cnot<R, RB, 2>(a, mp, ctrl); break;
+        case 3: reg_cnot<R, RB, 3>(a, mp, ctrl); break;
+        default: reg_cnot<R, RB, 4>(a, mp, ctrl); break;
+    }
+}
+
+template <class R, int RB, int I, int J>
+__device__ __forceinline__ void reg_swap(cx<R> (&a)[1 << RB]) {
+    if constexpr (I < RB && J < RB && I != J) {
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) {
+            if (!((k >> I) & 1) || ((k >> J) & 1)) continue;
+            const int k2 = k ^ (1 << I) ^ (1 << J);
+            const cx<R> t = a[k];
+            a[k] = a[k2];
+            a[k2] = t;
+        }
+    }
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_swap_any(cx<R> (&a)[1 << RB], int i, int j) {
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    switch (lo * 8 + hi) {
+        case 0 * 8 + 1: reg_swap<R, RB, 0, 1>(a); break;
+        case 0 * 8 + 2: reg_swap<R, RB, 0, 2>(a); break;
+        case 0 * 8 + 3: reg_swap<R, RB, 0, 3>(a); break;
+        case 0 * 8 + 4: reg_swap<R, RB, 0, 4>(a); break;
+        case 1 * 8 + 2: reg_swap<R, RB, 1, 2>(a); break;
+        case 1 * 8 + 3: reg_swap<R, RB, 1, 3>(a); break;
+        case 1 * 8 + 4: reg_swap<R, RB, 1, 4>(a); break;
+        case 2 * 8 + 3: reg_swap<R, RB, 2, 3>(a); break;
+        case 2 * 8 + 4: reg_swap<R, RB, 2, 4>(a); break;
+        default: reg_swap<R, RB, 3, 4>(a); break;
+    }
+}
+
+// Diagonal on global qubits q0 (, q1): a_k *= d[bit(q1)*2 + bit(q0)].
+template <class R, int RB>
+__device__ __forceinline__ void reg_diag(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, const TileOp& op) {
+    const R d0r = R(op.m[0]), d0i = R(op.m[1]), d1r = R(op.m[2]), d1i = R(op.m[3]);
+    const R d2r = R(op.m[4]), d2i = R(op.m[5]), d3r = R(op.m[6]), d3i = R(op.m[7]);
+    const int q0 = op.q0, q1 = op.q1;
+#pragma unroll
+    for (int k = 0; k < (1 << RB); ++k) {
+        const uint64_t gi = mp.gt | mp.gk[k];
+        const bool b0 = (gi >> q0) & 1;
+        const bool b1 = q1 >= 0 ? ((gi >> q1) & 1) : false;
+        const R dr = b1 ? (b0 ? d3r : d2r) : (b0 ? d1r : d0r);
+        const R di = b1 ? (b0 ? d3i : d2i) : (b0 ? d1i : d0i);
+        a[k] = cm(dr, di, a[k]);
+    }
+}
+
+// exp(-i theta Z/2) or exp(-i theta Z Z/2) from (c, s): c - is on even parity, c + is on odd.
+template <class R, int RB>
+__device__ __forceinline__ void reg_rotz(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, int q0, int q1, R c, R s) {
+#pragma unroll
+    for (int k = 0; k < (1 << RB); ++k) {
+        const uint64_t gi = mp.gt | mp.gk[k];
+        const bool odd = (((gi >> q0) ^ (q1 >= 0 ? (gi >> q1) : 0)) & 1) != 0;
+        a[k] = cm(c, odd ? s : -s, a[k]);
+    }
+}
+
+// Thread-local Im <lambda| G |psi> for a rotation generator.
+template <class R, int RB, int B>
+__device__ __forceinline__ double reg_overlap_xy(const cx<R> (&psi)[1 << RB], const cx<R> (&lam)[1 << RB],
+                                                 int axis) {
+    double acc = 0.0;
+    if constexpr (B < RB) {
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) {
+            const int kp = k ^ (1 << B);
+            const double lr = lam[k].re, li = lam[k].im, pr = psi[kp].re, pi = psi[kp].im;
+            const double re = lr * pr + li * pi, im = lr * pi - li * pr;
+            acc += axis == 0 ? im : ((k >> B) & 1 ? re : -re);
+        }
+    }
+    return acc;
+}
+
+template <class R, int RB>
+__device__ __forceinline__ double reg_overlap(const cx<R> (&psi)[1 << RB], const cx<R> (&lam)[1 << RB],
+                                              const RegMap<RB>& mp, const TileOp& op) {
+    if (op.kind == TOP_ROT && op.q1 != 2) {
+        switch (op.q0) {
+            case 0: return reg_overlap_xy<R, RB, 0>(psi, lam, op.q1);
+            case 1: return reg_overlap_xy<R, RB, 1>(psi, lam, op.q1);
+            case 2: return reg_overlap_xy<R, RB, 2>(psi, lam, op.q1);
+            case 3: return reg_overlap_xy<R, RB, 3>(psi, lam, op.q1);
+            default: return reg_overlap_xy<R, RB, 4>(psi, lam, op.q1);
+        }
+    }
+    const int q0 = op.q0, q1 = op.kind == TOP_ROTZZ ? op.q1 : -1;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < (1 << RB); ++k) {
+        const uint64_t gi = mp.gt | mp.gk[k];
+        const bool odd = (((gi >> q0) ^ (q1 >= 0 ? (gi >> q1) : 0)) & 1) != 0;
+        const double im = double(lam[k].re) * double(psi[k].im) - double(lam[k].im) * double(psi[k].re);
+        acc += odd ? -im : im;
+    }
+    return acc;
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, const TileOp& op,
+                                          const double* __restrict__ cs_row) {
+    switch (op.kind) {
+        case TOP_MAT1: {
+            const R m[8] = {R(op.m[0]), R(op.m[1]), R(op.m[2]), R(op.m[3]),
+                            R(op.m[4]), R(op.m[5]), R(op.m[6]), R(op.m[7])};
+            reg_mat1_any<R, RB>(a, op.q0, m);
+            break;
+        }
+        case TOP_DIAG: reg_diag<R, RB>(a, mp, op); break;
+        case TOP_CNOT: reg_cnot_any<R, RB>(a, mp, op.q0, op.q1); break;
+        case TOP_SWAP: reg_swap_any<R, RB>(a, op.q0, op.q1); break;
+        case TOP_ROT:
+        case TOP_ROTZZ: {
+            R c, s;
+            rot_cs<R>(op, cs_row, c, s);
+            if (op.kind == TOP_ROT && op.q1 == 0) {  // Rx = [[c, -is], [-is, c]]
+                const R m[8] = {c, R(0), R(0), -s, R(0), -s, c, R(0)};
+                reg_mat1_any<R, RB>(a, op.q0, m);
+            } else if (op.kind == TOP_ROT && op.q1 == 1) {  // Ry = [[c, -s], [s, c]]
+                const R m[8] = {c, R(0), -s, R(0), s, R(0), c, R(0)};
+                reg_mat1_any<R, RB>(a, op.q0, m);
+            } else {
+                reg_rotz<R, RB>(a, mp, op.q0, op.kind == TOP_ROTZZ ? op.q1 : -1, c, s);
+            }
+            break;
+        }
+        default: break;
+    }
+}
+
+// ops[0] is always a TOP_PHASE header.  NS = 1: s0 is the state; NS = 2: s0 =
+// psi, s1 = lambda, partial as k_tile_adjoint.
+template <class R, int RB, int NS>
+__global__ __launch_bounds__(kThreads) void k_tile_reg(cx<R>* __restrict__ s0, cx<R>* __restrict__ s1, TileArgs args,
+                                                       const TileOp* __restrict__ ops, const double* __restrict__ cs,
+                                                       double* __restrict__ partial) {
+    constexpr int NR = 1 << RB;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[kThreads / 64];
+    cx<R>* xb = reinterpret_cast<cx<R>*>(smem_raw);
+    const uint32_t T = 1u << args.M;
+
+    const uint64_t b = uint64_t(blockIdx.x) + uint64_t(blockIdx.y) * gridDim.x;
+    const uint64_t inst = b >> args.log_blocks;
+    const uint64_t bl = b & ((uint64_t(1) << args.log_blocks) - 1);
+    uint64_t base = 0;
+    {
+        const uint32_t nh = args.M - args.low;
+        uint32_t k = 0, bit = 0;
+        for (uint32_t g = args.low; g < args.n; ++g) {
+            if (k < nh && args.high[k] == g) {
+                ++k;
+                continue;
+            }
+            base |= ((bl >> bit) & 1) << g;
+            ++bit;
+        }
+    }
+    base += inst << args.n;
+    const double* cs_row = cs ? cs + 2 * inst * args.nslots : nullptr;
+    double* prow = (NS == 2) ? partial + (inst * (uint64_t(1) << args.log_blocks) + bl) * args.nslots : nullptr;
+
+    RegMap<RB> mp;
+    decode_map<RB>(uint32_t(ops[0].q0), args, base, mp);
+    cx<R> a[NS][NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        a[0][k] = s0[mp.gt | mp.gk[k]];
+        if constexpr (NS == 2) a[1][k] = s1[mp.gt | mp.gk[k]];
+    }
+    for (uint32_t o = 1; o < args.nops; ++o) {
+        const TileOp& op = ops[o];
+        if (op.kind == TOP_PHASE) {
+            __syncthreads();  // earlier readers of the exchange buffer are done
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int k = 0; k < NR; ++k) xb[s * T + (mp.jt | mp.jk[k])] = a[s][k];
+            decode_map<RB>(uint32_t(op.q0), args, base, mp);
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int k = 0; k < NR; ++k) a[s][k] = xb[s * T + (mp.jt | mp.jk[k])];
+            continue;
+        }
+        if constexpr (NS == 2) {
+            if ((op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD)) {
+                const double v = block_sum(reg_overlap<R, RB>(a[0], a[1], mp, op), red);
+                if (threadIdx.x == 0) prow[op.slot] = v;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) reg_apply<R, RB>(a[s], mp, op, cs_row);
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        s0[mp.gt | mp.gk[k]] = a[0][k];
+        if constexpr (NS == 2) s1[mp.gt | mp.gk[k]] = a[1][k];
+    }
+}
+
+}  // namespace qf

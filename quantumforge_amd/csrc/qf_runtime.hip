@@ -12,6 +12,7 @@
 
 #include "../../include/qf_hip.h"
 #include "qf_kernels.h"
+#include "qf_regtile.h"
 #include "qf_plan.h"
 #include "qf_runtime.h"
 
@@ -315,6 +316,30 @@ qf_status launch_pass(qf_backend* be, cx<R>* d, uint32_t qubits, uint32_t count,
             a.nops = uint32_t(p.ops.size());
             a.nslots = ctx.nslots;
             for (size_t k = 0; k < p.high.size(); ++k) a.high[k] = p.high[k];
+            const uint32_t gx_log0 = a.log_blocks < 16 ? a.log_blocks : 16;
+            const dim3 grid0(1u << gx_log0, count << (a.log_blocks - gx_log0));
+            if (p.reg_bits) {
+                const int ns = ctx.lam ? 2 : 1;
+                const size_t lds = size_t(ns) << p.tile_bits;
+                const size_t bytes = lds * sizeof(cx<R>);
+                const dim3 block(1u << (p.tile_bits - p.reg_bits));
+                const void* fn = nullptr;
+#define QF_REG_CASE(RBV, NSV)                                                                          \
+    if (p.reg_bits == RBV && ns == NSV) {                                                              \
+        fn = reinterpret_cast<const void*>(&qf::k_tile_reg<R, RBV, NSV>);                              \
+        if (bytes > 64 * 1024)                                                                         \
+            HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));  \
+        hipLaunchKernelGGL((qf::k_tile_reg<R, RBV, NSV>), grid0, block, bytes, st, d,                  \
+                           static_cast<cx<R>*>(ctx.lam), a, d_ops, ctx.cs, ctx.partial);               \
+    }
+                QF_REG_CASE(3, 1)
+                QF_REG_CASE(3, 2)
+                QF_REG_CASE(4, 1)
+                QF_REG_CASE(4, 2)
+#undef QF_REG_CASE
+                if (!fn) return fail(QF_INVALID_ARGUMENT, "unsupported register-kernel shape");
+                return launch_check("k_tile_reg");
+            }
             const uint32_t nh = p.tile_bits - p.low;
             const int tiles = ctx.lam ? 2 : 1;
             const size_t lds =
@@ -505,6 +530,8 @@ extern "C" qf_status qf_backend_set_plan_options(qf_backend* be, const qf_plan_o
     be->opts.tile_qubits = o->tile_qubits;
     be->opts.low_qubits = o->low_qubits;
     be->opts.max_block = o->max_block;
+    be->opts.kernel = o->kernel;
+    be->opts.reg_bits = o->reg_bits;
     return QF_OK;
 }
 
@@ -1032,6 +1059,8 @@ extern "C" qf_status qf_plan_build(const qf_op* ops, size_t count, uint32_t qubi
         po.tile_qubits = o->tile_qubits;
         po.low_qubits = o->low_qubits;
         po.max_block = o->max_block;
+        po.kernel = o->kernel;
+        po.reg_bits = o->reg_bits;
     }
     qf_plan* p = new qf_plan;
     std::string err = qf::build_plan(g, qubits, QF_C128, po, p->plan);
@@ -1050,6 +1079,7 @@ extern "C" qf_status qf_plan_stats_get(const qf_plan* p, qf_plan_stats* out) {
     out->ops_applied = p->plan.stats.ops_applied;
     out->dense_blocks = p->plan.stats.dense_blocks;
     out->max_ops_per_pass = p->plan.stats.max_ops_per_pass;
+    out->phases = p->plan.stats.phases;
     return QF_OK;
 }
 

@@ -82,69 +82,110 @@ def _c(m, k):
     return complex(m[2 * k], m[2 * k + 1])
 
 
+def _mat1(psi, idx, q, m):
+    mask = 1 << q
+    i0 = idx[(idx & mask) == 0]
+    a0, a1 = psi[i0].copy(), psi[i0 | mask].copy()
+    psi[i0] = _c(m, 0) * a0 + _c(m, 1) * a1
+    psi[i0 | mask] = _c(m, 2) * a0 + _c(m, 3) * a1
+
+
+def _cnot(psi, idx, c, t):
+    sel = idx[(((idx >> c) & 1) == 1) & (((idx >> t) & 1) == 0)]
+    tmp = psi[sel].copy()
+    psi[sel] = psi[sel | (1 << t)]
+    psi[sel | (1 << t)] = tmp
+
+
+def _swap(psi, idx, a, b):
+    sel = idx[(((idx >> a) & 1) == 1) & (((idx >> b) & 1) == 0)]
+    other = sel ^ (1 << a) ^ (1 << b)
+    tmp = psi[sel].copy()
+    psi[sel] = psi[other]
+    psi[other] = tmp
+
+
+def _diag(psi, idx, q0, q1, m):
+    kk = (idx >> q0) & 1
+    if q1 >= 0:
+        kk = kk | (((idx >> q1) & 1) << 1)
+    d = np.array([_c(m, j) for j in range(4)])
+    return psi * d[kk]
+
+
 def replay(desc, psi):
+    """Applies a described fused plan (LDS tile kernel or register kernel) to psi."""
     n = desc["qubits"]
-    dim = 1 << n
-    idx = np.arange(dim, dtype=np.int64)
+    idx = np.arange(1 << n, dtype=np.int64)
     psi = psi.astype(complex).copy()
     for p in desc["passes"]:
-        if p["kind"] != "tile":
-            raise AssertionError("fused plans contain tile passes only")
-        low, high = p["low"], p["high"]
-        bits = list(range(low)) + list(high)  # tile bit -> global qubit
+        assert p["kind"] == "tile", "fused plans contain tile passes only"
+        bits = list(range(p["low"])) + list(p["high"])  # tile bit -> global qubit
+        reg = p["reg_bits"]
+        regs = None
         for op in p["ops"]:
             k, m = op["kind"], op["m"]
-            if k == 0:  # MAT1 on tile bit
-                q = bits[op["q0"]]
-                mask = 1 << q
-                i0 = idx[(idx & mask) == 0]
-                a0, a1 = psi[i0].copy(), psi[i0 | mask].copy()
-                psi[i0] = _c(m, 0) * a0 + _c(m, 1) * a1
-                psi[i0 | mask] = _c(m, 2) * a0 + _c(m, 3) * a1
-            elif k == 1:  # DIAG on global qubits
-                q0, q1 = op["q0"], op["q1"]
-                kk = (idx >> q0) & 1
-                if q1 >= 0:
-                    kk = kk | (((idx >> q1) & 1) << 1)
-                d = np.array([_c(m, j) for j in range(4)])
-                psi = psi * d[kk]
-            elif k in (2, 3):
-                a, b = bits[op["q0"]], bits[op["q1"]]
-                if k == 2:
-                    sel = idx[((idx >> a) & 1) == 1]
-                    sel = sel[((sel >> b) & 1) == 0]
-                    tmp = psi[sel].copy()
-                    psi[sel] = psi[sel | (1 << b)]
-                    psi[sel | (1 << b)] = tmp
+            if reg:
+                if k == 7:
+                    regs = [bits[(op["q0"] >> (4 * i)) & 15] for i in range(reg)]
+                    assert len(set(regs)) == reg
+                    continue
+                assert regs is not None, "register plan must open with a phase header"
+                if k == 0:
+                    _mat1(psi, idx, regs[op["q0"]], m)
+                elif k == 1:
+                    psi = _diag(psi, idx, op["q0"], op["q1"], m)
+                elif k == 2:
+                    _cnot(psi, idx, op["q0"], regs[op["q1"]])
+                elif k == 3:
+                    _swap(psi, idx, regs[op["q0"]], regs[op["q1"]])
                 else:
-                    sel = idx[(((idx >> a) & 1) == 1) & (((idx >> b) & 1) == 0)]
-                    other = sel ^ (1 << a) ^ (1 << b)
-                    tmp = psi[sel].copy()
-                    psi[sel] = psi[other]
-                    psi[other] = tmp
+                    raise AssertionError(f"unexpected op kind {k}")
             else:
-                raise AssertionError(f"unexpected op kind {k}")
+                if k == 0:
+                    _mat1(psi, idx, bits[op["q0"]], m)
+                elif k == 1:
+                    psi = _diag(psi, idx, op["q0"], op["q1"], m)
+                elif k == 2:
+                    _cnot(psi, idx, bits[op["q0"]], bits[op["q1"]])
+                elif k == 3:
+                    _swap(psi, idx, bits[op["q0"]], bits[op["q1"]])
+                else:
+                    raise AssertionError(f"unexpected op kind {k}")
     return psi
 
 
 def check_tile_invariants(desc):
     n = desc["qubits"]
     for p in desc["passes"]:
-        M, low, high = p["tile_bits"], p["low"], p["high"]
+        M, low, high, reg = p["tile_bits"], p["low"], p["high"], p["reg_bits"]
         assert len(high) == M - low and high == sorted(high) and all(low <= h < n for h in high)
+        if reg:
+            assert p["ops"][0]["kind"] == 7
+            for op in p["ops"]:
+                if op["kind"] in (0, 3):
+                    assert 0 <= op["q0"] < reg and (op["kind"] == 0 or 0 <= op["q1"] < reg)
+                if op["kind"] == 2:
+                    assert 0 <= op["q0"] < n and 0 <= op["q1"] < reg
+            continue
         for op in p["ops"]:
             if op["kind"] in (0, 2, 3):
                 assert 0 <= op["q0"] < M and (op["kind"] == 0 or 0 <= op["q1"] < M)
 
 
-@pytest.mark.parametrize("n,depth,tile,low", [(8, 10, 4, 2), (12, 12, 6, 3), (14, 20, 8, 4), (16, 10, 11, 5),
-                                              (10, 6, 10, 5), (5, 4, 11, 5)])
-def test_fused_plan_replay_equals_oracle(n, depth, tile, low):
+@pytest.mark.parametrize("n,depth,tile,low,kernel,reg", [
+    (8, 10, 4, 2, 1, 0), (12, 12, 6, 3, 1, 0), (14, 20, 8, 4, 1, 0), (16, 10, 11, 5, 1, 0), (10, 6, 10, 5, 1, 0),
+    (5, 4, 11, 5, 0, 0), (16, 10, 11, 5, 2, 3), (16, 10, 12, 5, 2, 4), (14, 20, 10, 4, 2, 3), (12, 12, 9, 3, 2, 3),
+    (12, 8, 10, 5, 2, 4), (11, 6, 11, 6, 2, 3)])
+def test_fused_plan_replay_equals_oracle(n, depth, tile, low, kernel, reg):
     c = random_layer_circuit(n, depth, seed=n * 31 + depth, pairs_per_layer=n // 2)
     c.swap(0, n - 1)
+    c.swap(1, 2)
     c.push(Gate.T, 2)
     c.push(Gate.Sdg, 2)
-    plan = Plan(c, n, True, tile, low)
+    plan = Plan(c, n, True, tile, low, kernel, reg)
+    if kernel == 2:
+        assert plan.describe()["passes"][0]["reg_bits"] == reg
     desc = plan.describe()
     check_tile_invariants(desc)
     st = plan.stats()

@@ -127,17 +127,43 @@ def test_config1_random_circuit_exact_and_fused(backend):
     assert abs(dev.norm_sqr() - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("tile,low", [(6, 2), (8, 3), (10, 4), (11, 5), (12, 5), (13, 6)])
-def test_fused_tile_shapes(backend, tile, low):
+@pytest.mark.parametrize("tile,low,kernel,reg", [(6, 2, 1, 0), (8, 3, 1, 0), (10, 4, 1, 0), (11, 5, 1, 0),
+                                                 (12, 5, 1, 0), (13, 6, 1, 0), (11, 5, 2, 3), (12, 5, 2, 4),
+                                                 (10, 4, 2, 3), (9, 3, 2, 3), (11, 6, 2, 4), (12, 6, 2, 4)])
+def test_fused_tile_shapes(backend, tile, low, kernel, reg):
     n = 14
     c = random_layer_circuit(n, 12, seed=tile * 7 + low, pairs_per_layer=7)
     c.swap(0, 13)
     c.swap(3, 9)
+    c.swap(1, 2)
     want = O.run_circuit(n, c.to_ops())
-    backend.set_plan_options(fuse=True, tile_qubits=tile, low_qubits=low)
+    backend.set_plan_options(fuse=True, tile_qubits=tile, low_qubits=low, kernel=kernel, reg_bits=reg)
     got = backend.run_circuit(StateVector.zero(n), c)
     backend.set_plan_options(fuse=True)
     close(got.amplitudes, want, 1e-12)
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_fused_batch_and_complex64_both_kernels(backend, kernel):
+    n, B = 13, 3
+    c = random_layer_circuit(n, 10, seed=kernel, pairs_per_layer=6)
+    rng = np.random.default_rng(kernel)
+    states = []
+    for _ in range(B):
+        a = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        states.append(StateVector(a / np.linalg.norm(a), n))
+    backend.set_plan_options(fuse=True, kernel=kernel)
+    out = backend.run_circuit_batch(states, c)
+    for s, r in zip(states, out):
+        o = O.State(n, s.amplitudes)
+        o.run(c.to_ops())
+        close(r.amplitudes, o.amplitudes(), 1e-12)
+    dev = backend.zero_state(n, precision="complex64")
+    dev.run_circuit(c)
+    want = O.run_circuit(n, c.to_ops())
+    got = dev.download().amplitudes
+    assert 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real < 1e-9
+    backend.set_plan_options(fuse=True)
 
 
 def test_qft_fused(backend):
