@@ -59,6 +59,56 @@ def cpu_baseline(circuit, qubits: int, sample_ops: int):
     }
 
 
+def vqe_bench(be, sets: int, shift_sets: int, precision: str, with_cpu: bool) -> dict:
+    """16-qubit hardware-efficient ansatz (8 layers of Ry, Rz on every qubit,
+    linear CNOT ladder, 256 parameters ~ U[-pi, pi]) with a Heisenberg + Z-field
+    chain: batched adjoint gradients for `sets` parameter sets against the
+    plain parameter-shift GPU form (2 circuits per parameter) on a subset."""
+    import numpy as np
+
+    from quantumforge_amd import gradients as G
+    from quantumforge_amd.workloads import hea_circuit, heisenberg_hamiltonian, uniform_parameters
+
+    nq, layers = 16, 8
+    pc = hea_circuit(nq, layers)
+    H = heisenberg_hamiltonian(nq, seed=16)
+    params = uniform_parameters(pc.parameter_count, sets, seed=4096)
+    P = pc.parameter_count
+    G.batch_adjoint_gradient(be, pc, nq, params[: min(sets, 64)], H, precision)  # warmup
+    t0 = time.perf_counter()
+    e, g = G.batch_adjoint_gradient(be, pc, nq, params, H, precision)
+    t_adj = time.perf_counter() - t0
+    log(f"vqe: adjoint {sets} sets x {P} params in {t_adj * 1e3:.1f} ms")
+    k = max(1, min(shift_sets, sets))
+    G.batch_shift_gradient(be, pc, nq, params[:1], H, precision)  # warmup
+    t0 = time.perf_counter()
+    es, gs = G.batch_shift_gradient(be, pc, nq, params[:k], H, precision)
+    t_shift = time.perf_counter() - t0
+    log(f"vqe: parameter shift {k} sets in {t_shift * 1e3:.1f} ms")
+    adj_gps = sets * P / t_adj
+    shift_gps = k * P / t_shift
+    res = {
+        "workload": f"{nq}q HEA {layers} layers ({P} params), Heisenberg+Z chain, {sets} sets, {precision}",
+        "adjoint_grads_per_s": round(adj_gps, 1),
+        "adjoint_ms": round(t_adj * 1e3, 2),
+        "shift_grads_per_s": round(shift_gps, 1),
+        "shift_sets_timed": k,
+        "ratio_vs_shift": round(adj_gps / shift_gps, 2),
+        "max_abs_adjoint_minus_shift": float(np.max(np.abs(g[:k] - gs))),
+        "max_abs_energy_diff": float(np.max(np.abs(e[:k] - es))),
+    }
+    if with_cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        gc = O.pc_adjoint_gradient(pc.to_pops(), P, 0, params[0], [], nq, H.to_terms())
+        t_cpu = time.perf_counter() - t0
+        res["cpu_adjoint_grads_per_s"] = round(P / t_cpu, 1)
+        res["cpu_sample"] = "1 set, oracle restatement of adjoint_gradient_with_inputs (store-every-state)"
+        res["max_abs_gpu_vs_cpu_adjoint"] = float(np.max(np.abs(g[0] - gc)))
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,8 +120,12 @@ def main() -> int:
     ap.add_argument("--precision", default="complex128", choices=["complex128", "complex64"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--low", type=int, default=0)
+    ap.add_argument("--kernel", type=int, default=0, help="0 default, 1 LDS op-by-op tile kernel, 2 register kernel")
+    ap.add_argument("--reg-bits", type=int, default=0)
     ap.add_argument("--cpu-sample-ops", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--vqe-sets", type=int, default=4096, help="parameter sets of the 16-qubit VQE adjoint (0 = skip)")
+    ap.add_argument("--vqe-shift-sets", type=int, default=4, help="sets timed with the parameter-shift GPU form")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -87,11 +141,14 @@ def main() -> int:
 
     n = args.qubits
     circuit = random_layer_circuit(n, args.depth, seed=args.seed)
-    plan = Plan(circuit, n, fuse=True, tile_qubits=args.tile, low_qubits=args.low)
+    plan = Plan(circuit, n, fuse=True, tile_qubits=args.tile, low_qubits=args.low, kernel=args.kernel,
+                reg_bits=args.reg_bits)
     stats = plan.stats()
     desc = plan.describe()
     tile_bits = desc["passes"][0]["tile_bits"]
-    log(f"plan: {stats['passes']} passes for {stats['ops_in']} ops (tile {tile_bits} qubits)")
+    reg_bits = desc["passes"][0].get("reg_bits", 0)
+    log(f"plan: {stats['passes']} passes, {stats['phases']} register phases for {stats['ops_in']} ops "
+        f"(tile {tile_bits} qubits, register bits {reg_bits})")
 
     be = HipBackend(0)
     state = be.zero_state(n, precision=args.precision)
@@ -144,7 +201,7 @@ def main() -> int:
             "workload": f"random layered circuit, {n} qubits, depth {args.depth}, "
                         f"{'complex128' if amp_bytes == 16 else 'complex64'}, fused tile passes",
             "qubits": n, "depth": args.depth, "ops": stats["ops_in"], "passes_per_step": stats["passes"],
-            "tile_qubits": tile_bits,
+            "tile_qubits": tile_bits, "register_bits": reg_bits, "phases_per_step": stats["phases"],
         },
         "roofline": {
             "bound": "hbm",
@@ -153,13 +210,15 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_tile",
+            "kernel": "k_tile_reg" if reg_bits else "k_tile",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
         "cpu_baseline": None,
     }
     del state
+    if args.vqe_sets > 0:
+        out["vqe_adjoint"] = vqe_bench(be, args.vqe_sets, args.vqe_shift_sets, args.precision, not args.no_cpu)
     if not args.no_cpu:
         log("timing the CPU baseline (oracle restatement of the reference CPU path)")
         out["cpu_baseline"] = cpu_baseline(circuit, n, args.cpu_sample_ops)
