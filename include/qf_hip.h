@@ -98,7 +98,10 @@ typedef struct { float re, im; } qf_c64;
  * kind: 0 single-qubit {gate, a=target, theta for Rx/Ry/Rz},
  *       1 CNOT {a=control, b=target}, 2 CZ {a, b}, 3 SWAP {a, b},
  *       4 channel, 5 measure, 6 classically conditioned gate
- *       (4-6 are rejected with the reference's errors, src/lib.rs:2793-2798). */
+ *       (4-6 are rejected with the reference's errors, src/lib.rs:2793-2798),
+ *       7 exact controlled phase diag(1,1,1,e^{i theta}) {a=control, b=target}
+ *       (no CircuitOp counterpart: the crate lowers it to Rz/CNOT,
+ *       src/algorithms.rs:139-145, which equals it times e^{-i theta/4}). */
 typedef struct {
     int32_t kind;
     int32_t gate;

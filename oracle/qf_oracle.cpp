@@ -189,6 +189,16 @@ int apply_swap(State& st, uint32_t a, uint32_t b) {
     return OK;
 }
 
+// Exact controlled phase diag(1,1,1,e^{i phi}) (no reference counterpart: the
+// crate lowers it to Rz/CNOT, src/algorithms.rs:139-145, = this x e^{-i phi/4}).
+int apply_cp(State& st, uint32_t a, uint32_t b, double phi) {
+    int e = check_pair(st, a, b);
+    if (e) return e;
+    const C ph{std::cos(phi), std::sin(phi)};
+    for_free(st, a, b, (size_t(1) << a) | (size_t(1) << b), [&](size_t idx) { st.amps[idx] = cmul(ph, st.amps[idx]); });
+    return OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -285,6 +295,7 @@ int or_run(void* s, const or_op* ops, size_t nops) {
             case 3: e = apply_swap(st, op.a, op.b); break;
             case 4: return REQUIRES_DENSITY_MATRIX;
             case 5: case 6: return REQUIRES_SHOT_EXECUTION;
+            case 7: e = apply_cp(st, op.a, op.b, op.theta); break;
             default: return INVALID_ARGUMENT;
         }
         if (e) return e;

@@ -107,6 +107,12 @@ class Circuit:
     def swap(self, first: int, second: int) -> None:
         self._ops.append((3, 0, int(first), int(second), 0.0))
 
+    def cp(self, control: int, target: int, phi: float) -> None:
+        """Exact controlled phase diag(1, 1, 1, e^{i phi}) (one diagonal op; the
+        crate's Rz/CNOT lowering, src/algorithms.rs:139-145, equals it up to
+        the global phase e^{-i phi/4})."""
+        self._ops.append((7, 0, int(control), int(target), float(phi)))
+
     def channel(self, target: int, channel: object = None) -> None:
         """A noise channel: kept in the list; running it reports OperationRequiresDensityMatrix."""
         self._ops.append((4, 0, int(target), 0, 0.0))
@@ -129,7 +135,7 @@ class Circuit:
         """Highest qubit touched + 1 (Circuit::num_qubits, src/lib.rs:2760-2784)."""
         top = -1
         for kind, _g, a, b, _t in self._ops:
-            top = max(top, a, b if kind in (1, 2, 3) else -1)
+            top = max(top, a, b if kind in (1, 2, 3, 7) else -1)
         return top + 1
 
     def to_ops(self) -> np.ndarray:

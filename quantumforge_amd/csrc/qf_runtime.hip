@@ -263,6 +263,17 @@ qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector
                 g.b = op.b;
                 break;
             }
+            case 7: {
+                qf_status st = check_pair(qubits, op.a, op.b);
+                if (st) return st;
+                if (!std::isfinite(op.theta)) return fail(QF_NON_FINITE_ANGLE, "rotation angle must be finite");
+                g.kind = 4;
+                g.a = op.a;
+                g.b = op.b;
+                g.m[0] = g.m[1] = g.m[2] = cd{1, 0};
+                g.m[3] = cd{std::cos(op.theta), std::sin(op.theta)};
+                break;
+            }
             case 4: return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
             case 5: case 6:
                 return fail(QF_REQUIRES_SHOT_EXECUTION,

@@ -89,30 +89,30 @@ def random_layer_circuit(qubits: int, depth: int, seed: int, pairs_per_layer: in
 # ---------------------------------------------------------------------------
 # QFT-like layers (33/34q workload)
 # ---------------------------------------------------------------------------
-def qft_like_circuit(qubits: int, layers: int, seed: int, max_distance: int = 8) -> Tuple[Circuit, list]:
+def qft_like_circuit(qubits: int, layers: int, seed: int, max_distance: int = 8) -> Tuple[Circuit, Circuit]:
     """Per layer: H on every qubit; controlled phase CP(pi/2^d) on every pair
     at distance 1 <= d <= max_distance (control = higher qubit, as
     qft_circuit, src/algorithms.rs:278-284); RZ(2*pi*u) on every qubit.
-    Returns the circuit with CP lowered the reference's way
-    (controlled_phase, src/algorithms.rs:139-145) and the list of exact CP
-    gates (control, target, phi) in order for engines with a native CP."""
+    Returns (lowered, native): CP lowered the reference's way
+    (controlled_phase, src/algorithms.rs:139-145) and the same circuit with
+    exact CP ops; they agree up to the global phase e^{-i sum(phi)/4}."""
     rng = SmallRng(seed)
     c = Circuit()
-    native: list = []
+    native = Circuit()
     for _ in range(layers):
         for q in range(qubits):
             c.push(Gate.H, q)
-            native.append(("h", q))
+            native.push(Gate.H, q)
         for d in range(1, max_distance + 1):
             for t in range(qubits - d):
                 ctl = t + d
                 phi = math.pi / (1 << d)
                 controlled_phase(c, ctl, t, phi)
-                native.append(("cp", ctl, t, phi))
+                native.cp(ctl, t, phi)
         for q in range(qubits):
             th = 2.0 * math.pi * rng.next_f64()
             c.push(Gate.Rz(th), q)
-            native.append(("rz", q, th))
+            native.push(Gate.Rz(th), q)
     return c, native
 
 

@@ -225,3 +225,26 @@ def test_fast_variant_matches_restatement():
     c = random_layer_circuit(n, depth=8, seed=11, pairs_per_layer=5)
     ops = c.to_ops()
     close(O.fast_run(n, ops), O.run_circuit(n, ops), 0.0)
+
+
+def test_controlled_phase_matches_reference_lowering_up_to_global_phase():
+    # controlled_phase (src/algorithms.rs:139-145) = CP(phi) * e^{-i phi/4}; CP itself vs its definition.
+    from quantumforge_amd.workloads import controlled_phase, qft_like_circuit
+    n = 4
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=16) + 1j * rng.normal(size=16)
+    psi /= np.linalg.norm(psi)
+    for ctl, tgt, phi in [(3, 0, 0.7), (1, 2, -2.1), (0, 3, math.pi / 4)]:
+        lowered = Circuit()
+        controlled_phase(lowered, ctl, tgt, phi)
+        native = Circuit()
+        native.cp(ctl, tgt, phi)
+        a = O.State(n, psi)
+        a.run(lowered.to_ops())
+        b = O.State(n, psi)
+        b.run(native.to_ops())
+        close(a.amplitudes(), np.exp(-0.25j * phi) * b.amplitudes(), 1e-14)
+        close(b.amplitudes(), NP.cp(ctl, tgt, phi, n) @ psi, 1e-15)
+    low, nat = qft_like_circuit(6, 2, seed=5, max_distance=3)
+    total_phi = sum(op[4] for op in nat.operations if op[0] == 7)
+    close(O.run_circuit(6, low.to_ops()), np.exp(-0.25j * total_phi) * O.run_circuit(6, nat.to_ops()), 1e-13)
