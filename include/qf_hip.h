@@ -315,6 +315,40 @@ qf_status qf_state_run_parameterized(qf_state* state, const qf_pop* ops, size_t 
                                      uint32_t ninputs, const double* inputs, size_t inputs_len, const double* params,
                                      size_t params_len);
 
+/* One circuit of an uneven batch: its own ops, width, parameters and observable
+ * (no inputs).  No reference counterpart: the reference batches equal widths
+ * only (IncompatibleBatch, src/cuda.rs:875-877). */
+typedef struct {
+    const qf_pop* ops;
+    size_t nops;
+    uint32_t nparams;
+    uint32_t qubits;
+    const double* params;          /* [nparams] */
+    const qf_pauli_term* obs;
+    size_t nterms;
+} qf_uneven_instance;
+
+typedef struct {
+    uint32_t waves;                /* device-memory waves the instances were packed into */
+    uint32_t graph_nodes;          /* kernel launches in the captured per-wave graph */
+    uint32_t graph_replays;        /* replays of that graph (one per wave) */
+    uint32_t grouped_instances;    /* instances run through the grouped graph */
+    uint32_t tile_qubits, low_qubits, reg_bits;  /* autotuned forward tile shape */
+    uint32_t reserved;
+    double tune_ms;                /* tuned forward time of the widest instance */
+} qf_uneven_stats;
+
+/* Energies <obs_i> and adjoint gradients of `count` parameterized circuits of
+ * different widths.  grads is the concatenation of every instance's [nparams].
+ * Each instance is validated like qf_gradient.  stats may be NULL. */
+qf_status qf_uneven_adjoint_gradient(qf_backend* backend, const qf_uneven_instance* instances, uint32_t count,
+                                     int32_t precision, double* energies, double* grads, qf_uneven_stats* stats);
+
+/* Times the candidate register-kernel tile shapes of a circuit on the device
+ * and returns the fastest as plan options (cached per backend process). */
+qf_status qf_autotune(qf_backend* backend, const qf_op* ops, size_t count, uint32_t qubits, int32_t precision,
+                      qf_plan_options* best, double* best_ms);
+
 /* ------------------------------------------------------------------------ */
 /* Planning (no reference counterpart; see qf_plan_options)                  */
 /* ------------------------------------------------------------------------ */

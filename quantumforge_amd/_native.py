@@ -46,6 +46,17 @@ class GradOptions(C.Structure):
                 ("shots", C.c_uint64), ("tile_qubits", C.c_uint32), ("chunk", C.c_uint32)]
 
 
+class UnevenInstance(C.Structure):
+    _fields_ = [("ops", C.c_void_p), ("nops", C.c_size_t), ("nparams", C.c_uint32), ("qubits", C.c_uint32),
+                ("params", C.c_void_p), ("obs", C.c_void_p), ("nterms", C.c_size_t)]
+
+
+class UnevenStats(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("graph_nodes", C.c_uint32), ("graph_replays", C.c_uint32),
+                ("grouped_instances", C.c_uint32), ("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32),
+                ("reg_bits", C.c_uint32), ("reserved", C.c_uint32), ("tune_ms", C.c_double)]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 u32, i32, u64, dbl, sz = C.c_uint32, C.c_int32, C.c_uint64, C.c_double, C.c_size_t
@@ -99,6 +110,8 @@ SIGNATURES = {
     "qf_batch_gradient": (None, [P, P, sz, u32, u32, P, u32, P, u32, P, sz, C.POINTER(GradOptions), P, P]),
     "qf_batch_expectation": (None, [P, P, sz, u32, u32, P, u32, P, u32, P, sz, C.POINTER(GradOptions), P]),
     "qf_state_run_parameterized": (None, [P, P, sz, u32, u32, P, sz, P, sz]),
+    "qf_uneven_adjoint_gradient": (None, [P, C.POINTER(UnevenInstance), u32, i32, P, P, C.POINTER(UnevenStats)]),
+    "qf_autotune": (None, [P, P, sz, u32, i32, C.POINTER(PlanOptions), C.POINTER(dbl)]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqfhip.so")
 ARCH = os.environ.get("QF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_plan.cpp"]
+SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_plan.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

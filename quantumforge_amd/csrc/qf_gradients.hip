@@ -37,116 +37,9 @@ using namespace qfrt;
 struct qf_backend;
 struct qf_state;
 
-namespace {
+namespace qfrt {
 
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-
-// lambda = O psi (Pauli sum, terms sorted by x mask) and the energy partials
-// conj(psi_i) lambda_i, FP64 compensated, one row per instance (grid.y).
-template <class R>
-__global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
-                                                              uint32_t n, const qf::PauliDev* __restrict__ terms,
-                                                              uint32_t nterms, qf::dd* __restrict__ partial) {
-    const uint64_t dim = uint64_t(1) << n;
-    const uint64_t inst = blockIdx.y;
-    const cx<R>* __restrict__ p = psi + inst * dim;
-    cx<R>* __restrict__ l = lam ? lam + inst * dim : nullptr;
-    qf::dd er{0, 0}, ei{0, 0};
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
-        double ar = 0.0, ai = 0.0;
-        uint64_t cur = ~uint64_t(0);
-        double pr = 0.0, pi = 0.0;
-        for (uint32_t t = 0; t < nterms; ++t) {
-            const qf::PauliDev& T = terms[t];
-            if (T.x != cur) {
-                cur = T.x;
-                const cx<R> v = p[i ^ cur];
-                pr = v.re;
-                pi = v.im;
-            }
-            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
-            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
-            const double cr = sg * T.cr, ci = sg * T.ci;
-            ar += cr * pr - ci * pi;
-            ai += cr * pi + ci * pr;
-        }
-        if (l) l[i] = cx<R>{R(ar), R(ai)};
-        const cx<R> v = p[i];
-        const double vr = v.re, vi = v.im;
-        qf::neumaier(er, vr * ar + vi * ai);  // Re(conj(psi) lambda)
-        qf::neumaier(ei, vr * ai - vi * ar);  // Im(conj(psi) lambda)
-    }
-    qf::block_reduce(er, ei);
-    if (threadIdx.x == 0) {
-        partial[2 * (inst * gridDim.x + blockIdx.x)] = er;
-        partial[2 * (inst * gridDim.x + blockIdx.x) + 1] = ei;
-    }
-}
-
-// grads[inst][p] = sum over the occurrences of p, in reverse circuit order as
-// the reference's reverse loop adds them, of the per-block overlaps summed
-// in block order (Neumaier).
-__global__ __launch_bounds__(qf::kThreads) void k_grad_finish(const double* __restrict__ partial, uint32_t nblk,
-                                                              uint32_t nslots, const uint32_t* __restrict__ pstart,
-                                                              const uint32_t* __restrict__ pslots, uint32_t nparams,
-                                                              uint32_t count, double* __restrict__ grads) {
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= uint64_t(count) * nparams) return;
-    const uint64_t inst = t / nparams;
-    const uint32_t p = uint32_t(t % nparams);
-    double g = 0.0;
-    for (uint32_t k = pstart[p]; k < pstart[p + 1]; ++k) {
-        const uint32_t slot = pslots[k];
-        qf::dd acc{0, 0};
-        for (uint32_t b = 0; b < nblk; ++b) qf::neumaier(acc, partial[(inst * nblk + b) * nslots + slot]);
-        g += acc.s + acc.c;
-    }
-    grads[t] = g;
-}
-
-// Im <lambda| G |psi> for G = X/Y/Z on qubit q (backprop's per-gate overlap).
-template <class R>
-__global__ __launch_bounds__(qf::kThreads) void k_gen_overlap(const cx<R>* __restrict__ lam,
-                                                              const cx<R>* __restrict__ psi, uint64_t dim, int axis,
-                                                              uint32_t q, qf::dd* __restrict__ partial) {
-    qf::dd acc{0, 0}, unused{0, 0};
-    const uint64_t m = uint64_t(1) << q;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
-        const cx<R> l = lam[i];
-        const cx<R> p = axis == 2 ? psi[i] : psi[i ^ m];
-        const double lr = l.re, li = l.im, pr = p.re, pi = p.im;
-        const double re = lr * pr + li * pi, im = lr * pi - li * pr;
-        double v;
-        if (axis == 0)
-            v = im;
-        else if (axis == 1)
-            v = (i & m) ? re : -re;
-        else
-            v = (i & m) ? -im : im;
-        qf::neumaier(acc, v);
-    }
-    qf::block_reduce(acc, unused);
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = acc;
-        partial[2 * blockIdx.x + 1] = qf::dd{0, 0};
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Host side
-// ---------------------------------------------------------------------------
-struct PCirc {
-    const qf_pop* ops;
-    size_t nops;
-    uint32_t nparams, ninputs;
-    std::vector<int32_t> slot_param;  // rotation occurrence -> parameter
-    std::vector<size_t> slot_op;      // rotation occurrence -> op index
-};
-
+// Host-side helpers shared with the uneven-batch engine.
 qf_status qubit_error(uint32_t q, uint32_t qubits) {
     return fail(QF_QUBIT_OUT_OF_RANGE,
                 "qubit index " + std::to_string(q) + " is out of range for " + std::to_string(qubits) + " qubits");
@@ -280,6 +173,115 @@ void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double 
     }
 }
 
+
+// Observable terms sorted by x mask (k_pauli_apply reloads psi[i ^ x] only when x changes).
+std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t) {
+    std::vector<qf::PauliDev> s(t);
+    std::stable_sort(s.begin(), s.end(), [](const qf::PauliDev& a, const qf::PauliDev& b) { return a.x < b.x; });
+    return s;
+}
+
+}  // namespace qfrt
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// lambda = O psi (Pauli sum, terms sorted by x mask) and the energy partials
+// conj(psi_i) lambda_i, FP64 compensated, one row per instance (grid.y).
+template <class R>
+__global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                              uint32_t n, const qf::PauliDev* __restrict__ terms,
+                                                              uint32_t nterms, qf::dd* __restrict__ partial) {
+    const uint64_t dim = uint64_t(1) << n;
+    const uint64_t inst = blockIdx.y;
+    const cx<R>* __restrict__ p = psi + inst * dim;
+    cx<R>* __restrict__ l = lam ? lam + inst * dim : nullptr;
+    qf::dd er{0, 0}, ei{0, 0};
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        double ar = 0.0, ai = 0.0;
+        uint64_t cur = ~uint64_t(0);
+        double pr = 0.0, pi = 0.0;
+        for (uint32_t t = 0; t < nterms; ++t) {
+            const qf::PauliDev& T = terms[t];
+            if (T.x != cur) {
+                cur = T.x;
+                const cx<R> v = p[i ^ cur];
+                pr = v.re;
+                pi = v.im;
+            }
+            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
+            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
+            const double cr = sg * T.cr, ci = sg * T.ci;
+            ar += cr * pr - ci * pi;
+            ai += cr * pi + ci * pr;
+        }
+        if (l) l[i] = cx<R>{R(ar), R(ai)};
+        const cx<R> v = p[i];
+        const double vr = v.re, vi = v.im;
+        qf::neumaier(er, vr * ar + vi * ai);  // Re(conj(psi) lambda)
+        qf::neumaier(ei, vr * ai - vi * ar);  // Im(conj(psi) lambda)
+    }
+    qf::block_reduce(er, ei);
+    if (threadIdx.x == 0) {
+        partial[2 * (inst * gridDim.x + blockIdx.x)] = er;
+        partial[2 * (inst * gridDim.x + blockIdx.x) + 1] = ei;
+    }
+}
+
+// grads[inst][p] = sum over the occurrences of p, in reverse circuit order as
+// the reference's reverse loop adds them, of the per-block overlaps summed
+// in block order (Neumaier).
+__global__ __launch_bounds__(qf::kThreads) void k_grad_finish(const double* __restrict__ partial, uint32_t nblk,
+                                                              uint32_t nslots, const uint32_t* __restrict__ pstart,
+                                                              const uint32_t* __restrict__ pslots, uint32_t nparams,
+                                                              uint32_t count, double* __restrict__ grads) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= uint64_t(count) * nparams) return;
+    const uint64_t inst = t / nparams;
+    const uint32_t p = uint32_t(t % nparams);
+    double g = 0.0;
+    for (uint32_t k = pstart[p]; k < pstart[p + 1]; ++k) {
+        const uint32_t slot = pslots[k];
+        qf::dd acc{0, 0};
+        for (uint32_t b = 0; b < nblk; ++b) qf::neumaier(acc, partial[(inst * nblk + b) * nslots + slot]);
+        g += acc.s + acc.c;
+    }
+    grads[t] = g;
+}
+
+// Im <lambda| G |psi> for G = X/Y/Z on qubit q (backprop's per-gate overlap).
+template <class R>
+__global__ __launch_bounds__(qf::kThreads) void k_gen_overlap(const cx<R>* __restrict__ lam,
+                                                              const cx<R>* __restrict__ psi, uint64_t dim, int axis,
+                                                              uint32_t q, qf::dd* __restrict__ partial) {
+    qf::dd acc{0, 0}, unused{0, 0};
+    const uint64_t m = uint64_t(1) << q;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const cx<R> l = lam[i];
+        const cx<R> p = axis == 2 ? psi[i] : psi[i ^ m];
+        const double lr = l.re, li = l.im, pr = p.re, pi = p.im;
+        const double re = lr * pr + li * pi, im = lr * pi - li * pr;
+        double v;
+        if (axis == 0)
+            v = im;
+        else if (axis == 1)
+            v = (i & m) ? re : -re;
+        else
+            v = (i & m) ? -im : im;
+        qf::neumaier(acc, v);
+    }
+    qf::block_reduce(acc, unused);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = acc;
+        partial[2 * blockIdx.x + 1] = qf::dd{0, 0};
+    }
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() {
@@ -300,12 +302,6 @@ void fill_zero_states(hipStream_t st, void* d, uint64_t total, uint32_t qubits) 
                        static_cast<cx<R>*>(d), total, qubits);
 }
 
-// Observable terms sorted by x mask (k_pauli_apply reloads psi[i ^ x] only when x changes).
-std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t) {
-    std::vector<qf::PauliDev> s(t);
-    std::stable_sort(s.begin(), s.end(), [](const qf::PauliDev& a, const qf::PauliDev& b) { return a.x < b.x; });
-    return s;
-}
 
 struct GradRun {
     qf_backend* be;

@@ -258,19 +258,16 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
     }
 }
 
-// ops[0] is always a TOP_PHASE header.  NS = 1: s0 is the state; NS = 2: s0 =
-// psi, s1 = lambda, partial as k_tile_adjoint.
+// One workgroup's tile: block index b (instance = b >> log_blocks).  ops[0]
+// is always a TOP_PHASE header.  NS = 1: s0 is the state; NS = 2: s0 = psi,
+// s1 = lambda, partial as k_tile_adjoint.  `xb` is the LDS exchange buffer
+// (NS x 2^M amplitudes), `red` 4 doubles of LDS.
 template <class R, int RB, int NS>
-__global__ __launch_bounds__(kThreads) void k_tile_reg(cx<R>* __restrict__ s0, cx<R>* __restrict__ s1, TileArgs args,
-                                                       const TileOp* __restrict__ ops, const double* __restrict__ cs,
-                                                       double* __restrict__ partial) {
+__device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __restrict__ s1, const TileArgs& args,
+                                              const TileOp* __restrict__ ops, const double* __restrict__ cs,
+                                              double* __restrict__ partial, uint64_t b, cx<R>* xb, double* red) {
     constexpr int NR = 1 << RB;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[kThreads / 64];
-    cx<R>* xb = reinterpret_cast<cx<R>*>(smem_raw);
     const uint32_t T = 1u << args.M;
-
-    const uint64_t b = uint64_t(blockIdx.x) + uint64_t(blockIdx.y) * gridDim.x;
     const uint64_t inst = b >> args.log_blocks;
     const uint64_t bl = b & ((uint64_t(1) << args.log_blocks) - 1);
     uint64_t base = 0;
@@ -328,6 +325,16 @@ __global__ __launch_bounds__(kThreads) void k_tile_reg(cx<R>* __restrict__ s0, c
         s0[mp.gt | mp.gk[k]] = a[0][k];
         if constexpr (NS == 2) s1[mp.gt | mp.gk[k]] = a[1][k];
     }
+}
+
+template <class R, int RB, int NS>
+__global__ __launch_bounds__(kThreads) void k_tile_reg(cx<R>* __restrict__ s0, cx<R>* __restrict__ s1, TileArgs args,
+                                                       const TileOp* __restrict__ ops, const double* __restrict__ cs,
+                                                       double* __restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[kThreads / 64];
+    const uint64_t b = uint64_t(blockIdx.x) + uint64_t(blockIdx.y) * gridDim.x;
+    tile_reg_body<R, RB, NS>(s0, s1, args, ops, cs, partial, b, reinterpret_cast<cx<R>*>(smem_raw), red);
 }
 
 }  // namespace qf

@@ -50,6 +50,22 @@ qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, u
 qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, uint64_t dim, uint32_t count, int kind,
                  uint32_t q, const std::vector<qf::PauliDev>& terms, std::vector<double>& out);
 qf_backend* state_backend(const qf_state* st);
+
+// A parameterized circuit as the gradient code sees it (qf_pop records).
+struct PCirc {
+    const qf_pop* ops = nullptr;
+    size_t nops = 0;
+    uint32_t nparams = 0, ninputs = 0;
+    std::vector<int32_t> slot_param;  // rotation occurrence -> parameter
+    std::vector<size_t> slot_op;      // rotation occurrence -> op index
+};
+qf_status validate_values(const PCirc& pc, const double* inputs, size_t inputs_len, const double* params,
+                          size_t params_len);
+qf_status validate_ops(PCirc& pc, uint32_t qubits);
+std::vector<qf::GateIn> param_gates(const PCirc& pc, const double* inputs, bool reverse);
+void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row);
+std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t);
+void adjoint_gate_id(int gate, double theta, int* g, double* th);
 qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits, std::vector<qf::PauliDev>& out);
 
 }  // namespace qfrt

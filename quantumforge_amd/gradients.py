@@ -221,3 +221,44 @@ def run_parameterized(backend, circuit: ParameterizedCircuit, parameters, qubits
     check(lib().qf_state_run_parameterized(st._h, ptr(ops), len(ops), circuit.parameter_count, circuit.input_count,
                                            ptr(x), len(x), ptr(p), len(p)))
     return st
+
+
+def uneven_adjoint_gradient(backend, instances: Sequence[Tuple[ParameterizedCircuit, int, Sequence[float], object]],
+                            precision: str = "complex128"):
+    """Energies and adjoint gradients of circuits of different widths in one
+    device schedule (grouped launches replayed from one captured hipGraph per
+    memory wave; tile shape from the autotuner).
+
+    instances: (circuit, qubits, parameters, observable) tuples.
+    Returns (energies [count], [gradient arrays], stats dict)."""
+    keep = []
+    recs = (N.UnevenInstance * len(instances))()
+    for k, (pc, q, params, obs) in enumerate(instances):
+        ops = pc.to_pops()
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        t = _terms(obs, q)
+        keep.append((ops, p, t))
+        recs[k] = N.UnevenInstance(ops.ctypes.data if len(ops) else None, len(ops), pc.parameter_count, q,
+                                   p.ctypes.data if p.size else None, t.ctypes.data if len(t) else None, len(t))
+    energies = np.zeros(len(instances), dtype=np.float64)
+    total = sum(pc.parameter_count for pc, _q, _p, _o in instances)
+    grads = np.zeros(max(total, 1), dtype=np.float64)
+    stats = N.UnevenStats()
+    prec = 0 if precision in ("complex128", "c128") else 1
+    check(lib().qf_uneven_adjoint_gradient(backend._h, recs, len(instances), prec, ptr(energies), ptr(grads),
+                                           C.byref(stats)))
+    out, off = [], 0
+    for pc, _q, _p, _o in instances:
+        out.append(grads[off: off + pc.parameter_count].copy())
+        off += pc.parameter_count
+    return energies, out, {f: getattr(stats, f) for f, _ in N.UnevenStats._fields_}
+
+
+def autotune(backend, circuit, qubits: int, precision: str = "complex128") -> Tuple[dict, float]:
+    """Fastest register-kernel tile shape for a circuit (timed on the device)."""
+    ops = circuit.to_ops()
+    best = N.PlanOptions()
+    ms = C.c_double()
+    prec = 0 if precision in ("complex128", "c128") else 1
+    check(lib().qf_autotune(backend._h, ptr(ops), len(ops), qubits, prec, C.byref(best), C.byref(ms)))
+    return {f: int(getattr(best, f)) for f, _ in N.PlanOptions._fields_}, ms.value

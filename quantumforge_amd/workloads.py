@@ -327,3 +327,24 @@ def maxcut_cost_hamiltonian(problem: MaxCutProblem) -> PauliSum:
         word[u] = word[v] = "Z"
         H.add(w, "".join(word))
     return H
+
+
+def qaoa_uneven_instances(count: int, seed: int, p: int = 6, qubit_choices=(18, 20, 22, 24, 26),
+                          weight_range=(0.5, 1.0)):
+    """The uneven QAOA MaxCut batch: instance i draws its width uniformly from
+    `qubit_choices` (even widths: 3-regular graphs need an even vertex count,
+    src/mps_qaoa.rs:59-62), a random 3-regular graph and U[0.5, 1.0) weights
+    (with_random_weights), and gamma, beta ~ U[0, pi] per layer.  All draws
+    come from SmallRng(seed) (graph and weight seeds are drawn from it too).
+    Returns a list of dicts with problem, gammas, betas."""
+    rng = SmallRng(seed)
+    out = []
+    for _ in range(count):
+        n = qubit_choices[rng.next_u64() % len(qubit_choices)]
+        gseed = rng.next_u64()
+        wseed = rng.next_u64()
+        prob = MaxCutProblem.random_3_regular(n, gseed).with_random_weights(wseed, *weight_range)
+        gammas = [math.pi * rng.next_f64() for _ in range(p)]
+        betas = [math.pi * rng.next_f64() for _ in range(p)]
+        out.append({"problem": prob, "gammas": gammas, "betas": betas, "p": p})
+    return out

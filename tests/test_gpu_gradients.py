@@ -188,3 +188,37 @@ def test_torch_autograd_vqe_energy(backend):
         assert abs(e[s].item() - O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, p, [], n, t)) < 1e-12
         want = O.pc_shift_gradient(pc.to_pops(), pc.parameter_count, 0, p, [], n, t) * w[s].item()
         assert np.max(np.abs(params.grad.numpy()[s] - want)) < 1e-10
+
+
+def test_uneven_qaoa_batch_vs_oracle(backend):
+    from quantumforge_amd.workloads import qaoa_occurrence_angles, qaoa_parameterized, qaoa_uneven_instances
+    specs = qaoa_uneven_instances(7, seed=11, p=2, qubit_choices=(8, 12, 14, 16))
+    instances, meta = [], []
+    for s in specs:
+        prob = s["problem"]
+        pc = qaoa_parameterized(prob.num_qubits, prob.edges, s["p"])
+        ang = qaoa_occurrence_angles(prob.edges, prob.num_qubits, s["p"], s["gammas"], s["betas"])
+        instances.append((pc, prob.num_qubits, ang, maxcut_cost_hamiltonian(prob)))
+    e, g, stats = G.uneven_adjoint_gradient(backend, instances)
+    assert stats["graph_replays"] == stats["waves"] >= 1
+    assert stats["grouped_instances"] + sum(1 for _, q, _, _ in instances if q < stats["tile_qubits"]) == 7
+    for k, (pc, q, ang, H) in enumerate(instances):
+        t = H.to_terms()
+        want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+        want_g = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+        assert abs(e[k] - want_e) < 1e-11, k
+        assert np.max(np.abs(g[k] - want_g)) < 1e-10, k
+
+
+def test_uneven_heterogeneous_circuits(backend):
+    # different circuit families and widths in one uneven call
+    insts = []
+    for n, layers in ((13, 2), (15, 1), (12, 3)):
+        pc = hea_circuit(n, layers)
+        H = heisenberg_hamiltonian(n, seed=n)
+        insts.append((pc, n, uniform_parameters(pc.parameter_count, 1, seed=n)[0], H))
+    e, g, stats = G.uneven_adjoint_gradient(backend, insts)
+    for k, (pc, n, p, H) in enumerate(insts):
+        eb, gb = G.batch_adjoint_gradient(backend, pc, n, p[None, :], H)
+        assert abs(e[k] - eb[0]) < 1e-12
+        assert np.max(np.abs(g[k] - gb[0])) < 1e-11
