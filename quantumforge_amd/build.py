@@ -32,6 +32,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast-honor-pragmas",
+    # Keep LLVM from sinking the arms of the register-bit switches into one
+    # block with computed array offsets: that turns the per-thread amplitude
+    # registers of the tile kernels into scratch memory.
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-I", os.path.join(ROOT, "include"),
 ]
 

@@ -272,9 +272,12 @@ __device__ __forceinline__ void tile_apply(cx<R>* __restrict__ tile, const TileO
             x = ((x ^ l) << 1) | l;
             const uint32_t idx[4] = {x, x | (1u << a), x | (1u << c), x | (1u << a) | (1u << c)};
             cx<R> v[4];
+#pragma unroll
             for (int r = 0; r < 4; ++r) v[r] = tile[idx[r]];
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
                 cx<R> acc{R(0), R(0)};
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const cx<R> t = cmul_d(op.m, r * 4 + k, v[k]);
                     acc.re += t.re;

@@ -568,6 +568,9 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     if (opt.kernel == 2 && !regk) return "register kernel needs tile_qubits <= qubits and 6 <= tile - reg_bits <= 8";
     if (M > qubits) M = qubits;
     if (low > M) low = M;
+    // A tile smaller than the state must leave room for the two qubits of a
+    // gate outside the contiguous low block.
+    if (M < qubits && low + 2 > M) low = M >= 2 ? M - 2 : 0;
 
     Scheduler sch(nodes, qubits, regk ? 1 : 0);
     std::vector<uint32_t> all_q(qubits);

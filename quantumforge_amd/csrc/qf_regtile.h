@@ -66,6 +66,16 @@ __device__ __forceinline__ void decode_map(uint32_t packed, const TileArgs& a, u
     m.gt = base + tile_to_global(m.jt, a);
 }
 
+// LDS slot of tile element j during a phase exchange: XOR-folds the higher
+// index bits into the low bits that select the LDS bank row (256 B), so lanes
+// whose tile indices differ only above the register bits do not collide.
+template <class R>
+__device__ __forceinline__ uint32_t lds_slot(uint32_t j) {
+    constexpr uint32_t B = sizeof(cx<R>) == 16 ? 4 : 5;  // log2(elements per 256 B)
+    constexpr uint32_t MASK = (1u << B) - 1;
+    return j ^ (((j >> B) ^ (j >> (2 * B))) & MASK);
+}
+
 template <class R>
 __device__ __forceinline__ cx<R> cm(R mr, R mi, cx<R> v) {
     return cx<R>{mr * v.re - mi * v.im, mr * v.im + mi * v.re};
@@ -302,13 +312,13 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
-                for (int k = 0; k < NR; ++k) xb[s * T + (mp.jt | mp.jk[k])] = a[s][k];
+                for (int k = 0; k < NR; ++k) xb[s * T + lds_slot<R>(mp.jt | mp.jk[k])] = a[s][k];
             decode_map<RB>(uint32_t(op.q0), args, base, mp);
             __syncthreads();
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
-                for (int k = 0; k < NR; ++k) a[s][k] = xb[s * T + (mp.jt | mp.jk[k])];
+                for (int k = 0; k < NR; ++k) a[s][k] = xb[s * T + lds_slot<R>(mp.jt | mp.jk[k])];
             continue;
         }
         if constexpr (NS == 2) {
