@@ -58,6 +58,12 @@ enum {
 // Gate ids (qf_gate in include/qf_hip.h) in the order of `enum Gate`, src/lib.rs:2239-2265.
 enum { G_I, G_X, G_Y, G_Z, G_H, G_S, G_SDG, G_T, G_TDG, G_RX, G_RY, G_RZ };
 
+// cos and sin as two separate libm calls, like the reference's f64::cos and
+// f64::sin.  Kept out of line so that the compiler cannot merge a cos/sin pair
+// of one argument into sincos, whose last bit can differ from cos's.
+__attribute__((noinline)) static double lcos(double x) { return std::cos(x); }
+__attribute__((noinline)) static double lsin(double x) { return std::sin(x); }
+
 // Gate::matrix, src/lib.rs:2267-2333.  (m00, m01, m10, m11) row major.
 void gate_matrix(int gate, double theta, C m[4]) {
     const C z{0, 0}, o{1, 0}, i{0, 1};
@@ -73,19 +79,19 @@ void gate_matrix(int gate, double theta, C m[4]) {
         case G_T: m[0] = o; m[1] = z; m[2] = z; m[3] = C{s, s}; break;
         case G_TDG: m[0] = o; m[1] = z; m[2] = z; m[3] = C{s, -s}; break;
         case G_RX: {
-            double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            double c = lcos(theta * 0.5), si = lsin(theta * 0.5);
             m[0] = C{c, 0}; m[1] = C{0, -si}; m[2] = C{0, -si}; m[3] = C{c, 0};
             break;
         }
         case G_RY: {
-            double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            double c = lcos(theta * 0.5), si = lsin(theta * 0.5);
             m[0] = C{c, 0}; m[1] = C{-si, 0}; m[2] = C{si, 0}; m[3] = C{c, 0};
             break;
         }
         default: {  // G_RZ
             double half = theta * 0.5;
-            m[0] = C{std::cos(half), -std::sin(half)}; m[1] = z; m[2] = z;
-            m[3] = C{std::cos(half), std::sin(half)};
+            m[0] = C{lcos(half), -lsin(half)}; m[1] = z; m[2] = z;
+            m[3] = C{lcos(half), lsin(half)};
             break;
         }
     }
@@ -194,7 +200,7 @@ int apply_swap(State& st, uint32_t a, uint32_t b) {
 int apply_cp(State& st, uint32_t a, uint32_t b, double phi) {
     int e = check_pair(st, a, b);
     if (e) return e;
-    const C ph{std::cos(phi), std::sin(phi)};
+    const C ph{lcos(phi), lsin(phi)};
     for_free(st, a, b, (size_t(1) << a) | (size_t(1) << b), [&](size_t idx) { st.amps[idx] = cmul(ph, st.amps[idx]); });
     return OK;
 }

@@ -168,8 +168,8 @@ void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double 
     for (size_t s = 0; s < pc.slot_param.size(); ++s) {
         double th = params[pc.slot_param[s]];
         if (int64_t(s) == shift_slot) th = th + delta;
-        row[2 * s] = std::cos(th * 0.5);
-        row[2 * s + 1] = std::sin(th * 0.5);
+        row[2 * s] = host_cos(th * 0.5);
+        row[2 * s + 1] = host_sin(th * 0.5);
     }
 }
 

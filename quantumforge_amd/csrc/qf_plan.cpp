@@ -410,6 +410,16 @@ TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
     return op;
 }
 
+// Arithmetic form of a 2x2 matrix (see MAT_*).
+int32_t mat_form(const cd m[4]) {
+    if (m[0].re == 0 && m[0].im == 0 && m[3].re == 0 && m[3].im == 0 && m[1].re == 1 && m[1].im == 0 &&
+        m[2].re == 1 && m[2].im == 0)
+        return MAT_SWAP;
+    if (m[0].im == 0 && m[1].im == 0 && m[2].im == 0 && m[3].im == 0) return MAT_REAL;
+    if (m[0].im == 0 && m[3].im == 0 && m[1].re == 0 && m[2].re == 0) return MAT_CROSS;
+    return MAT_GENERAL;
+}
+
 TileOp make_reg_op(const Node& n, const std::vector<int>& reg) {
     TileOp op;
     std::memset(&op, 0, sizeof(op));
@@ -418,6 +428,7 @@ TileOp make_reg_op(const Node& n, const std::vector<int>& reg) {
             op.kind = TOP_MAT1;
             op.q0 = reg[n.q[0]];
             op.q1 = -1;
+            op.flags = mat_form(n.m);
             std::memcpy(op.m, n.m, 4 * sizeof(cd));
             break;
         case N_DIAG1:

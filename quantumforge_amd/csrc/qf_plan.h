@@ -41,6 +41,14 @@ enum TileOpKind : int32_t {
 //   TOP_DIAG, TOP_ROT (Z), TOP_ROTZZ: global qubits (bits taken from the
 //   element's global index).
 
+// Structure of a TOP_MAT1 matrix in the register kernel (TileOp::flags).
+enum : int32_t {
+    MAT_GENERAL = 0,  // dense complex 2x2
+    MAT_REAL = 1,     // all entries real
+    MAT_CROSS = 2,    // m00, m11 real; m01, m10 purely imaginary (Rx-like)
+    MAT_SWAP = 3,     // exactly [[0, 1], [1, 0]]
+};
+
 // TileOp flags (rotations).
 enum : int32_t {
     TF_NEGATE = 1,   // apply the inverse rotation (angle -theta)

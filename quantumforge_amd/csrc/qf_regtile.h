@@ -81,8 +81,11 @@ __device__ __forceinline__ cx<R> cm(R mr, R mi, cx<R> v) {
     return cx<R>{mr * v.re - mi * v.im, mr * v.im + mi * v.re};
 }
 
-// 2x2 on register bit B.
-template <class R, int RB, int B>
+// 2x2 on register bit B.  FORM selects the arithmetic by the matrix's
+// structure (MAT_* in qf_plan.h): general complex (16 flop per amplitude),
+// real (8), "cross" = real diagonal + imaginary off-diagonal such as Rx (8),
+// or the X permutation (0).  m holds (m00, m01, m10, m11) as re, im pairs.
+template <class R, int RB, int B, int FORM>
 __device__ __forceinline__ void reg_mat1(cx<R> (&a)[1 << RB], const R (&m)[8]) {
     if constexpr (B < RB) {
 #pragma unroll
@@ -90,22 +93,43 @@ __device__ __forceinline__ void reg_mat1(cx<R> (&a)[1 << RB], const R (&m)[8]) {
             if (k & (1 << B)) continue;
             const int k1 = k | (1 << B);
             const cx<R> a0 = a[k], a1 = a[k1];
-            const cx<R> x0 = cm(m[0], m[1], a0), x1 = cm(m[2], m[3], a1);
-            const cx<R> y0 = cm(m[4], m[5], a0), y1 = cm(m[6], m[7], a1);
-            a[k] = cx<R>{x0.re + x1.re, x0.im + x1.im};
-            a[k1] = cx<R>{y0.re + y1.re, y0.im + y1.im};
+            if constexpr (FORM == MAT_SWAP) {
+                a[k] = a1;
+                a[k1] = a0;
+            } else if constexpr (FORM == MAT_REAL) {
+                a[k] = cx<R>{m[0] * a0.re + m[2] * a1.re, m[0] * a0.im + m[2] * a1.im};
+                a[k1] = cx<R>{m[4] * a0.re + m[6] * a1.re, m[4] * a0.im + m[6] * a1.im};
+            } else if constexpr (FORM == MAT_CROSS) {  // m00, m11 real; m01 = i*m[3], m10 = i*m[5]
+                a[k] = cx<R>{m[0] * a0.re - m[3] * a1.im, m[0] * a0.im + m[3] * a1.re};
+                a[k1] = cx<R>{m[6] * a1.re - m[5] * a0.im, m[6] * a1.im + m[5] * a0.re};
+            } else {
+                const cx<R> x0 = cm(m[0], m[1], a0), x1 = cm(m[2], m[3], a1);
+                const cx<R> y0 = cm(m[4], m[5], a0), y1 = cm(m[6], m[7], a1);
+                a[k] = cx<R>{x0.re + x1.re, x0.im + x1.im};
+                a[k1] = cx<R>{y0.re + y1.re, y0.im + y1.im};
+            }
         }
     }
 }
 
-template <class R, int RB>
-__device__ __forceinline__ void reg_mat1_any(cx<R> (&a)[1 << RB], int bit, const R (&m)[8]) {
+template <class R, int RB, int FORM>
+__device__ __forceinline__ void reg_mat1_bit(cx<R> (&a)[1 << RB], int bit, const R (&m)[8]) {
     switch (bit) {
-        case 0: reg_mat1<R, RB, 0>(a, m); break;
-        case 1: reg_mat1<R, RB, 1>(a, m); break;
-        case 2: reg_mat1<R, RB, 2>(a, m); break;
-        case 3: reg_mat1<R, RB, 3>(a, m); break;
-        default: reg_mat1<R, RB, 4>(a, m); break;
+        case 0: reg_mat1<R, RB, 0, FORM>(a, m); break;
+        case 1: reg_mat1<R, RB, 1, FORM>(a, m); break;
+        case 2: reg_mat1<R, RB, 2, FORM>(a, m); break;
+        case 3: reg_mat1<R, RB, 3, FORM>(a, m); break;
+        default: reg_mat1<R, RB, 4, FORM>(a, m); break;
+    }
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_mat1_any(cx<R> (&a)[1 << RB], int bit, const R (&m)[8], int form = MAT_GENERAL) {
+    switch (form) {
+        case MAT_REAL: reg_mat1_bit<R, RB, MAT_REAL>(a, bit, m); break;
+        case MAT_CROSS: reg_mat1_bit<R, RB, MAT_CROSS>(a, bit, m); break;
+        case MAT_SWAP: reg_mat1_bit<R, RB, MAT_SWAP>(a, bit, m); break;
+        default: reg_mat1_bit<R, RB, MAT_GENERAL>(a, bit, m); break;
     }
 }
 
@@ -243,7 +267,7 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
         case TOP_MAT1: {
             const R m[8] = {R(op.m[0]), R(op.m[1]), R(op.m[2]), R(op.m[3]),
                             R(op.m[4]), R(op.m[5]), R(op.m[6]), R(op.m[7])};
-            reg_mat1_any<R, RB>(a, op.q0, m);
+            reg_mat1_any<R, RB>(a, op.q0, m, op.flags);
             break;
         }
         case TOP_DIAG: reg_diag<R, RB>(a, mp, op); break;
@@ -255,10 +279,10 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
             rot_cs<R>(op, cs_row, c, s);
             if (op.kind == TOP_ROT && op.q1 == 0) {  // Rx = [[c, -is], [-is, c]]
                 const R m[8] = {c, R(0), R(0), -s, R(0), -s, c, R(0)};
-                reg_mat1_any<R, RB>(a, op.q0, m);
+                reg_mat1_any<R, RB>(a, op.q0, m, MAT_CROSS);
             } else if (op.kind == TOP_ROT && op.q1 == 1) {  // Ry = [[c, -s], [s, c]]
                 const R m[8] = {c, R(0), -s, R(0), s, R(0), c, R(0)};
-                reg_mat1_any<R, RB>(a, op.q0, m);
+                reg_mat1_any<R, RB>(a, op.q0, m, MAT_REAL);
             } else {
                 reg_rotz<R, RB>(a, mp, op.q0, op.kind == TOP_ROTZZ ? op.q1 : -1, c, s);
             }

@@ -29,6 +29,11 @@ struct LaunchCtx {
 
 qf_status fail(qf_status st, const std::string& msg);
 qf_status hip_fail(hipError_t e, const char* what);
+// cos and sin as two separate libm calls (the reference's f64::cos, f64::sin),
+// out of line so that no compiler merges a pair into sincos, whose last bit
+// can differ from cos's.
+double host_cos(double x);
+double host_sin(double x);
 void gate_matrix(int gate, double theta, qf::cd m[4]);
 bool is_rotation(int gate);
 qf_status check_pair(uint32_t qubits, uint32_t a, uint32_t b);

@@ -83,6 +83,9 @@ extern "C" const char* qf_status_name(qf_status s) {
 // ---------------------------------------------------------------------------
 namespace qfrt {
 
+__attribute__((noinline)) double host_cos(double x) { return std::cos(x); }
+__attribute__((noinline)) double host_sin(double x) { return std::sin(x); }
+
 void gate_matrix(int gate, double theta, cd m[4]) {
     const cd z{0, 0}, o{1, 0}, i{0, 1};
     const double s = 0.70710678118654752440;  // FRAC_1_SQRT_2
@@ -97,19 +100,19 @@ void gate_matrix(int gate, double theta, cd m[4]) {
         case QF_GATE_T: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{s, s}; return;
         case QF_GATE_TDG: m[0] = o; m[1] = z; m[2] = z; m[3] = cd{s, -s}; return;
         case QF_GATE_RX: {
-            const double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            const double c = host_cos(theta * 0.5), si = host_sin(theta * 0.5);
             m[0] = cd{c, 0}; m[1] = cd{0, -si}; m[2] = cd{0, -si}; m[3] = cd{c, 0};
             return;
         }
         case QF_GATE_RY: {
-            const double c = std::cos(theta * 0.5), si = std::sin(theta * 0.5);
+            const double c = host_cos(theta * 0.5), si = host_sin(theta * 0.5);
             m[0] = cd{c, 0}; m[1] = cd{-si, 0}; m[2] = cd{si, 0}; m[3] = cd{c, 0};
             return;
         }
         default: {  // RZ
             const double half = theta * 0.5;
-            m[0] = cd{std::cos(half), -std::sin(half)}; m[1] = z; m[2] = z;
-            m[3] = cd{std::cos(half), std::sin(half)};
+            m[0] = cd{host_cos(half), -host_sin(half)}; m[1] = z; m[2] = z;
+            m[3] = cd{host_cos(half), host_sin(half)};
             return;
         }
     }
@@ -271,7 +274,7 @@ qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector
                 g.a = op.a;
                 g.b = op.b;
                 g.m[0] = g.m[1] = g.m[2] = cd{1, 0};
-                g.m[3] = cd{std::cos(op.theta), std::sin(op.theta)};
+                g.m[3] = cd{host_cos(op.theta), host_sin(op.theta)};
                 break;
             }
             case 4: return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
@@ -781,7 +784,7 @@ extern "C" qf_status qf_state_apply_cp(qf_state* st, uint32_t control, uint32_t 
     g.a = control;
     g.b = target;
     g.m[0] = g.m[1] = g.m[2] = cd{1, 0};
-    g.m[3] = cd{std::cos(phi), std::sin(phi)};
+    g.m[3] = cd{host_cos(phi), host_sin(phi)};
     if ((s = activate(st->be))) return s;
     qf::PlanOptions po;
     exact_opts(po);
