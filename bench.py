@@ -120,12 +120,15 @@ def main() -> int:
     ap.add_argument("--precision", default="complex128", choices=["complex128", "complex64"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--low", type=int, default=0)
-    ap.add_argument("--kernel", type=int, default=0, help="0 default, 1 LDS op-by-op tile kernel, 2 register kernel")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="0 default, 1 LDS op-by-op tile kernel, 2 register kernel, 3 compiled passes")
     ap.add_argument("--reg-bits", type=int, default=0)
     ap.add_argument("--cpu-sample-ops", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vqe-sets", type=int, default=4096, help="parameter sets of the 16-qubit VQE adjoint (0 = skip)")
     ap.add_argument("--vqe-shift-sets", type=int, default=4, help="sets timed with the parameter-shift GPU form")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (3 launches) to scale HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -134,6 +137,16 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_calibration_mib > 0:
+        nbytes = args.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}}))
+        return 0
 
     from quantumforge_amd import StateVector
     from quantumforge_amd.backend import HipBackend, Plan
@@ -147,8 +160,14 @@ def main() -> int:
     desc = plan.describe()
     tile_bits = desc["passes"][0]["tile_bits"]
     reg_bits = desc["passes"][0].get("reg_bits", 0)
+    compiled = bool(desc.get("compiled", 0))
     log(f"plan: {stats['passes']} passes, {stats['phases']} register phases for {stats['ops_in']} ops "
-        f"(tile {tile_bits} qubits, register bits {reg_bits})")
+        f"(tile {tile_bits} qubits, register bits {reg_bits}, {'compiled' if compiled else 'generic'} kernels)")
+    compile_info = None
+    if compiled:
+        compile_info = plan.compile(args.precision)
+        log(f"compiled {stats['passes']} pass kernels in {compile_info['compile_ms']:.0f} ms"
+            f"{' (on-disk cache)' if compile_info['from_cache'] else ''}")
 
     be = HipBackend(0)
     state = be.zero_state(n, precision=args.precision)
@@ -202,6 +221,8 @@ def main() -> int:
                         f"{'complex128' if amp_bytes == 16 else 'complex64'}, fused tile passes",
             "qubits": n, "depth": args.depth, "ops": stats["ops_in"], "passes_per_step": stats["passes"],
             "tile_qubits": tile_bits, "register_bits": reg_bits, "phases_per_step": stats["phases"],
+            "pass_kernels": "compiled" if compiled else ("register" if reg_bits else "lds"),
+            "compile_ms": round(compile_info["compile_ms"], 1) if compile_info else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -210,7 +231,8 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_tile_reg" if reg_bits else "k_tile",
+            "kernel": ("qf_pass_* (one compiled kernel per tile pass)" if compiled
+                       else ("k_tile_reg" if reg_bits else "k_tile")),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },

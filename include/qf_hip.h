@@ -145,7 +145,8 @@ typedef struct {
     uint32_t tile_qubits;  /* qubits held per workgroup tile (0 = default) */
     uint32_t low_qubits;   /* contiguous low qubits always in a tile (0 = default) */
     uint32_t max_block;    /* widest dense fused block k (2..5; 0 = default) */
-    int32_t kernel;        /* 0 default, 1 LDS op-by-op tile kernel, 2 register-resident tile kernel */
+    int32_t kernel;        /* 0 default, 1 LDS op-by-op tile kernel, 2 register-resident tile kernel,
+                              3 compiled passes (one generated kernel per pass; default from 18 qubits) */
     uint32_t reg_bits;     /* register kernel: 2^reg_bits amplitudes per thread (0 = default) */
 } qf_plan_options;
 
@@ -360,6 +361,23 @@ qf_status qf_plan_stats_get(const qf_plan* plan, qf_plan_stats* out);
 /* JSON text of the plan's passes (inspection and planner tests).  Writes at
  * most cap bytes including the terminating NUL; *needed gets the full length + 1. */
 qf_status qf_plan_describe(const qf_plan* plan, char* buf, size_t cap, size_t* needed);
+/* Plan of a parameterized circuit as the gradient engine runs it (host only):
+ * rotation occurrences read (cos, sin) of their half angle from a per-instance
+ * table in occurrence order; reverse != 0 gives the inverted circuit in reverse
+ * order with every occurrence recorded (the adjoint reverse sweep).  Validates
+ * like qf_gradient.  For inspection (describe, source, compile). */
+qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                                      const double* inputs, size_t inputs_len, uint32_t qubits, int32_t reverse,
+                                      const qf_plan_options* options, qf_plan** out);
+/* Source of the plan's compiled passes (kernel = 3, or the default for states
+ * of 18+ qubits): one generated kernel per register-kernel tile pass, in pass
+ * order.  Same buffer convention as qf_plan_describe. */
+qf_status qf_plan_source(const qf_plan* plan, int32_t precision, char* buf, size_t cap, size_t* needed);
+/* Compiles the plan's passes for gfx950 now (host only; otherwise the first
+ * run compiles them).  compile_ms and from_cache may be NULL; from_cache is 1
+ * when the code objects came from the on-disk cache (QF_JIT_CACHE, default
+ * $HOME/.cache/quantumforge_amd/jit; "0" disables it). */
+qf_status qf_plan_compile(const qf_plan* plan, int32_t precision, double* compile_ms, int32_t* from_cache);
 qf_status qf_plan_destroy(qf_plan* plan);
 
 #ifdef __cplusplus

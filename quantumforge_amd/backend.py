@@ -36,6 +36,24 @@ class Plan:
         self._h = h
         self.qubits = qubits
 
+    @classmethod
+    def parameterized(cls, pc, qubits: int, inputs=None, reverse: bool = False, tile_qubits: int = 0,
+                      low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0) -> "Plan":
+        """The plan the gradient engine runs for a ParameterizedCircuit
+        (qf_plan_build_parameterized): rotations read their angles per
+        instance; reverse gives the inverted, recorded reverse sweep."""
+        self = cls.__new__(cls)
+        self._ops = pc.to_pops()
+        inp = np.ascontiguousarray(np.asarray(inputs if inputs is not None else [], dtype=np.float64))
+        opts = N.PlanOptions(1, tile_qubits, low_qubits, 0, kernel, reg_bits)
+        h = C.c_void_p()
+        check(lib().qf_plan_build_parameterized(ptr(self._ops), len(self._ops), pc.parameter_count,
+                                                pc.input_count, ptr(inp), inp.size, qubits, int(reverse),
+                                                C.byref(opts), C.byref(h)))
+        self._h = h
+        self.qubits = qubits
+        return self
+
     def stats(self) -> dict:
         s = N.PlanStats()
         check(lib().qf_plan_stats_get(self._h, C.byref(s)))
@@ -47,6 +65,28 @@ class Plan:
         buf = C.create_string_buffer(need.value)
         check(lib().qf_plan_describe(self._h, buf, need.value, C.byref(need)))
         return json.loads(buf.value.decode())
+
+    @property
+    def compiled(self) -> bool:
+        """Whether the register passes run as compiled kernels."""
+        return bool(self.describe().get("compiled", 0))
+
+    def source(self, precision: str = "complex128") -> str:
+        """Generated source of the compiled passes (qf_plan_source)."""
+        pr = 0 if precision == "complex128" else 1
+        need = C.c_size_t()
+        check(lib().qf_plan_source(self._h, pr, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        check(lib().qf_plan_source(self._h, pr, buf, need.value, C.byref(need)))
+        return buf.value.decode()
+
+    def compile(self, precision: str = "complex128") -> dict:
+        """Compiles the passes for gfx950 now (host only); returns the compile
+        time and whether the code came from the on-disk cache."""
+        ms = C.c_double()
+        cached = C.c_int32()
+        check(lib().qf_plan_compile(self._h, 0 if precision == "complex128" else 1, C.byref(ms), C.byref(cached)))
+        return {"compile_ms": ms.value, "from_cache": bool(cached.value)}
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqfhip.so")
 ARCH = os.environ.get("QF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_plan.cpp"]
+SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_plan.cpp", "qf_jit.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -37,6 +37,8 @@ HIPCC_FLAGS = [
     # registers of the tile kernels into scratch memory.
     "-mllvm", "-simplifycfg-sink-common=false",
     "-I", os.path.join(ROOT, "include"),
+    # The pass compiler (qf_jit.cpp) compiles generated kernels with hiprtc.
+    "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

@@ -825,3 +825,34 @@ extern "C" qf_status qf_state_run_parameterized(qf_state* st, const qf_pop* ops,
     QF_HIP_TRY(hipStreamSynchronize(strm));  // cs is freed on return
     return QF_OK;
 }
+
+// Plan of a parameterized circuit as the gradient engine builds it: rotation
+// occurrences read their angles from the per-instance table (forward), or run
+// inverted and recorded in reverse order (reverse != 0).
+extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops, uint32_t nparams, uint32_t ninputs,
+                                                 const double* inputs, size_t inputs_len, uint32_t qubits,
+                                                 int32_t reverse, const qf_plan_options* options, qf_plan** out) {
+    if (!out || (!ops && nops) || (!inputs && inputs_len)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    qf_status s = check_qubits(qubits);
+    if (s) return s;
+    PCirc pc;
+    pc.ops = ops;
+    pc.nops = nops;
+    pc.nparams = nparams;
+    pc.ninputs = ninputs;
+    if ((s = validate_ops(pc, qubits))) return s;
+    if (inputs_len != ninputs)
+        return fail(QF_INVALID_INPUT_COUNT, "expected " + std::to_string(ninputs) + " encoded inputs, got " +
+                                                std::to_string(inputs_len));
+    qf::PlanOptions po = plan_options_from(options);
+    po.fuse = 1;
+    qf_plan* p = new qf_plan;
+    const std::string e = qf::build_plan(param_gates(pc, inputs, reverse != 0), qubits, QF_C128, po, p->plan);
+    if (!e.empty()) {
+        delete p;
+        return fail(QF_INVALID_ARGUMENT, e);
+    }
+    *out = p;
+    return QF_OK;
+}

@@ -1,0 +1,686 @@
+// Pass compiler: source generation for compiled tile passes, hiprtc
+// compilation and the in-process / on-disk caches (interface in qf_jit.h).
+#include "qf_jit.h"
+
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <thread>
+
+namespace qf {
+namespace {
+
+std::string hexlit(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%a", v);
+    return std::string("R(") + b + ")";
+}
+
+// One product coefficient * variable of a linear combination.  coef is a
+// source expression, or empty for 1.
+struct Term {
+    std::string coef;
+    bool neg;
+    std::string var;
+};
+
+Term lit_term(double c, const std::string& var) {
+    const double a = std::fabs(c);
+    return Term{a == 1.0 ? std::string() : hexlit(a), std::signbit(c), var};
+}
+
+std::string lincomb(const std::vector<Term>& ts) {
+    std::string s;
+    for (const Term& t : ts) {
+        const std::string body = t.coef.empty() ? t.var : t.coef + " * " + t.var;
+        if (s.empty())
+            s = t.neg ? "-" + body : body;
+        else
+            s += (t.neg ? " - " : " + ") + body;
+    }
+    return s.empty() ? "R(0)" : s;
+}
+
+struct Bit {
+    enum Kind { REG, THREAD, BLOCK } kind;
+    int idx;  // register index, thread bit, or global qubit
+};
+
+class PassGen {
+public:
+    PassGen(const Pass& p, uint32_t n, int precision, const std::string& name)
+        : p_(p), n_(n), dbl_(precision == 0), name_(name) {}
+
+    bool run(std::string* err) {
+        if (p_.kind != Pass::TILE || p_.reg_bits == 0 || p_.ops.empty() || p_.ops[0].kind != TOP_PHASE)
+            return fail(err, "not a register-kernel tile pass");
+        M_ = p_.tile_bits;
+        RB_ = p_.reg_bits;
+        if (RB_ > 6 || M_ < RB_ || M_ - RB_ > 10 || M_ > 16 || M_ > n_) return fail(err, "unsupported tile shape");
+        NR_ = 1u << RB_;
+        NT_ = 1u << (M_ - RB_);
+        LOGB_ = n_ - M_;
+        tile_of_q_.assign(n_, -1);
+        q_of_tile_.assign(M_, 0);
+        for (uint32_t b = 0; b < p_.low; ++b) {
+            tile_of_q_[b] = int(b);
+            q_of_tile_[b] = b;
+        }
+        for (size_t k = 0; k < p_.high.size(); ++k) {
+            tile_of_q_[p_.high[k]] = int(p_.low + k);
+            q_of_tile_[p_.low + k] = p_.high[k];
+        }
+        bool any_rot = false;
+        size_t phases = 0;
+        for (const TileOp& op : p_.ops) {
+            if (op.kind == TOP_ROT || op.kind == TOP_ROTZZ) any_rot = true;
+            if (op.kind == TOP_PHASE) ++phases;
+        }
+        const char* R = dbl_ ? "double" : "float";
+        o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << ") void " << name_
+           << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots) {\n";
+        if (phases > 1) o_ << "  __shared__ C xb[" << (1u << M_) << "];\n";
+        o_ << "  typedef " << R << " R;\n";
+        o_ << "  const unsigned t = threadIdx.x;\n";
+        o_ << "  const unsigned long long b = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * "
+              "gridDim.x;\n";
+        o_ << "  const unsigned long long inst = b >> " << LOGB_ << ";\n";
+        o_ << "  const unsigned long long bl = b & " << mask64(LOGB_) << "ull;\n";
+        o_ << "  const unsigned long long base = (inst << " << n_ << ")" << block_scatter() << ";\n";
+        if (any_rot) o_ << "  const double* __restrict__ csr = cs + inst * 2ull * nslots;\n";
+        else o_ << "  (void)cs; (void)nslots;\n";
+        for (uint32_t k = 0; k < NR_; ++k) o_ << "  R r" << k << ", i" << k << ";\n";
+        var_.resize(NR_);
+        for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
+        set_phase(p_.ops[0]);
+        // Load with the first phase's mapping.
+        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_global() << ";\n";
+        for (uint32_t k = 0; k < NR_; ++k)
+            o_ << "    { const C v = sp[" << reg_global(k) << "ull]; r" << k << " = v.re; i" << k << " = v.im; }\n";
+        o_ << "  }\n";
+        bool lds_used = false;
+        for (size_t i = 1; i < p_.ops.size(); ++i) {
+            const TileOp& op = p_.ops[i];
+            ++opn_;
+            switch (op.kind) {
+                case TOP_PHASE: exchange(op, lds_used); lds_used = true; break;
+                case TOP_MAT1: mat1(op); break;
+                case TOP_DIAG: diag(op); break;
+                case TOP_CNOT: cnot(op); break;
+                case TOP_SWAP: swap(op); break;
+                case TOP_ROT: rot(op); break;
+                case TOP_ROTZZ: rotzz(op); break;
+                default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
+            }
+        }
+        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_global() << ";\n";
+        for (uint32_t k = 0; k < NR_; ++k)
+            o_ << "    sp[" << reg_global(k) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
+        o_ << "  }\n}\n";
+        src = o_.str();
+        return true;
+    }
+
+    std::string src;
+
+private:
+    const Pass& p_;
+    uint32_t n_;
+    bool dbl_;
+    std::string name_;
+    uint32_t M_ = 0, RB_ = 0, NR_ = 0, NT_ = 0, LOGB_ = 0;
+    std::vector<int> tile_of_q_;
+    std::vector<uint32_t> q_of_tile_;
+    int rb_[8] = {0};
+    std::vector<int> reg_of_tile_, thr_of_tile_;
+    std::vector<int> var_;
+    std::ostringstream o_;
+    int opn_ = 0;
+
+    static bool fail(std::string* err, const std::string& m) {
+        if (err) *err = m;
+        return false;
+    }
+    static std::string mask64(uint32_t bits) {
+        char b[32];
+        std::snprintf(b, sizeof b, "0x%llxu", bits >= 64 ? ~0ull : ((1ull << bits) - 1));
+        std::string s(b);
+        s.pop_back();  // the caller appends the suffix
+        return s;
+    }
+    std::string rv(uint32_t k) const { return "r" + std::to_string(var_[k]); }
+    std::string iv(uint32_t k) const { return "i" + std::to_string(var_[k]); }
+
+    // Bits of the block index go, in order, to the qubits outside the tile.
+    std::string block_scatter() const {
+        std::string s;
+        uint32_t bit = 0;
+        uint32_t g = 0;
+        while (g < n_) {
+            if (tile_of_q_[g] >= 0) {
+                ++g;
+                continue;
+            }
+            uint32_t len = 0;
+            while (g + len < n_ && tile_of_q_[g + len] < 0) ++len;
+            s += " | ((bl >> " + std::to_string(bit) + ") & " + mask64(len) + "ull) << " + std::to_string(g);
+            bit += len;
+            g += len;
+        }
+        return s;
+    }
+
+    void set_phase(const TileOp& hdr) {
+        const uint32_t packed = uint32_t(hdr.q0);
+        reg_of_tile_.assign(M_, -1);
+        thr_of_tile_.assign(M_, -1);
+        for (uint32_t i = 0; i < RB_; ++i) {
+            rb_[i] = int((packed >> (4 * i)) & 15u);
+            reg_of_tile_[rb_[i]] = int(i);
+        }
+        int u = 0;
+        for (uint32_t tb = 0; tb < M_; ++tb)
+            if (reg_of_tile_[tb] < 0) thr_of_tile_[tb] = u++;
+    }
+
+    // Thread bits go, in order, to the tile bits that are not register bits.
+    // Runs of consecutive thread bits landing on consecutive positions are one
+    // shift-and-mask each.  `global` selects global qubit positions instead of
+    // tile bit positions.
+    std::string thread_scatter(bool global) const {
+        std::vector<std::pair<int, uint32_t>> pos;  // (thread bit, position)
+        for (uint32_t tb = 0; tb < M_; ++tb)
+            if (thr_of_tile_[tb] >= 0) pos.push_back({thr_of_tile_[tb], global ? q_of_tile_[tb] : tb});
+        std::string s;
+        size_t i = 0;
+        while (i < pos.size()) {
+            size_t j = i + 1;
+            while (j < pos.size() && pos[j].first == pos[j - 1].first + 1 && pos[j].second == pos[j - 1].second + 1)
+                ++j;
+            const uint32_t len = uint32_t(j - i);
+            std::string term = "((t >> " + std::to_string(pos[i].first) + ") & " + mask64(len) + "u)";
+            if (global) term = "((unsigned long long)" + term + " << " + std::to_string(pos[i].second) + ")";
+            else term = "(" + term + " << " + std::to_string(pos[i].second) + ")";
+            s += s.empty() ? term : " | " + term;
+            i = j;
+        }
+        return s.empty() ? (global ? "0ull" : "0u") : "(" + s + ")";
+    }
+    std::string thread_global() const { return thread_scatter(true); }
+
+    uint64_t reg_global(uint32_t k) const {
+        uint64_t g = 0;
+        for (uint32_t i = 0; i < RB_; ++i)
+            if ((k >> i) & 1) g |= uint64_t(1) << q_of_tile_[rb_[i]];
+        return g;
+    }
+    uint32_t reg_tile(uint32_t k) const {
+        uint32_t j = 0;
+        for (uint32_t i = 0; i < RB_; ++i)
+            if ((k >> i) & 1) j |= 1u << rb_[i];
+        return j;
+    }
+    // LDS slot swizzle: XOR-folds the higher index bits into the bits that
+    // select the bank row (256 bytes), linear over XOR so slot(a ^ b) =
+    // slot(a) ^ slot(b).
+    uint32_t swz_bits() const { return dbl_ ? 4 : 5; }
+    uint32_t slot(uint32_t j) const {
+        const uint32_t B = swz_bits(), m = (1u << B) - 1;
+        return j ^ (((j >> B) ^ (j >> (2 * B))) & m);
+    }
+    std::string slot_expr(const std::string& j) const {
+        const uint32_t B = swz_bits();
+        return j + " ^ (((" + j + " >> " + std::to_string(B) + ") ^ (" + j + " >> " + std::to_string(2 * B) +
+               ")) & " + std::to_string((1u << B) - 1) + "u)";
+    }
+
+    void exchange(const TileOp& hdr, bool lds_used) {
+        if (lds_used) o_ << "  __syncthreads();\n";
+        o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
+        o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
+        for (uint32_t k = 0; k < NR_; ++k)
+            o_ << "    xb[st ^ " << slot(reg_tile(k)) << "u] = C{" << rv(k) << ", " << iv(k) << "};\n";
+        o_ << "  }\n  __syncthreads();\n";
+        set_phase(hdr);
+        for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
+        o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
+        o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
+        for (uint32_t k = 0; k < NR_; ++k)
+            o_ << "    { const C v = xb[st ^ " << slot(reg_tile(k)) << "u]; r" << k << " = v.re; i" << k
+               << " = v.im; }\n";
+        o_ << "  }\n";
+    }
+
+    Bit bit_of(int g) const {
+        const int tb = tile_of_q_[g];
+        if (tb < 0) return Bit{Bit::BLOCK, g};
+        if (reg_of_tile_[tb] >= 0) return Bit{Bit::REG, reg_of_tile_[tb]};
+        return Bit{Bit::THREAD, thr_of_tile_[tb]};
+    }
+    std::string bit_expr(const Bit& b) const {
+        if (b.kind == Bit::THREAD) return "((t >> " + std::to_string(b.idx) + ") & 1u)";
+        return "((unsigned)(base >> " + std::to_string(b.idx) + ") & 1u)";
+    }
+
+    // a_k *= (dr + i di), literal.
+    void mul_lit(uint32_t k, double dr, double di) {
+        const std::string r = rv(k), i = iv(k);
+        if (dr == 1.0 && di == 0.0) return;
+        if (di == 0.0) {
+            o_ << "  " << r << " = " << lincomb({lit_term(dr, r)}) << "; " << i << " = " << lincomb({lit_term(dr, i)})
+               << ";\n";
+            return;
+        }
+        if (dr == 0.0) {
+            o_ << "  { const R x = " << r << "; " << r << " = " << lincomb({lit_term(-di, i)}) << "; " << i << " = "
+               << lincomb({lit_term(di, "x")}) << "; }\n";
+            return;
+        }
+        o_ << "  { const R x = " << r << "; " << r << " = " << lincomb({lit_term(dr, r), lit_term(-di, i)}) << "; "
+           << i << " = " << lincomb({lit_term(dr, i), lit_term(di, "x")}) << "; }\n";
+    }
+    // a_k *= (DR + i DI), run-time values (DI empty: real factor).
+    void mul_rt(uint32_t k, const std::string& DR, const std::string& DI, bool neg_im = false) {
+        const std::string r = rv(k), i = iv(k);
+        if (DI.empty()) {
+            o_ << "  " << r << " = " << DR << " * " << r << "; " << i << " = " << DR << " * " << i << ";\n";
+            return;
+        }
+        o_ << "  { const R x = " << r << "; " << r << " = " << lincomb({{DR, false, r}, {DI, !neg_im, i}}) << "; " << i
+           << " = " << lincomb({{DR, false, i}, {DI, neg_im, "x"}}) << "; }\n";
+    }
+
+    void swap_vars(uint32_t a, uint32_t b) { std::swap(var_[a], var_[b]); }
+
+    void mat1(const TileOp& op) {
+        const uint32_t i = uint32_t(op.q0);
+        const double* m = op.m;
+        const bool offz = m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0;
+        const bool diagz = m[0] == 0 && m[1] == 0 && m[6] == 0 && m[7] == 0;
+        if (offz) {
+            for (uint32_t k = 0; k < NR_; ++k)
+                if ((k >> i) & 1) mul_lit(k, m[6], m[7]);
+                else mul_lit(k, m[0], m[1]);
+            return;
+        }
+        if (diagz) {
+            for (uint32_t k = 0; k < NR_; ++k)
+                if (!((k >> i) & 1)) swap_vars(k, k | (1u << i));
+            for (uint32_t k = 0; k < NR_; ++k)
+                if ((k >> i) & 1) mul_lit(k, m[4], m[5]);
+                else mul_lit(k, m[2], m[3]);
+            return;
+        }
+        for (uint32_t k = 0; k < NR_; ++k) {
+            if ((k >> i) & 1) continue;
+            const uint32_t k1 = k | (1u << i);
+            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+            o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
+            o_ << "    " << rA << " = "
+               << lincomb({lit_term(m[0], "a0r"), lit_term(-m[1], "a0i"), lit_term(m[2], "a1r"), lit_term(-m[3], "a1i")})
+               << ";\n";
+            o_ << "    " << iA << " = "
+               << lincomb({lit_term(m[0], "a0i"), lit_term(m[1], "a0r"), lit_term(m[2], "a1i"), lit_term(m[3], "a1r")})
+               << ";\n";
+            o_ << "    " << rB << " = "
+               << lincomb({lit_term(m[4], "a0r"), lit_term(-m[5], "a0i"), lit_term(m[6], "a1r"), lit_term(-m[7], "a1i")})
+               << ";\n";
+            o_ << "    " << iB << " = "
+               << lincomb({lit_term(m[4], "a0i"), lit_term(m[5], "a0r"), lit_term(m[6], "a1i"), lit_term(m[7], "a1r")})
+               << "; }\n";
+        }
+    }
+
+    // Diagonal d[bit(g1) * 2 + bit(g0)] (g1 < 0: one qubit, d[bit(g0)]).
+    void diag(const TileOp& op) {
+        const int g0 = op.q0, g1 = op.q1;
+        const Bit b0 = bit_of(g0);
+        const bool two = g1 >= 0;
+        const Bit b1 = two ? bit_of(g1) : Bit{Bit::REG, -1};
+        auto d = [&](int idx, int c) { return op.m[2 * idx + c]; };
+        const std::string pfx = "d" + std::to_string(opn_);
+        const bool rt0 = b0.kind != Bit::REG, rt1 = two && b1.kind != Bit::REG;
+        if (rt0) o_ << "  const bool " << pfx << "c0 = " << bit_expr(b0) << ";\n";
+        if (rt1) o_ << "  const bool " << pfx << "c1 = " << bit_expr(b1) << ";\n";
+        // Factor per compile-time combination (x0, x1) of the register bits.
+        std::map<int, std::pair<std::string, std::string>> made;
+        for (uint32_t k = 0; k < NR_; ++k) {
+            const int x0 = rt0 ? -1 : int((k >> b0.idx) & 1);
+            const int x1 = !two ? 0 : (rt1 ? -1 : int((k >> b1.idx) & 1));
+            // Candidate indices over the run-time bits.
+            std::vector<int> cand;
+            for (int v1 = 0; v1 < 2; ++v1)
+                for (int v0 = 0; v0 < 2; ++v0) {
+                    if (x0 >= 0 && v0 != x0) continue;
+                    if (!two && v1) continue;
+                    if (two && x1 >= 0 && v1 != x1) continue;
+                    cand.push_back(v1 * 2 + v0);
+                }
+            bool same = true, real = true;
+            for (int c : cand) {
+                if (d(c, 0) != d(cand[0], 0) || d(c, 1) != d(cand[0], 1)) same = false;
+                if (d(c, 1) != 0.0) real = false;
+            }
+            if (same) {
+                mul_lit(k, d(cand[0], 0), d(cand[0], 1));
+                continue;
+            }
+            const int key = (x0 + 1) * 3 + (x1 + 1);
+            auto it = made.find(key);
+            if (it == made.end()) {
+                const std::string nm = pfx + "_" + std::to_string(key);
+                auto sel = [&](int comp) {
+                    // Select among the candidates with the run-time bits.
+                    auto val = [&](int v0, int v1) {
+                        const int i0 = x0 >= 0 ? x0 : v0, i1 = !two ? 0 : (x1 >= 0 ? x1 : v1);
+                        return hexlit(d(i1 * 2 + i0, comp));
+                    };
+                    if (rt0 && rt1)
+                        return pfx + "c1 ? (" + pfx + "c0 ? " + val(1, 1) + " : " + val(0, 1) + ") : (" + pfx + "c0 ? " +
+                               val(1, 0) + " : " + val(0, 0) + ")";
+                    if (rt0) return pfx + "c0 ? " + val(1, 0) + " : " + val(0, 0);
+                    return pfx + "c1 ? " + val(0, 1) + " : " + val(0, 0);
+                };
+                o_ << "  const R " << nm << "r = " << sel(0) << ";\n";
+                if (!real) o_ << "  const R " << nm << "i = " << sel(1) << ";\n";
+                it = made.emplace(key, std::make_pair(nm + "r", real ? std::string() : nm + "i")).first;
+            }
+            mul_rt(k, it->second.first, it->second.second);
+        }
+    }
+
+    void cnot(const TileOp& op) {
+        const uint32_t ti = uint32_t(op.q1);
+        const Bit c = bit_of(op.q0);
+        if (c.kind == Bit::REG) {
+            for (uint32_t k = 0; k < NR_; ++k)
+                if (((k >> c.idx) & 1) && !((k >> ti) & 1)) swap_vars(k, k | (1u << ti));
+            return;
+        }
+        const std::string cn = "x" + std::to_string(opn_);
+        o_ << "  const bool " << cn << " = " << bit_expr(c) << ";\n";
+        for (uint32_t k = 0; k < NR_; ++k) {
+            if ((k >> ti) & 1) continue;
+            const uint32_t k1 = k | (1u << ti);
+            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+            o_ << "  { const R ur = " << rA << ", ui = " << iA << "; " << rA << " = " << cn << " ? " << rB << " : "
+               << rA << "; " << iA << " = " << cn << " ? " << iB << " : " << iA << "; " << rB << " = " << cn
+               << " ? ur : " << rB << "; " << iB << " = " << cn << " ? ui : " << iB << "; }\n";
+        }
+    }
+
+    void swap(const TileOp& op) {
+        const uint32_t a = uint32_t(op.q0), b = uint32_t(op.q1);
+        for (uint32_t k = 0; k < NR_; ++k)
+            if (((k >> a) & 1) && !((k >> b) & 1)) swap_vars(k, k ^ (1u << a) ^ (1u << b));
+    }
+
+    // (c, s) of a rotation occurrence from the instance's angle-table row.
+    void load_cs(const TileOp& op, std::string& c, std::string& s) {
+        c = "c" + std::to_string(opn_);
+        s = "s" + std::to_string(opn_);
+        o_ << "  const R " << c << " = R(csr[" << 2 * op.slot << "]);\n";
+        o_ << "  const R " << s << " = " << ((op.flags & TF_NEGATE) ? "-" : "") << "R(csr[" << 2 * op.slot + 1
+           << "]);\n";
+    }
+
+    void rot(const TileOp& op) {
+        std::string c, s;
+        load_cs(op, c, s);
+        if (op.q1 == 2) {  // exp(-i theta Z / 2): (c - is) on bit 0, (c + is) on bit 1
+            const Bit b = bit_of(op.q0);
+            if (b.kind == Bit::REG) {
+                for (uint32_t k = 0; k < NR_; ++k) mul_rt(k, c, s, !((k >> b.idx) & 1));
+                return;
+            }
+            const std::string sz = "z" + std::to_string(opn_);
+            o_ << "  const R " << sz << " = " << bit_expr(b) << " ? " << s << " : -" << s << ";\n";
+            for (uint32_t k = 0; k < NR_; ++k) mul_rt(k, c, sz);
+            return;
+        }
+        const uint32_t i = uint32_t(op.q0);
+        for (uint32_t k = 0; k < NR_; ++k) {
+            if ((k >> i) & 1) continue;
+            const uint32_t k1 = k | (1u << i);
+            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+            o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
+            if (op.q1 == 0) {  // Rx = [[c, -is], [-is, c]]
+                o_ << "    " << rA << " = " << c << " * a0r + " << s << " * a1i; " << iA << " = " << c << " * a0i - "
+                   << s << " * a1r;\n";
+                o_ << "    " << rB << " = " << c << " * a1r + " << s << " * a0i; " << iB << " = " << c << " * a1i - "
+                   << s << " * a0r; }\n";
+            } else {  // Ry = [[c, -s], [s, c]]
+                o_ << "    " << rA << " = " << c << " * a0r - " << s << " * a1r; " << iA << " = " << c << " * a0i - "
+                   << s << " * a1i;\n";
+                o_ << "    " << rB << " = " << s << " * a0r + " << c << " * a1r; " << iB << " = " << s << " * a0i + "
+                   << c << " * a1i; }\n";
+            }
+        }
+    }
+
+    // exp(-i theta Z_g0 Z_g1 / 2): (c - is) on even parity, (c + is) on odd.
+    void rotzz(const TileOp& op) {
+        std::string c, s;
+        load_cs(op, c, s);
+        const Bit b0 = bit_of(op.q0), b1 = bit_of(op.q1);
+        std::string rt;
+        for (const Bit& b : {b0, b1})
+            if (b.kind != Bit::REG) rt += (rt.empty() ? "" : " ^ ") + bit_expr(b);
+        std::string sz = s;
+        if (!rt.empty()) {
+            sz = "z" + std::to_string(opn_);
+            o_ << "  const R " << sz << " = (" << rt << ") ? " << s << " : -" << s << ";\n";
+        }
+        for (uint32_t k = 0; k < NR_; ++k) {
+            int par = 0;
+            if (b0.kind == Bit::REG) par ^= int((k >> b0.idx) & 1);
+            if (b1.kind == Bit::REG) par ^= int((k >> b1.idx) & 1);
+            // Without run-time bits the sign is fixed: odd -> +s, even -> -s.
+            // With them, sz already carries the run-time parity; a set
+            // compile-time parity flips it.
+            const bool neg = rt.empty() ? !par : par;
+            mul_rt(k, c, sz, neg);
+        }
+    }
+};
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull) {
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string cache_dir() {
+    const char* e = std::getenv("QF_JIT_CACHE");
+    if (e && (!std::strcmp(e, "0") || !std::strcmp(e, "off") || !*e)) return "";
+    if (e) return e;
+    const char* home = std::getenv("HOME");
+    if (!home || !*home) return "";
+    return std::string(home) + "/.cache/quantumforge_amd/jit";
+}
+
+void make_dirs(const std::string& path) {
+    std::string cur;
+    for (size_t i = 0; i < path.size(); ++i) {
+        cur += path[i];
+        if (path[i] == '/' && cur.size() > 1) ::mkdir(cur.c_str(), 0755);
+    }
+    ::mkdir(path.c_str(), 0755);
+}
+
+bool read_file(const std::string& path, std::vector<char>& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !out.empty();
+}
+
+void write_file_atomic(const std::string& dir, const std::string& name, const std::vector<char>& data) {
+    if (dir.empty()) return;
+    make_dirs(dir);
+    const std::string tmp = dir + "/." + name + "." + std::to_string(::getpid()) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return;
+        f.write(data.data(), std::streamsize(data.size()));
+        if (!f) return;
+    }
+    std::rename(tmp.c_str(), (dir + "/" + name).c_str());
+}
+
+std::string compile_one(const std::string& src, const std::string& fname, std::vector<char>& code) {
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), fname.c_str(), 0, nullptr, nullptr);
+    if (r != HIPRTC_SUCCESS) return std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r);
+    const std::vector<std::string> opts = jit_compile_options();
+    std::vector<const char*> argv;
+    for (const auto& s : opts) argv.push_back(s.c_str());
+    r = hiprtcCompileProgram(prog, int(argv.size()), argv.data());
+    std::string log;
+    size_t logn = 0;
+    if (hiprtcGetProgramLogSize(prog, &logn) == HIPRTC_SUCCESS && logn > 1) {
+        log.resize(logn);
+        hiprtcGetProgramLog(prog, &log[0]);
+    }
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return std::string("hiprtcCompileProgram: ") + hiprtcGetErrorString(r) + "\n" + log.substr(0, 4000);
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return "";
+}
+
+std::mutex g_mu;
+std::map<std::string, std::shared_ptr<JitProgram>> g_cache;
+
+}  // namespace
+
+std::vector<std::string> jit_compile_options() {
+    return {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast", "-Xclang", "-target-feature", "-Xclang",
+            "-packed-fp32-ops"};
+}
+
+std::string jit_pass_source(const Pass& p, uint32_t qubits, int precision, const std::string& name,
+                            std::string* err) {
+    PassGen g(p, qubits, precision, name);
+    if (!g.run(err)) return "";
+    return g.src;
+}
+
+std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, std::string* err) {
+    // Generate every compiled pass with placeholder names; the source text
+    // (with the compile options) is the cache key, and the final kernel names
+    // carry the key.
+    std::vector<std::string> bodies;
+    prog->kernel_of_pass.assign(plan.passes.size(), -1);
+    prog->threads.clear();
+    for (size_t i = 0; i < plan.passes.size(); ++i) {
+        const Pass& p = plan.passes[i];
+        if (p.kind != Pass::TILE || p.reg_bits == 0) continue;
+        std::string e;
+        std::string body = jit_pass_source(p, plan.qubits, precision, "qf_pass_@" + std::to_string(bodies.size()), &e);
+        if (body.empty()) {
+            if (err) *err = "pass " + std::to_string(i) + ": " + e;
+            return "";
+        }
+        prog->kernel_of_pass[i] = int(bodies.size());
+        prog->threads.push_back(1u << (p.tile_bits - p.reg_bits));
+        bodies.push_back(std::move(body));
+    }
+    const std::string prelude = precision == 0 ? "struct alignas(16) C { double re, im; };\n"
+                                               : "struct alignas(8) C { float re, im; };\n";
+    std::string all = prelude;
+    for (const auto& b : bodies) all += b;
+    std::string optstr;
+    for (const auto& o : jit_compile_options()) optstr += o + " ";
+    char keyb[32];
+    std::snprintf(keyb, sizeof keyb, "%016llx", (unsigned long long)fnv1a(optstr, fnv1a(all)));
+    prog->key = keyb;
+    prog->names.clear();
+    prog->bodies.clear();
+    for (size_t i = 0; i < bodies.size(); ++i) {
+        prog->names.push_back("qf_pass_" + prog->key + "_" + std::to_string(i));
+        const std::string tag = "qf_pass_@" + std::to_string(i);
+        bodies[i].replace(bodies[i].find(tag), tag.size(), prog->names[i]);
+        prog->bodies.push_back(std::move(bodies[i]));
+    }
+    prog->prelude = prelude;
+    std::string out = prelude;
+    for (const auto& b : prog->bodies) out += b;
+    return out;
+}
+
+std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err) {
+    auto prog = std::make_shared<JitProgram>();
+    std::string gen_err;
+    if (jit_plan_source(plan, precision, prog.get(), &gen_err).empty()) {
+        if (err) *err = gen_err;
+        return nullptr;
+    }
+    const std::string key = prog->key;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) return it->second;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    // Chunks of 8 kernels, compiled by a few host threads; each chunk is its
+    // own code object (kernel names are unique).
+    const size_t nk = prog->bodies.size();
+    const size_t nchunk = (nk + 7) / 8;
+    prog->codes.assign(nchunk, {});
+    prog->code_of_kernel.assign(nk, 0);
+    std::vector<std::string> srcs(nchunk, prog->prelude);
+    for (size_t i = 0; i < nk; ++i) {
+        const size_t c = i / 8;
+        prog->code_of_kernel[i] = int(c);
+        srcs[c] += prog->bodies[i];
+    }
+    const std::string dir = cache_dir();
+    bool cached = !dir.empty() && nchunk > 0;
+    for (size_t c = 0; c < nchunk && cached; ++c)
+        cached = read_file(dir + "/" + key + "_" + std::to_string(c) + ".co", prog->codes[c]);
+    if (cached) {
+        prog->from_disk = true;
+    } else if (nchunk > 0) {
+        std::vector<std::string> errs(nchunk);
+        std::atomic<size_t> next{0};
+        const size_t hw = std::max<size_t>(1, std::min<size_t>(8, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (size_t w = 0; w < std::min(hw, nchunk); ++w)
+            pool.emplace_back([&] {
+                for (size_t c = next++; c < nchunk; c = next++)
+                    errs[c] = compile_one(srcs[c], "qf_pass_" + key + "_" + std::to_string(c) + ".hip", prog->codes[c]);
+            });
+        for (auto& th : pool) th.join();
+        for (size_t c = 0; c < nchunk; ++c)
+            if (!errs[c].empty()) {
+                if (err) *err = errs[c];
+                return nullptr;
+            }
+        for (size_t c = 0; c < nchunk; ++c) write_file_atomic(dir, key + "_" + std::to_string(c) + ".co", prog->codes[c]);
+    }
+    prog->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto ins = g_cache.emplace(key, prog);
+    return ins.first->second;
+}
+
+}  // namespace qf

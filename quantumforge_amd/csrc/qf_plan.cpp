@@ -491,7 +491,8 @@ std::string describe_plan(const Plan& plan) {
         }
         s += "]";
     };
-    s += "{\"qubits\":" + std::to_string(plan.qubits) + ",\"passes\":[";
+    s += "{\"qubits\":" + std::to_string(plan.qubits) + ",\"compiled\":" + std::to_string(plan.compiled) +
+         ",\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const Pass& p = plan.passes[i];
         if (i) s += ",";
@@ -504,7 +505,8 @@ std::string describe_plan(const Plan& plan) {
                 const TileOp& o = p.ops[k];
                 if (k) s += ",";
                 s += "{\"kind\":" + std::to_string(o.kind) + ",\"q0\":" + std::to_string(o.q0) + ",\"q1\":" +
-                     std::to_string(o.q1) + ",\"m\":";
+                     std::to_string(o.q1) + ",\"flags\":" + std::to_string(o.flags) + ",\"slot\":" +
+                     std::to_string(o.slot) + ",\"m\":";
                 cplx(o.m, o.kind == TOP_MAT2 ? 16 : 4);
                 s += "}";
             }
@@ -574,9 +576,16 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     uint32_t low = opt.low_qubits ? opt.low_qubits : default_low_bits(precision);
     if (M > 14) return "tile_qubits must be at most 14";
     // Register kernel: 2^RB amplitudes per thread, 2^(M-RB) <= 256 threads.
+    // Compiled passes take the same plan with 2 <= RB <= 5 and up to 1024 threads.
     uint32_t RB = opt.reg_bits ? opt.reg_bits : (precision == 0 ? 3 : 4);
+    const bool want_compiled =
+        opt.kernel == 3 || (opt.kernel == 0 && qubits >= kCompileMinQubits && !opt.reg_bits);
     bool regk = opt.kernel != 1 && M <= qubits && RB >= 3 && RB <= 4 && M >= RB + 6 && M - RB <= 8;
+    if (opt.kernel == 3) regk = M <= qubits && RB >= 2 && RB <= 5 && M >= RB + 6 && M - RB <= 10;
     if (opt.kernel == 2 && !regk) return "register kernel needs tile_qubits <= qubits and 6 <= tile - reg_bits <= 8";
+    if (opt.kernel == 3 && !regk)
+        return "compiled passes need tile_qubits <= qubits, 2 <= reg_bits <= 5 and 6 <= tile - reg_bits <= 10";
+    out.compiled = regk && want_compiled ? (opt.kernel == 3 ? 2 : 1) : 0;
     if (M > qubits) M = qubits;
     if (low > M) low = M;
     // A tile smaller than the state must leave room for the two qubits of a

@@ -13,10 +13,13 @@
 //     global index, so they never need their qubits inside the tile.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
 namespace qf {
+
+struct JitProgram;
 
 struct cd {
     double re, im;
@@ -88,9 +91,14 @@ struct PlanOptions {
     uint32_t tile_qubits = 0;  // 0 = default for the precision
     uint32_t low_qubits = 0;   // 0 = default
     uint32_t max_block = 0;
-    int kernel = 0;            // 0 = default, 1 = LDS op-by-op tile kernel, 2 = register kernel
+    int kernel = 0;            // 0 = default, 1 = LDS op-by-op tile kernel, 2 = register kernel,
+                               // 3 = compiled passes (register-kernel plan, one generated kernel per pass)
     uint32_t reg_bits = 0;     // register kernel: amplitudes per thread = 2^reg_bits (0 = default)
 };
+
+// Circuits of at least this many qubits run compiled passes by default
+// (smaller states are launch-bound and not worth a compile).
+constexpr uint32_t kCompileMinQubits = 18;
 
 struct PlanStats {
     uint64_t passes = 0, ops_in = 0, ops_applied = 0, dense_blocks = 0, max_ops_per_pass = 0, phases = 0;
@@ -112,6 +120,11 @@ struct Plan {
     int precision = 0;
     std::vector<Pass> passes;
     PlanStats stats;
+    int compiled = 0;          // 1: register passes run as compiled kernels (2: explicitly requested)
+    // Compiled programs per precision, attached on first launch (jit_state:
+    // 0 not tried, 1 compiled, -1 the compile failed and the register kernel runs).
+    mutable std::shared_ptr<JitProgram> jit[2];
+    mutable int jit_state[2] = {0, 0};
 };
 
 // Builds a plan; returns an empty string on success or an error message.

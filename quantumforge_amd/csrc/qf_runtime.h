@@ -15,6 +15,10 @@
         if (e_ != hipSuccess) return ::qfrt::hip_fail(e_, #expr);   \
     } while (0)
 
+struct qf_plan {
+    qf::Plan plan;
+};
+
 namespace qfrt {
 
 // Extra inputs of a plan launch: the per-instance angle table (cos, sin of
@@ -28,6 +32,7 @@ struct LaunchCtx {
 };
 
 qf_status fail(qf_status st, const std::string& msg);
+qf::PlanOptions plan_options_from(const qf_plan_options* o);
 qf_status hip_fail(hipError_t e, const char* what);
 // cos and sin as two separate libm calls (the reference's f64::cos, f64::sin),
 // out of line so that no compiler merges a pair into sincos, whose last bit

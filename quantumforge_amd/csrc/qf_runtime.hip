@@ -7,12 +7,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/qf_hip.h"
 #include "qf_kernels.h"
 #include "qf_regtile.h"
+#include "qf_jit.h"
 #include "qf_plan.h"
 #include "qf_runtime.h"
 
@@ -120,6 +123,19 @@ void gate_matrix(int gate, double theta, cd m[4]) {
 
 bool is_rotation(int gate) { return gate == QF_GATE_RX || gate == QF_GATE_RY || gate == QF_GATE_RZ; }
 
+qf::PlanOptions plan_options_from(const qf_plan_options* o) {
+    qf::PlanOptions po;
+    if (o) {
+        po.fuse = o->fuse;
+        po.tile_qubits = o->tile_qubits;
+        po.low_qubits = o->low_qubits;
+        po.max_block = o->max_block;
+        po.kernel = o->kernel;
+        po.reg_bits = o->reg_bits;
+    }
+    return po;
+}
+
 }  // namespace qfrt
 
 // ---------------------------------------------------------------------------
@@ -156,9 +172,6 @@ struct qf_batch {
     uint64_t dim = 0;
 };
 
-struct qf_plan {
-    qf::Plan plan;
-};
 
 namespace qfrt {
 
@@ -397,15 +410,97 @@ qf_status upload_plans(qf_backend* be, const std::vector<const qf::Plan*>& plans
     return QF_OK;
 }
 
-// Launches every pass of a plan whose ops were uploaded at d_ops.
+// ---------------------------------------------------------------------------
+// Compiled passes (qf_jit.h): one program per plan and precision, loaded as
+// modules once per device.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_jit_mu;
+std::map<std::pair<std::string, int>, std::vector<hipFunction_t>> g_jit_fns;
+}  // namespace
+
+// The plan's compiled program for `precision` (compiled on first use), or
+// nullptr when the plan runs the register kernel.  A failed compile is an
+// error when compiled passes were requested explicitly; otherwise it is
+// reported once and the register kernel runs the same plan.
+qf_status plan_program(const qf::Plan& plan, int precision, const qf::JitProgram** out) {
+    *out = nullptr;
+    if (!plan.compiled) return QF_OK;
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    const int pr = precision == QF_C128 ? 0 : 1;
+    if (plan.jit_state[pr] == 0) {
+        std::string err;
+        plan.jit[pr] = qf::jit_compile_plan(plan, pr, &err);
+        plan.jit_state[pr] = plan.jit[pr] ? 1 : -1;
+        if (!plan.jit[pr]) {
+            if (plan.compiled == 2) {
+                plan.jit_state[pr] = 0;
+                return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
+            }
+            std::fprintf(stderr, "quantumforge_amd: pass compiler failed, running the register kernel: %s\n",
+                         err.c_str());
+        }
+    }
+    *out = plan.jit[pr].get();
+    return QF_OK;
+}
+
+qf_status jit_functions(const qf::JitProgram& prog, const std::vector<hipFunction_t>** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    const auto key = std::make_pair(prog.key, dev);
+    auto it = g_jit_fns.find(key);
+    if (it == g_jit_fns.end()) {
+        std::vector<hipModule_t> mods(prog.codes.size(), nullptr);
+        for (size_t c = 0; c < prog.codes.size(); ++c) HIP_TRY(hipModuleLoadData(&mods[c], prog.codes[c].data()));
+        std::vector<hipFunction_t> fns(prog.names.size(), nullptr);
+        for (size_t i = 0; i < prog.names.size(); ++i)
+            HIP_TRY(hipModuleGetFunction(&fns[i], mods[prog.code_of_kernel[i]], prog.names[i].c_str()));
+        it = g_jit_fns.emplace(key, std::move(fns)).first;
+    }
+    *out = &it->second;
+    return QF_OK;
+}
+
+qf_status launch_compiled(qf_backend* be, void* d, uint32_t qubits, uint32_t count, const qf::Pass& p,
+                          hipFunction_t fn, uint32_t threads, const LaunchCtx& ctx) {
+    const uint32_t log_blocks = qubits - p.tile_bits;
+    const uint32_t gx_log = log_blocks < 16 ? log_blocks : 16;
+    const uint64_t gy = uint64_t(count) << (log_blocks - gx_log);
+    if (gy > 65535) return fail(QF_INVALID_ARGUMENT, "too many states for one compiled-pass launch");
+    void* sp = d;
+    const double* cs = ctx.cs;
+    unsigned nslots = ctx.nslots;
+    void* args[] = {&sp, &cs, &nslots};
+    HIP_TRY(hipModuleLaunchKernel(fn, 1u << gx_log, uint32_t(gy), 1, threads, 1, 1, 0, be->stream, args, nullptr));
+    return QF_OK;
+}
+
+// Launches every pass of a plan whose ops were uploaded at d_ops (d_ops may
+// be null when every tile pass runs compiled).
 qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan,
                       const qf::TileOp* d_ops, const LaunchCtx& ctx) {
     if (plan.qubits != qubits) return fail(QF_INCOMPATIBLE_DEVICE_STATE, "plan was built for a different qubit count");
+    const qf::JitProgram* prog = nullptr;
+    const std::vector<hipFunction_t>* fns = nullptr;
+    if (!ctx.lam && plan.compiled) {
+        qf_status s = plan_program(plan, precision, &prog);
+        if (s) return s;
+        if (prog && (s = jit_functions(*prog, &fns))) return s;
+    }
     const qf::TileOp* cur = d_ops;
-    for (const qf::Pass& p : plan.passes) {
-        qf_status s = precision == QF_C128
-                          ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, p, cur, ctx)
-                          : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, p, cur, ctx);
+    for (size_t i = 0; i < plan.passes.size(); ++i) {
+        const qf::Pass& p = plan.passes[i];
+        qf_status s;
+        const int kern = prog ? prog->kernel_of_pass[i] : -1;
+        if (kern >= 0) {
+            s = launch_compiled(be, d, qubits, count, p, (*fns)[kern], prog->threads[kern], ctx);
+        } else {
+            if (p.kind == qf::Pass::TILE && !cur) return fail(QF_INVALID_ARGUMENT, "tile pass without uploaded ops");
+            s = precision == QF_C128 ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, p, cur, ctx)
+                                     : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, p, cur, ctx);
+        }
         if (s) return s;
         if (cur) cur += p.ops.size();
     }
@@ -414,6 +509,12 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
 
 // Runs a plan on `count` contiguous states of `qubits` qubits.
 qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
+    if (plan.compiled) {
+        const qf::JitProgram* prog = nullptr;
+        qf_status s = plan_program(plan, precision, &prog);
+        if (s) return s;
+        if (prog) return launch_plan(be, d, precision, qubits, count, plan, nullptr, LaunchCtx{});
+    }
     std::vector<const qf::TileOp*> bases;
     qf_status s = upload_plans(be, {&plan}, bases);
     if (s) return s;
@@ -1067,15 +1168,7 @@ extern "C" qf_status qf_plan_build(const qf_op* ops, size_t count, uint32_t qubi
     std::vector<qf::GateIn> g;
     s = lower_ops(ops, count, qubits, g);
     if (s) return s;
-    qf::PlanOptions po;
-    if (o) {
-        po.fuse = o->fuse;
-        po.tile_qubits = o->tile_qubits;
-        po.low_qubits = o->low_qubits;
-        po.max_block = o->max_block;
-        po.kernel = o->kernel;
-        po.reg_bits = o->reg_bits;
-    }
+    const qf::PlanOptions po = plan_options_from(o);
     qf_plan* p = new qf_plan;
     std::string err = qf::build_plan(g, qubits, QF_C128, po, p->plan);
     if (!err.empty()) {
@@ -1106,6 +1199,34 @@ extern "C" qf_status qf_plan_describe(const qf_plan* p, char* buf, size_t cap, s
         std::memcpy(buf, s.data(), n);
         buf[n] = '\0';
     }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_plan_source(const qf_plan* p, int32_t precision, char* buf, size_t cap, size_t* needed) {
+    if (!p) return fail(QF_INVALID_ARGUMENT, "null plan");
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    qf::JitProgram prog;
+    std::string err;
+    const std::string s = qf::jit_plan_source(p->plan, precision == QF_C128 ? 0 : 1, &prog, &err);
+    if (s.empty()) return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
+    if (needed) *needed = s.size() + 1;
+    if (buf && cap) {
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return QF_OK;
+}
+
+extern "C" qf_status qf_plan_compile(const qf_plan* p, int32_t precision, double* compile_ms, int32_t* from_cache) {
+    if (!p) return fail(QF_INVALID_ARGUMENT, "null plan");
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    if (!p->plan.compiled) return fail(QF_INVALID_ARGUMENT, "the plan does not run compiled passes");
+    std::string err;
+    auto prog = qf::jit_compile_plan(p->plan, precision == QF_C128 ? 0 : 1, &err);
+    if (!prog) return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
+    if (compile_ms) *compile_ms = prog->compile_ms;
+    if (from_cache) *from_cache = prog->from_disk ? 1 : 0;
     return QF_OK;
 }
 

@@ -1,0 +1,99 @@
+"""CPU emulation of compiled-pass kernels (test infrastructure).
+
+The pass compiler emits plain HIP C++ that uses only threadIdx, blockIdx,
+gridDim, a static __shared__ array and __syncthreads().  Here that source is
+compiled with g++ under a small emulation of those built-ins: every workgroup
+runs as real host threads (one per work item) meeting at a std::barrier, one
+workgroup after another.  This checks what the generated code computes (index
+scatter, register renaming, phase exchanges through the shared array) against
+the oracle without a GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+PRELUDE = r"""
+#include <barrier>
+#include <cstdint>
+#include <thread>
+#include <vector>
+struct qf_dim3 { unsigned x, y, z; };
+static thread_local qf_dim3 qf_tid, qf_bid;
+static qf_dim3 qf_grid;
+static std::barrier<>* qf_bar;
+#define threadIdx qf_tid
+#define blockIdx qf_bid
+#define gridDim qf_grid
+#define __global__
+#define __launch_bounds__(n)
+#define __shared__ static
+#define __syncthreads() qf_bar->arrive_and_wait()
+"""
+
+LAUNCHER = r"""
+typedef void (*qf_kfn)(C*, const double*, unsigned);
+extern "C" void qf_emu_launch(qf_kfn fn, C* s, const double* cs, unsigned nslots, unsigned gx, unsigned gy,
+                              unsigned nt) {
+    qf_grid = {gx, gy, 1};
+    for (unsigned by = 0; by < gy; ++by)
+        for (unsigned bx = 0; bx < gx; ++bx) {
+            std::barrier<> bar(nt);
+            qf_bar = &bar;
+            std::vector<std::thread> th;
+            th.reserve(nt);
+            for (unsigned t = 0; t < nt; ++t)
+                th.emplace_back([=] {
+                    qf_tid = {t, 0, 0};
+                    qf_bid = {bx, by, 0};
+                    fn(s, cs, nslots);
+                });
+            for (auto& x : th) x.join();
+        }
+}
+"""
+
+KERNEL_RE = re.compile(r'extern "C" __global__ __launch_bounds__\((\d+)\) void (\w+)\(')
+
+
+class EmulatedProgram:
+    """g++ build of a plan's generated source; run() applies every pass."""
+
+    def __init__(self, source: str, workdir: str | None = None) -> None:
+        self.kernels = [(name, int(nt)) for nt, name in KERNEL_RE.findall(source)]
+        self.complex64 = "float re, im" in source
+        full = PRELUDE + source + LAUNCHER
+        key = hashlib.sha1(full.encode()).hexdigest()[:16]
+        workdir = workdir or tempfile.mkdtemp(prefix="qf_emu_")
+        src = os.path.join(workdir, f"emu_{key}.cpp")
+        so = os.path.join(workdir, f"emu_{key}.so")
+        if not os.path.exists(so):
+            with open(src, "w") as f:
+                f.write(full)
+            subprocess.run(["g++", "-O1", "-std=c++20", "-shared", "-fPIC", "-pthread", "-ffp-contract=off",
+                            "-w", src, "-o", so], check=True, capture_output=True, text=True)
+        self.lib = C.CDLL(so)
+        self.launch = self.lib.qf_emu_launch
+        self.launch.restype = None
+        self.launch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+
+    def run(self, amps: np.ndarray, qubits: int, tile_bits: list[int], count: int = 1,
+            cs: np.ndarray | None = None, nslots: int = 0) -> np.ndarray:
+        """Applies the passes in order to `count` contiguous states (a copy);
+        tile_bits[i] is the tile size of compiled kernel i."""
+        dt = np.complex64 if self.complex64 else np.complex128
+        s = np.ascontiguousarray(amps.astype(dt, copy=True))
+        csp = C.c_void_p(cs.ctypes.data) if cs is not None and cs.size else C.c_void_p(0)
+        for (name, nt), M in zip(self.kernels, tile_bits):
+            fn = C.cast(getattr(self.lib, name), C.c_void_p)
+            logb = qubits - M
+            gx_log = min(logb, 16)
+            gy = count << (logb - gx_log)
+            self.launch(fn, C.c_void_p(s.ctypes.data), csp, nslots, 1 << gx_log, gy, nt)
+        return s

@@ -1,0 +1,139 @@
+"""Compiled passes on the CPU: the generated kernel source, run under a g++
+emulation of the HIP built-ins (tests/jit_emu.py), against the oracle; the
+plan flags and errors of the pass compiler; hiprtc compiling a plan for gfx950
+without a device."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from quantumforge_amd import ParameterizedCircuit, QuantumError, RotationAxis
+from quantumforge_amd.backend import Plan
+from quantumforge_amd.workloads import (MaxCutProblem, hea_circuit, qaoa_parameterized, qft_like_circuit,
+                                        random_layer_circuit, uniform_parameters)
+
+from .conftest import ROOT, close
+from .jit_emu import EmulatedProgram
+
+EMU_DIR = os.path.join(ROOT, "tests", ".emu_cache")
+
+
+def emulate(plan: Plan, amps: np.ndarray, qubits: int, count: int = 1, cs=None, nslots=0, precision="complex128"):
+    os.makedirs(EMU_DIR, exist_ok=True)
+    tb = [p["tile_bits"] for p in plan.describe()["passes"] if p["kind"] == "tile"]
+    emu = EmulatedProgram(plan.source(precision), EMU_DIR)
+    return emu.run(amps, qubits, tb, count=count, cs=cs, nslots=nslots)
+
+
+def random_state(n, seed, count=1):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(count, 1 << n)) + 1j * rng.normal(size=(count, 1 << n))
+    return a / np.linalg.norm(a, axis=1, keepdims=True)
+
+
+# (qubits, depth, seed, tile_qubits, reg_bits): register sets of 2..5 bits,
+# 64..512 threads, tiles from 8 qubits to the whole state.
+SHAPES = [(12, 10, 1, 9, 3), (13, 12, 2, 10, 4), (12, 8, 5, 8, 2), (11, 12, 7, 11, 5)]
+
+
+@pytest.mark.parametrize("n,depth,seed,M,RB", SHAPES)
+def test_generated_kernels_match_oracle_on_random_circuits(n, depth, seed, M, RB):
+    c = random_layer_circuit(n, depth, seed=seed, pairs_per_layer=n // 2)
+    plan = Plan(c, n, kernel=3, tile_qubits=M, reg_bits=RB)
+    assert plan.compiled
+    a0 = random_state(n, seed)[0]
+    st = O.State(n, a0)
+    st.run(c.to_ops())
+    close(emulate(plan, a0, n), st.amplitudes(), 1e-13)
+
+
+def test_generated_kernels_on_a_batch_and_controlled_phases():
+    # Two instances (the instance index lands above the state's qubits) of a
+    # QFT-like circuit: controlled phases become diagonal ops on any qubit.
+    n = 12
+    _, c = qft_like_circuit(n, 2, seed=4, max_distance=6)
+    plan = Plan(c, n, kernel=3, tile_qubits=9, reg_bits=3)
+    a0 = random_state(n, 8, count=2)
+    out = emulate(plan, a0.ravel(), n, count=2).reshape(2, -1)
+    for i in range(2):
+        st = O.State(n, a0[i])
+        st.run(c.to_ops())
+        close(out[i], st.amplitudes(), 1e-13)
+
+
+def test_generated_kernels_complex64():
+    n = 12
+    c = random_layer_circuit(n, 8, seed=11, pairs_per_layer=6)
+    plan = Plan(c, n, kernel=3, tile_qubits=10, reg_bits=4)
+    a0 = random_state(n, 3)[0]
+    st = O.State(n, a0)
+    st.run(c.to_ops())
+    out = emulate(plan, a0.astype(np.complex64), n, precision="complex64")
+    assert out.dtype == np.complex64
+    close(out, st.amplitudes(), 2e-5)
+
+
+def mixed_rotations(n: int) -> ParameterizedCircuit:
+    pc = ParameterizedCircuit(3 * n)
+    k = 0
+    for layer in range(3):
+        for q in range(n):
+            pc.rotation([RotationAxis.X, RotationAxis.Y, RotationAxis.Z][(q + layer) % 3], q, k)
+            k += 1
+        for q in range(0, n - 1, 2 if layer % 2 else 1):
+            t = (q + 1 + layer) % n
+            if t != q:
+                pc.cnot(q, t)
+    return pc
+
+
+@pytest.mark.parametrize("case", ["hea", "qaoa", "mixed"])
+def test_generated_kernels_read_rotation_angles_per_instance(case):
+    # Rotations (X/Y on register bits, Z and ZZ on any qubit) read
+    # (cos, sin) of their half angle from the instance's table row.
+    if case == "hea":
+        n, pc, M, RB = 12, hea_circuit(12, 3), 9, 3
+    elif case == "qaoa":
+        n, M, RB = 12, 10, 4
+        pc = qaoa_parameterized(n, MaxCutProblem.random_3_regular(n, 5).edges, 2)
+    else:
+        n, pc, M, RB = 11, mixed_rotations(11), 9, 3
+    P = pc.parameter_count
+    plan = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB)
+    pops = pc.to_pops()
+    slots = [int(r["index"]) for r in pops if r["kind"] == 1]
+    sets = uniform_parameters(P, 2, seed=7)
+    cs = np.concatenate([np.stack([np.cos(s[slots] / 2), np.sin(s[slots] / 2)], axis=1).ravel() for s in sets])
+    a0 = np.zeros(2 << n, complex)
+    a0[0] = a0[1 << n] = 1
+    out = emulate(plan, a0, n, count=2, cs=cs, nslots=len(slots)).reshape(2, -1)
+    for i in range(2):
+        close(out[i], O.pc_run(pops, P, 0, sets[i], [], n), 1e-13)
+
+
+def test_compiled_flag_defaults_and_errors():
+    c = random_layer_circuit(18, 2, seed=1, pairs_per_layer=9)
+    assert Plan(c, 18).describe()["compiled"] == 1          # default from 18 qubits
+    assert Plan(c, 18, kernel=2).describe()["compiled"] == 0
+    c17 = random_layer_circuit(17, 2, seed=1, pairs_per_layer=8)
+    assert Plan(c17, 17).describe()["compiled"] == 0
+    assert Plan(c17, 17, kernel=3).describe()["compiled"] == 2
+    assert Plan(c17, 17, fuse=False).describe()["compiled"] == 0
+    with pytest.raises(QuantumError, match="compiled passes need"):
+        Plan(c17, 17, kernel=3, tile_qubits=8, reg_bits=3)
+    with pytest.raises(QuantumError, match="does not run compiled passes"):
+        Plan(c17, 17, kernel=2).compile()
+
+
+def test_hiprtc_compiles_a_plan_for_gfx950_without_a_device(tmp_path, monkeypatch):
+    monkeypatch.setenv("QF_JIT_CACHE", str(tmp_path))
+    c = random_layer_circuit(14, 3, seed=21, pairs_per_layer=7)
+    plan = Plan(c, 14, kernel=3, tile_qubits=10, reg_bits=3)
+    r = plan.compile()
+    assert r["compile_ms"] > 0
+    files = os.listdir(tmp_path)
+    assert files and all(f.endswith(".co") for f in files)
+    # The code object is an AMDGPU ELF.
+    blob = open(os.path.join(tmp_path, files[0]), "rb").read()
+    assert blob[:4] == b"\x7fELF"
