@@ -586,6 +586,9 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     if (opt.kernel == 3 && !regk)
         return "compiled passes need tile_qubits <= qubits, 2 <= reg_bits <= 5 and 6 <= tile - reg_bits <= 10";
     out.compiled = regk && want_compiled ? (opt.kernel == 3 ? 2 : 1) : 0;
+    // Compiled passes need only one 128-byte run of contiguous amplitudes
+    // per tile for full cache lines; the freed tile slots take gate qubits.
+    if (out.compiled && !opt.low_qubits) low = precision == 0 ? 3 : 4;
     if (M > qubits) M = qubits;
     if (low > M) low = M;
     // A tile smaller than the state must leave room for the two qubits of a
