@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -28,6 +30,43 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def inplace_stream_rate(torch, nbytes: int) -> float:
+    """GB/s of a plain in-place elementwise multiply (torch's kernel) over
+    `nbytes` of device memory: the same one-read-one-write pattern as a tile
+    pass, i.e. the streaming rate this GPU reaches for it."""
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    x.mul_(1.0)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(5):
+        x.mul_(1.0)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / 5
+    del x
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def measured_traffic(n, args, tile_bits, low, reg_bits, passes, compiled):
+    """HBM bytes per tile-pass launch measured with rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, copy-calibrated) for this exact workload, from
+    profiles/hbm_traffic.json; None when the workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    want = {"qubits": n, "depth": args.depth, "seed": args.seed, "precision": args.precision,
+            "pass_kernels": "compiled" if compiled else ("register" if reg_bits else "lds"),
+            "tile_qubits": tile_bits, "low_qubits": low, "register_bits": reg_bits, "passes": passes}
+    for e in table.get("entries", []):
+        if all(e.get(k) == v for k, v in want.items()):
+            return e["bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(circuit, qubits: int, sample_ops: int):
@@ -109,6 +148,84 @@ def vqe_bench(be, sets: int, shift_sets: int, precision: str, with_cpu: bool) ->
     return res
 
 
+def circuit_section(be, torch, circuit, n: int, precision: str, steps: int, warmup: int, label: str) -> dict:
+    """Times `steps` runs of `circuit` on a resident |0..0> state with its
+    default plan (HIP events on the engine's stream)."""
+    from quantumforge_amd.backend import Plan
+
+    plan = Plan(circuit, n, precision=precision)
+    stats = plan.stats()
+    compiled = bool(plan.describe().get("compiled", 0))
+    if compiled:
+        plan.compile(precision)
+    st = be.zero_state(n, precision=precision)
+    stream = torch.cuda.ExternalStream(be.stream())
+    for _ in range(warmup):
+        st.run_circuit(circuit, plan)
+    be.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        st.run_circuit(circuit, plan)
+    b.record(stream)
+    be.synchronize()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    amp = 16 if precision == "complex128" else 8
+    per_pass = ms / max(1, stats["passes"])
+    res = {"workload": label, "qubits": n, "precision": precision, "ops": stats["ops_in"],
+           "passes": stats["passes"], "pass_kernels": "compiled" if compiled else "generic",
+           "ms_per_circuit": round(ms, 3), "ms_per_pass": round(per_pass, 4),
+           "GBs_per_pass": round(2 * (1 << n) * amp / (per_pass * 1e-3) / 1e9, 1),
+           "norm_after": st.norm_sqr(), "steps": steps, "warmup": warmup}
+    del st
+    return res
+
+
+def qft_like_bench(be, torch, steps: int) -> list:
+    """33 qubits complex128 and 34 qubits complex64 (128 GiB each): 2 layers of
+    H on every qubit, exact controlled phases CP(pi/2^d) at distances d <= 8,
+    RZ(2 pi u) on every qubit (SmallRng seed 33)."""
+    from quantumforge_amd.workloads import qft_like_circuit
+
+    out = []
+    for n, prec in ((33, "complex128"), (34, "complex64")):
+        _, native = qft_like_circuit(n, 2, seed=33)
+        r = circuit_section(be, torch, native, n, prec, steps, 1,
+                            f"QFT-like, {n} qubits, 2 layers, exact controlled phases, {prec}")
+        r["layers_per_s"] = round(2 / (r["ms_per_circuit"] * 1e-3), 3)
+        log(f"qft-like {n}q {prec}: {r['ms_per_circuit']:.1f} ms/circuit, {r['passes']} passes, "
+            f"{r['GBs_per_pass']:.0f} GB/s per pass")
+        out.append(r)
+    return out
+
+
+def uneven_bench(be, count: int, seed: int) -> dict:
+    """QAOA MaxCut adjoint gradients of `count` instances of different widths
+    (even n in 18..26, random 3-regular graphs with U[0.5, 1) weights, p = 6,
+    gamma, beta ~ U[0, pi]) through the uneven-batch engine."""
+    from quantumforge_amd import gradients as G
+    from quantumforge_amd.workloads import (maxcut_cost_hamiltonian, qaoa_occurrence_angles, qaoa_parameterized,
+                                            qaoa_uneven_instances)
+
+    specs = qaoa_uneven_instances(count, seed=seed, p=6)
+    insts = []
+    for sp in specs:
+        prob = sp["problem"]
+        pc = qaoa_parameterized(prob.num_qubits, prob.edges, sp["p"])
+        ang = qaoa_occurrence_angles(prob.edges, prob.num_qubits, sp["p"], sp["gammas"], sp["betas"])
+        insts.append((pc, prob.num_qubits, ang, maxcut_cost_hamiltonian(prob)))
+    t0 = time.perf_counter()
+    e, g, stats = G.uneven_adjoint_gradient(be, insts)
+    dt = time.perf_counter() - t0
+    occ = sum(pc.parameter_count for pc, _, _, _ in insts)
+    log(f"uneven: {count} instances, {occ} rotation occurrences, {dt:.2f} s, stats {stats}")
+    return {"workload": f"{count} QAOA MaxCut instances, even n in 18..26, p=6, complex128, seed {seed}",
+            "seconds": round(dt, 3), "instances_per_s": round(count / dt, 2),
+            "occurrence_grads_per_s": round(occ / dt, 1), "stats": stats,
+            "mean_qubits": float(np.mean([q for _, q, _, _ in insts]))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +244,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vqe-sets", type=int, default=4096, help="parameter sets of the 16-qubit VQE adjoint (0 = skip)")
     ap.add_argument("--vqe-shift-sets", type=int, default=4, help="sets timed with the parameter-shift GPU form")
+    ap.add_argument("--qft-steps", type=int, default=2, help="timed runs of the 33q/34q QFT-like circuits (0 = skip)")
+    ap.add_argument("--small-steps", type=int, default=20, help="timed runs of the 20-qubit circuit (0 = skip)")
+    ap.add_argument("--uneven-instances", type=int, default=0,
+                    help="uneven QAOA adjoint batch size (0 = skip; the full workload is 512)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 launches) to scale HBM counters")
     args = ap.parse_args()
@@ -155,7 +276,7 @@ def main() -> int:
     n = args.qubits
     circuit = random_layer_circuit(n, args.depth, seed=args.seed)
     plan = Plan(circuit, n, fuse=True, tile_qubits=args.tile, low_qubits=args.low, kernel=args.kernel,
-                reg_bits=args.reg_bits)
+                reg_bits=args.reg_bits, precision=args.precision)
     stats = plan.stats()
     desc = plan.describe()
     tile_bits = desc["passes"][0]["tile_bits"]
@@ -195,6 +316,9 @@ def main() -> int:
 
     norm = state.norm_sqr()
     log(f"norm after {args.warmup + args.steps} runs: {norm:.15f}")
+    del state
+    ceiling = inplace_stream_rate(torch, (1 << n) * amp_bytes)
+    log(f"in-place streaming rate of a plain torch multiply over the same bytes: {ceiling:.0f} GB/s")
 
     ms_per_step = gpu_ms / args.steps
     layers_per_s = args.depth / (ms_per_step / 1e3)
@@ -235,12 +359,24 @@ def main() -> int:
                        else ("k_tile_reg" if reg_bits else "k_tile")),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "inplace_stream_GBs": round(ceiling, 1),
         },
         "cpu_baseline": None,
     }
-    del state
+    out["roofline"]["traffic"] = measured_traffic(n, args, tile_bits, desc["passes"][0].get("low", 0), reg_bits,
+                                                  stats["passes"], compiled)
     if args.vqe_sets > 0:
         out["vqe_adjoint"] = vqe_bench(be, args.vqe_sets, args.vqe_shift_sets, args.precision, not args.no_cpu)
+    if args.small_steps > 0:
+        c20 = random_layer_circuit(20, args.depth, seed=args.seed)
+        r = circuit_section(be, torch, c20, 20, "complex128", args.small_steps, 2,
+                            f"random layered circuit, 20 qubits, depth {args.depth}, complex128 (16 MiB: cache-resident)")
+        r["layers_per_s"] = round(args.depth / (r["ms_per_circuit"] * 1e-3), 2)
+        out["small_20q"] = r
+    if args.qft_steps > 0:
+        out["qft_like"] = qft_like_bench(be, torch, args.qft_steps)
+    if args.uneven_instances > 0:
+        out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512)
     if not args.no_cpu:
         log("timing the CPU baseline (oracle restatement of the reference CPU path)")
         out["cpu_baseline"] = cpu_baseline(circuit, n, args.cpu_sample_ops)

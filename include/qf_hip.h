@@ -148,6 +148,7 @@ typedef struct {
     int32_t kernel;        /* 0 default, 1 LDS op-by-op tile kernel, 2 register-resident tile kernel,
                               3 compiled passes (one generated kernel per pass; default from 18 qubits) */
     uint32_t reg_bits;     /* register kernel: 2^reg_bits amplitudes per thread (0 = default) */
+    int32_t precision;     /* QF_C128 or QF_C64: the amplitude type the defaults are chosen for */
 } qf_plan_options;
 
 /* Summary of a plan (counts the HBM sweeps the plan streams). */

@@ -33,7 +33,8 @@ ROT_DTYPE = np.dtype([("axis", "<i4"), ("qubit", "<u4"), ("theta", "<f8")])
 
 class PlanOptions(C.Structure):
     _fields_ = [("fuse", C.c_int32), ("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32),
-                ("max_block", C.c_uint32), ("kernel", C.c_int32), ("reg_bits", C.c_uint32)]
+                ("max_block", C.c_uint32), ("kernel", C.c_int32), ("reg_bits", C.c_uint32),
+                ("precision", C.c_int32)]
 
 
 class PlanStats(C.Structure):

@@ -28,9 +28,11 @@ class Plan:
     """Fused execution plan of a circuit (qf_plan_build)."""
 
     def __init__(self, circuit: Circuit, qubits: int, fuse: bool = True, tile_qubits: int = 0,
-                 low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0) -> None:
+                 low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0, precision="complex128") -> None:
+        """precision picks the default tile shape (complex64 tiles hold twice
+        the amplitudes in the same LDS); the plan runs on either precision."""
         self._ops = circuit.to_ops()
-        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits)
+        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits, _precision(precision))
         h = C.c_void_p()
         check(lib().qf_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
         self._h = h
@@ -38,14 +40,14 @@ class Plan:
 
     @classmethod
     def parameterized(cls, pc, qubits: int, inputs=None, reverse: bool = False, tile_qubits: int = 0,
-                      low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0) -> "Plan":
+                      low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0, precision="complex128") -> "Plan":
         """The plan the gradient engine runs for a ParameterizedCircuit
         (qf_plan_build_parameterized): rotations read their angles per
         instance; reverse gives the inverted, recorded reverse sweep."""
         self = cls.__new__(cls)
         self._ops = pc.to_pops()
         inp = np.ascontiguousarray(np.asarray(inputs if inputs is not None else [], dtype=np.float64))
-        opts = N.PlanOptions(1, tile_qubits, low_qubits, 0, kernel, reg_bits)
+        opts = N.PlanOptions(1, tile_qubits, low_qubits, 0, kernel, reg_bits, _precision(precision))
         h = C.c_void_p()
         check(lib().qf_plan_build_parameterized(ptr(self._ops), len(self._ops), pc.parameter_count,
                                                 pc.input_count, ptr(inp), inp.size, qubits, int(reverse),

@@ -848,7 +848,12 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
     qf::PlanOptions po = plan_options_from(options);
     po.fuse = 1;
     qf_plan* p = new qf_plan;
-    const std::string e = qf::build_plan(param_gates(pc, inputs, reverse != 0), qubits, QF_C128, po, p->plan);
+    const int precision = options ? options->precision : QF_C128;
+    if (precision != QF_C128 && precision != QF_C64) {
+        delete p;
+        return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    }
+    const std::string e = qf::build_plan(param_gates(pc, inputs, reverse != 0), qubits, precision, po, p->plan);
     if (!e.empty()) {
         delete p;
         return fail(QF_INVALID_ARGUMENT, e);

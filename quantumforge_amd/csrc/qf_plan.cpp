@@ -578,17 +578,19 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     // Register kernel: 2^RB amplitudes per thread, 2^(M-RB) <= 256 threads.
     // Compiled passes take the same plan with 2 <= RB <= 5 and up to 1024 threads.
     uint32_t RB = opt.reg_bits ? opt.reg_bits : (precision == 0 ? 3 : 4);
-    const bool want_compiled =
-        opt.kernel == 3 || (opt.kernel == 0 && qubits >= kCompileMinQubits && !opt.reg_bits);
+    const bool want_compiled = opt.kernel == 3 || (opt.kernel == 0 && qubits >= kCompileMinQubits);
     bool regk = opt.kernel != 1 && M <= qubits && RB >= 3 && RB <= 4 && M >= RB + 6 && M - RB <= 8;
-    if (opt.kernel == 3) regk = M <= qubits && RB >= 2 && RB <= 5 && M >= RB + 6 && M - RB <= 10;
+    if (want_compiled) regk = M <= qubits && RB >= 2 && RB <= 5 && M >= RB + 6 && M - RB <= 10;
     if (opt.kernel == 2 && !regk) return "register kernel needs tile_qubits <= qubits and 6 <= tile - reg_bits <= 8";
     if (opt.kernel == 3 && !regk)
         return "compiled passes need tile_qubits <= qubits, 2 <= reg_bits <= 5 and 6 <= tile - reg_bits <= 10";
     out.compiled = regk && want_compiled ? (opt.kernel == 3 ? 2 : 1) : 0;
     // Compiled passes need only one 128-byte run of contiguous amplitudes
     // per tile for full cache lines; the freed tile slots take gate qubits.
+    // complex64 tiles of 13 qubits fill the same 64 KiB of LDS as 12-qubit
+    // complex128 ones and need fewer passes.
     if (out.compiled && !opt.low_qubits) low = precision == 0 ? 3 : 4;
+    if (out.compiled && !opt.tile_qubits && precision != 0 && qubits >= 13 && M + 1 - RB <= 10) ++M;
     if (M > qubits) M = qubits;
     if (low > M) low = M;
     // A tile smaller than the state must leave room for the two qubits of a

@@ -1169,8 +1169,10 @@ extern "C" qf_status qf_plan_build(const qf_op* ops, size_t count, uint32_t qubi
     s = lower_ops(ops, count, qubits, g);
     if (s) return s;
     const qf::PlanOptions po = plan_options_from(o);
+    const int precision = o ? o->precision : QF_C128;
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
     qf_plan* p = new qf_plan;
-    std::string err = qf::build_plan(g, qubits, QF_C128, po, p->plan);
+    std::string err = qf::build_plan(g, qubits, precision, po, p->plan);
     if (!err.empty()) {
         delete p;
         return fail(QF_INVALID_ARGUMENT, err);

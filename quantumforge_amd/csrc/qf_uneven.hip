@@ -374,6 +374,7 @@ extern "C" qf_status qf_autotune(qf_backend* be, const qf_op* ops, size_t count,
     best->low_qubits = r.low;
     best->reg_bits = r.reg;
     best->kernel = r.tile ? 2 : 1;
+    best->precision = precision;
     if (best_ms) *best_ms = r.ms;
     return QF_OK;
 }
