@@ -35,7 +35,10 @@ struct Term {
     std::string var;
 };
 
+// A zero coefficient yields a term with no variable, which lincomb drops
+// (x * 0 cannot be folded by the compiler: x may be infinite or NaN).
 Term lit_term(double c, const std::string& var) {
+    if (c == 0.0) return Term{std::string(), false, std::string()};
     const double a = std::fabs(c);
     return Term{a == 1.0 ? std::string() : hexlit(a), std::signbit(c), var};
 }
@@ -43,6 +46,7 @@ Term lit_term(double c, const std::string& var) {
 std::string lincomb(const std::vector<Term>& ts) {
     std::string s;
     for (const Term& t : ts) {
+        if (t.var.empty()) continue;
         const std::string body = t.coef.empty() ? t.var : t.coef + " * " + t.var;
         if (s.empty())
             s = t.neg ? "-" + body : body;
@@ -101,6 +105,17 @@ public:
         if (any_rot) o_ << "  const double* __restrict__ csr = cs + inst * 2ull * nslots;\n";
         else o_ << "  (void)cs; (void)nslots;\n";
         for (uint32_t k = 0; k < NR_; ++k) o_ << "  R r" << k << ", i" << k << ";\n";
+        acc_.assign(1 + 2 * NR_, Acc{});
+        acc_[0].r = "ar";
+        acc_[0].i = "ai";
+        o_ << "  R ar = R(1), ai = R(0);\n";
+        for (uint32_t b = 0; b < RB_; ++b)
+            for (int v = 0; v < 2; ++v) {
+                Acc& a = acc_[1 + 2 * b + v];
+                a.r = "a" + std::to_string(b) + std::to_string(v) + "r";
+                a.i = "a" + std::to_string(b) + std::to_string(v) + "i";
+                o_ << "  R " << a.r << " = R(1), " << a.i << " = R(0);\n";
+            }
         var_.resize(NR_);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         set_phase(p_.ops[0]);
@@ -124,6 +139,7 @@ public:
                 default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
             }
         }
+        acc_flush_all();
         o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_global() << ";\n";
         for (uint32_t k = 0; k < NR_; ++k)
             o_ << "    sp[" << reg_global(k) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
@@ -147,6 +163,18 @@ private:
     std::vector<int> var_;
     std::ostringstream o_;
     int opn_ = 0;
+    // Diagonal factor accumulators.  acc_[0]: the product of the diagonal
+    // factors that depend on no register bit: it scales all of a thread's
+    // amplitudes alike, so it commutes with every op of the phase and is
+    // applied once, before the next exchange or the store.  acc_[1 + 2i + v]:
+    // factors that depend on register bit i (value v) and on thread or block
+    // bits: they commute with everything but ops that are not diagonal on
+    // bit i, and are applied before those.
+    struct Acc {
+        std::string r, i;
+        bool on = false, real = true;
+    };
+    std::vector<Acc> acc_;
 
     static bool fail(std::string* err, const std::string& m) {
         if (err) *err = m;
@@ -245,7 +273,50 @@ private:
                ")) & " + std::to_string((1u << B) - 1) + "u)";
     }
 
+    // a *= (fr + i fi) (fi empty: real factor).
+    void acc_mul(Acc& a, const std::string& fr, const std::string& fi) {
+        if (!a.on) {
+            o_ << "  " << a.r << " = " << fr << "; " << a.i << " = " << (fi.empty() ? "R(0)" : fi) << ";\n";
+            a.on = true;
+            a.real = fi.empty();
+            return;
+        }
+        if (fi.empty()) {
+            o_ << "  " << a.r << " = " << a.r << " * " << fr << ";";
+            if (!a.real) o_ << " " << a.i << " = " << a.i << " * " << fr << ";";
+            o_ << "\n";
+        } else if (a.real) {
+            o_ << "  " << a.i << " = " << a.r << " * " << fi << "; " << a.r << " = " << a.r << " * " << fr << ";\n";
+            a.real = false;
+        } else {
+            o_ << "  { const R x = " << a.r << "; " << a.r << " = " << a.r << " * " << fr << " - " << a.i << " * " << fi
+               << "; " << a.i << " = x * " << fi << " + " << a.i << " * " << fr << "; }\n";
+        }
+    }
+    void acc_mul_lit(Acc& a, double fr, double fi) {
+        if (fr == 1.0 && fi == 0.0) return;
+        acc_mul(a, hexlit(fr), fi == 0.0 ? "" : hexlit(fi));
+    }
+    // Applies accumulator a to the registers k with (k >> bit & 1) == v
+    // (bit < 0: all registers) and clears it.
+    void acc_apply(Acc& a, int bit, int v) {
+        if (!a.on) return;
+        for (uint32_t k = 0; k < NR_; ++k)
+            if (bit < 0 || int((k >> bit) & 1) == v) mul_rt(k, a.r, a.real ? "" : a.i);
+        a.on = false;
+        a.real = true;
+    }
+    void acc_flush_bit(uint32_t b) {
+        acc_apply(acc_[1 + 2 * b], int(b), 0);
+        acc_apply(acc_[2 + 2 * b], int(b), 1);
+    }
+    void acc_flush_all() {
+        acc_apply(acc_[0], -1, 0);
+        for (uint32_t b = 0; b < RB_; ++b) acc_flush_bit(b);
+    }
+
     void exchange(const TileOp& hdr, bool lds_used) {
+        acc_flush_all();
         if (lds_used) o_ << "  __syncthreads();\n";
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
@@ -314,6 +385,7 @@ private:
                 else mul_lit(k, m[0], m[1]);
             return;
         }
+        acc_flush_bit(i);
         if (diagz) {
             for (uint32_t k = 0; k < NR_; ++k)
                 if (!((k >> i) & 1)) swap_vars(k, k | (1u << i));
@@ -345,20 +417,27 @@ private:
     // Diagonal d[bit(g1) * 2 + bit(g0)] (g1 < 0: one qubit, d[bit(g0)]).
     void diag(const TileOp& op) {
         const int g0 = op.q0, g1 = op.q1;
-        const Bit b0 = bit_of(g0);
         const bool two = g1 >= 0;
+        const Bit b0 = bit_of(g0);
         const Bit b1 = two ? bit_of(g1) : Bit{Bit::REG, -1};
         auto d = [&](int idx, int c) { return op.m[2 * idx + c]; };
         const std::string pfx = "d" + std::to_string(opn_);
         const bool rt0 = b0.kind != Bit::REG, rt1 = two && b1.kind != Bit::REG;
+        const int nreg = (rt0 ? 0 : 1) + (two && !rt1 ? 1 : 0);
+        if (nreg == 2 || (nreg == 1 && !rt0 && !rt1)) {
+            // Only register bits: a literal factor per register.
+            for (uint32_t k = 0; k < NR_; ++k) {
+                const int x0 = int((k >> b0.idx) & 1), x1 = two ? int((k >> b1.idx) & 1) : 0;
+                mul_lit(k, d(x1 * 2 + x0, 0), d(x1 * 2 + x0, 1));
+            }
+            return;
+        }
         if (rt0) o_ << "  const bool " << pfx << "c0 = " << bit_expr(b0) << ";\n";
         if (rt1) o_ << "  const bool " << pfx << "c1 = " << bit_expr(b1) << ";\n";
-        // Factor per compile-time combination (x0, x1) of the register bits.
-        std::map<int, std::pair<std::string, std::string>> made;
-        for (uint32_t k = 0; k < NR_; ++k) {
-            const int x0 = rt0 ? -1 : int((k >> b0.idx) & 1);
-            const int x1 = !two ? 0 : (rt1 ? -1 : int((k >> b1.idx) & 1));
-            // Candidate indices over the run-time bits.
+        // Factor for a fixed value of the (at most one) register bit, as a
+        // select among literals over the run-time bits.
+        auto factor = [&](Acc& a, int xreg) {
+            const int x0 = rt0 ? -1 : xreg, x1 = !two ? 0 : (rt1 ? -1 : xreg);
             std::vector<int> cand;
             for (int v1 = 0; v1 < 2; ++v1)
                 for (int v0 = 0; v0 < 2; ++v0) {
@@ -373,36 +452,38 @@ private:
                 if (d(c, 1) != 0.0) real = false;
             }
             if (same) {
-                mul_lit(k, d(cand[0], 0), d(cand[0], 1));
-                continue;
+                acc_mul_lit(a, d(cand[0], 0), d(cand[0], 1));
+                return;
             }
-            const int key = (x0 + 1) * 3 + (x1 + 1);
-            auto it = made.find(key);
-            if (it == made.end()) {
-                const std::string nm = pfx + "_" + std::to_string(key);
-                auto sel = [&](int comp) {
-                    // Select among the candidates with the run-time bits.
-                    auto val = [&](int v0, int v1) {
-                        const int i0 = x0 >= 0 ? x0 : v0, i1 = !two ? 0 : (x1 >= 0 ? x1 : v1);
-                        return hexlit(d(i1 * 2 + i0, comp));
-                    };
-                    if (rt0 && rt1)
-                        return pfx + "c1 ? (" + pfx + "c0 ? " + val(1, 1) + " : " + val(0, 1) + ") : (" + pfx + "c0 ? " +
-                               val(1, 0) + " : " + val(0, 0) + ")";
-                    if (rt0) return pfx + "c0 ? " + val(1, 0) + " : " + val(0, 0);
-                    return pfx + "c1 ? " + val(0, 1) + " : " + val(0, 0);
-                };
-                o_ << "  const R " << nm << "r = " << sel(0) << ";\n";
-                if (!real) o_ << "  const R " << nm << "i = " << sel(1) << ";\n";
-                it = made.emplace(key, std::make_pair(nm + "r", real ? std::string() : nm + "i")).first;
-            }
-            mul_rt(k, it->second.first, it->second.second);
+            auto val = [&](int v0, int v1, int comp) {
+                const int i0 = x0 >= 0 ? x0 : v0, i1 = !two ? 0 : (x1 >= 0 ? x1 : v1);
+                return hexlit(d(i1 * 2 + i0, comp));
+            };
+            auto sel = [&](int comp) {
+                if (rt0 && rt1)
+                    return pfx + "c1 ? (" + pfx + "c0 ? " + val(1, 1, comp) + " : " + val(0, 1, comp) + ") : (" + pfx +
+                           "c0 ? " + val(1, 0, comp) + " : " + val(0, 0, comp) + ")";
+                if (rt0) return pfx + "c0 ? " + val(1, 0, comp) + " : " + val(0, 0, comp);
+                return pfx + "c1 ? " + val(0, 1, comp) + " : " + val(0, 0, comp);
+            };
+            const std::string nm = pfx + "_" + std::to_string(xreg + 1);
+            o_ << "  const R " << nm << "r = " << sel(0) << ";\n";
+            if (!real) o_ << "  const R " << nm << "i = " << sel(1) << ";\n";
+            acc_mul(a, nm + "r", real ? std::string() : nm + "i");
+        };
+        if (nreg == 0) {
+            factor(acc_[0], -1);
+            return;
         }
+        const uint32_t rb = uint32_t(rt0 ? b1.idx : b0.idx);
+        factor(acc_[1 + 2 * rb], 0);
+        factor(acc_[2 + 2 * rb], 1);
     }
 
     void cnot(const TileOp& op) {
         const uint32_t ti = uint32_t(op.q1);
         const Bit c = bit_of(op.q0);
+        acc_flush_bit(ti);
         if (c.kind == Bit::REG) {
             for (uint32_t k = 0; k < NR_; ++k)
                 if (((k >> c.idx) & 1) && !((k >> ti) & 1)) swap_vars(k, k | (1u << ti));
@@ -422,6 +503,8 @@ private:
 
     void swap(const TileOp& op) {
         const uint32_t a = uint32_t(op.q0), b = uint32_t(op.q1);
+        acc_flush_bit(a);
+        acc_flush_bit(b);
         for (uint32_t k = 0; k < NR_; ++k)
             if (((k >> a) & 1) && !((k >> b) & 1)) swap_vars(k, k ^ (1u << a) ^ (1u << b));
     }
@@ -446,10 +529,11 @@ private:
             }
             const std::string sz = "z" + std::to_string(opn_);
             o_ << "  const R " << sz << " = " << bit_expr(b) << " ? " << s << " : -" << s << ";\n";
-            for (uint32_t k = 0; k < NR_; ++k) mul_rt(k, c, sz);
+            acc_mul(acc_[0], c, sz);
             return;
         }
         const uint32_t i = uint32_t(op.q0);
+        acc_flush_bit(i);
         for (uint32_t k = 0; k < NR_; ++k) {
             if ((k >> i) & 1) continue;
             const uint32_t k1 = k | (1u << i);
@@ -481,6 +565,18 @@ private:
         if (!rt.empty()) {
             sz = "z" + std::to_string(opn_);
             o_ << "  const R " << sz << " = (" << rt << ") ? " << s << " : -" << s << ";\n";
+        }
+        if (b0.kind != Bit::REG && b1.kind != Bit::REG) {
+            acc_mul(acc_[0], c, sz);
+            return;
+        }
+        if ((b0.kind == Bit::REG) != (b1.kind == Bit::REG)) {
+            // One register bit r: parity = run-time parity ^ bit r, so the
+            // factor is (c, sz) for r = 0 and (c, -sz) for r = 1.
+            const uint32_t r = uint32_t(b0.kind == Bit::REG ? b0.idx : b1.idx);
+            acc_mul(acc_[1 + 2 * r], c, sz);
+            acc_mul(acc_[2 + 2 * r], c, "(-" + sz + ")");
+            return;
         }
         for (uint32_t k = 0; k < NR_; ++k) {
             int par = 0;

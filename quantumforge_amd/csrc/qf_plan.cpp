@@ -587,10 +587,11 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     out.compiled = regk && want_compiled ? (opt.kernel == 3 ? 2 : 1) : 0;
     // Compiled passes need only one 128-byte run of contiguous amplitudes
     // per tile for full cache lines; the freed tile slots take gate qubits.
-    // complex64 tiles of 13 qubits fill the same 64 KiB of LDS as 12-qubit
-    // complex128 ones and need fewer passes.
+    // Compiled tiles are one qubit larger than the generic kernels' (12
+    // qubits complex128, 13 complex64: 64 KiB of LDS, 512-thread workgroups
+    // with 3 or 4 register bits): fewer passes at the same streaming rate.
     if (out.compiled && !opt.low_qubits) low = precision == 0 ? 3 : 4;
-    if (out.compiled && !opt.tile_qubits && precision != 0 && qubits >= 13 && M + 1 - RB <= 10) ++M;
+    if (out.compiled && !opt.tile_qubits && qubits >= M + 1 && M + 1 - RB <= 10) ++M;
     if (M > qubits) M = qubits;
     if (low > M) low = M;
     // A tile smaller than the state must leave room for the two qubits of a
