@@ -10,6 +10,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -455,21 +456,64 @@ private:
                 acc_mul_lit(a, d(cand[0], 0), d(cand[0], 1));
                 return;
             }
-            auto val = [&](int v0, int v1, int comp) {
+            // Two distinct values: one select on the set of run-time bit
+            // combinations giving the second value.  More: a product of
+            // single-select factors (nested selects over the same conditions
+            // become divergent branches in the compiled code).
+            auto idx = [&](int v0, int v1) {
                 const int i0 = x0 >= 0 ? x0 : v0, i1 = !two ? 0 : (x1 >= 0 ? x1 : v1);
-                return hexlit(d(i1 * 2 + i0, comp));
+                return i1 * 2 + i0;
             };
-            auto sel = [&](int comp) {
-                if (rt0 && rt1)
-                    return pfx + "c1 ? (" + pfx + "c0 ? " + val(1, 1, comp) + " : " + val(0, 1, comp) + ") : (" + pfx +
-                           "c0 ? " + val(1, 0, comp) + " : " + val(0, 0, comp) + ")";
-                if (rt0) return pfx + "c0 ? " + val(1, 0, comp) + " : " + val(0, 0, comp);
-                return pfx + "c1 ? " + val(0, 1, comp) + " : " + val(0, 0, comp);
+            auto eq = [&](int a, int b) { return d(a, 0) == d(b, 0) && d(a, 1) == d(b, 1); };
+            auto lit = [&](double v) { return hexlit(v); };
+            std::vector<std::pair<int, int>> combos;  // run-time (v0, v1)
+            for (int v1 = 0; v1 < (rt1 ? 2 : 1); ++v1)
+                for (int v0 = 0; v0 < (rt0 ? 2 : 1); ++v0) combos.push_back({v0, v1});
+            const int base = idx(0, 0);
+            int other = -1;
+            bool two_values = true;
+            for (auto& cb : combos) {
+                const int j = idx(cb.first, cb.second);
+                if (eq(j, base)) continue;
+                if (other < 0) other = j;
+                else if (!eq(j, other)) two_values = false;
+            }
+            auto cond_of = [&](int v0, int v1) {
+                std::string c;
+                if (rt0) c += std::string(v0 ? "" : "!") + pfx + "c0";
+                if (rt1) c += std::string(c.empty() ? "" : " && ") + (v1 ? "" : "!") + pfx + "c1";
+                return "(" + c + ")";
             };
             const std::string nm = pfx + "_" + std::to_string(xreg + 1);
-            o_ << "  const R " << nm << "r = " << sel(0) << ";\n";
-            if (!real) o_ << "  const R " << nm << "i = " << sel(1) << ";\n";
-            acc_mul(a, nm + "r", real ? std::string() : nm + "i");
+            if (two_values) {
+                std::string cond;
+                for (auto& cb : combos)
+                    if (!eq(idx(cb.first, cb.second), base))
+                        cond += (cond.empty() ? "" : " || ") + cond_of(cb.first, cb.second);
+                o_ << "  const bool " << nm << "s = " << cond << ";\n";
+                o_ << "  const R " << nm << "r = " << nm << "s ? " << lit(d(other, 0)) << " : " << lit(d(base, 0)) << ";\n";
+                if (!real)
+                    o_ << "  const R " << nm << "i = " << nm << "s ? " << lit(d(other, 1)) << " : " << lit(d(base, 1))
+                       << ";\n";
+                acc_mul(a, nm + "r", real ? std::string() : nm + "i");
+                return;
+            }
+            // Both bits run-time, three or four values: d00 * (x0 ? d01/d00 : 1)
+            // * (x1 ? d10/d00 : 1) * (x0 x1 ? d11 d00 / (d01 d10) : 1).
+            typedef std::complex<double> Z;
+            const Z z00(d(0, 0), d(0, 1)), z01(d(1, 0), d(1, 1)), z10(d(2, 0), d(2, 1)), z11(d(3, 0), d(3, 1));
+            const Z f1 = z01 / z00, f2 = z10 / z00, f3 = z11 * z00 / (z01 * z10);
+            acc_mul_lit(a, z00.real(), z00.imag());
+            const Z fs[3] = {f1, f2, f3};
+            const char* conds[3] = {"c0", "c1", ""};
+            for (int q = 0; q < 3; ++q) {
+                if (fs[q] == Z(1, 0)) continue;
+                const std::string fn = nm + "f" + std::to_string(q);
+                const std::string cnd = q < 2 ? pfx + conds[q] : "(" + pfx + "c0 && " + pfx + "c1)";
+                o_ << "  const R " << fn << "r = " << cnd << " ? " << lit(fs[q].real()) << " : R(1);\n";
+                o_ << "  const R " << fn << "i = " << cnd << " ? " << lit(fs[q].imag()) << " : R(0);\n";
+                acc_mul(a, fn + "r", fn + "i");
+            }
         };
         if (nreg == 0) {
             factor(acc_[0], -1);

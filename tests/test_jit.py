@@ -62,6 +62,36 @@ def test_generated_kernels_on_a_batch_and_controlled_phases():
         close(out[i], st.amplitudes(), 1e-13)
 
 
+def test_generated_kernels_general_two_qubit_diagonals():
+    # Rz on both qubits next to a controlled phase merge into one diagonal
+    # with four distinct entries; on qubits outside the register set it is
+    # applied as a product of single-select factors.
+    import math
+
+    from quantumforge_amd import Circuit, Gate
+
+    n = 12
+    c = Circuit()
+    rng = np.random.default_rng(5)
+    for q in range(n):
+        c.push(Gate.H, q)
+    for layer in range(3):
+        for a in range(n):
+            b = (a + 1 + 2 * layer) % n
+            c.push(Gate.Rz(float(rng.uniform(0, 2 * math.pi))), a)
+            c.push(Gate.Rz(float(rng.uniform(0, 2 * math.pi))), b)
+            c.cp(a, b, float(rng.uniform(0, 2 * math.pi)))
+        for q in range(0, n, 3):
+            c.push(Gate.Rx(0.3 + q), q)
+    plan = Plan(c, n, kernel=3, tile_qubits=9, reg_bits=3, low_qubits=2)
+    kinds = [o["kind"] for p in plan.describe()["passes"] for o in p["ops"]]
+    assert kinds.count(1) > 0
+    a0 = random_state(n, 12)[0]
+    st = O.State(n, a0)
+    st.run(c.to_ops())
+    close(emulate(plan, a0, n), st.amplitudes(), 1e-13)
+
+
 def test_generated_kernels_complex64():
     n = 12
     c = random_layer_circuit(n, 8, seed=11, pairs_per_layer=6)
