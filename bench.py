@@ -226,6 +226,30 @@ def uneven_bench(be, count: int, seed: int) -> dict:
             "mean_qubits": float(np.mean([q for _, q, _, _ in insts]))}
 
 
+def pcie_bench(be, qubits: int) -> dict:
+    """Host<->device rate of the boundary's host-buffer entry points
+    (qf_state_upload / qf_state_download, pageable NumPy memory) for one
+    complex128 state of `qubits` qubits."""
+    from quantumforge_amd import StateVector
+
+    amps = np.zeros(1 << qubits, dtype=np.complex128)
+    amps[0] = 1.0
+    sv = StateVector(amps, qubits)
+    dev = be.upload(sv)  # warm the path
+    be.synchronize()
+    t0 = time.perf_counter()
+    dev = be.upload(sv)
+    be.synchronize()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = dev.download()
+    t_down = time.perf_counter() - t0
+    assert out.amplitudes[0] == 1.0
+    nbytes = amps.nbytes
+    return {"qubits": qubits, "bytes": nbytes, "upload_GBs": round(nbytes / t_up / 1e9, 2),
+            "download_GBs": round(nbytes / t_down / 1e9, 2), "memory": "pageable host (NumPy)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +270,7 @@ def main() -> int:
     ap.add_argument("--vqe-shift-sets", type=int, default=4, help="sets timed with the parameter-shift GPU form")
     ap.add_argument("--qft-steps", type=int, default=2, help="timed runs of the 33q/34q QFT-like circuits (0 = skip)")
     ap.add_argument("--small-steps", type=int, default=20, help="timed runs of the 20-qubit circuit (0 = skip)")
+    ap.add_argument("--pcie-qubits", type=int, default=26, help="state size of the upload/download timing (0 = skip)")
     ap.add_argument("--uneven-instances", type=int, default=0,
                     help="uneven QAOA adjoint batch size (0 = skip; the full workload is 512)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -375,6 +400,9 @@ def main() -> int:
         out["small_20q"] = r
     if args.qft_steps > 0:
         out["qft_like"] = qft_like_bench(be, torch, args.qft_steps)
+    if args.pcie_qubits > 0:
+        out["host_boundary"] = pcie_bench(be, args.pcie_qubits)
+        log(f"host boundary: {out['host_boundary']}")
     if args.uneven_instances > 0:
         out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512)
     if not args.no_cpu:
