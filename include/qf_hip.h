@@ -65,7 +65,10 @@ enum {
     QF_INVALID_BATCH_SAMPLES = 18,      /* CudaError::InvalidBatchSamples */
     QF_MEMORY_BUDGET_EXCEEDED = 19,     /* QuantumError::MemoryBudgetExceeded */
     QF_INVALID_FD_STEP = 20,            /* QuantumError::InvalidFiniteDifferenceStep */
-    QF_SIMULATION_SIZE_OVERFLOW = 21    /* QuantumError::SimulationSizeOverflow */
+    QF_SIMULATION_SIZE_OVERFLOW = 21,   /* QuantumError::SimulationSizeOverflow */
+    QF_INVALID_PROBABILITY = 22,        /* QuantumError::InvalidProbability */
+    QF_INVALID_SHOT_COUNT = 23,         /* QuantumError::InvalidShotCount */
+    QF_CLASSICAL_BIT_OUT_OF_RANGE = 24  /* QuantumError::ClassicalBitOutOfRange */
 };
 
 /* Gate ids, in the order of `enum Gate` (src/lib.rs:2239-2265). */
@@ -97,8 +100,12 @@ typedef struct { float re, im; } qf_c64;
 /* One operation of a Circuit (CircuitOp, src/lib.rs:2386-2440).
  * kind: 0 single-qubit {gate, a=target, theta for Rx/Ry/Rz},
  *       1 CNOT {a=control, b=target}, 2 CZ {a, b}, 3 SWAP {a, b},
- *       4 channel, 5 measure, 6 classically conditioned gate
- *       (4-6 are rejected with the reference's errors, src/lib.rs:2793-2798),
+ *       4 channel, 5 measure {a=target, b=classical bit},
+ *       6 classically conditioned gate {gate, a=target, theta, b=classical bit
+ *       | value << 16} (CircuitOp::Measure / ConditionalSingleQubit,
+ *       src/lib.rs:2386-2440; value is 0 or 1, if_classical_bit keeps value & 1);
+ *       4-6 are rejected by the state-vector runs with the reference's errors
+ *       (src/lib.rs:2793-2798) and run by qf_run_circuit_shots,
  *       7 exact controlled phase diag(1,1,1,e^{i theta}) {a=control, b=target}
  *       (no CircuitOp counterpart: the crate lowers it to Rz/CNOT,
  *       src/algorithms.rs:139-145, which equals it times e^{-i theta/4}). */
@@ -350,6 +357,38 @@ qf_status qf_uneven_adjoint_gradient(qf_backend* backend, const qf_uneven_instan
  * and returns the fastest as plan options (cached per backend process). */
 qf_status qf_autotune(qf_backend* backend, const qf_op* ops, size_t count, uint32_t qubits, int32_t precision,
                       qf_plan_options* best, double* best_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement and shots                                                      */
+/* ------------------------------------------------------------------------ */
+/* CudaDeviceState::measure, src/cuda.rs:1386-1442: P(1) of `qubit` (FP64),
+ * outcome = uniform < P(1), InvalidProbability when the drawn branch has
+ * probability <= DBL_EPSILON, collapse dividing the kept branch by
+ * sqrt(probability).  The reference draws `uniform` from its SmallRng inside
+ * the call; here the caller draws it (rng.next_f64()) and passes it. */
+qf_status qf_state_measure(qf_state* state, uint32_t qubit, double uniform, uint8_t* outcome);
+/* CudaDeviceStateBatch::measure_with_uniforms, src/cuda.rs:1511-1609: one
+ * uniform in [0, 1) per state (else InvalidBatchSamples); every outcome is
+ * checked before any state collapses. */
+qf_status qf_batch_measure_with_uniforms(qf_batch* batch, uint32_t qubit, const double* uniforms, size_t count,
+                                         uint8_t* outcomes);
+/* CudaDeviceStateBatch::apply_gate_masked (the classically conditioned gate
+ * of a shot batch, src/cuda.rs:1239-1249): the gate on states with enabled[i]. */
+qf_status qf_batch_apply_gate_masked(qf_batch* batch, int32_t gate, uint32_t target, double theta,
+                                     const uint8_t* enabled, size_t count);
+/* CudaBackend::run_circuit_shots (batched = 0, src/cuda.rs:1085-1159) and
+ * run_circuit_shots_batched (batched = 1, src/cuda.rs:1161-1269) from a
+ * complex128 initial state: SmallRng(seed) uniforms in the CPU's shot-major
+ * order (Circuit::run_shots, src/lib.rs:2843-2911), so the registers equal the
+ * CPU's.  Writes the classical register of every shot, [shots][width] bytes
+ * (least-significant classical bit first), into `registers`; the caller
+ * counts them (ShotResult::counts).  With registers == NULL only validates and
+ * sets *classical_width.  Errors: InvalidShotCount, ClassicalBitOutOfRange,
+ * OperationRequiresDensityMatrix (noise channels: trajectories are not built),
+ * the gate errors, InvalidProbability. */
+qf_status qf_run_circuit_shots(qf_backend* backend, const qf_c128* initial, uint32_t qubits, const qf_op* ops,
+                               size_t nops, uint64_t shots, uint64_t seed, int32_t batched, uint8_t* registers,
+                               uint32_t* classical_width);
 
 /* ------------------------------------------------------------------------ */
 /* Planning (no reference counterpart; see qf_plan_options)                  */

@@ -75,6 +75,8 @@ def op_unitary(op, n: int) -> np.ndarray:
         return cz(a, b, n)
     if kind == 3:
         return swap(a, b, n)
+    if kind == 7:
+        return cp(a, b, th, n)
     raise ValueError("non-unitary op")
 
 
@@ -86,6 +88,48 @@ def run(ops, n: int, psi0: np.ndarray | None = None) -> np.ndarray:
     for op in ops:
         psi = op_unitary(op, n) @ psi
     return psi
+
+
+def measure(psi: np.ndarray, q: int, u: float):
+    """Projective measurement of qubit q with uniform u (StateVector::measure,
+    src/lib.rs:1321-1350, as a projector): returns (outcome, collapsed state)."""
+    n_idx = np.arange(len(psi))
+    bit = (n_idx >> q) & 1
+    p_one = min(max(float(np.sum(np.abs(psi[bit == 1]) ** 2)), 0.0), 1.0)
+    o = int(u < p_one)
+    p = p_one if o else 1.0 - p_one
+    if p <= np.finfo(float).eps:
+        raise ValueError("invalid probability")
+    out = np.where(bit == o, psi, 0) / np.sqrt(p)
+    return o, out
+
+
+def run_shots(ops, n: int, psi0: np.ndarray, shots: int, seed: int) -> np.ndarray:
+    """Circuit::run_shots (noiseless), src/lib.rs:2843-2911, with dense
+    unitaries: one SmallRng uniform per measurement, shot-major."""
+    from quantumforge_amd.workloads import SmallRng
+
+    width = 0
+    for op in ops:
+        if int(op["kind"]) == 5:
+            width = max(width, int(op["b"]) + 1)
+        elif int(op["kind"]) == 6:
+            width = max(width, (int(op["b"]) & 0xFFFF) + 1)
+    rng = SmallRng(seed)
+    regs = np.zeros((shots, width), dtype=np.uint8)
+    for shot in range(shots):
+        psi = psi0.astype(complex).copy()
+        for op in ops:
+            k = int(op["kind"])
+            if k == 5:
+                regs[shot, int(op["b"])], psi = measure(psi, int(op["a"]), rng.next_f64())
+            elif k == 6:
+                b = int(op["b"])
+                if regs[shot, b & 0xFFFF] == ((b >> 16) & 1):
+                    psi = embed_1q(gate(int(op["gate"]), float(op["theta"])), int(op["a"]), n) @ psi
+            else:
+                psi = op_unitary(op, n) @ psi
+    return regs
 
 
 def pauli_word_matrix(word: str) -> np.ndarray:

@@ -83,6 +83,10 @@ def lib():
         L.or_pc_fd_gradient.restype = i32
         L.or_fast_run.argtypes = [P, u32, P, sz]
         L.or_fast_run.restype = i32
+        L.or_measure.argtypes = [P, u32, dbl, C.POINTER(C.c_uint8)]
+        L.or_measure.restype = i32
+        L.or_run_shots.argtypes = [P, u32, P, sz, u64, u64, P, C.POINTER(u32)]
+        L.or_run_shots.restype = i32
         _lib = L
     return _lib
 
@@ -142,6 +146,12 @@ class State:
     def run(self, ops: np.ndarray):
         ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
         _chk(lib().or_run(self.h, _p(ops), len(ops)))
+
+    def measure(self, q: int, u: float) -> int:
+        """StateVector::measure with the uniform draw passed in."""
+        o = C.c_uint8()
+        _chk(lib().or_measure(self.h, q, float(u), C.byref(o)))
+        return int(o.value)
 
     def norm_sqr(self) -> float:
         return lib().or_norm_sqr(self.h)
@@ -257,3 +267,20 @@ class Rng:
 
     def next_f64(self) -> float:
         return float(lib().or_rng_next_f64(C.byref(self.s)))
+
+
+def run_shots(initial: np.ndarray, qubits: int, ops: np.ndarray, shots: int, seed: int) -> np.ndarray:
+    """Circuit::run_shots (noiseless): the classical register of every shot,
+    [shots][width] uint8, least-significant classical bit first."""
+    initial = np.ascontiguousarray(initial, dtype=np.complex128)
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    w = C.c_uint32()
+    _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, None, C.byref(w)))
+    out = np.zeros((shots, w.value), dtype=np.uint8)
+    if w.value:
+        _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, _p(out), C.byref(w)))
+    else:
+        _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, _p(np.zeros(1, np.uint8)),
+                                C.byref(w)))
+    return out
+

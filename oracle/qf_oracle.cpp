@@ -14,6 +14,7 @@
 // function (golden vectors, closed forms, brute force) lives in
 // tests/test_oracle_*.py.
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -53,6 +54,9 @@ enum {
     REQUIRES_SHOT_EXECUTION = 14,
     INVALID_ARGUMENT = 15,
     INVALID_FD_STEP = 20,
+    INVALID_PROBABILITY = 22,
+    INVALID_SHOT_COUNT = 23,
+    CLASSICAL_BIT_OUT_OF_RANGE = 24,
 };
 
 // Gate ids (qf_gate in include/qf_hip.h) in the order of `enum Gate`, src/lib.rs:2239-2265.
@@ -328,6 +332,85 @@ int or_max_threads() {
 #else
     return 1;
 #endif
+}
+
+// ---- measurement and shots
+// StateVector::measure, src/lib.rs:1321-1350: P(1) summed left to right over
+// the indices with bit q set, outcome = u < P(1), branch probability checked
+// against f64::EPSILON, kept amplitudes divided by sqrt(probability).
+int measure_state(State& st, uint32_t q, double u, uint8_t* outcome) {
+    if (q >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    const size_t mask = size_t(1) << q;
+    double p_one = 0.0;
+    for (size_t i = 0; i < st.amps.size(); ++i)
+        if (i & mask) p_one += cnorm(st.amps[i]);
+    p_one = std::min(std::max(p_one, 0.0), 1.0);
+    const uint8_t o = uint8_t(u < p_one);
+    const double probability = o ? p_one : 1.0 - p_one;
+    if (probability <= std::numeric_limits<double>::epsilon()) return INVALID_PROBABILITY;
+    const double scale = std::sqrt(probability);
+    for (size_t i = 0; i < st.amps.size(); ++i) {
+        if (uint8_t((i & mask) != 0) != o) st.amps[i] = C{0.0, 0.0};
+        else st.amps[i] = C{st.amps[i].re / scale, st.amps[i].im / scale};  // Complex64 / f64, lib.rs:502-505
+    }
+    *outcome = o;
+    return OK;
+}
+
+int or_measure(void* s, uint32_t q, double u, uint8_t* outcome) {
+    return measure_state(*static_cast<State*>(s), q, u, outcome);
+}
+
+// Circuit::run_shots (noiseless), src/lib.rs:2822-2911: op kinds as or_op,
+// 5 = measure {a target, b classical bit}, 6 = conditional gate {gate, a
+// target, theta, b = classical bit | value << 16}.  Writes every shot's
+// classical register ([shots][width], least-significant bit first).
+int or_run_shots(const double* initial, uint32_t qubits, const or_op* ops, size_t nops, uint64_t shots,
+                 uint64_t seed, uint8_t* registers, uint32_t* width_out) {
+    if (shots == 0) return INVALID_SHOT_COUNT;
+    uint32_t width = 0;
+    for (size_t i = 0; i < nops; ++i) {
+        if (ops[i].kind != 5 && ops[i].kind != 6) continue;
+        const uint32_t bit = ops[i].kind == 5 ? ops[i].b : (ops[i].b & 0xFFFFu);
+        if (bit >= 4096) return CLASSICAL_BIT_OUT_OF_RANGE;
+        width = std::max(width, bit + 1);
+    }
+    *width_out = width;
+    if (!registers) return OK;
+    uint64_t rng = or_rng_seeded(seed);
+    const size_t dim = size_t(1) << qubits;
+    std::vector<uint8_t> classical(width);
+    for (uint64_t shot = 0; shot < shots; ++shot) {
+        State st;
+        st.qubits = qubits;
+        st.amps.resize(dim);
+        std::memcpy(st.amps.data(), initial, dim * sizeof(C));
+        std::fill(classical.begin(), classical.end(), 0);
+        for (size_t i = 0; i < nops; ++i) {
+            const or_op& op = ops[i];
+            int e = OK;
+            switch (op.kind) {
+                case 0: e = apply_1q(st, op.gate, op.theta, op.a); break;
+                case 1: e = apply_cnot(st, op.a, op.b); break;
+                case 2: e = apply_cz(st, op.a, op.b); break;
+                case 3: e = apply_swap(st, op.a, op.b); break;
+                case 4: return REQUIRES_DENSITY_MATRIX;
+                case 5: {
+                    const double u = or_rng_next_f64(&rng);
+                    e = measure_state(st, op.a, u, &classical[op.b]);
+                    break;
+                }
+                case 6:
+                    if (classical[op.b & 0xFFFFu] == ((op.b >> 16) & 1u)) e = apply_1q(st, op.gate, op.theta, op.a);
+                    break;
+                case 7: e = apply_cp(st, op.a, op.b, op.theta); break;
+                default: return INVALID_ARGUMENT;
+            }
+            if (e) return e;
+        }
+        if (width) std::memcpy(registers + shot * width, classical.data(), width);
+    }
+    return OK;
 }
 
 // ---- reductions

@@ -6,12 +6,12 @@ include/qf_hip.h (backend), gradients (gradients), workload builders
 (workloads) and the torch integration (torch_ops, imported on demand).
 """
 from ._native import QuantumError
-from .core import (Circuit, Gate, ParameterizedCircuit, PauliSum, RotationAxis, SingleQubitObservable,
+from .core import (Circuit, Gate, ParameterizedCircuit, PauliSum, RotationAxis, ShotResult, SingleQubitObservable,
                    StateVector, rotation_gate)
 
 __all__ = [
     "QuantumError", "Circuit", "Gate", "ParameterizedCircuit", "PauliSum", "RotationAxis",
-    "SingleQubitObservable", "StateVector", "rotation_gate", "HipBackend", "HipDeviceState",
+    "ShotResult", "SingleQubitObservable", "StateVector", "rotation_gate", "HipBackend", "HipDeviceState",
     "HipDeviceStateBatch", "Plan",
 ]
 

@@ -21,6 +21,7 @@ STATUS = {
     13: "OperationRequiresDensityMatrix", 14: "OperationRequiresShotExecution", 15: "InvalidArgument",
     16: "IncompatibleDeviceState", 17: "StateSizeNotPowerOfTwo", 18: "InvalidBatchSamples",
     19: "MemoryBudgetExceeded", 20: "InvalidFiniteDifferenceStep", 21: "SimulationSizeOverflow",
+    22: "InvalidProbability", 23: "InvalidShotCount", 24: "ClassicalBitOutOfRange",
 }
 
 # Record layouts identical to the C structs (checked by tests/test_abi.py).
@@ -105,6 +106,10 @@ SIGNATURES = {
     "qf_plan_build": (None, [P, sz, u32, C.POINTER(PlanOptions), PP]),
     "qf_plan_stats_get": (None, [P, C.POINTER(PlanStats)]),
     "qf_plan_describe": (None, [P, C.c_char_p, sz, C.POINTER(sz)]),
+    "qf_state_measure": (None, [P, u32, dbl, C.POINTER(C.c_uint8)]),
+    "qf_batch_measure_with_uniforms": (None, [P, u32, P, sz, P]),
+    "qf_batch_apply_gate_masked": (None, [P, i32, u32, dbl, P, sz]),
+    "qf_run_circuit_shots": (None, [P, P, u32, P, sz, u64, u64, i32, P, C.POINTER(u32)]),
     "qf_plan_build_parameterized": (None, [P, sz, u32, u32, P, sz, u32, i32, C.POINTER(PlanOptions), PP]),
     "qf_plan_source": (None, [P, i32, C.c_char_p, sz, C.POINTER(sz)]),
     "qf_plan_compile": (None, [P, i32, C.POINTER(dbl), C.POINTER(i32)]),

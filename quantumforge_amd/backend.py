@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _native as N
 from ._native import QuantumError, check, lib, ptr
-from .core import Circuit, Gate, PauliSum, RotationAxis, SingleQubitObservable, StateVector
+from .core import Circuit, Gate, PauliSum, RotationAxis, ShotResult, SingleQubitObservable, StateVector
 
 C128, C64 = 0, 1
 _PRECISION = {"complex128": C128, "c128": C128, C128: C128, "complex64": C64, "c64": C64, C64: C64}
@@ -173,6 +173,30 @@ class HipBackend:
         batch.run_circuit(circuit)
         return batch.download()
 
+    def _shots(self, initial: StateVector, circuit: Circuit, shots: int, seed: int, batched: bool) -> ShotResult:
+        amps = np.ascontiguousarray(initial.amplitudes, dtype=np.complex128)
+        ops = circuit.to_ops()
+        width = C.c_uint32()
+        check(lib().qf_run_circuit_shots(self._h, ptr(amps), initial.qubits, ptr(ops), len(ops), int(shots),
+                                         int(seed), int(batched), None, C.byref(width)))
+        regs = np.zeros((int(shots), width.value), dtype=np.uint8)
+        buf = regs if regs.size else np.zeros(1, dtype=np.uint8)
+        check(lib().qf_run_circuit_shots(self._h, ptr(amps), initial.qubits, ptr(ops), len(ops), int(shots),
+                                         int(seed), int(batched), ptr(buf), C.byref(width)))
+        return ShotResult.from_registers(regs)
+
+    def run_circuit_shots(self, initial: StateVector, circuit: Circuit, shots: int, seed: int) -> ShotResult:
+        """CudaBackend::run_circuit_shots, src/cuda.rs:1085-1159: shots one
+        after another on a resident state, measurements and classical
+        feed-forward on the device, SmallRng(seed) draws in the CPU's order."""
+        return self._shots(initial, circuit, shots, seed, False)
+
+    def run_circuit_shots_batched(self, initial: StateVector, circuit: Circuit, shots: int,
+                                  seed: int) -> ShotResult:
+        """CudaBackend::run_circuit_shots_batched, src/cuda.rs:1161-1269: all
+        shots as one batch of states (split to fit device memory)."""
+        return self._shots(initial, circuit, shots, seed, True)
+
     def apply_rotation_chain(self, state: StateVector, gates: Sequence[Tuple[RotationAxis, int, float]]) -> None:
         """Updates `state` in place (CudaBackend::apply_rotation_chain)."""
         dev = self.upload(state)
@@ -256,6 +280,13 @@ class HipDeviceState:
         check(lib().qf_state_download_c64(self._h, ptr(out)))
         return out
 
+    def measure(self, qubit: int, rng) -> int:
+        """CudaDeviceState::measure, src/cuda.rs:1386-1442: draws
+        rng.next_f64() and collapses the resident state."""
+        o = C.c_uint8()
+        check(lib().qf_state_measure(self._h, int(qubit), float(rng.next_f64()), C.byref(o)))
+        return int(o.value)
+
     def norm_sqr(self) -> float:
         v = C.c_double()
         check(lib().qf_state_norm_sqr(self._h, C.byref(v)))
@@ -320,6 +351,24 @@ class HipDeviceStateBatch:
     def run_circuit(self, circuit: Circuit, plan: Optional[Plan] = None) -> None:
         ops = circuit.to_ops()
         check(lib().qf_batch_run_circuit(self._h, ptr(ops), len(ops), plan._h if plan is not None else None))
+
+    def measure(self, qubit: int, rng) -> List[int]:
+        """CudaDeviceStateBatch::measure, src/cuda.rs:1500-1509: one
+        rng.next_f64() per state, in state order."""
+        return self.measure_with_uniforms(qubit, [rng.next_f64() for _ in range(self.count)])
+
+    def measure_with_uniforms(self, qubit: int, uniforms: Sequence[float]) -> List[int]:
+        """CudaDeviceStateBatch::measure_with_uniforms, src/cuda.rs:1511-1609."""
+        u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64))
+        out = np.zeros(max(1, self.count), dtype=np.uint8)
+        check(lib().qf_batch_measure_with_uniforms(self._h, int(qubit), ptr(u), u.size, ptr(out)))
+        return [int(x) for x in out[: self.count]]
+
+    def apply_gate_masked(self, gate: Gate, target: int, enabled: Sequence[int]) -> None:
+        """The gate on the states whose enabled flag is set (classically
+        conditioned gates of a shot batch)."""
+        e = np.ascontiguousarray(np.asarray(enabled, dtype=np.uint8))
+        check(lib().qf_batch_apply_gate_masked(self._h, gate.id, int(target), float(gate.theta), ptr(e), e.size))
 
     def synchronize(self) -> None:
         self._backend.synchronize()

@@ -118,8 +118,14 @@ class Circuit:
         self._ops.append((4, 0, int(target), 0, 0.0))
 
     def measure(self, target: int, classical_bit: int) -> None:
-        """A measurement: running it reports OperationRequiresShotExecution."""
+        """Circuit::measure, src/lib.rs:2548-2553.  State-vector runs report
+        OperationRequiresShotExecution; shot runs execute it."""
         self._ops.append((5, 0, int(target), int(classical_bit), 0.0))
+
+    def if_classical_bit(self, classical_bit: int, value: int, gate: Gate, target: int) -> None:
+        """Circuit::if_classical_bit, src/lib.rs:2567-2574 (value & 1): the
+        gate runs when the classical bit equals value (shot runs only)."""
+        self._ops.append((6, gate.id, int(target), int(classical_bit) | ((int(value) & 1) << 16), float(gate.theta)))
 
     def extend(self, other: "Circuit") -> None:
         self._ops.extend(other._ops)
@@ -151,6 +157,30 @@ class Circuit:
         for r in ops:
             c._ops.append((int(r["kind"]), int(r["gate"]), int(r["a"]), int(r["b"]), float(r["theta"])))
         return c
+
+
+class ShotResult:
+    """ShotResult, src/lib.rs:2443-2452: shot count and the classical
+    registers (tuples of bits, least-significant classical bit first) mapped
+    to their counts."""
+
+    def __init__(self, shots: int, counts: dict) -> None:
+        self.shots = int(shots)
+        self.counts = dict(counts)
+
+    @staticmethod
+    def from_registers(registers: np.ndarray) -> "ShotResult":
+        counts: dict = {}
+        for row in registers:
+            key = tuple(int(x) for x in row)
+            counts[key] = counts.get(key, 0) + 1
+        return ShotResult(len(registers), counts)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, ShotResult) and self.shots == other.shots and self.counts == other.counts
+
+    def __repr__(self) -> str:
+        return f"ShotResult(shots={self.shots}, counts={self.counts})"
 
 
 # ---------------------------------------------------------------------------

@@ -102,6 +102,60 @@ __global__ __launch_bounds__(kThreads) void k_sweep_diag(cx<R>* __restrict__ s, 
   }
 }
 
+// Projective collapse after measuring qubit q of each of the `total >>
+// qubits` states: amplitudes whose bit q differs from the instance's outcome
+// become 0, the others are divided by the instance's sqrt(probability)
+// (StateVector::measure, src/lib.rs:1342-1348, which divides too).
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_collapse(cx<R>* __restrict__ s, uint64_t total, uint32_t qubits,
+                                                       uint32_t q, const uint8_t* __restrict__ outcomes,
+                                                       const double* __restrict__ scales) {
+#pragma clang fp contract(off)
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = global_thread(); i < total; i += stride) {
+        const uint64_t inst = i >> qubits;
+        const uint32_t bit = uint32_t((i >> q) & 1);
+        cx<R> v = s[i];
+        if (bit != outcomes[inst]) {
+            v = cx<R>{R(0), R(0)};
+        } else {
+            const R sc = R(scales[inst]);
+            v = cx<R>{v.re / sc, v.im / sc};
+        }
+        s[i] = v;
+    }
+}
+
+// One-qubit gate on the states whose enabled[inst] is set (classically
+// conditioned gates of a shot batch), with k_sweep_1q's unfused arithmetic.
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_sweep_1q_masked(cx<R>* __restrict__ s, uint64_t npairs,
+                                                              uint32_t qubits, uint32_t q,
+                                                              const uint8_t* __restrict__ enabled, cd m00, cd m01,
+                                                              cd m10, cd m11) {
+#pragma clang fp contract(off)
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t p = global_thread(); p < npairs; p += stride) {
+        const uint64_t i0 = insert_zero(p, q);
+        if (!enabled[i0 >> qubits]) continue;
+        const uint64_t i1 = i0 | (uint64_t(1) << q);
+        const cx<R> a0 = s[i0], a1 = s[i1];
+        const R ar0 = a0.re, ai0 = a0.im, ar1 = a1.re, ai1 = a1.im;
+        const R b00r = R(m00.re), b00i = R(m00.im), b01r = R(m01.re), b01i = R(m01.im);
+        const R b10r = R(m10.re), b10i = R(m10.im), b11r = R(m11.re), b11i = R(m11.im);
+        R p0r = b00r * ar0 - b00i * ai0;
+        R p0i = b00r * ai0 + b00i * ar0;
+        R p1r = b01r * ar1 - b01i * ai1;
+        R p1i = b01r * ai1 + b01i * ar1;
+        R q0r = b10r * ar0 - b10i * ai0;
+        R q0i = b10r * ai0 + b10i * ar0;
+        R q1r = b11r * ar1 - b11i * ai1;
+        R q1i = b11r * ai1 + b11i * ar1;
+        s[i0] = cx<R>{p0r + p1r, p0i + p1i};
+        s[i1] = cx<R>{q0r + q1r, q0i + q1i};
+    }
+}
+
 // |0..0> for `count` states of 2^qubits amplitudes each (a fill kernel, so it
 // may sit inside a captured graph).
 template <class R>
@@ -445,7 +499,9 @@ __device__ __forceinline__ void block_reduce(dd& re, dd& im) {
     }
 }
 
-enum RedKind : int { RED_NORM = 0, RED_Z = 1, RED_XY = 2, RED_PAULI = 3, RED_INNER = 4 };
+// RED_P1: sum of |a_i|^2 over the indices with bit q set (the probability of
+// measuring 1 on qubit q, StateVector::measure, src/lib.rs:1328-1335).
+enum RedKind : int { RED_NORM = 0, RED_Z = 1, RED_XY = 2, RED_PAULI = 3, RED_INNER = 4, RED_P1 = 5 };
 
 // Device form of a Pauli term: c' = coefficient * i^(number of Y), sign from
 // parity(index & z_mask); contribution conj(psi[i ^ x]) * c' * sign * psi[i].
@@ -464,8 +520,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const cx<R>* __restrict__ s
     const cx<R>* __restrict__ a = s + inst * dim;
     dd re{0, 0}, im{0, 0};
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    if (kind == RED_NORM || kind == RED_Z) {
+    if (kind == RED_NORM || kind == RED_Z || kind == RED_P1) {
         for (uint64_t i = global_thread(); i < dim; i += stride) {
+            if (kind == RED_P1 && !((i >> q) & 1)) continue;
             const cx<R> v = a[i];
             const double x = double(v.re) * double(v.re) + double(v.im) * double(v.im);
             neumaier(re, (kind == RED_Z && ((i >> q) & 1)) ? -x : x);

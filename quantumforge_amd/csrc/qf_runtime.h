@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,42 @@
 
 struct qf_plan {
     qf::Plan plan;
+};
+
+struct qf_backend {
+    int32_t ordinal = 0;
+    hipStream_t stream = nullptr;
+    std::atomic<int> refs{1};
+    qf::PlanOptions opts;
+    // Stream-ordered scratch; reused after a stream synchronize.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double* d_out = nullptr;
+    size_t d_out_count = 0;
+    void* d_ops = nullptr;
+    size_t d_ops_bytes = 0;
+    // Per-instance measurement outcomes / masks and collapse scales.
+    void* meas_a = nullptr;
+    size_t meas_a_bytes = 0;
+    void* meas_b = nullptr;
+    size_t meas_b_bytes = 0;
+};
+
+struct qf_state {
+    qf_backend* be = nullptr;
+    void* d = nullptr;
+    uint32_t qubits = 0;
+    int32_t precision = QF_C128;
+    uint64_t dim = 0;
+};
+
+struct qf_batch {
+    qf_backend* be = nullptr;
+    void* d = nullptr;
+    uint32_t qubits = 0;
+    uint32_t count = 0;
+    int32_t precision = QF_C128;
+    uint64_t dim = 0;
 };
 
 namespace qfrt {

@@ -71,6 +71,9 @@ extern "C" const char* qf_status_name(qf_status s) {
         case QF_MEMORY_BUDGET_EXCEEDED: return "MemoryBudgetExceeded";
         case QF_INVALID_FD_STEP: return "InvalidFiniteDifferenceStep";
         case QF_SIMULATION_SIZE_OVERFLOW: return "SimulationSizeOverflow";
+        case QF_INVALID_PROBABILITY: return "InvalidProbability";
+        case QF_INVALID_SHOT_COUNT: return "InvalidShotCount";
+        case QF_CLASSICAL_BIT_OUT_OF_RANGE: return "ClassicalBitOutOfRange";
         default: return "Unknown";
     }
 }
@@ -141,38 +144,6 @@ qf::PlanOptions plan_options_from(const qf_plan_options* o) {
 // ---------------------------------------------------------------------------
 // Handles
 // ---------------------------------------------------------------------------
-struct qf_backend {
-    int32_t ordinal = 0;
-    hipStream_t stream = nullptr;
-    std::atomic<int> refs{1};
-    qf::PlanOptions opts;
-    // Stream-ordered scratch; reused after a stream synchronize.
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    double* d_out = nullptr;
-    size_t d_out_count = 0;
-    void* d_ops = nullptr;
-    size_t d_ops_bytes = 0;
-};
-
-struct qf_state {
-    qf_backend* be = nullptr;
-    void* d = nullptr;
-    uint32_t qubits = 0;
-    int32_t precision = QF_C128;
-    uint64_t dim = 0;
-};
-
-struct qf_batch {
-    qf_backend* be = nullptr;
-    void* d = nullptr;
-    uint32_t qubits = 0;
-    uint32_t count = 0;
-    int32_t precision = QF_C128;
-    uint64_t dim = 0;
-};
-
-
 namespace qfrt {
 
 qf_backend* state_backend(const qf_state* st) { return st->be; }
@@ -185,6 +156,8 @@ void backend_release(qf_backend* be) {
         if (be->scratch) (void)hipFree(be->scratch);
         if (be->d_out) (void)hipFree(be->d_out);
         if (be->d_ops) (void)hipFree(be->d_ops);
+        if (be->meas_a) (void)hipFree(be->meas_a);
+        if (be->meas_b) (void)hipFree(be->meas_b);
         (void)hipStreamDestroy(be->stream);
         delete be;
     }
