@@ -89,11 +89,28 @@ def rotation_gate(axis: RotationAxis, theta: float) -> Gate:
 # ---------------------------------------------------------------------------
 # Circuit (src/lib.rs:2386-2802)
 # ---------------------------------------------------------------------------
+@dataclass(frozen=True)
+class NoiseChannel:
+    """NoiseChannel, src/lib.rs:1561-1600: name is one of Depolarizing
+    (probability), AmplitudeDamping (gamma), PhaseDamping (gamma),
+    ThermalDamping (gamma_down, beta_delta)."""
+
+    name: str
+    params: Tuple[float, ...] = ()
+
+    KINDS = ("Depolarizing", "AmplitudeDamping", "PhaseDamping", "ThermalDamping")
+
+    @property
+    def kind(self) -> int:
+        return NoiseChannel.KINDS.index(self.name)
+
+
 class Circuit:
     """Ordered list of operations (CircuitOp, src/lib.rs:2386-2440)."""
 
     def __init__(self) -> None:
         self._ops: List[tuple] = []
+        self._channels: dict = {}  # op index -> NoiseChannel
 
     def push(self, gate: Gate, target: int) -> None:
         self._ops.append((0, gate.id, int(target), 0, float(gate.theta)))
@@ -113,9 +130,12 @@ class Circuit:
         the global phase e^{-i phi/4})."""
         self._ops.append((7, 0, int(control), int(target), float(phi)))
 
-    def channel(self, target: int, channel: object = None) -> None:
-        """A noise channel: kept in the list; running it reports OperationRequiresDensityMatrix."""
-        self._ops.append((4, 0, int(target), 0, 0.0))
+    def channel(self, target: int, channel: Optional[NoiseChannel] = None) -> None:
+        """A noise channel: kept in the list; state-vector and shot runs report
+        OperationRequiresDensityMatrix."""
+        ch = channel if isinstance(channel, NoiseChannel) else NoiseChannel("Depolarizing", (0.0,))
+        self._channels[len(self._ops)] = ch
+        self._ops.append((4, ch.kind, int(target), 0, float(ch.params[0]) if ch.params else 0.0))
 
     def measure(self, target: int, classical_bit: int) -> None:
         """Circuit::measure, src/lib.rs:2548-2553.  State-vector runs report
@@ -128,6 +148,8 @@ class Circuit:
         self._ops.append((6, gate.id, int(target), int(classical_bit) | ((int(value) & 1) << 16), float(gate.theta)))
 
     def extend(self, other: "Circuit") -> None:
+        for i, ch in other._channels.items():
+            self._channels[len(self._ops) + i] = ch
         self._ops.extend(other._ops)
 
     def __len__(self) -> int:
