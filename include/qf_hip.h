@@ -68,7 +68,9 @@ enum {
     QF_SIMULATION_SIZE_OVERFLOW = 21,   /* QuantumError::SimulationSizeOverflow */
     QF_INVALID_PROBABILITY = 22,        /* QuantumError::InvalidProbability */
     QF_INVALID_SHOT_COUNT = 23,         /* QuantumError::InvalidShotCount */
-    QF_CLASSICAL_BIT_OUT_OF_RANGE = 24  /* QuantumError::ClassicalBitOutOfRange */
+    QF_CLASSICAL_BIT_OUT_OF_RANGE = 24, /* QuantumError::ClassicalBitOutOfRange */
+    QF_NON_HERMITIAN_DENSITY_MATRIX = 25,/* QuantumError::NonHermitianDensityMatrix (host-side entropy) */
+    QF_INVALID_SUBSYSTEM_SELECTION = 26 /* QuantumError::InvalidSubsystemSelection (host-side partial trace) */
 };
 
 /* Gate ids, in the order of `enum Gate` (src/lib.rs:2239-2265). */
@@ -76,6 +78,13 @@ enum {
     QF_GATE_I = 0, QF_GATE_X = 1, QF_GATE_Y = 2, QF_GATE_Z = 3, QF_GATE_H = 4,
     QF_GATE_S = 5, QF_GATE_SDG = 6, QF_GATE_T = 7, QF_GATE_TDG = 8,
     QF_GATE_RX = 9, QF_GATE_RY = 10, QF_GATE_RZ = 11
+};
+
+/* Noise channels, in the declaration order of NoiseChannel
+ * (src/lib.rs:1561-1600); parameters as NoiseChannel::params(). */
+enum {
+    QF_CHANNEL_DEPOLARIZING = 0, QF_CHANNEL_AMPLITUDE_DAMPING = 1, QF_CHANNEL_PHASE_DAMPING = 2,
+    QF_CHANNEL_THERMAL_DAMPING = 3
 };
 
 /* Two-qubit op ids, numbered as the reference kernel's `op` argument
@@ -100,7 +109,9 @@ typedef struct { float re, im; } qf_c64;
 /* One operation of a Circuit (CircuitOp, src/lib.rs:2386-2440).
  * kind: 0 single-qubit {gate, a=target, theta for Rx/Ry/Rz},
  *       1 CNOT {a=control, b=target}, 2 CZ {a, b}, 3 SWAP {a, b},
- *       4 channel, 5 measure {a=target, b=classical bit},
+ *       4 channel {gate = QF_CHANNEL_*, a=target, theta = first parameter,
+ *       theta2 = ThermalDamping's beta_delta} (NoiseChannel,
+ *       src/lib.rs:1561-1600), 5 measure {a=target, b=classical bit},
  *       6 classically conditioned gate {gate, a=target, theta, b=classical bit
  *       | value << 16} (CircuitOp::Measure / ConditionalSingleQubit,
  *       src/lib.rs:2386-2440; value is 0 or 1, if_classical_bit keeps value & 1);
@@ -114,6 +125,7 @@ typedef struct {
     int32_t gate;
     uint32_t a, b;
     double theta;
+    double theta2;
 } qf_op;
 
 /* One operation of a ParameterizedCircuit (ParameterizedOp, src/lib.rs:2942-2996).
@@ -172,6 +184,7 @@ typedef struct qf_backend qf_backend;
 typedef struct qf_state qf_state;
 typedef struct qf_batch qf_batch;
 typedef struct qf_plan qf_plan;
+typedef struct qf_density qf_density;
 
 /* ------------------------------------------------------------------------ */
 /* Errors                                                                    */
@@ -376,6 +389,18 @@ qf_status qf_batch_measure_with_uniforms(qf_batch* batch, uint32_t qubit, const 
  * of a shot batch, src/cuda.rs:1239-1249): the gate on states with enabled[i]. */
 qf_status qf_batch_apply_gate_masked(qf_batch* batch, int32_t gate, uint32_t target, double theta,
                                      const uint8_t* enabled, size_t count);
+/* One Kraus-trajectory step of a noise channel (QF_CHANNEL_*, p0 / p1 as
+ * qf_density_apply_channel) on qubit `target` of every state, one uniform in
+ * [0, 1) per state (else InvalidBatchSamples), with the branch rules of
+ * StateVector::apply_channel_trajectory (src/lib.rs:1356-1505): depolarizing
+ * applies X, Y or Z; the damping channels take each state's norm and P(1)
+ * and renormalize the drawn branch (InvalidProbability for a branch of
+ * probability <= DBL_EPSILON).  The per-batch step of
+ * run_circuit_shots_batched (CudaDeviceStateBatch::
+ * apply_channel_trajectory_with_uniforms, src/cuda.rs:1670-1790, which is
+ * private there); ThermalDamping is included. */
+qf_status qf_batch_apply_channel_trajectory(qf_batch* batch, int32_t channel, uint32_t target, double p0, double p1,
+                                            const double* uniforms, size_t count);
 /* CudaBackend::run_circuit_shots (batched = 0, src/cuda.rs:1085-1159) and
  * run_circuit_shots_batched (batched = 1, src/cuda.rs:1161-1269) from a
  * complex128 initial state: SmallRng(seed) uniforms in the CPU's shot-major
@@ -383,12 +408,85 @@ qf_status qf_batch_apply_gate_masked(qf_batch* batch, int32_t gate, uint32_t tar
  * CPU's.  Writes the classical register of every shot, [shots][width] bytes
  * (least-significant classical bit first), into `registers`; the caller
  * counts them (ShotResult::counts).  With registers == NULL only validates and
- * sets *classical_width.  Errors: InvalidShotCount, ClassicalBitOutOfRange,
- * OperationRequiresDensityMatrix (noise channels: trajectories are not built),
- * the gate errors, InvalidProbability. */
+ * sets *classical_width.  Noise channels: the batched form runs Kraus
+ * trajectories (one uniform per channel and shot in the same shot-major order,
+ * the initial state normalized first, branch rules of
+ * StateVector::apply_channel_trajectory, src/lib.rs:1356-1505, so the
+ * registers equal Circuit::run_shots_with_noise); the sequential form rejects
+ * them with OperationRequiresDensityMatrix (src/cuda.rs:1109-1113).  Errors:
+ * InvalidShotCount, ClassicalBitOutOfRange, the gate errors,
+ * InvalidProbability. */
 qf_status qf_run_circuit_shots(qf_backend* backend, const qf_c128* initial, uint32_t qubits, const qf_op* ops,
                                size_t nops, uint64_t shots, uint64_t seed, int32_t batched, uint8_t* registers,
                                uint32_t* classical_width);
+
+/* ------------------------------------------------------------------------ */
+/* Density matrices (CudaDeviceDensityMatrix, src/cuda.rs:2007-2237)         */
+/* ------------------------------------------------------------------------ */
+/* A density matrix of n qubits is held as its row-major entries, which is a
+ * state vector of 2n qubits: entry (row, col) at index row * 2^n + col, row
+ * qubit q at vector bit n + q, column qubit q at bit q.  Gates, channels and
+ * whole circuits run in place (no scratch matrix) through the fused tile
+ * planner: rho -> U rho U^dagger is U on the row bits and conj(U) on the
+ * column bits, and a channel is one 4x4 superoperator on the bit pair
+ * (q, n + q).  Complex128 only; 1 <= n <= 15 (validate_density_size,
+ * src/cuda.rs:2239-2253: ZeroQubits, StateTooLarge). */
+
+/* CudaBackend::upload_density, src/cuda.rs:969-1022: `entries` row major,
+ * len = 4^qubits (else StateTooLarge, as the reference). */
+qf_status qf_density_upload(qf_backend* backend, const qf_c128* entries, size_t len, uint32_t qubits,
+                            qf_density** out);
+/* DensityMatrix::from_state (src/lib.rs:1642-1656) of a resident complex128
+ * state, computed on the device: entries = psi[row] * conj(psi[col]). */
+qf_status qf_density_from_state(const qf_state* state, qf_density** out);
+/* DensityMatrix::maximally_mixed (src/lib.rs:1659-1704) on the device. */
+qf_status qf_density_maximally_mixed(qf_backend* backend, uint32_t qubits, qf_density** out);
+qf_status qf_density_destroy(qf_density* density);
+/* CudaDeviceDensityMatrix::qubits / download, src/cuda.rs:2009-2011, 2215-2227. */
+qf_status qf_density_qubits(const qf_density* density, uint32_t* out);
+qf_status qf_density_download(const qf_density* density, qf_c128* out, size_t len);
+/* CudaDeviceDensityMatrix::apply_gate, src/cuda.rs:2015-2029: QubitOutOfRange,
+ * NonFiniteAngle for a non-finite rotation angle. */
+qf_status qf_density_apply_gate(qf_density* density, int32_t gate, uint32_t target, double theta);
+/* apply_cnot / apply_cz / apply_swap, src/cuda.rs:2069-2121 (pair errors of
+ * the CPU path: QubitOutOfRange, InvalidQubitPair). */
+qf_status qf_density_apply_cnot(qf_density* density, uint32_t control, uint32_t target);
+qf_status qf_density_apply_cz(qf_density* density, uint32_t first, uint32_t second);
+qf_status qf_density_apply_swap(qf_density* density, uint32_t first, uint32_t second);
+/* CudaDeviceDensityMatrix::apply_channel, src/cuda.rs:2123-2159, channel
+ * QF_CHANNEL_*, p0 = first parameter, p1 = ThermalDamping's beta_delta:
+ * QubitOutOfRange, InvalidProbability (validate_probability and
+ * NoiseChannel::thermal_rates, src/lib.rs:2229-2235, 1611-1627).
+ * ThermalDamping runs here as on the CPU path (the reference's CUDA backend
+ * reports UnsupportedOperation for it). */
+qf_status qf_density_apply_channel(qf_density* density, int32_t channel, uint32_t target, double p0, double p1);
+/* CudaDeviceDensityMatrix::measure, src/cuda.rs:2161-2213, with the uniform
+ * passed in (the caller's rng.next_f64()): outcome = uniform < P(1) (clamped
+ * to [0, 1]), InvalidProbability when the drawn branch has probability <=
+ * DBL_EPSILON, kept entries divided by the probability as
+ * DensityMatrix::measure does (src/lib.rs:1924-1950). */
+qf_status qf_density_measure(qf_density* density, uint32_t qubit, double uniform, uint8_t* outcome);
+/* DensityMatrix::trace, purity, expectation and probability_one
+ * (src/lib.rs:1732-1921) as FP64 reductions over the resident entries. */
+qf_status qf_density_trace(const qf_density* density, double* re, double* im);
+qf_status qf_density_purity(const qf_density* density, double* out);
+qf_status qf_density_expectation(const qf_density* density, int32_t observable, uint32_t qubit, double* out);
+qf_status qf_density_probability_one(const qf_density* density, uint32_t qubit, double* out);
+/* Runs a circuit (gates, CNOT/CZ/SWAP, controlled phase, channels) on a
+ * resident density matrix in place, fused into tile passes over the 2n-qubit
+ * view.  Ops are validated in order before anything runs; measurements and
+ * conditional gates give OperationRequiresShotExecution (Circuit::run_density,
+ * src/lib.rs:2804-2819). */
+qf_status qf_density_run_circuit(qf_density* density, const qf_op* ops, size_t nops);
+/* CudaBackend::run_density_circuit, src/cuda.rs:1025-1047: upload, run,
+ * download into `out` (4^qubits entries). */
+qf_status qf_run_density_circuit(qf_backend* backend, const qf_c128* initial, size_t len, uint32_t qubits,
+                                 const qf_op* ops, size_t nops, qf_c128* out, size_t out_len);
+/* The plan qf_density_run_circuit would run (host only; for inspection,
+ * qf_plan_compile and repeated runs), and a run of such a plan. */
+qf_status qf_density_plan_build(const qf_op* ops, size_t nops, uint32_t qubits, const qf_plan_options* options,
+                                qf_plan** out);
+qf_status qf_density_run_plan(qf_density* density, const qf_plan* plan);
 
 /* ------------------------------------------------------------------------ */
 /* Planning (no reference counterpart; see qf_plan_options)                  */

@@ -104,9 +104,119 @@ def measure(psi: np.ndarray, q: int, u: float):
     return o, out
 
 
-def run_shots(ops, n: int, psi0: np.ndarray, shots: int, seed: int) -> np.ndarray:
-    """Circuit::run_shots (noiseless), src/lib.rs:2843-2911, with dense
-    unitaries: one SmallRng uniform per measurement, shot-major."""
+def channel_params(op) -> tuple:
+    p = (float(op["theta"]),)
+    if int(op["gate"]) == 3:
+        p += (float(op["theta2"]) if "theta2" in (op.dtype.names or ()) else 0.0,)
+    return p
+
+
+def kraus(kind: int, params) -> list:
+    """Kraus operators of a noise channel (NoiseChannel, src/lib.rs:1561-1600):
+    0 depolarizing (1-p) rho + p/3 (X rho X + Y rho Y + Z rho Z), 1 amplitude
+    damping |1> -> |0> with probability gamma, 2 phase damping, 3 thermal
+    damping with gamma_up = gamma_down * exp(-beta_delta)."""
+    p = float(params[0])
+    if kind == 0:
+        return [np.sqrt(1 - p) * I2, np.sqrt(p / 3) * PX, np.sqrt(p / 3) * PY, np.sqrt(p / 3) * PZ]
+    if kind == 1:
+        return [np.diag([1.0, np.sqrt(1 - p)]).astype(complex), np.array([[0, np.sqrt(p)], [0, 0]], dtype=complex)]
+    if kind == 2:
+        return [np.diag([1.0, np.sqrt(1 - p)]).astype(complex), np.diag([0.0, np.sqrt(p)]).astype(complex)]
+    gu = p * np.exp(-float(params[1]))
+    return [np.diag([np.sqrt(1 - gu), np.sqrt(1 - p)]).astype(complex),
+            np.array([[0, np.sqrt(p)], [0, 0]], dtype=complex), np.array([[0, 0], [np.sqrt(gu), 0]], dtype=complex)]
+
+
+def apply_channel(rho: np.ndarray, kind: int, params, q: int, n: int) -> np.ndarray:
+    out = np.zeros_like(rho)
+    for K in kraus(kind, params):
+        E = embed_1q(K, q, n)
+        out = out + E @ rho @ E.conj().T
+    return out
+
+
+def density_run(ops, n: int, rho: np.ndarray) -> np.ndarray:
+    """Circuit::run_density (src/lib.rs:2804-2819) with dense matrices:
+    U rho U^dagger for gates, sum_k E_k rho E_k^dagger for channels."""
+    rho = np.asarray(rho, dtype=complex).reshape(1 << n, 1 << n).copy()
+    for op in ops:
+        k = int(op["kind"])
+        if k == 4:
+            rho = apply_channel(rho, int(op["gate"]), channel_params(op), int(op["a"]), n)
+        elif k in (5, 6):
+            raise ValueError("measurement needs shot execution")
+        else:
+            U = op_unitary(op, n)
+            rho = U @ rho @ U.conj().T
+    return rho
+
+
+def density_measure(rho: np.ndarray, q: int, u: float):
+    """Projective measurement of a density matrix (DensityMatrix::measure,
+    src/lib.rs:1924-1950, as P rho P / p): returns (outcome, rho')."""
+    d = rho.shape[0]
+    bit = (np.arange(d) >> q) & 1
+    p_one = min(max(float(np.sum(np.real(np.diag(rho))[bit == 1])), 0.0), 1.0)
+    o = int(u < p_one)
+    p = p_one if o else 1.0 - p_one
+    if p <= np.finfo(float).eps:
+        raise ValueError("invalid probability")
+    P = np.diag((bit == o).astype(float))
+    return o, P @ rho @ P / p
+
+
+def trajectory(psi: np.ndarray, kind: int, params, q: int, rng) -> np.ndarray:
+    """One Kraus-trajectory step (StateVector::apply_channel_trajectory,
+    src/lib.rs:1356-1505) with vector operations: pick branch k with
+    probability ||K_k psi||^2 / ||psi||^2 by the rule of that function (one
+    uniform), return K_k psi normalized."""
+    bit = ((np.arange(len(psi)) >> q) & 1).astype(bool)
+    p = float(params[0])
+    if kind == 0:
+        u = rng.next_f64()
+        if u >= 1.0 - p and p > 0.0:
+            b = min(max(int(np.floor((u - (1.0 - p)) * (3.0 / p))), 0), 2)
+            return embed_1q([PX, PY, PZ][b], q, int(np.log2(len(psi)))) @ psi
+        return psi
+    norm = float(np.sum(np.abs(psi) ** 2))
+    p1 = float(np.sum(np.abs(psi[bit]) ** 2))
+    if kind in (1, 2):
+        jp = min(max(p * p1 / norm, 0.0), 1.0)
+        jump = rng.next_f64() < jp
+        bn = p * p1 if jump else norm - p * p1
+        if bn <= np.finfo(float).eps:
+            raise ValueError("invalid probability")
+        inv = 1.0 / np.sqrt(bn)
+        out = np.zeros_like(psi)
+        if jump and kind == 1:
+            out[~bit] = psi[bit] * (np.sqrt(p) * inv)
+        elif jump:
+            out[bit] = psi[bit] * (np.sqrt(p) * inv)
+        else:
+            out[~bit] = psi[~bit] * inv
+            out[bit] = psi[bit] * (np.sqrt(1.0 - p) * inv)
+        return out
+    gd, gu = p, p * np.exp(-float(params[1]))
+    pr1 = min(max(p1 / norm, 0.0), 1.0)
+    pr0 = 1.0 - pr1
+    down, up, stay = pr1 * gd, pr0 * gu, pr0 * (1 - gu) + pr1 * (1 - gd)
+    u = rng.next_f64()
+    out = np.zeros_like(psi)
+    if u < down:
+        out[~bit] = psi[bit] * (np.sqrt(gd) / np.sqrt(down))
+    elif u < down + up:
+        out[bit] = psi[~bit] * (np.sqrt(gu) / np.sqrt(up))
+    else:
+        out[~bit] = psi[~bit] * (np.sqrt(1 - gu) / np.sqrt(stay))
+        out[bit] = psi[bit] * (np.sqrt(1 - gd) / np.sqrt(stay))
+    return out
+
+
+def run_shots(ops, n: int, psi0: np.ndarray, shots: int, seed: int, noise: bool = False) -> np.ndarray:
+    """Circuit::run_shots / run_shots_with_noise, src/lib.rs:2822-2911, with
+    dense unitaries: one SmallRng uniform per measurement and per channel,
+    shot-major; with noise the initial state is normalized per shot."""
     from quantumforge_amd.workloads import SmallRng
 
     width = 0
@@ -119,6 +229,8 @@ def run_shots(ops, n: int, psi0: np.ndarray, shots: int, seed: int) -> np.ndarra
     regs = np.zeros((shots, width), dtype=np.uint8)
     for shot in range(shots):
         psi = psi0.astype(complex).copy()
+        if noise:
+            psi = psi / np.sqrt(float(np.sum(np.abs(psi) ** 2)))
         for op in ops:
             k = int(op["kind"])
             if k == 5:
@@ -127,6 +239,10 @@ def run_shots(ops, n: int, psi0: np.ndarray, shots: int, seed: int) -> np.ndarra
                 b = int(op["b"])
                 if regs[shot, b & 0xFFFF] == ((b >> 16) & 1):
                     psi = embed_1q(gate(int(op["gate"]), float(op["theta"])), int(op["a"]), n) @ psi
+            elif k == 4:
+                if not noise:
+                    raise ValueError("channels need run_shots_with_noise")
+                psi = trajectory(psi, int(op["gate"]), channel_params(op), int(op["a"]), rng)
             else:
                 psi = op_unitary(op, n) @ psi
     return regs

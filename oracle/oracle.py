@@ -15,7 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-OP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("theta", "<f8")])
+OP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("theta", "<f8"),
+                     ("theta2", "<f8")])
 POP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("index", "<u4"),
                       ("reserved", "<u4"), ("theta", "<f8")])
 PAULI_DTYPE = np.dtype([("re", "<f8"), ("im", "<f8"), ("x_mask", "<u8"), ("z_mask", "<u8")])
@@ -85,14 +86,50 @@ def lib():
         L.or_fast_run.restype = i32
         L.or_measure.argtypes = [P, u32, dbl, C.POINTER(C.c_uint8)]
         L.or_measure.restype = i32
-        L.or_run_shots.argtypes = [P, u32, P, sz, u64, u64, P, C.POINTER(u32)]
+        L.or_run_shots.argtypes = [P, u32, P, sz, u64, u64, P, C.POINTER(u32), i32]
         L.or_run_shots.restype = i32
+        L.or_channel_trajectory.argtypes = [P, i32, dbl, dbl, u32, C.POINTER(u64)]
+        L.or_channel_trajectory.restype = i32
+        L.or_normalize.argtypes = [P]
+        L.or_normalize.restype = i32
+        L.or_density_from.restype = P
+        L.or_density_from.argtypes = [P, u32]
+        L.or_density_from_state.restype = P
+        L.or_density_from_state.argtypes = [P, u32]
+        L.or_density_free.argtypes = [P]
+        L.or_density_free.restype = None
+        L.or_density_get.argtypes = [P, P]
+        L.or_density_get.restype = None
+        L.or_density_apply_gate.argtypes = [P, i32, dbl, u32]
+        L.or_density_apply_gate.restype = i32
+        L.or_density_apply_2q.argtypes = [P, i32, u32, u32, dbl]
+        L.or_density_apply_2q.restype = i32
+        L.or_density_apply_channel.argtypes = [P, i32, dbl, dbl, u32]
+        L.or_density_apply_channel.restype = i32
+        L.or_density_measure.argtypes = [P, u32, dbl, C.POINTER(C.c_uint8)]
+        L.or_density_measure.restype = i32
+        L.or_density_probability_one.argtypes = [P, u32, C.POINTER(dbl)]
+        L.or_density_probability_one.restype = i32
+        L.or_density_run.argtypes = [P, P, sz]
+        L.or_density_run.restype = i32
         _lib = L
     return _lib
 
 
 def _p(a):
     return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)
+
+
+def as_ops(ops) -> np.ndarray:
+    """Op records in the current layout; arrays written with fewer fields
+    (fixtures from before `theta2`) are copied over field by field."""
+    ops = np.asarray(ops)
+    if ops.dtype == OP_DTYPE:
+        return np.ascontiguousarray(ops)
+    out = np.zeros(ops.shape, dtype=OP_DTYPE)
+    for name in ops.dtype.names:
+        out[name] = ops[name]
+    return out
 
 
 class OracleError(Exception):
@@ -144,7 +181,7 @@ class State:
         _chk(lib().or_apply_swap(self.h, a, b))
 
     def run(self, ops: np.ndarray):
-        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        ops = as_ops(ops)
         _chk(lib().or_run(self.h, _p(ops), len(ops)))
 
     def measure(self, q: int, u: float) -> int:
@@ -155,6 +192,9 @@ class State:
 
     def norm_sqr(self) -> float:
         return lib().or_norm_sqr(self.h)
+
+    def normalize(self) -> None:
+        _chk(lib().or_normalize(self.h))
 
     def probabilities(self) -> np.ndarray:
         out = np.empty(1 << self.qubits, dtype=np.float64)
@@ -233,7 +273,7 @@ def pc_expectation(pops, nparams, ninputs, params, inputs, qubits, terms) -> flo
 
 def time_run(qubits: int, ops: np.ndarray, threads: int = 0) -> float:
     """Seconds for the restated reference CPU path to run `ops` from |0..0>."""
-    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    ops = as_ops(ops)
     t = lib().or_time_run(qubits, _p(ops), len(ops), threads)
     if t < 0:
         raise OracleError(int(-t))
@@ -249,7 +289,7 @@ def fast_run(qubits: int, ops: np.ndarray, amps: np.ndarray | None = None) -> np
     if amps is None:
         amps = np.zeros(1 << qubits, dtype=np.complex128)
         amps[0] = 1
-    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    ops = as_ops(ops)
     _chk(lib().or_fast_run(_p(amps), qubits, _p(ops), len(ops)))
     return amps
 
@@ -269,18 +309,78 @@ class Rng:
         return float(lib().or_rng_next_f64(C.byref(self.s)))
 
 
-def run_shots(initial: np.ndarray, qubits: int, ops: np.ndarray, shots: int, seed: int) -> np.ndarray:
-    """Circuit::run_shots (noiseless): the classical register of every shot,
-    [shots][width] uint8, least-significant classical bit first."""
+def run_shots(initial: np.ndarray, qubits: int, ops: np.ndarray, shots: int, seed: int,
+              noise: bool = False) -> np.ndarray:
+    """Circuit::run_shots (noise=False) or run_shots_with_noise (noise=True):
+    the classical register of every shot, [shots][width] uint8,
+    least-significant classical bit first."""
     initial = np.ascontiguousarray(initial, dtype=np.complex128)
-    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    ops = as_ops(ops)
     w = C.c_uint32()
-    _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, None, C.byref(w)))
+    nz = int(bool(noise))
+    _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, None, C.byref(w), nz))
     out = np.zeros((shots, w.value), dtype=np.uint8)
-    if w.value:
-        _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, _p(out), C.byref(w)))
-    else:
-        _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, _p(np.zeros(1, np.uint8)),
-                                C.byref(w)))
+    buf = out if w.value else np.zeros(1, np.uint8)
+    _chk(lib().or_run_shots(_p(initial), qubits, _p(ops), len(ops), shots, seed, _p(buf), C.byref(w), nz))
     return out
+
+
+def channel_trajectory(state: "State", kind: int, params, qubit: int, rng: "Rng") -> None:
+    """StateVector::apply_channel_trajectory on an oracle state, drawing from `rng`."""
+    p = tuple(params) + (0.0,) * (2 - len(params))
+    _chk(lib().or_channel_trajectory(state.h, kind, p[0], p[1], qubit, C.byref(rng.s)))
+
+
+class Density:
+    """Owned oracle density matrix (row-major entries[row * dim + col])."""
+
+    def __init__(self, qubits: int, entries: np.ndarray | None = None, state: np.ndarray | None = None):
+        self.qubits = qubits
+        if state is not None:
+            a = np.ascontiguousarray(state, dtype=np.complex128)
+            self.h = lib().or_density_from_state(_p(a), qubits)
+        else:
+            e = np.ascontiguousarray(entries, dtype=np.complex128).reshape(-1)
+            self.h = lib().or_density_from(_p(e), qubits)
+        if not self.h:
+            raise OracleError(15)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_density_free(self.h)
+            self.h = None
+
+    def entries(self) -> np.ndarray:
+        d = 1 << self.qubits
+        out = np.zeros(d * d, dtype=np.complex128)
+        lib().or_density_get(self.h, _p(out))
+        return out
+
+    def matrix(self) -> np.ndarray:
+        d = 1 << self.qubits
+        return self.entries().reshape(d, d)
+
+    def apply(self, gate: int, qubit: int, theta: float = 0.0) -> None:
+        _chk(lib().or_density_apply_gate(self.h, gate, theta, qubit))
+
+    def apply_2q(self, kind: int, a: int, b: int, phi: float = 0.0) -> None:
+        _chk(lib().or_density_apply_2q(self.h, kind, a, b, phi))
+
+    def apply_channel(self, kind: int, params, qubit: int) -> None:
+        p = tuple(params) + (0.0,) * (2 - len(params))
+        _chk(lib().or_density_apply_channel(self.h, kind, p[0], p[1], qubit))
+
+    def measure(self, qubit: int, u: float) -> int:
+        o = C.c_uint8()
+        _chk(lib().or_density_measure(self.h, qubit, u, C.byref(o)))
+        return int(o.value)
+
+    def probability_one(self, qubit: int) -> float:
+        v = C.c_double()
+        _chk(lib().or_density_probability_one(self.h, qubit, C.byref(v)))
+        return v.value
+
+    def run(self, ops: np.ndarray) -> None:
+        ops = as_ops(ops)
+        _chk(lib().or_density_run(self.h, _p(ops), len(ops)))
 

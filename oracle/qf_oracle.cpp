@@ -20,10 +20,13 @@
 #include <map>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <chrono>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+extern "C" double or_rng_next_f64(uint64_t* s);
 
 namespace {
 
@@ -209,16 +212,326 @@ int apply_cp(State& st, uint32_t a, uint32_t b, double phi) {
     return OK;
 }
 
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+// StateVector::norm_sqr, src/lib.rs:1000-1002: sequential left-to-right sum.
+double norm_sqr(const State& st) {
+    double acc = 0.0;
+    for (const C& a : st.amps) acc += cnorm(a);
+    return acc;
+}
+
+// StateVector::normalize, src/lib.rs:1005-1014.
+int normalize(State& st) {
+    const double norm = std::sqrt(norm_sqr(st));
+    if (norm == 0.0 || !std::isfinite(norm)) return INVALID_PROBABILITY;
+    for (C& a : st.amps) a = C{a.re / norm, a.im / norm};
+    return OK;
+}
+
+// Noise channels (NoiseChannel, src/lib.rs:1561-1627), numbered as their
+// declaration order; p0 / p1 are NoiseChannel::params().
+enum { CH_DEPOLARIZING = 0, CH_AMPLITUDE_DAMPING = 1, CH_PHASE_DAMPING = 2, CH_THERMAL_DAMPING = 3 };
+
+// validate_probability, src/lib.rs:2229-2235.
+bool valid_probability(double p) { return std::isfinite(p) && p >= 0.0 && p <= 1.0; }
+
+// NoiseChannel::thermal_rates, src/lib.rs:1611-1627: (gamma_up, gamma_down).
+int thermal_rates(double gamma_down, double beta_delta, double* up, double* down) {
+    if (!valid_probability(gamma_down)) return INVALID_PROBABILITY;
+    if (!std::isfinite(beta_delta) || beta_delta < 0.0) return INVALID_PROBABILITY;
+    *up = gamma_down * std::exp(-beta_delta);
+    *down = gamma_down;
+    return OK;
+}
+
+int check_channel(int kind, double p0, double p1) {
+    switch (kind) {
+        case CH_DEPOLARIZING:
+        case CH_AMPLITUDE_DAMPING:
+        case CH_PHASE_DAMPING: return valid_probability(p0) ? OK : INVALID_PROBABILITY;
+        case CH_THERMAL_DAMPING: {
+            double u, d;
+            return thermal_rates(p0, p1, &u, &d);
+        }
+        default: return INVALID_ARGUMENT;
+    }
+}
+
+// Rust's saturating float -> u8 cast (`as u8`): NaN and negatives give 0.
+int as_u8(double x) { return !(x > 0.0) ? 0 : (x >= 255.0 ? 255 : int(x)); }
+
+// StateVector::apply_channel_trajectory, src/lib.rs:1356-1505: one uniform
+// per channel, drawn from `rng` after validation (and, for the damping
+// channels, after the norm and P(1) sums).
+int channel_trajectory(State& st, int kind, double p0, double p1, uint32_t q, uint64_t* rng) {
+    if (q >= st.qubits) return QUBIT_OUT_OF_RANGE;
+    const size_t mask = size_t(1) << q;
+    const size_t n = st.amps.size();
+    if (kind == CH_DEPOLARIZING) {
+        if (!valid_probability(p0)) return INVALID_PROBABILITY;
+        const double sample = or_rng_next_f64(rng);
+        if (sample >= 1.0 - p0 && p0 > 0.0) {
+            const int branch = std::min(as_u8(std::floor((sample - (1.0 - p0)) * (3.0 / p0))), 2);
+            return apply_1q(st, branch == 0 ? G_X : (branch == 1 ? G_Y : G_Z), 0.0, q);
+        }
+        return OK;
+    }
+    double up = 0.0, down = 0.0;
+    if (kind == CH_THERMAL_DAMPING) {
+        const int e = thermal_rates(p0, p1, &up, &down);
+        if (e) return e;
+    } else if (kind == CH_AMPLITUDE_DAMPING || kind == CH_PHASE_DAMPING) {
+        if (!valid_probability(p0)) return INVALID_PROBABILITY;
+    } else {
+        return INVALID_ARGUMENT;
+    }
+    const double norm = norm_sqr(st);
+    if (!std::isfinite(norm) || norm <= kEps) return INVALID_PROBABILITY;
+    double p_one = 0.0;
+    for (size_t i = 0; i < n; ++i)
+        if (i & mask) p_one += cnorm(st.amps[i]);
+    if (kind != CH_THERMAL_DAMPING) {
+        const double gamma = p0;
+        const double jump_probability = std::min(std::max(gamma * p_one / norm, 0.0), 1.0);
+        const bool jump = or_rng_next_f64(rng) < jump_probability;
+        const double branch_norm = jump ? gamma * p_one : norm - gamma * p_one;
+        if (!std::isfinite(branch_norm) || branch_norm <= kEps) return INVALID_PROBABILITY;
+        const double inverse = 1.0 / std::sqrt(branch_norm);
+        if (jump) {
+            const double scale = std::sqrt(gamma) * inverse;
+            for (size_t base = 0; base < n; ++base) {
+                if (base & mask) continue;
+                const C value = cmulf(st.amps[base | mask], scale);
+                if (kind == CH_AMPLITUDE_DAMPING) {
+                    st.amps[base] = value;
+                    st.amps[base | mask] = C{0.0, 0.0};
+                } else {
+                    st.amps[base] = C{0.0, 0.0};
+                    st.amps[base | mask] = value;
+                }
+            }
+        } else {
+            const double excited = std::sqrt(1.0 - gamma) * inverse;
+            for (size_t i = 0; i < n; ++i) st.amps[i] = cmulf(st.amps[i], (i & mask) ? excited : inverse);
+        }
+        return OK;
+    }
+    const double pr1 = std::min(std::max(p_one / norm, 0.0), 1.0);
+    const double pr0 = 1.0 - pr1;
+    const double go_down = pr1 * down, go_up = pr0 * up;
+    const double stay = pr0 * (1.0 - up) + pr1 * (1.0 - down);
+    const double sample = or_rng_next_f64(rng);
+    double branch_norm;
+    int branch;
+    if (sample < go_down) {
+        branch_norm = go_down;
+        branch = 1;
+    } else if (sample < go_down + go_up) {
+        branch_norm = go_up;
+        branch = 2;
+    } else {
+        branch_norm = stay;
+        branch = 0;
+    }
+    if (!std::isfinite(branch_norm) || branch_norm <= kEps) return INVALID_PROBABILITY;
+    const double inverse = 1.0 / std::sqrt(branch_norm);
+    if (branch == 1) {
+        const double scale = std::sqrt(down) * inverse;
+        for (size_t base = 0; base < n; ++base) {
+            if (base & mask) continue;
+            st.amps[base] = cmulf(st.amps[base | mask], scale);
+            st.amps[base | mask] = C{0.0, 0.0};
+        }
+    } else if (branch == 2) {
+        const double scale = std::sqrt(up) * inverse;
+        for (size_t base = 0; base < n; ++base) {
+            if (base & mask) continue;
+            st.amps[base | mask] = cmulf(st.amps[base], scale);
+            st.amps[base] = C{0.0, 0.0};
+        }
+    } else {
+        const double ground = std::sqrt(1.0 - up) * inverse, excited = std::sqrt(1.0 - down) * inverse;
+        for (size_t i = 0; i < n; ++i) st.amps[i] = cmulf(st.amps[i], (i & mask) ? excited : ground);
+    }
+    return OK;
+}
+
+// ---- DensityMatrix, src/lib.rs:1553-2227: row-major entries[row * dim + col].
+struct Density {
+    std::vector<C> e;
+    size_t dim;
+    uint32_t qubits;
+};
+
+// DensityMatrix::local_transform, src/lib.rs:2157-2198: rows with m, then
+// columns with conj(m), through a full temporary.
+std::vector<C> local_transform(const Density& d, const C m[4], uint32_t q) {
+    const size_t mask = size_t(1) << q, D = d.dim;
+    std::vector<C> left(d.e.size());
+    for (size_t row = 0; row < D; ++row) {
+        const size_t row0 = row & ~mask, row1 = row0 | mask;
+        for (size_t col = 0; col < D; ++col) {
+            const C a = d.e[row0 * D + col], b = d.e[row1 * D + col];
+            left[row * D + col] = (row & mask) == 0 ? cadd(cmul(m[0], a), cmul(m[1], b))
+                                                    : cadd(cmul(m[2], a), cmul(m[3], b));
+        }
+    }
+    std::vector<C> out(d.e.size());
+    for (size_t row = 0; row < D; ++row)
+        for (size_t col = 0; col < D; ++col) {
+            const size_t col0 = col & ~mask, col1 = col0 | mask;
+            const C a = left[row * D + col0], b = left[row * D + col1];
+            out[row * D + col] = (col & mask) == 0 ? cadd(cmul(a, cconj(m[0])), cmul(b, cconj(m[1])))
+                                                   : cadd(cmul(a, cconj(m[2])), cmul(b, cconj(m[3])));
+        }
+    return out;
+}
+
+// DensityMatrix::apply, src/lib.rs:2044-2054.
+int density_apply(Density& d, int gate, double theta, uint32_t q) {
+    if (q >= d.qubits) return QUBIT_OUT_OF_RANGE;
+    C m[4];
+    gate_matrix(gate, theta, m);
+    d.e = local_transform(d, m, q);
+    return OK;
+}
+
+// check_qubit_pair, src/lib.rs:1532-1550.
+int density_pair(const Density& d, uint32_t a, uint32_t b) {
+    if (a >= d.qubits || b >= d.qubits) return QUBIT_OUT_OF_RANGE;
+    return a == b ? INVALID_QUBIT_PAIR : OK;
+}
+
+// apply_basis_permutation, src/lib.rs:2033-2042.
+template <class F>
+void density_permute(Density& d, F perm) {
+    const std::vector<C> original(d.e);
+    const size_t D = d.dim;
+    for (size_t row = 0; row < D; ++row)
+        for (size_t col = 0; col < D; ++col) d.e[row * D + col] = original[perm(row) * D + perm(col)];
+}
+
+// DensityMatrix::apply_cnot / apply_cz / apply_swap, src/lib.rs:1984-2031.
+int density_2q(Density& d, int kind, uint32_t a, uint32_t b, double phi) {
+    const int e = density_pair(d, a, b);
+    if (e) return e;
+    const size_t ma = size_t(1) << a, mb = size_t(1) << b, D = d.dim;
+    if (kind == 1) {
+        density_permute(d, [&](size_t i) { return (i & ma) ? i ^ mb : i; });
+    } else if (kind == 3) {
+        density_permute(d, [&](size_t i) { return ((i & ma) == 0) != ((i & mb) == 0) ? i ^ ma ^ mb : i; });
+    } else if (kind == 2) {
+        for (size_t row = 0; row < D; ++row)
+            for (size_t col = 0; col < D; ++col) {
+                C& x = d.e[row * D + col];
+                if ((row & ma) && (row & mb)) x = cmulf(x, -1.0);
+                if ((col & ma) && (col & mb)) x = cmulf(x, -1.0);
+            }
+    } else {  // exact controlled phase (no reference counterpart), rho -> CP rho CP^dagger
+        const C ph{lcos(phi), lsin(phi)};
+        for (size_t row = 0; row < D; ++row)
+            for (size_t col = 0; col < D; ++col) {
+                C& x = d.e[row * D + col];
+                if ((row & ma) && (row & mb)) x = cmul(ph, x);
+                if ((col & ma) && (col & mb)) x = cmul(x, cconj(ph));
+            }
+    }
+    return OK;
+}
+
+// DensityMatrix::apply_kraus, src/lib.rs:2131-2144.
+void apply_kraus(Density& d, const std::vector<std::array<C, 4>>& ks, uint32_t q) {
+    std::vector<C> sum(d.e.size(), C{0.0, 0.0});
+    for (const auto& k : ks) {
+        const std::vector<C> branch = local_transform(d, k.data(), q);
+        for (size_t i = 0; i < sum.size(); ++i) sum[i] = cadd(sum[i], branch[i]);
+    }
+    d.e.swap(sum);
+}
+
+// DensityMatrix::apply_channel, src/lib.rs:2057-2129, and depolarize,
+// src/lib.rs:2200-2226.
+int density_channel(Density& d, int kind, double p0, double p1, uint32_t q) {
+    if (q >= d.qubits) return QUBIT_OUT_OF_RANGE;
+    const C z{0.0, 0.0}, one{1.0, 0.0};
+    switch (kind) {
+        case CH_DEPOLARIZING: {
+            if (!valid_probability(p0)) return INVALID_PROBABILITY;
+            const Density original = d;
+            std::vector<C> mixed(d.e.size(), z);
+            for (int gate : {int(G_X), int(G_Y), int(G_Z)}) {
+                C m[4];
+                gate_matrix(gate, 0.0, m);
+                const std::vector<C> branch = local_transform(original, m, q);
+                for (size_t i = 0; i < mixed.size(); ++i) mixed[i] = cadd(mixed[i], cmulf(branch[i], p0 / 3.0));
+            }
+            for (size_t i = 0; i < mixed.size(); ++i) mixed[i] = cadd(mixed[i], cmulf(original.e[i], 1.0 - p0));
+            d.e.swap(mixed);
+            return OK;
+        }
+        case CH_AMPLITUDE_DAMPING:
+        case CH_PHASE_DAMPING: {
+            if (!valid_probability(p0)) return INVALID_PROBABILITY;
+            const std::array<C, 4> k0{one, z, z, C{std::sqrt(1.0 - p0), 0.0}};
+            const std::array<C, 4> k1 = kind == CH_AMPLITUDE_DAMPING ? std::array<C, 4>{z, C{std::sqrt(p0), 0.0}, z, z}
+                                                                     : std::array<C, 4>{z, z, z, C{std::sqrt(p0), 0.0}};
+            apply_kraus(d, {k0, k1}, q);
+            return OK;
+        }
+        case CH_THERMAL_DAMPING: {
+            double up, down;
+            const int e = thermal_rates(p0, p1, &up, &down);
+            if (e) return e;
+            const std::array<C, 4> k0{C{std::sqrt(1.0 - up), 0.0}, z, z, C{std::sqrt(1.0 - down), 0.0}};
+            const std::array<C, 4> k1{z, C{std::sqrt(down), 0.0}, z, z};
+            const std::array<C, 4> k2{z, z, C{std::sqrt(up), 0.0}, z};
+            apply_kraus(d, {k0, k1, k2}, q);
+            return OK;
+        }
+        default: return INVALID_ARGUMENT;
+    }
+}
+
+// DensityMatrix::probability_one, src/lib.rs:1908-1921.
+double density_p1(const Density& d, uint32_t q) {
+    const size_t mask = size_t(1) << q;
+    double acc = 0.0;
+    for (size_t i = 0; i < d.dim; ++i)
+        if (i & mask) acc += d.e[i * d.dim + i].re;
+    return std::min(std::max(acc, 0.0), 1.0);
+}
+
+// DensityMatrix::measure, src/lib.rs:1924-1950.
+int density_measure(Density& d, uint32_t q, double u, uint8_t* outcome) {
+    if (q >= d.qubits) return QUBIT_OUT_OF_RANGE;
+    const double p_one = density_p1(d, q);
+    const uint8_t o = uint8_t(u < p_one);
+    const double probability = o ? p_one : 1.0 - p_one;
+    if (probability <= kEps) return INVALID_PROBABILITY;
+    const size_t mask = size_t(1) << q, D = d.dim;
+    for (size_t row = 0; row < D; ++row)
+        for (size_t col = 0; col < D; ++col) {
+            C& x = d.e[row * D + col];
+            if (uint8_t((row & mask) != 0) != o || uint8_t((col & mask) != 0) != o) x = C{0.0, 0.0};
+            else x = C{x.re / probability, x.im / probability};
+        }
+    *outcome = o;
+    return OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 // ---- op records (layout identical to qf_op / qf_pop / qf_pauli_term in include/qf_hip.h)
 struct or_op {
-    int32_t kind;   // 0 single-qubit, 1 CNOT, 2 CZ, 3 SWAP, 4 channel, 5 measure, 6 conditional
+    int32_t kind;   // 0 single-qubit, 1 CNOT, 2 CZ, 3 SWAP, 4 channel {gate = channel kind, a, theta, theta2},
+                    // 5 measure, 6 conditional, 7 controlled phase
     int32_t gate;
     uint32_t a, b;
     double theta;
+    double theta2;  // second channel parameter (ThermalDamping beta_delta)
 };
 struct or_pop {
     int32_t kind;   // 0 fixed, 1 rotation, 2 encoding rotation, 3 CNOT, 4 CZ, 5 SWAP, 6 channel
@@ -361,12 +674,15 @@ int or_measure(void* s, uint32_t q, double u, uint8_t* outcome) {
     return measure_state(*static_cast<State*>(s), q, u, outcome);
 }
 
-// Circuit::run_shots (noiseless), src/lib.rs:2822-2911: op kinds as or_op,
-// 5 = measure {a target, b classical bit}, 6 = conditional gate {gate, a
-// target, theta, b = classical bit | value << 16}.  Writes every shot's
-// classical register ([shots][width], least-significant bit first).
+// Circuit::run_shots / run_shots_with_noise (allow_noise = 1),
+// src/lib.rs:2822-2911: op kinds as or_op, 5 = measure {a target, b classical
+// bit}, 6 = conditional gate {gate, a target, theta, b = classical bit | value
+// << 16}, 4 = channel (a Kraus trajectory with noise, else
+// OperationRequiresDensityMatrix).  With noise each shot's state is normalized
+// first.  Writes every shot's classical register ([shots][width],
+// least-significant bit first).
 int or_run_shots(const double* initial, uint32_t qubits, const or_op* ops, size_t nops, uint64_t shots,
-                 uint64_t seed, uint8_t* registers, uint32_t* width_out) {
+                 uint64_t seed, uint8_t* registers, uint32_t* width_out, int allow_noise) {
     if (shots == 0) return INVALID_SHOT_COUNT;
     uint32_t width = 0;
     for (size_t i = 0; i < nops; ++i) {
@@ -385,6 +701,10 @@ int or_run_shots(const double* initial, uint32_t qubits, const or_op* ops, size_
         st.qubits = qubits;
         st.amps.resize(dim);
         std::memcpy(st.amps.data(), initial, dim * sizeof(C));
+        if (allow_noise) {
+            const int e = normalize(st);
+            if (e) return e;
+        }
         std::fill(classical.begin(), classical.end(), 0);
         for (size_t i = 0; i < nops; ++i) {
             const or_op& op = ops[i];
@@ -394,7 +714,10 @@ int or_run_shots(const double* initial, uint32_t qubits, const or_op* ops, size_
                 case 1: e = apply_cnot(st, op.a, op.b); break;
                 case 2: e = apply_cz(st, op.a, op.b); break;
                 case 3: e = apply_swap(st, op.a, op.b); break;
-                case 4: return REQUIRES_DENSITY_MATRIX;
+                case 4:
+                    if (!allow_noise) return REQUIRES_DENSITY_MATRIX;
+                    e = channel_trajectory(st, op.gate, op.theta, op.theta2, op.a, &rng);
+                    break;
                 case 5: {
                     const double u = or_rng_next_f64(&rng);
                     e = measure_state(st, op.a, u, &classical[op.b]);
@@ -409,6 +732,75 @@ int or_run_shots(const double* initial, uint32_t qubits, const or_op* ops, size_
             if (e) return e;
         }
         if (width) std::memcpy(registers + shot * width, classical.data(), width);
+    }
+    return OK;
+}
+
+int or_channel_trajectory(void* s, int kind, double p0, double p1, uint32_t q, uint64_t* rng) {
+    return channel_trajectory(*static_cast<State*>(s), kind, p0, p1, q, rng);
+}
+int or_normalize(void* s) { return normalize(*static_cast<State*>(s)); }
+
+// ---- density matrices (DensityMatrix, src/lib.rs:1553-2227)
+void* or_density_from(const double* entries, uint32_t qubits) {  // DensityMatrix::from_entries, src/lib.rs:1707-1715
+    if (qubits == 0 || qubits > 14) return nullptr;
+    Density* d = new Density;
+    d->qubits = qubits;
+    d->dim = size_t(1) << qubits;
+    d->e.resize(d->dim * d->dim);
+    std::memcpy(d->e.data(), entries, d->e.size() * sizeof(C));
+    return d;
+}
+void* or_density_from_state(const double* amps, uint32_t qubits) {  // DensityMatrix::from_state, src/lib.rs:1642-1656
+    if (qubits == 0 || qubits > 14) return nullptr;
+    Density* d = new Density;
+    d->qubits = qubits;
+    d->dim = size_t(1) << qubits;
+    d->e.resize(d->dim * d->dim);
+    const C* a = reinterpret_cast<const C*>(amps);
+    for (size_t row = 0; row < d->dim; ++row)
+        for (size_t col = 0; col < d->dim; ++col) d->e[row * d->dim + col] = cmul(a[row], cconj(a[col]));
+    return d;
+}
+void or_density_free(void* d) { delete static_cast<Density*>(d); }
+void or_density_get(const void* d, double* out) {
+    const Density* x = static_cast<const Density*>(d);
+    std::memcpy(out, x->e.data(), x->e.size() * sizeof(C));
+}
+int or_density_apply_gate(void* d, int gate, double theta, uint32_t q) {
+    return density_apply(*static_cast<Density*>(d), gate, theta, q);
+}
+// kind: 1 CNOT {a control, b target}, 2 CZ, 3 SWAP, 7 controlled phase.
+int or_density_apply_2q(void* d, int kind, uint32_t a, uint32_t b, double phi) {
+    return density_2q(*static_cast<Density*>(d), kind, a, b, phi);
+}
+int or_density_apply_channel(void* d, int kind, double p0, double p1, uint32_t q) {
+    return density_channel(*static_cast<Density*>(d), kind, p0, p1, q);
+}
+int or_density_measure(void* d, uint32_t q, double u, uint8_t* outcome) {
+    return density_measure(*static_cast<Density*>(d), q, u, outcome);
+}
+int or_density_probability_one(const void* d, uint32_t q, double* out) {
+    const Density& x = *static_cast<const Density*>(d);
+    if (q >= x.qubits) return QUBIT_OUT_OF_RANGE;
+    *out = density_p1(x, q);
+    return OK;
+}
+// Circuit::run_density, src/lib.rs:2804-2819 (ops run and validated in order).
+int or_density_run(void* d, const or_op* ops, size_t nops) {
+    Density& x = *static_cast<Density*>(d);
+    for (size_t i = 0; i < nops; ++i) {
+        const or_op& op = ops[i];
+        int e = OK;
+        switch (op.kind) {
+            case 0: e = density_apply(x, op.gate, op.theta, op.a); break;
+            case 1: case 2: case 3: e = density_2q(x, op.kind, op.a, op.b, 0.0); break;
+            case 7: e = density_2q(x, 7, op.a, op.b, op.theta); break;
+            case 4: e = density_channel(x, op.gate, op.theta, op.theta2, op.a); break;
+            case 5: case 6: return REQUIRES_SHOT_EXECUTION;
+            default: return INVALID_ARGUMENT;
+        }
+        if (e) return e;
     }
     return OK;
 }

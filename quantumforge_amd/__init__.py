@@ -6,20 +6,20 @@ include/qf_hip.h (backend), gradients (gradients), workload builders
 (workloads) and the torch integration (torch_ops, imported on demand).
 """
 from ._native import QuantumError
-from .core import (Circuit, Gate, ParameterizedCircuit, PauliSum, RotationAxis, ShotResult, SingleQubitObservable,
-                   StateVector, rotation_gate)
+from .core import (Circuit, DensityMatrix, Gate, NoiseChannel, ParameterizedCircuit, PauliSum, RotationAxis,
+                   ShotResult, SingleQubitObservable, StateVector, rotation_gate)
 
 __all__ = [
-    "QuantumError", "Circuit", "Gate", "ParameterizedCircuit", "PauliSum", "RotationAxis",
-    "ShotResult", "SingleQubitObservable", "StateVector", "rotation_gate", "HipBackend", "HipDeviceState",
-    "HipDeviceStateBatch", "Plan",
+    "QuantumError", "Circuit", "DensityMatrix", "Gate", "NoiseChannel", "ParameterizedCircuit", "PauliSum",
+    "RotationAxis", "ShotResult", "SingleQubitObservable", "StateVector", "rotation_gate", "HipBackend",
+    "HipDeviceState", "HipDeviceStateBatch", "HipDeviceDensityMatrix", "Plan",
 ]
 
 
 def __getattr__(name):
     # The backend loads libqfhip.so; import it lazily so the pure-host parts
     # (circuit building, workloads) import without the library.
-    if name in ("HipBackend", "HipDeviceState", "HipDeviceStateBatch", "Plan"):
+    if name in ("HipBackend", "HipDeviceState", "HipDeviceStateBatch", "HipDeviceDensityMatrix", "Plan"):
         from . import backend
 
         return getattr(backend, name)

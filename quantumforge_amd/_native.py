@@ -22,10 +22,12 @@ STATUS = {
     16: "IncompatibleDeviceState", 17: "StateSizeNotPowerOfTwo", 18: "InvalidBatchSamples",
     19: "MemoryBudgetExceeded", 20: "InvalidFiniteDifferenceStep", 21: "SimulationSizeOverflow",
     22: "InvalidProbability", 23: "InvalidShotCount", 24: "ClassicalBitOutOfRange",
+    25: "NonHermitianDensityMatrix", 26: "InvalidSubsystemSelection",
 }
 
 # Record layouts identical to the C structs (checked by tests/test_abi.py).
-OP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("theta", "<f8")])
+OP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("theta", "<f8"),
+                     ("theta2", "<f8")])
 POP_DTYPE = np.dtype([("kind", "<i4"), ("gate", "<i4"), ("a", "<u4"), ("b", "<u4"), ("index", "<u4"),
                       ("reserved", "<u4"), ("theta", "<f8")])
 PAULI_DTYPE = np.dtype([("re", "<f8"), ("im", "<f8"), ("x_mask", "<u8"), ("z_mask", "<u8")])
@@ -110,6 +112,27 @@ SIGNATURES = {
     "qf_batch_measure_with_uniforms": (None, [P, u32, P, sz, P]),
     "qf_batch_apply_gate_masked": (None, [P, i32, u32, dbl, P, sz]),
     "qf_run_circuit_shots": (None, [P, P, u32, P, sz, u64, u64, i32, P, C.POINTER(u32)]),
+    "qf_batch_apply_channel_trajectory": (None, [P, i32, u32, dbl, dbl, P, sz]),
+    "qf_density_upload": (None, [P, P, sz, u32, PP]),
+    "qf_density_from_state": (None, [P, PP]),
+    "qf_density_maximally_mixed": (None, [P, u32, PP]),
+    "qf_density_destroy": (None, [P]),
+    "qf_density_qubits": (None, [P, C.POINTER(u32)]),
+    "qf_density_download": (None, [P, P, sz]),
+    "qf_density_apply_gate": (None, [P, i32, u32, dbl]),
+    "qf_density_apply_cnot": (None, [P, u32, u32]),
+    "qf_density_apply_cz": (None, [P, u32, u32]),
+    "qf_density_apply_swap": (None, [P, u32, u32]),
+    "qf_density_apply_channel": (None, [P, i32, u32, dbl, dbl]),
+    "qf_density_measure": (None, [P, u32, dbl, C.POINTER(C.c_uint8)]),
+    "qf_density_trace": (None, [P, C.POINTER(dbl), C.POINTER(dbl)]),
+    "qf_density_purity": (None, [P, C.POINTER(dbl)]),
+    "qf_density_expectation": (None, [P, i32, u32, C.POINTER(dbl)]),
+    "qf_density_probability_one": (None, [P, u32, C.POINTER(dbl)]),
+    "qf_density_run_circuit": (None, [P, P, sz]),
+    "qf_run_density_circuit": (None, [P, P, sz, u32, P, sz, P, sz]),
+    "qf_density_plan_build": (None, [P, sz, u32, C.POINTER(PlanOptions), PP]),
+    "qf_density_run_plan": (None, [P, P]),
     "qf_plan_build_parameterized": (None, [P, sz, u32, u32, P, sz, u32, i32, C.POINTER(PlanOptions), PP]),
     "qf_plan_source": (None, [P, i32, C.c_char_p, sz, C.POINTER(sz)]),
     "qf_plan_compile": (None, [P, i32, C.POINTER(dbl), C.POINTER(i32)]),

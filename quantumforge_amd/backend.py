@@ -1,6 +1,6 @@
 """HIP backend: the drop-in for the reference's `cuda.rs` (CudaBackend,
-CudaDeviceState, CudaDeviceStateBatch, src/cuda.rs:484-2005), over the C ABI
-of include/qf_hip.h."""
+CudaDeviceState, CudaDeviceStateBatch, CudaDeviceDensityMatrix,
+src/cuda.rs:484-2237), over the C ABI of include/qf_hip.h."""
 from __future__ import annotations
 
 import ctypes as C
@@ -11,7 +11,8 @@ import numpy as np
 
 from . import _native as N
 from ._native import QuantumError, check, lib, ptr
-from .core import Circuit, Gate, PauliSum, RotationAxis, ShotResult, SingleQubitObservable, StateVector
+from .core import (Circuit, DensityMatrix, Gate, NoiseChannel, PauliSum, RotationAxis, ShotResult,
+                   SingleQubitObservable, StateVector)
 
 C128, C64 = 0, 1
 _PRECISION = {"complex128": C128, "c128": C128, C128: C128, "complex64": C64, "c64": C64, C64: C64}
@@ -54,6 +55,20 @@ class Plan:
                                                 C.byref(opts), C.byref(h)))
         self._h = h
         self.qubits = qubits
+        return self
+
+    @classmethod
+    def density(cls, circuit: Circuit, qubits: int, tile_qubits: int = 0, low_qubits: int = 0, kernel: int = 0,
+                reg_bits: int = 0) -> "Plan":
+        """The plan a density-matrix circuit runs as (qf_density_plan_build):
+        tile passes over the 2n-qubit view of the row-major entries."""
+        self = cls.__new__(cls)
+        self._ops = circuit.to_ops()
+        opts = N.PlanOptions(1, tile_qubits, low_qubits, 0, kernel, reg_bits, C128)
+        h = C.c_void_p()
+        check(lib().qf_density_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.qubits = 2 * qubits
         return self
 
     def stats(self) -> dict:
@@ -196,6 +211,34 @@ class HipBackend:
         """CudaBackend::run_circuit_shots_batched, src/cuda.rs:1161-1269: all
         shots as one batch of states (split to fit device memory)."""
         return self._shots(initial, circuit, shots, seed, True)
+
+    # --- density matrices (src/cuda.rs:969-1047) ---------------------------
+    def upload_density(self, density: DensityMatrix) -> "HipDeviceDensityMatrix":
+        """CudaBackend::upload_density, src/cuda.rs:969-1022."""
+        e = np.ascontiguousarray(density.entries, dtype=np.complex128)
+        h = C.c_void_p()
+        check(lib().qf_density_upload(self._h, ptr(e), e.size, density.qubits, C.byref(h)))
+        return HipDeviceDensityMatrix(h)
+
+    def density_from_state(self, state: "HipDeviceState") -> "HipDeviceDensityMatrix":
+        """|psi><psi| of a resident complex128 state, built on the device."""
+        h = C.c_void_p()
+        check(lib().qf_density_from_state(state._h, C.byref(h)))
+        return HipDeviceDensityMatrix(h)
+
+    def maximally_mixed_density(self, qubits: int) -> "HipDeviceDensityMatrix":
+        h = C.c_void_p()
+        check(lib().qf_density_maximally_mixed(self._h, int(qubits), C.byref(h)))
+        return HipDeviceDensityMatrix(h)
+
+    def run_density_circuit(self, initial: DensityMatrix, circuit: Circuit) -> DensityMatrix:
+        """CudaBackend::run_density_circuit, src/cuda.rs:1025-1047."""
+        e = np.ascontiguousarray(initial.entries, dtype=np.complex128)
+        out = np.empty_like(e)
+        ops = circuit.to_ops()
+        check(lib().qf_run_density_circuit(self._h, ptr(e), e.size, initial.qubits, ptr(ops), len(ops), ptr(out),
+                                           out.size))
+        return DensityMatrix(out, initial.qubits)
 
     def apply_rotation_chain(self, state: StateVector, gates: Sequence[Tuple[RotationAxis, int, float]]) -> None:
         """Updates `state` in place (CudaBackend::apply_rotation_chain)."""
@@ -370,6 +413,15 @@ class HipDeviceStateBatch:
         e = np.ascontiguousarray(np.asarray(enabled, dtype=np.uint8))
         check(lib().qf_batch_apply_gate_masked(self._h, gate.id, int(target), float(gate.theta), ptr(e), e.size))
 
+    def apply_channel_trajectory_with_uniforms(self, channel: NoiseChannel, target: int,
+                                               uniforms: Sequence[float]) -> None:
+        """One Kraus-trajectory step per state (apply_channel_trajectory_with_uniforms,
+        src/cuda.rs:1670-1790, with the CPU path's branch rules)."""
+        u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64))
+        p = tuple(channel.params) + (0.0,) * (2 - len(channel.params))
+        check(lib().qf_batch_apply_channel_trajectory(self._h, channel.kind, int(target), float(p[0]), float(p[1]),
+                                                      ptr(u), u.size))
+
     def synchronize(self) -> None:
         self._backend.synchronize()
 
@@ -383,3 +435,81 @@ class HipDeviceStateBatch:
         out = np.empty(self.count, dtype=np.float64)
         check(lib().qf_batch_expectation_pauli_sum(self._h, ptr(t), len(t), ptr(out)))
         return out
+
+
+class HipDeviceDensityMatrix:
+    """CudaDeviceDensityMatrix, src/cuda.rs:2007-2237: a resident density
+    matrix (row-major entries, complex128)."""
+
+    def __init__(self, handle: C.c_void_p) -> None:
+        self._h = handle
+        q = C.c_uint32()
+        check(lib().qf_density_qubits(handle, C.byref(q)))
+        self.qubits_ = int(q.value)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            lib().qf_density_destroy(h)
+            self._h = None
+
+    def qubits(self) -> int:
+        return self.qubits_
+
+    def apply_gate(self, gate: Gate, qubit: int) -> None:
+        check(lib().qf_density_apply_gate(self._h, gate.id, int(qubit), float(gate.theta)))
+
+    def apply_cnot(self, control: int, target: int) -> None:
+        check(lib().qf_density_apply_cnot(self._h, int(control), int(target)))
+
+    def apply_cz(self, first: int, second: int) -> None:
+        check(lib().qf_density_apply_cz(self._h, int(first), int(second)))
+
+    def apply_swap(self, first: int, second: int) -> None:
+        check(lib().qf_density_apply_swap(self._h, int(first), int(second)))
+
+    def apply_channel(self, channel: NoiseChannel, qubit: int) -> None:
+        p = tuple(channel.params) + (0.0,) * (2 - len(channel.params))
+        check(lib().qf_density_apply_channel(self._h, channel.kind, int(qubit), float(p[0]), float(p[1])))
+
+    def run_circuit(self, circuit: Circuit, plan: Optional[Plan] = None) -> None:
+        if plan is not None:
+            check(lib().qf_density_run_plan(self._h, plan._h))
+            return
+        ops = circuit.to_ops()
+        check(lib().qf_density_run_circuit(self._h, ptr(ops), len(ops)))
+
+    def measure(self, qubit: int, rng) -> int:
+        """CudaDeviceDensityMatrix::measure, src/cuda.rs:2161-2213: one rng.next_f64()."""
+        return self.measure_with_uniform(qubit, rng.next_f64())
+
+    def measure_with_uniform(self, qubit: int, uniform: float) -> int:
+        o = C.c_uint8()
+        check(lib().qf_density_measure(self._h, int(qubit), float(uniform), C.byref(o)))
+        return int(o.value)
+
+    def download(self) -> DensityMatrix:
+        d = 1 << self.qubits_
+        out = np.empty(d * d, dtype=np.complex128)
+        check(lib().qf_density_download(self._h, ptr(out), out.size))
+        return DensityMatrix(out, self.qubits_)
+
+    def trace(self) -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().qf_density_trace(self._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def purity(self) -> float:
+        v = C.c_double()
+        check(lib().qf_density_purity(self._h, C.byref(v)))
+        return v.value
+
+    def expectation(self, observable: SingleQubitObservable, qubit: int) -> float:
+        v = C.c_double()
+        check(lib().qf_density_expectation(self._h, int(observable), int(qubit), C.byref(v)))
+        return v.value
+
+    def probability_one(self, qubit: int) -> float:
+        v = C.c_double()
+        check(lib().qf_density_probability_one(self._h, int(qubit), C.byref(v)))
+        return v.value

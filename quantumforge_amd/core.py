@@ -167,17 +167,28 @@ class Circuit:
         return top + 1
 
     def to_ops(self) -> np.ndarray:
-        """The op records of include/qf_hip.h (qf_op)."""
+        """The op records of include/qf_hip.h (qf_op); a channel's second
+        parameter (ThermalDamping beta_delta) goes to theta2."""
         arr = np.zeros(len(self._ops), dtype=OP_DTYPE)
         for i, (k, g, a, b, t) in enumerate(self._ops):
-            arr[i] = (k, g, a, b, t)
+            ch = self._channels.get(i) if k == 4 else None
+            t2 = float(ch.params[1]) if ch is not None and len(ch.params) > 1 else 0.0
+            arr[i] = (k, g, a, b, t, t2)
         return arr
 
     @staticmethod
     def from_ops(ops: np.ndarray) -> "Circuit":
         c = Circuit()
+        names = ops.dtype.names or ()
         for r in ops:
-            c._ops.append((int(r["kind"]), int(r["gate"]), int(r["a"]), int(r["b"]), float(r["theta"])))
+            k = int(r["kind"])
+            if k == 4:
+                kind = int(r["gate"])
+                params = (float(r["theta"]),)
+                if kind == 3:
+                    params += (float(r["theta2"]) if "theta2" in names else 0.0,)
+                c._channels[len(c._ops)] = NoiseChannel(NoiseChannel.KINDS[kind], params)
+            c._ops.append((k, int(r["gate"]), int(r["a"]), int(r["b"]), float(r["theta"])))
         return c
 
 
@@ -357,3 +368,115 @@ class StateVector:
 
     def copy(self) -> "StateVector":
         return StateVector(self.amplitudes.copy(), self.qubits)
+
+
+class DensityMatrix:
+    """DensityMatrix, src/lib.rs:1553-2227: the row-major entries of a
+    2^n x 2^n matrix.  A host value: gates, channels and measurement run on a
+    HIP backend (HipBackend.upload_density / run_density_circuit); the
+    queries below are host conveniences on downloaded matrices."""
+
+    def __init__(self, entries: np.ndarray, qubits: int) -> None:
+        d = 1 << qubits
+        e = np.ascontiguousarray(np.asarray(entries, dtype=np.complex128).reshape(-1))
+        if e.size != d * d:
+            raise ValueError(f"a {qubits}-qubit density matrix has {d * d} entries, got {e.size}")
+        self.entries = e
+        self.qubits = qubits
+
+    @staticmethod
+    def from_entries(entries: Sequence[complex], qubits: int) -> "DensityMatrix":
+        """DensityMatrix::from_entries, src/lib.rs:1707-1715."""
+        return DensityMatrix(np.array(entries, dtype=np.complex128), qubits)
+
+    @staticmethod
+    def from_state(state: StateVector) -> "DensityMatrix":
+        """DensityMatrix::from_state, src/lib.rs:1642-1656: |psi><psi|."""
+        a = np.asarray(state.amplitudes, dtype=np.complex128)
+        return DensityMatrix(np.outer(a, a.conj()), state.qubits)
+
+    @staticmethod
+    def maximally_mixed(qubits: int) -> "DensityMatrix":
+        """DensityMatrix::maximally_mixed, src/lib.rs:1659-1704."""
+        if qubits == 0:
+            raise QuantumError(1, "a density matrix must contain at least one qubit")
+        d = 1 << qubits
+        return DensityMatrix(np.eye(d, dtype=np.complex128) / d, qubits)
+
+    def dimension(self) -> int:
+        return 1 << self.qubits
+
+    def matrix(self) -> np.ndarray:
+        d = self.dimension()
+        return self.entries.reshape(d, d)
+
+    def trace(self) -> complex:
+        """DensityMatrix::trace, src/lib.rs:1732-1736."""
+        return complex(np.trace(self.matrix()))
+
+    def purity(self) -> float:
+        """DensityMatrix::purity, src/lib.rs:1738-1742: sum of |rho_ij|^2."""
+        return float(np.sum(np.abs(self.entries) ** 2))
+
+    def _check(self, qubit: int) -> None:
+        if qubit >= self.qubits:
+            raise QuantumError(2, f"qubit index {qubit} is out of range for {self.qubits} qubits")
+
+    def probability_one(self, qubit: int) -> float:
+        """DensityMatrix::probability_one, src/lib.rs:1908-1921."""
+        self._check(qubit)
+        diag = np.real(np.diag(self.matrix()))
+        ones = ((np.arange(self.dimension()) >> qubit) & 1) == 1
+        return min(max(float(np.sum(diag[ones])), 0.0), 1.0)
+
+    def expectation_z(self, qubit: int) -> float:
+        """DensityMatrix::expectation_z, src/lib.rs:1858-1873."""
+        self._check(qubit)
+        diag = np.real(np.diag(self.matrix()))
+        sign = 1.0 - 2.0 * ((np.arange(self.dimension()) >> qubit) & 1)
+        return float(np.sum(sign * diag))
+
+    def expectation(self, observable: "SingleQubitObservable", qubit: int) -> float:
+        """DensityMatrix::expectation, src/lib.rs:1875-1905."""
+        self._check(qubit)
+        if observable == SingleQubitObservable.Z:
+            return self.expectation_z(qubit)
+        m = self.matrix()
+        rows = np.arange(self.dimension())
+        rows = rows[((rows >> qubit) & 1) == 0]
+        coh = m[rows, rows | (1 << qubit)]
+        return float(2.0 * np.sum(coh.real)) if observable == SingleQubitObservable.X else float(-2.0 * np.sum(coh.imag))
+
+    def partial_trace(self, keep_qubits: Sequence[int]) -> "DensityMatrix":
+        """DensityMatrix::partial_trace, src/lib.rs:1953-1981: traces out every
+        qubit not listed; the kept qubits are ordered as given."""
+        keep = [int(q) for q in keep_qubits]
+        if any(q >= self.qubits for q in keep) or len(set(keep)) != len(keep):
+            raise QuantumError(26, "invalid subsystem selection for the partial trace")
+        traced = [q for q in range(self.qubits) if q not in keep]
+
+        def expand(k: np.ndarray, e: np.ndarray) -> np.ndarray:
+            full = np.zeros(np.broadcast(k, e).shape, dtype=np.int64)
+            for pos, q in enumerate(keep):
+                full |= ((k >> pos) & 1) << q
+            for pos, q in enumerate(traced):
+                full |= ((e >> pos) & 1) << q
+            return full
+
+        kept = np.arange(1 << len(keep))[:, None]
+        env = np.arange(1 << len(traced))[None, :]
+        full = expand(kept, env)  # [kept, env]
+        m = self.matrix()
+        out = m[full[:, None, :], full[None, :, :]].sum(axis=-1)
+        return DensityMatrix(out, len(keep))
+
+    def von_neumann_entropy(self) -> float:
+        """DensityMatrix::von_neumann_entropy, src/lib.rs:1749-1796: -sum
+        lambda ln lambda over the eigenvalues (Hermitian eigensolver),
+        NonHermitianDensityMatrix beyond 1e-9."""
+        m = self.matrix()
+        if np.max(np.abs(m - m.conj().T), initial=0.0) > 1e-9:
+            raise QuantumError(25, "density matrix is not Hermitian")
+        ev = np.maximum(np.linalg.eigvalsh(m), 0.0)
+        ev = ev[ev > 1e-15]
+        return float(-np.sum(ev * np.log(ev)))

@@ -135,6 +135,7 @@ public:
                 case TOP_DIAG: diag(op); break;
                 case TOP_CNOT: cnot(op); break;
                 case TOP_SWAP: swap(op); break;
+                case TOP_MAT2: mat2(op); break;
                 case TOP_ROT: rot(op); break;
                 case TOP_ROTZZ: rotzz(op); break;
                 default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
@@ -412,6 +413,37 @@ private:
             o_ << "    " << iB << " = "
                << lincomb({lit_term(m[4], "a0i"), lit_term(m[5], "a0r"), lit_term(m[6], "a1i"), lit_term(m[7], "a1r")})
                << "; }\n";
+        }
+    }
+
+    // Dense 4x4 on register bits q0 < q1 over the index bit(q0) + 2 bit(q1),
+    // literal coefficients; zero entries (most of a channel superoperator's)
+    // are dropped.
+    void mat2(const TileOp& op) {
+        const uint32_t i = uint32_t(op.q0), j = uint32_t(op.q1);
+        acc_flush_bit(i);
+        acc_flush_bit(j);
+        const double* m = op.m;
+        for (uint32_t k = 0; k < NR_; ++k) {
+            if (((k >> i) & 1) || ((k >> j) & 1)) continue;
+            const uint32_t idx[4] = {k, k | (1u << i), k | (1u << j), k | (1u << i) | (1u << j)};
+            o_ << "  { const R v0r = " << rv(idx[0]) << ", v0i = " << iv(idx[0]) << ", v1r = " << rv(idx[1])
+               << ", v1i = " << iv(idx[1]) << ", v2r = " << rv(idx[2]) << ", v2i = " << iv(idx[2])
+               << ", v3r = " << rv(idx[3]) << ", v3i = " << iv(idx[3]) << ";\n";
+            for (int r = 0; r < 4; ++r) {
+                std::vector<Term> re, im;
+                for (int c = 0; c < 4; ++c) {
+                    const double mr = m[2 * (4 * r + c)], mi = m[2 * (4 * r + c) + 1];
+                    const std::string vr = "v" + std::to_string(c) + "r", vi = "v" + std::to_string(c) + "i";
+                    re.push_back(lit_term(mr, vr));
+                    re.push_back(lit_term(-mi, vi));
+                    im.push_back(lit_term(mr, vi));
+                    im.push_back(lit_term(mi, vr));
+                }
+                o_ << "    " << rv(idx[r]) << " = " << lincomb(re) << "; " << iv(idx[r]) << " = " << lincomb(im)
+                   << ";\n";
+            }
+            o_ << "  }\n";
         }
     }
 

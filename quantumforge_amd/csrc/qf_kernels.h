@@ -501,7 +501,16 @@ __device__ __forceinline__ void block_reduce(dd& re, dd& im) {
 
 // RED_P1: sum of |a_i|^2 over the indices with bit q set (the probability of
 // measuring 1 on qubit q, StateVector::measure, src/lib.rs:1328-1335).
-enum RedKind : int { RED_NORM = 0, RED_Z = 1, RED_XY = 2, RED_PAULI = 3, RED_INNER = 4, RED_P1 = 5 };
+// RED_NORM_P1: (sum |a_i|^2, the same over bit q set) in one pass (Kraus
+// trajectories, src/lib.rs:1384-1395).
+// Density matrices (dim = D = 2^n, entries row-major, one instance):
+// RED_DTRACE (sum rho_rr), RED_DP1 (sum of Re rho_rr over rows with bit q
+// set, and sum of +-Re rho_rr = <Z_q>), RED_DXY (sum of rho[r0][r0 | 2^q]
+// over rows r0 with bit q clear: <X> = 2 Re, <Y> = -2 Im).
+enum RedKind : int {
+    RED_NORM = 0, RED_Z = 1, RED_XY = 2, RED_PAULI = 3, RED_INNER = 4, RED_P1 = 5,
+    RED_NORM_P1 = 6, RED_DTRACE = 7, RED_DP1 = 8, RED_DXY = 9
+};
 
 // Device form of a Pauli term: c' = coefficient * i^(number of Y), sign from
 // parity(index & z_mask); contribution conj(psi[i ^ x]) * c' * sign * psi[i].
@@ -526,6 +535,32 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const cx<R>* __restrict__ s
             const cx<R> v = a[i];
             const double x = double(v.re) * double(v.re) + double(v.im) * double(v.im);
             neumaier(re, (kind == RED_Z && ((i >> q) & 1)) ? -x : x);
+        }
+    } else if (kind == RED_NORM_P1) {
+        for (uint64_t i = global_thread(); i < dim; i += stride) {
+            const cx<R> v = a[i];
+            const double x = double(v.re) * double(v.re) + double(v.im) * double(v.im);
+            neumaier(re, x);
+            if ((i >> q) & 1) neumaier(im, x);
+        }
+    } else if (kind == RED_DTRACE || kind == RED_DP1) {
+        for (uint64_t r = global_thread(); r < dim; r += stride) {
+            const cx<R> v = a[r * (dim + 1)];
+            if (kind == RED_DTRACE) {
+                neumaier(re, double(v.re));
+                neumaier(im, double(v.im));
+            } else {
+                const bool one = (r >> q) & 1;
+                if (one) neumaier(re, double(v.re));
+                neumaier(im, one ? -double(v.re) : double(v.re));
+            }
+        }
+    } else if (kind == RED_DXY) {
+        for (uint64_t p = global_thread(); p < dim / 2; p += stride) {
+            const uint64_t r0 = insert_zero(p, q);
+            const cx<R> v = a[r0 * dim + (r0 | (uint64_t(1) << q))];
+            neumaier(re, double(v.re));
+            neumaier(im, double(v.im));
         }
     } else if (kind == RED_XY) {  // sum over bit-q-clear i of conj(a_i) a_{i|m}; x2 done on host
         for (uint64_t p = global_thread(); p < dim / 2; p += stride) {
@@ -580,6 +615,44 @@ __global__ __launch_bounds__(kThreads) void k_reduce_finish(const dd* __restrict
     if (threadIdx.x == 0) {
         out[2 * inst] = re.s + re.c;
         out[2 * inst + 1] = im.s + im.c;
+    }
+}
+
+// One Kraus-trajectory branch per state (StateVector::apply_channel_trajectory,
+// src/lib.rs:1356-1505) on the pair (a0, a1) of qubit q; code per state:
+//   0 nothing, 1 X, 2 Y, 3 Z (depolarizing branches, exact permutations and
+//   negations of Gate::X/Y/Z applied as matrices),
+//   4 a0 *= s0, a1 *= s1 (no jump),
+//   5 a0 = a1 * s0, a1 = 0 (decay |1> -> |0>),
+//   6 a1 = a0 * s0, a0 = 0 (excitation |0> -> |1>),
+//   7 a0 = 0, a1 *= s0 (phase-damping jump).
+// scales holds (s0, s1) per state.  Complex64 * f64 products as the CPU path.
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_trajectory(cx<R>* __restrict__ s, uint64_t npairs, uint32_t qubits,
+                                                         uint32_t q, const uint8_t* __restrict__ code,
+                                                         const double* __restrict__ scales) {
+#pragma clang fp contract(off)
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t p = global_thread(); p < npairs; p += stride) {
+        const uint64_t inst = p >> (qubits - 1);
+        const int c = code[inst];
+        if (c == 0) continue;
+        const uint64_t i0 = (inst << qubits) | insert_zero(p & ((uint64_t(1) << (qubits - 1)) - 1), q);
+        const uint64_t i1 = i0 | (uint64_t(1) << q);
+        const cx<R> a0 = s[i0], a1 = s[i1];
+        const R s0 = R(scales[2 * inst]), s1 = R(scales[2 * inst + 1]);
+        cx<R> b0 = a0, b1 = a1;
+        switch (c) {
+            case 1: b0 = a1; b1 = a0; break;
+            case 2: b0 = cx<R>{a1.im, -a1.re}; b1 = cx<R>{-a0.im, a0.re}; break;
+            case 3: b1 = cx<R>{-a1.re, -a1.im}; break;
+            case 4: b0 = cx<R>{a0.re * s0, a0.im * s0}; b1 = cx<R>{a1.re * s1, a1.im * s1}; break;
+            case 5: b0 = cx<R>{a1.re * s0, a1.im * s0}; b1 = cx<R>{R(0), R(0)}; break;
+            case 6: b1 = cx<R>{a0.re * s0, a0.im * s0}; b0 = cx<R>{R(0), R(0)}; break;
+            default: b0 = cx<R>{R(0), R(0)}; b1 = cx<R>{a1.re * s0, a1.im * s0}; break;
+        }
+        s[i0] = b0;
+        s[i1] = b1;
     }
 }
 

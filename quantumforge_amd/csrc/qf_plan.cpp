@@ -36,13 +36,14 @@ void mat2_mul(const cd x[4], const cd y[4], cd r[4]) {
 
 bool is_zero(cd a) { return a.re == 0.0 && a.im == 0.0; }
 
-enum NodeKind { N_MAT1, N_DIAG1, N_DIAG2, N_CNOT, N_SWAP, N_ROT, N_ROTZZ };
+enum NodeKind { N_MAT1, N_DIAG1, N_DIAG2, N_CNOT, N_SWAP, N_ROT, N_ROTZZ, N_MAT2 };
 
 struct Node {
     NodeKind kind;
     uint32_t q[2];
     int nq;
     cd m[4];  // MAT1 matrix, DIAG1 (d0, d1), DIAG2 d[bit(q1)*2 + bit(q0)]
+    cd m4[16];  // MAT2: row-major 4x4 over bit(q1)*2 + bit(q0)
     int axis = 0, param = 0, slot = 0, flags = 0;  // N_ROT / N_ROTZZ
     bool dead = false;
     int prev[2] = {-1, -1};  // previous live node on q[0], q[1]
@@ -85,6 +86,14 @@ std::vector<Node> lower(const std::vector<GateIn>& gates, uint32_t qubits) {
                 for (int k = 0; k < 4; ++k) D.m[k] = mul(((k >> s) & 1) ? n.m[3] : n.m[0], D.m[k]);
                 continue;
             }
+            push(n);
+        } else if (g.kind == 6) {  // dense two-qubit matrix
+            Node n{};
+            n.kind = N_MAT2;
+            n.nq = 2;
+            n.q[0] = g.a;
+            n.q[1] = g.b;
+            std::memcpy(n.m4, g.m4, sizeof(n.m4));
             push(n);
         } else if (g.kind == 5) {  // rotation with a per-instance angle
             Node n{};
@@ -406,8 +415,22 @@ TileOp make_tile_op(const Node& n, const std::vector<int>& tile_bit) {
             op.param = n.param;
             op.slot = n.slot;
             break;
+        case N_MAT2:
+            op.kind = TOP_MAT2;
+            op.q0 = tile_bit[n.q[0]];
+            op.q1 = tile_bit[n.q[1]];
+            std::memcpy(op.m, n.m4, sizeof(n.m4));
+            break;
     }
     return op;
+}
+
+// A dense 4x4 over (bit(hi) * 2 + bit(lo)) re-indexed for the bits in the
+// other order: swaps the two index bits of rows and columns.
+void swap_mat2_bits(const cd in[16], cd out[16]) {
+    auto sw = [](int k) { return ((k & 1) << 1) | (k >> 1); };
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[sw(r) * 4 + sw(c)] = in[r * 4 + c];
 }
 
 // Arithmetic form of a 2x2 matrix (see MAT_*).
@@ -470,6 +493,23 @@ TileOp make_reg_op(const Node& n, const std::vector<int>& reg) {
             op.param = n.param;
             op.slot = n.slot;
             break;
+        case N_MAT2: {
+            // Register form: q0 < q1 (the kernels instantiate ordered pairs only).
+            op.kind = TOP_MAT2;
+            const int r0 = reg[n.q[0]], r1 = reg[n.q[1]];
+            if (r0 < r1) {
+                op.q0 = r0;
+                op.q1 = r1;
+                std::memcpy(op.m, n.m4, sizeof(n.m4));
+            } else {
+                op.q0 = r1;
+                op.q1 = r0;
+                cd t[16];
+                swap_mat2_bits(n.m4, t);
+                std::memcpy(op.m, t, sizeof(t));
+            }
+            break;
+        }
     }
     return op;
 }
@@ -544,7 +584,7 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     }
     if (!opt.fuse) {
         for (const GateIn& g : gates)
-            if (g.kind == 5) return "per-instance rotations need the tile lowering";
+            if (g.kind == 5 || g.kind == 6) return "per-instance rotations and dense two-qubit ops need the tile lowering";
         for (const GateIn& g : gates) {
             Pass p{};
             if (g.kind == 0) {

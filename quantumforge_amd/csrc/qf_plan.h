@@ -40,6 +40,7 @@ enum TileOpKind : int32_t {
 // Register-resident tile kernel ("phases"): each thread keeps 2^RB amplitudes
 // in registers; the ops of a phase address register bits:
 //   TOP_MAT1, TOP_ROT (X/Y), TOP_SWAP: q0 (and q1) are register indices;
+//   TOP_MAT2: register indices q0 < q1;
 //   TOP_CNOT: q0 = control as a global qubit, q1 = target register index;
 //   TOP_DIAG, TOP_ROT (Z), TOP_ROTZZ: global qubits (bits taken from the
 //   element's global index).
@@ -106,9 +107,11 @@ struct PlanStats {
 
 // One circuit gate in planner form.
 struct GateIn {
-    int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.), 5 rotation
+    int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.), 5 rotation,
+                       // 6 dense 4x4 on (a, b) (a density matrix's channel superoperator)
     uint32_t a, b;
     cd m[4];           // 1q: matrix; diag2: d[bits(b,a)]
+    cd m4[16]{};       // kind 6: row-major 4x4, basis index bit(b) * 2 + bit(a)
     int32_t axis = 0;  // rotation: 0 X, 1 Y, 2 Z
     int32_t param = 0; // rotation: angle-table column
     int32_t slot = 0;  // rotation: occurrence index

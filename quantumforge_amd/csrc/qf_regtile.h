@@ -191,6 +191,55 @@ __device__ __forceinline__ void reg_swap_any(cx<R> (&a)[1 << RB], int i, int j) 
     }
 }
 
+// Dense 4x4 on register bits I < J: a_r' = sum_c m[r][c] a_c over the index
+// r = bit(I) + 2 bit(J); m holds 16 complex entries (re, im) row major.
+template <class R, int RB, int I, int J>
+__device__ __forceinline__ void reg_mat2(cx<R> (&a)[1 << RB], const double* __restrict__ m) {
+    if constexpr (I < RB && J < RB && I < J) {
+        R mr[16], mi[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            mr[e] = R(m[2 * e]);
+            mi[e] = R(m[2 * e + 1]);
+        }
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) {
+            if ((k >> I) & 1 || (k >> J) & 1) continue;
+            const int idx[4] = {k, k | (1 << I), k | (1 << J), k | (1 << I) | (1 << J)};
+            cx<R> v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = a[idx[c]];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cx<R> acc = cm(mr[4 * r], mi[4 * r], v[0]);
+#pragma unroll
+                for (int c = 1; c < 4; ++c) {
+                    const cx<R> t = cm(mr[4 * r + c], mi[4 * r + c], v[c]);
+                    acc.re += t.re;
+                    acc.im += t.im;
+                }
+                a[idx[r]] = acc;
+            }
+        }
+    }
+}
+
+template <class R, int RB>
+__device__ __forceinline__ void reg_mat2_any(cx<R> (&a)[1 << RB], int i, int j, const double* __restrict__ m) {
+    switch (i * 8 + j) {
+        case 0 * 8 + 1: reg_mat2<R, RB, 0, 1>(a, m); break;
+        case 0 * 8 + 2: reg_mat2<R, RB, 0, 2>(a, m); break;
+        case 0 * 8 + 3: reg_mat2<R, RB, 0, 3>(a, m); break;
+        case 0 * 8 + 4: reg_mat2<R, RB, 0, 4>(a, m); break;
+        case 1 * 8 + 2: reg_mat2<R, RB, 1, 2>(a, m); break;
+        case 1 * 8 + 3: reg_mat2<R, RB, 1, 3>(a, m); break;
+        case 1 * 8 + 4: reg_mat2<R, RB, 1, 4>(a, m); break;
+        case 2 * 8 + 3: reg_mat2<R, RB, 2, 3>(a, m); break;
+        case 2 * 8 + 4: reg_mat2<R, RB, 2, 4>(a, m); break;
+        default: reg_mat2<R, RB, 3, 4>(a, m); break;
+    }
+}
+
 // Diagonal on global qubits q0 (, q1): a_k *= d[bit(q1)*2 + bit(q0)].
 template <class R, int RB>
 __device__ __forceinline__ void reg_diag(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, const TileOp& op) {
@@ -273,6 +322,7 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
         case TOP_DIAG: reg_diag<R, RB>(a, mp, op); break;
         case TOP_CNOT: reg_cnot_any<R, RB>(a, mp, op.q0, op.q1); break;
         case TOP_SWAP: reg_swap_any<R, RB>(a, op.q0, op.q1); break;
+        case TOP_MAT2: reg_mat2_any<R, RB>(a, op.q0, op.q1, op.m); break;
         case TOP_ROT:
         case TOP_ROTZZ: {
             R c, s;

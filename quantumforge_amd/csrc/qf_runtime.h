@@ -47,6 +47,15 @@ struct qf_state {
     uint64_t dim = 0;
 };
 
+// A density matrix of `qubits` qubits: dim x dim complex128 entries, row
+// major (a state vector of 2 * qubits qubits).
+struct qf_density {
+    qf_backend* be = nullptr;
+    void* d = nullptr;
+    uint32_t qubits = 0;
+    uint64_t dim = 0;
+};
+
 struct qf_batch {
     qf_backend* be = nullptr;
     void* d = nullptr;
@@ -97,6 +106,12 @@ qf_status apply_gates(qf_backend* be, void* d, int precision, uint32_t qubits, u
 qf_status reduce(qf_backend* be, const void* d, const void* d2, int precision, uint64_t dim, uint32_t count, int kind,
                  uint32_t q, const std::vector<qf::PauliDev>& terms, std::vector<double>& out);
 qf_backend* state_backend(const qf_state* st);
+void backend_release(qf_backend* be);
+// Noise-channel parameters (QF_CHANNEL_*): InvalidArgument, InvalidProbability.
+qf_status check_channel(int32_t kind, double p0, double p1);
+// One Kraus-trajectory step on `count` contiguous states, one uniform each.
+qf_status trajectory_states(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, int32_t channel,
+                            double p0, double p1, uint32_t q, const double* uniforms);
 
 // A parameterized circuit as the gradient code sees it (qf_pop records).
 struct PCirc {

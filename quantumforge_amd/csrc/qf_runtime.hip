@@ -74,6 +74,8 @@ extern "C" const char* qf_status_name(qf_status s) {
         case QF_INVALID_PROBABILITY: return "InvalidProbability";
         case QF_INVALID_SHOT_COUNT: return "InvalidShotCount";
         case QF_CLASSICAL_BIT_OUT_OF_RANGE: return "ClassicalBitOutOfRange";
+        case QF_NON_HERMITIAN_DENSITY_MATRIX: return "NonHermitianDensityMatrix";
+        case QF_INVALID_SUBSYSTEM_SELECTION: return "InvalidSubsystemSelection";
         default: return "Unknown";
     }
 }

@@ -1,7 +1,8 @@
-// Projective measurement, classically conditioned gates and seeded shots on
-// resident states (CudaDeviceState::measure, CudaDeviceStateBatch::
-// measure_with_uniforms / apply_gate_masked, CudaBackend::run_circuit_shots
-// and run_circuit_shots_batched, src/cuda.rs:1085-1269, 1386-1442, 1500-1624).
+// Projective measurement, classically conditioned gates, Kraus trajectories
+// and seeded shots on resident states (CudaDeviceState::measure,
+// CudaDeviceStateBatch::measure_with_uniforms / apply_gate_masked /
+// apply_channel_trajectory_with_uniforms, CudaBackend::run_circuit_shots and
+// run_circuit_shots_batched, src/cuda.rs:1085-1269, 1386-1442, 1500-1790).
 //
 // A measurement is one FP64 reduction of P(1) per state (RED_P1), the outcome
 // and its probability decided on the host exactly as the CPU path does
@@ -141,6 +142,103 @@ qf_status masked_gate(qf_backend* be, void* d, int precision, uint32_t qubits, u
     return QF_OK;
 }
 
+// Rust's saturating float -> u8 cast (`as u8`): NaN and negatives give 0.
+static int as_u8(double x) { return !(x > 0.0) ? 0 : (x >= 255.0 ? 255 : int(x)); }
+
+// One Kraus-trajectory step of `channel` on qubit `q` of `count` contiguous
+// states, one uniform each, with the branch rules of
+// StateVector::apply_channel_trajectory (src/lib.rs:1356-1505): the norm and
+// P(1) of each state come from one FP64 reduction, the branch and its scales
+// are decided on the host, and one kernel applies every state's branch.  All
+// states are checked before any changes.
+qf_status trajectory_states(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, int32_t channel,
+                            double p0, double p1, uint32_t q, const double* uniforms) {
+#pragma clang fp contract(off)
+    std::vector<uint8_t> code(count, 0);
+    std::vector<double> sc(2 * size_t(count), 0.0);
+    if (channel == QF_CHANNEL_DEPOLARIZING) {
+        for (uint32_t i = 0; i < count; ++i) {
+            const double u = uniforms[i];
+            if (u >= 1.0 - p0 && p0 > 0.0)
+                code[i] = uint8_t(1 + std::min(as_u8(std::floor((u - (1.0 - p0)) * (3.0 / p0))), 2));
+        }
+    } else {
+        const uint64_t dim = uint64_t(1) << qubits;
+        std::vector<double> nrm(count), pone(count), r;
+        for (uint32_t c0 = 0; c0 < count; c0 += 65535) {
+            const uint32_t n = std::min<uint32_t>(65535, count - c0);
+            const char* base = static_cast<const char*>(d) + size_t(c0) * dim * amp_bytes(precision);
+            qf_status s = reduce(be, base, nullptr, precision, dim, n, qf::RED_NORM_P1, q, {}, r);
+            if (s) return s;
+            for (uint32_t i = 0; i < n; ++i) {
+                nrm[c0 + i] = r[2 * i];
+                pone[c0 + i] = r[2 * i + 1];
+            }
+        }
+        const double up = channel == QF_CHANNEL_THERMAL_DAMPING ? p0 * std::exp(-p1) : 0.0, down = p0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const double norm = nrm[i], p_one = pone[i], u = uniforms[i];
+            if (!std::isfinite(norm) || norm <= DBL_EPSILON)
+                return fail(QF_INVALID_PROBABILITY, "state norm must be positive, got " + num(norm));
+            double branch_norm;
+            if (channel != QF_CHANNEL_THERMAL_DAMPING) {
+                const double gamma = p0;
+                const double jump_probability = std::min(std::max(gamma * p_one / norm, 0.0), 1.0);
+                const bool jump = u < jump_probability;
+                branch_norm = jump ? gamma * p_one : norm - gamma * p_one;
+                if (!std::isfinite(branch_norm) || branch_norm <= DBL_EPSILON)
+                    return fail(QF_INVALID_PROBABILITY, "trajectory branch probability " + num(branch_norm));
+                const double inverse = 1.0 / std::sqrt(branch_norm);
+                if (jump) {
+                    code[i] = channel == QF_CHANNEL_AMPLITUDE_DAMPING ? 5 : 7;
+                    sc[2 * i] = std::sqrt(gamma) * inverse;
+                } else {
+                    code[i] = 4;
+                    sc[2 * i] = inverse;
+                    sc[2 * i + 1] = std::sqrt(1.0 - gamma) * inverse;
+                }
+                continue;
+            }
+            const double pr1 = std::min(std::max(p_one / norm, 0.0), 1.0), pr0 = 1.0 - pr1;
+            const double go_down = pr1 * down, go_up = pr0 * up;
+            const double stay = pr0 * (1.0 - up) + pr1 * (1.0 - down);
+            const int branch = u < go_down ? 1 : (u < go_down + go_up ? 2 : 0);
+            branch_norm = branch == 1 ? go_down : (branch == 2 ? go_up : stay);
+            if (!std::isfinite(branch_norm) || branch_norm <= DBL_EPSILON)
+                return fail(QF_INVALID_PROBABILITY, "trajectory branch probability " + num(branch_norm));
+            const double inverse = 1.0 / std::sqrt(branch_norm);
+            if (branch == 1) {
+                code[i] = 5;
+                sc[2 * i] = std::sqrt(down) * inverse;
+            } else if (branch == 2) {
+                code[i] = 6;
+                sc[2 * i] = std::sqrt(up) * inverse;
+            } else {
+                code[i] = 4;
+                sc[2 * i] = std::sqrt(1.0 - up) * inverse;
+                sc[2 * i + 1] = std::sqrt(1.0 - down) * inverse;
+            }
+        }
+    }
+    if (std::all_of(code.begin(), code.end(), [](uint8_t c) { return c == 0; })) return QF_OK;
+    uint8_t* d_code = nullptr;
+    double* d_sc = nullptr;
+    qf_status s = upload_small(be, &be->meas_a, &be->meas_a_bytes, code, &d_code);
+    if (s) return s;
+    if ((s = upload_small(be, &be->meas_b, &be->meas_b_bytes, sc, &d_sc))) return s;
+    const uint64_t npairs = (uint64_t(count) << qubits) / 2;
+    if (precision == QF_C128)
+        hipLaunchKernelGGL(qf::k_trajectory<double>, dim3(grid_for(npairs)), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<cx<double>*>(d), npairs, qubits, q, d_code, d_sc);
+    else
+        hipLaunchKernelGGL(qf::k_trajectory<float>, dim3(grid_for(npairs)), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<cx<float>*>(d), npairs, qubits, q, d_code, d_sc);
+    if ((s = launch_check("k_trajectory"))) return s;
+    // The host copies must outlive the uploads.
+    HIP_TRY_RET(hipStreamSynchronize(be->stream));
+    return QF_OK;
+}
+
 }  // namespace qfrt
 
 namespace {
@@ -155,20 +253,22 @@ qf_status check_gate(int gate, uint32_t target, double theta, uint32_t qubits) {
 }
 
 // One step of a shot circuit: a run of unitary ops (exact plan), a
-// measurement, or a classically conditioned gate.
+// measurement, a classically conditioned gate or a noise channel.
 struct Step {
-    enum Kind { UNITARY, MEASURE, CONDITIONAL } kind;
+    enum Kind { UNITARY, MEASURE, CONDITIONAL, CHANNEL } kind;
     qf::Plan plan;
     uint32_t target = 0, cbit = 0;
     uint8_t value = 0;
-    int gate = 0;
-    double theta = 0;
+    int gate = 0;  // CONDITIONAL: gate id; CHANNEL: QF_CHANNEL_*
+    double theta = 0, theta2 = 0;
 };
 
-qf_status parse_shot_circuit(const qf_op* ops, size_t nops, uint32_t qubits, std::vector<Step>& steps,
-                             uint32_t& width, uint32_t& nmeasure) {
+// nrandom counts the ops that draw one uniform per shot: measurements and
+// (with noise) channels.
+qf_status parse_shot_circuit(const qf_op* ops, size_t nops, uint32_t qubits, bool allow_noise,
+                             std::vector<Step>& steps, uint32_t& width, uint32_t& nrandom) {
     width = 0;
-    nmeasure = 0;
+    nrandom = 0;
     // Classical-bit range and channels in one pass in op order
     // (run_circuit_shots, src/cuda.rs:1097-1119).
     for (size_t i = 0; i < nops; ++i) {
@@ -181,7 +281,7 @@ qf_status parse_shot_circuit(const qf_op* ops, size_t nops, uint32_t qubits, std
                 return fail(QF_CLASSICAL_BIT_OUT_OF_RANGE, "classical bit index " + std::to_string(cbit) +
                                                                " exceeds the supported maximum of 4095");
             width = std::max(width, cbit + 1);
-        } else if (op.kind == 4) {
+        } else if (op.kind == 4 && !allow_noise) {
             return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
         }
     }
@@ -198,7 +298,23 @@ qf_status parse_shot_circuit(const qf_op* ops, size_t nops, uint32_t qubits, std
             st.target = op.a;
             st.cbit = op.b;
             steps.push_back(std::move(st));
-            ++nmeasure;
+            ++nrandom;
+            ++i;
+            continue;
+        }
+        if (op.kind == 4) {
+            if (op.a >= qubits)
+                return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(op.a) + " is out of range for " +
+                                                       std::to_string(qubits) + " qubits");
+            qf_status s = check_channel(op.gate, op.theta, op.theta2);
+            if (s) return s;
+            Step st{Step::CHANNEL};
+            st.gate = op.gate;
+            st.target = op.a;
+            st.theta = op.theta;
+            st.theta2 = op.theta2;
+            steps.push_back(std::move(st));
+            ++nrandom;
             ++i;
             continue;
         }
@@ -216,7 +332,7 @@ qf_status parse_shot_circuit(const qf_op* ops, size_t nops, uint32_t qubits, std
             continue;
         }
         size_t j = i;
-        while (j < nops && ops[j].kind != 5 && ops[j].kind != 6) ++j;
+        while (j < nops && ops[j].kind != 4 && ops[j].kind != 5 && ops[j].kind != 6) ++j;
         std::vector<qf::GateIn> g;
         qf_status s = lower_ops(ops + i, j - i, qubits, g);
         if (s) return s;
@@ -271,6 +387,9 @@ qf_status shots_sequential(qf_backend* be, const qf_c128* initial, uint32_t qubi
                     s = en ? masked_gate(be, d, QF_C128, qubits, 1, st.gate, st.target, st.theta, &en) : QF_OK;
                     break;
                 }
+                case Step::CHANNEL:
+                    s = fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
+                    break;
             }
             if (s) return done(s);
         }
@@ -290,13 +409,28 @@ __global__ void k_broadcast(cx<double>* __restrict__ dst, const cx<double>* __re
 // src/cuda.rs:1161-1269): the uniforms are drawn shot-major up front, then
 // every step runs on the whole batch.  Batches are sized to free device
 // memory; shots are independent, so the split does not change any result.
-qf_status shots_batched(qf_backend* be, const qf_c128* initial, uint32_t qubits, const std::vector<Step>& steps,
-                        uint64_t shots, uint64_t seed, uint32_t width, uint32_t nmeasure, uint8_t* registers) {
+// With noise channels the initial state is first normalized on the host as
+// StateVector::normalize does (src/lib.rs:1005-1014).
+qf_status shots_batched(qf_backend* be, const qf_c128* initial_in, uint32_t qubits, const std::vector<Step>& steps,
+                        uint64_t shots, uint64_t seed, uint32_t width, uint32_t nrandom, uint8_t* registers) {
     SmallRng rng(seed);
-    std::vector<double> draws(size_t(shots) * nmeasure);
+    std::vector<double> draws(size_t(shots) * nrandom);
     for (double& x : draws) x = rng.next_f64();
     const uint64_t dim = uint64_t(1) << qubits;
     const size_t per = size_t(dim) * sizeof(qf_c128);
+    const qf_c128* initial = initial_in;
+    std::vector<qf_c128> normalized;
+    if (std::any_of(steps.begin(), steps.end(), [](const Step& st) { return st.kind == Step::CHANNEL; })) {
+#pragma clang fp contract(off)
+        double acc = 0.0;
+        for (uint64_t i = 0; i < dim; ++i) acc += initial_in[i].re * initial_in[i].re + initial_in[i].im * initial_in[i].im;
+        const double norm = std::sqrt(acc);
+        if (norm == 0.0 || !std::isfinite(norm))
+            return fail(QF_INVALID_PROBABILITY, "initial state norm must be positive, got " + num(norm));
+        normalized.resize(dim);
+        for (uint64_t i = 0; i < dim; ++i) normalized[i] = qf_c128{initial_in[i].re / norm, initial_in[i].im / norm};
+        initial = normalized.data();
+    }
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = size_t(8) << 30;
     uint64_t chunk = uint64_t(double(freeb) * 0.7) / (per ? per : 1);
@@ -335,9 +469,16 @@ qf_status shots_batched(qf_backend* be, const qf_c128* initial, uint32_t qubits,
                 case Step::MEASURE: {
                     u.resize(n);
                     o.resize(n);
-                    for (uint32_t k = 0; k < n; ++k) u[k] = draws[(first + k) * nmeasure + mi];
+                    for (uint32_t k = 0; k < n; ++k) u[k] = draws[(first + k) * nrandom + mi];
                     s = measure_states(be, d, QF_C128, qubits, n, st.target, u.data(), o.data());
                     for (uint32_t k = 0; k < n; ++k) classical[size_t(k) * width + st.cbit] = o[k];
+                    ++mi;
+                    break;
+                }
+                case Step::CHANNEL: {
+                    u.resize(n);
+                    for (uint32_t k = 0; k < n; ++k) u[k] = draws[(first + k) * nrandom + mi];
+                    s = trajectory_states(be, d, QF_C128, qubits, n, st.gate, st.theta, st.theta2, st.target, u.data());
                     ++mi;
                     break;
                 }
@@ -406,11 +547,27 @@ extern "C" qf_status qf_run_circuit_shots(qf_backend* be, const qf_c128* initial
     qf_status s = check_qubits(qubits);
     if (s) return s;
     std::vector<Step> steps;
-    uint32_t width = 0, nmeasure = 0;
-    if ((s = parse_shot_circuit(ops, nops, qubits, steps, width, nmeasure))) return s;
+    uint32_t width = 0, nrandom = 0;
+    if ((s = parse_shot_circuit(ops, nops, qubits, batched != 0, steps, width, nrandom))) return s;
     *classical_width = width;
     if (!registers) return QF_OK;
     if ((s = activate(be))) return s;
-    return batched ? shots_batched(be, initial, qubits, steps, shots, seed, width, nmeasure, registers)
+    return batched ? shots_batched(be, initial, qubits, steps, shots, seed, width, nrandom, registers)
                    : shots_sequential(be, initial, qubits, steps, shots, seed, width, registers);
+}
+
+extern "C" qf_status qf_batch_apply_channel_trajectory(qf_batch* b, int32_t channel, uint32_t target, double p0,
+                                                      double p1, const double* uniforms, size_t count) {
+    if (!b || (!uniforms && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (target >= b->qubits)
+        return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(target) + " is out of range for " +
+                                               std::to_string(b->qubits) + " qubits");
+    bool ok = count == b->count;
+    for (size_t i = 0; ok && i < count; ++i)
+        ok = std::isfinite(uniforms[i]) && uniforms[i] >= 0.0 && uniforms[i] < 1.0;
+    if (!ok) return fail(QF_INVALID_BATCH_SAMPLES, "a trajectory step needs one uniform sample per state");
+    qf_status s = check_channel(channel, p0, p1);
+    if (s) return s;
+    if ((s = activate(b->be))) return s;
+    return trajectory_states(b->be, b->d, b->precision, b->qubits, b->count, channel, p0, p1, target, uniforms);
 }

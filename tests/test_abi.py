@@ -40,12 +40,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_record_layouts():
-    assert N.OP_DTYPE.itemsize == 24
+    assert N.OP_DTYPE.itemsize == 32
     assert N.POP_DTYPE.itemsize == 32
     assert N.PAULI_DTYPE.itemsize == 32
     assert N.ROT_DTYPE.itemsize == 16
     src = open(HEADER).read()
-    assert "int32_t kind;\n    int32_t gate;\n    uint32_t a, b;\n    double theta;\n} qf_op;" in src
+    assert "int32_t kind;\n    int32_t gate;\n    uint32_t a, b;\n    double theta;\n    double theta2;\n} qf_op;" in src
+    assert N.OP_DTYPE.names == ("kind", "gate", "a", "b", "theta", "theta2")
 
 
 def test_status_names_match_reference_variants():
@@ -105,6 +106,15 @@ def _swap(psi, idx, a, b):
     psi[other] = tmp
 
 
+def _mat2(psi, idx, a, b, m):
+    """Dense 4x4 over (bit(b) * 2 + bit(a)), row-major 16 complex."""
+    base = idx[(((idx >> a) & 1) == 0) & (((idx >> b) & 1) == 0)]
+    sel = [base, base | (1 << a), base | (1 << b), base | (1 << a) | (1 << b)]
+    v = [psi[s].copy() for s in sel]
+    for r in range(4):
+        psi[sel[r]] = sum(_c(m, 4 * r + c) * v[c] for c in range(4))
+
+
 def _diag(psi, idx, q0, q1, m):
     kk = (idx >> q0) & 1
     if q1 >= 0:
@@ -139,6 +149,9 @@ def replay(desc, psi):
                     _cnot(psi, idx, op["q0"], regs[op["q1"]])
                 elif k == 3:
                     _swap(psi, idx, regs[op["q0"]], regs[op["q1"]])
+                elif k == 4:
+                    assert op["q0"] < op["q1"], "register-form dense 4x4 ops have q0 < q1"
+                    _mat2(psi, idx, regs[op["q0"]], regs[op["q1"]], m)
                 else:
                     raise AssertionError(f"unexpected op kind {k}")
             else:
@@ -150,6 +163,8 @@ def replay(desc, psi):
                     _cnot(psi, idx, bits[op["q0"]], bits[op["q1"]])
                 elif k == 3:
                     _swap(psi, idx, bits[op["q0"]], bits[op["q1"]])
+                elif k == 4:
+                    _mat2(psi, idx, bits[op["q0"]], bits[op["q1"]], m)
                 else:
                     raise AssertionError(f"unexpected op kind {k}")
     return psi
@@ -163,13 +178,13 @@ def check_tile_invariants(desc):
         if reg:
             assert p["ops"][0]["kind"] == 7
             for op in p["ops"]:
-                if op["kind"] in (0, 3):
+                if op["kind"] in (0, 3, 4):
                     assert 0 <= op["q0"] < reg and (op["kind"] == 0 or 0 <= op["q1"] < reg)
                 if op["kind"] == 2:
                     assert 0 <= op["q0"] < n and 0 <= op["q1"] < reg
             continue
         for op in p["ops"]:
-            if op["kind"] in (0, 2, 3):
+            if op["kind"] in (0, 2, 3, 4):
                 assert 0 <= op["q0"] < M and (op["kind"] == 0 or 0 <= op["q1"] < M)
 
 
