@@ -250,6 +250,113 @@ def pcie_bench(be, qubits: int) -> dict:
             "download_GBs": round(nbytes / t_down / 1e9, 2), "memory": "pageable host (NumPy)"}
 
 
+def density_bench(be, torch, n: int, layers: int, steps: int, with_cpu: bool) -> dict:
+    """A density matrix of n qubits (row-major entries = a 2n-qubit state,
+    4^n * 16 bytes resident) under `layers` noisy layers
+    (workloads.noisy_layer_circuit, seed 7): compiled tile passes over the
+    2n-qubit view, gates and channel superoperators fused.  One step = the
+    whole circuit on the resident matrix, timed with HIP events on the
+    engine's stream."""
+    from quantumforge_amd.backend import Plan
+    from quantumforge_amd.workloads import noisy_layer_circuit
+
+    c = noisy_layer_circuit(n, layers, seed=7)
+    plan = Plan.density(c, n)
+    stats = plan.stats()
+    desc = plan.describe()
+    compiled = bool(desc.get("compiled", 0))
+    info = plan.compile() if compiled else None
+    dev = be.density_from_state(be.zero_state(n))
+    stream = torch.cuda.ExternalStream(be.stream())
+    dev.run_circuit(c, plan)
+    be.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        dev.run_circuit(c, plan)
+    b.record(stream)
+    be.synchronize()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    trace, purity = dev.trace(), dev.purity()
+    del dev
+    nbytes = (1 << (2 * n)) * 16
+    per_pass = ms / max(1, stats["passes"])
+    achieved = 2 * nbytes / (per_pass * 1e-3) / 1e9
+    res = {"workload": f"density matrix, {n} qubits ({2 * n}-qubit view, {nbytes >> 30} GiB), {layers} noisy layers "
+                       f"(rotations, CNOT/CZ, one channel per qubit and layer), complex128",
+           "qubits": n, "layers": layers, "ops": len(c), "passes": stats["passes"],
+           "pass_kernels": "compiled" if compiled else "generic",
+           "compile_ms": round(info["compile_ms"], 1) if info else None,
+           "ms_per_circuit": round(ms, 3), "layers_per_s": round(layers / (ms * 1e-3), 3),
+           "ms_per_pass": round(per_pass, 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": 2 * nbytes},
+           "trace_after": [trace.real, trace.imag], "purity_after": purity, "steps": steps}
+    p0 = desc["passes"][0]
+    want = {"workload": "density", "density_qubits": n, "layers": layers, "seed": 7, "precision": "complex128",
+            "pass_kernels": "compiled" if compiled else "generic", "tile_qubits": p0["tile_bits"],
+            "low_qubits": p0.get("low", 0), "register_bits": p0.get("reg_bits", 0), "passes": stats["passes"]}
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    res["roofline"]["traffic"] = None
+    if os.path.exists(path):
+        with open(path) as f:
+            for e in json.load(f).get("entries", []):
+                if all(e.get(k) == v for k, v in want.items()):
+                    res["roofline"]["traffic"] = e["bytes_per_launch"]
+    log(f"density {n}q: {ms:.1f} ms per {layers}-layer circuit, {stats['passes']} passes, "
+        f"{achieved:.0f} GB/s per pass")
+    if with_cpu:
+        from oracle import oracle as O
+
+        m = min(n, 11)
+        c1 = noisy_layer_circuit(m, 1, seed=7)
+        psi = np.zeros(1 << m, dtype=np.complex128)
+        psi[0] = 1.0
+        d = O.Density(m, state=psi)
+        t0 = time.perf_counter()
+        d.run(c1.to_ops())
+        t = time.perf_counter() - t0
+        scale = float(4 ** (n - m)) * (len(c) / layers) / len(c1)
+        res["cpu_baseline"] = {
+            "value": round(1.0 / (t * scale), 6), "unit": "layers/s", "cores": 1, "kind": "port",
+            "sample": f"one noisy layer of {len(c1)} ops on a {m}-qubit density matrix ({t:.1f} s; the reference's "
+                      f"DensityMatrix loops are sequential), scaled by the entry count 4^{n - m} to {n} qubits"}
+    return res
+
+
+def trajectory_bench(be, n: int, shots: int, layers: int, with_cpu: bool) -> dict:
+    """Batched noisy shots (Kraus trajectories): `shots` trajectories of an
+    n-qubit state through `layers` noisy layers, then every qubit measured,
+    seed 11; one call of run_circuit_shots_batched, wall clock."""
+    from quantumforge_amd import StateVector
+    from quantumforge_amd.workloads import noisy_layer_circuit
+
+    c = noisy_layer_circuit(n, layers, seed=11)
+    for q in range(n):
+        c.measure(q, q)
+    init = StateVector.zero(n)
+    be.run_circuit_shots_batched(init, c, 8, 1)  # warm the plans
+    be.synchronize()
+    t0 = time.perf_counter()
+    r = be.run_circuit_shots_batched(init, c, shots, 11)
+    dt = time.perf_counter() - t0
+    res = {"workload": f"{shots} noisy trajectories, {n} qubits, {layers} noisy layers, measure all, complex128",
+           "qubits": n, "shots": shots, "ops": len(c), "seconds": round(dt, 3), "shots_per_s": round(shots / dt, 1),
+           "distinct_outcomes": len(r.counts)}
+    log(f"trajectories {n}q: {shots} shots in {dt:.2f} s")
+    if with_cpu:
+        from oracle import oracle as O
+
+        k = 64
+        t0 = time.perf_counter()
+        O.run_shots(init.amplitudes, n, c.to_ops(), k, 11, noise=True)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(k / t, 3), "unit": "shots/s", "cores": O.max_threads(),
+                               "kind": "port", "sample": f"{k} shots of the same circuit ({t:.1f} s)"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +378,11 @@ def main() -> int:
     ap.add_argument("--qft-steps", type=int, default=2, help="timed runs of the 33q/34q QFT-like circuits (0 = skip)")
     ap.add_argument("--small-steps", type=int, default=20, help="timed runs of the 20-qubit circuit (0 = skip)")
     ap.add_argument("--pcie-qubits", type=int, default=26, help="state size of the upload/download timing (0 = skip)")
+    ap.add_argument("--density-qubits", type=int, default=15, help="density-matrix section size (0 = skip)")
+    ap.add_argument("--density-layers", type=int, default=4)
+    ap.add_argument("--density-steps", type=int, default=2)
+    ap.add_argument("--traj-qubits", type=int, default=16, help="noisy-trajectory section size (0 = skip)")
+    ap.add_argument("--traj-shots", type=int, default=4096)
     ap.add_argument("--uneven-instances", type=int, default=0,
                     help="uneven QAOA adjoint batch size (0 = skip; the full workload is 512)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -403,6 +515,11 @@ def main() -> int:
     if args.pcie_qubits > 0:
         out["host_boundary"] = pcie_bench(be, args.pcie_qubits)
         log(f"host boundary: {out['host_boundary']}")
+    if args.density_qubits > 0:
+        out["density"] = density_bench(be, torch, args.density_qubits, args.density_layers, args.density_steps,
+                                       not args.no_cpu)
+    if args.traj_qubits > 0:
+        out["trajectories"] = trajectory_bench(be, args.traj_qubits, args.traj_shots, 2, not args.no_cpu)
     if args.uneven_instances > 0:
         out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512)
     if not args.no_cpu:

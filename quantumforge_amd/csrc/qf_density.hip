@@ -183,6 +183,16 @@ qf_status lower_density(const qf_op* ops, size_t nops, uint32_t n, std::vector<q
                 g.a = op.a;      // column bit: low bit of the superoperator index
                 g.b = n + op.a;  // row bit: high bit
                 channel_superop(op.gate, op.theta, op.theta2, g.m4);
+                // A diagonal superoperator (phase damping) is a two-qubit
+                // diagonal: applied from the element index, no register bits.
+                bool diagonal = true;
+                for (int r = 0; r < 4; ++r)
+                    for (int c = 0; c < 4; ++c)
+                        if (r != c && (g.m4[r * 4 + c].re != 0.0 || g.m4[r * 4 + c].im != 0.0)) diagonal = false;
+                if (diagonal) {
+                    g.kind = 4;
+                    for (int k = 0; k < 4; ++k) g.m[k] = g.m4[k * 5];
+                }
                 out.push_back(g);
                 break;
             }

@@ -12,7 +12,7 @@ import math
 from dataclasses import dataclass, field
 from typing import List, Sequence, Tuple
 
-from .core import Circuit, Gate, ParameterizedCircuit, PauliSum, RotationAxis
+from .core import Circuit, Gate, NoiseChannel, ParameterizedCircuit, PauliSum, RotationAxis
 
 MASK64 = (1 << 64) - 1
 
@@ -83,6 +83,31 @@ def random_layer_circuit(qubits: int, depth: int, seed: int, pairs_per_layer: in
                 c.cnot(a, b)
             else:
                 c.cz(a, b)
+    return c
+
+
+def noisy_layer_circuit(qubits: int, layers: int, seed: int, strength: float = 0.02) -> Circuit:
+    """Noisy layers for density matrices and trajectories: per layer an
+    RX/RY/RZ rotation on every qubit (axis floor(3u), angle 2*pi*u), CNOT on
+    (q, q+1) for even q, CZ(n-1, 0), then one channel on every qubit, cycling
+    depolarizing (p), amplitude damping (gamma), phase damping (gamma) and
+    thermal damping (gamma_down, beta_delta = 1) with strength p = gamma =
+    gamma_down = `strength`."""
+    rng = SmallRng(seed)
+    c = Circuit()
+    two_pi = 2.0 * math.pi
+    chans = [NoiseChannel("Depolarizing", (strength,)), NoiseChannel("AmplitudeDamping", (strength,)),
+             NoiseChannel("PhaseDamping", (strength,)), NoiseChannel("ThermalDamping", (strength, 1.0))]
+    for layer in range(layers):
+        for q in range(qubits):
+            axis = ("Rx", "Ry", "Rz")[min(int(3.0 * rng.next_f64()), 2)]
+            c.push(Gate(axis, two_pi * rng.next_f64()), q)
+        for q in range(0, qubits - 1, 2):
+            c.cnot(q, q + 1)
+        if qubits > 2:
+            c.cz(qubits - 1, 0)
+        for q in range(qubits):
+            c.channel(q, chans[(q + layer) % 4])
     return c
 
 

@@ -65,6 +65,18 @@ def test_compiled_density_passes_match_oracle(n, M, RB):
     close(emulate(plan, rho0, 2 * n), oracle_density(c, n, psi), 1e-13)
 
 
+def test_noisy_layer_workload_plan_matches_oracle():
+    from quantumforge_amd.workloads import noisy_layer_circuit
+
+    n = 5
+    c = noisy_layer_circuit(n, 2, seed=7, strength=0.2)
+    assert sum(1 for op in c.to_ops() if op["kind"] == 4) == 2 * n
+    psi = random_state(n, 1)
+    rho0 = np.outer(psi, psi.conj()).reshape(-1)
+    close(replay(Plan.density(c, n, kernel=2, tile_qubits=10, reg_bits=3).describe(), rho0),
+          oracle_density(c, n, psi), 1e-13)
+
+
 def test_density_default_plan_is_compiled_from_nine_qubits():
     c = noisy_layers(9, 1, seed=1)
     assert Plan.density(c, 9).compiled
