@@ -357,6 +357,61 @@ def trajectory_bench(be, n: int, shots: int, layers: int, with_cpu: bool) -> dic
     return res
 
 
+def lanczos_bench(be, sites: int, csr_sites: int, with_cpu: bool) -> dict:
+    """The DMRG TFIM Lanczos on the device, matrix-free (`sites` sites,
+    2^sites doubles per Krylov vector, 30 steps from 24 sites, 100 below),
+    and sparse_lanczos_gpu on the same chain stored as CSR (`csr_sites`
+    sites, 100 steps).  Wall clock of one call each.  Algorithmic bytes of step
+    j: the operator's vectors (read v_j and v_{j-1}, write w: 3 x 8 B per
+    entry), one reorthogonalization pass (read the j + 1 basis vectors twice,
+    w three times) and the division (2 x 8 B)."""
+    from quantumforge_amd import SparseOperator
+
+    def model_bytes(dim, steps):
+        return sum((3 + 2 * (j + 1) + 3 + 2) * 8 * dim for j in range(steps))
+
+    be.tfim_lanczos(12)  # warm the kernels
+    res = {}
+    if sites > 0:
+        t0 = time.perf_counter()
+        e, a, b, m = be.tfim_lanczos(sites, details=True)
+        dt = time.perf_counter() - t0
+        dim = 1 << sites
+        steps = int(np.count_nonzero(a[:m])) or m
+        ach = model_bytes(dim, steps) / dt / 1e9
+        res["tfim"] = {"workload": f"TFIM chain, {sites} sites (2^{sites} doubles per vector), {steps} Lanczos steps, "
+                                   f"matrix-free", "sites": sites, "steps": steps, "seconds": round(dt, 4),
+                       "ms_per_step": round(dt / steps * 1e3, 3), "lowest": e,
+                       "model_GBs": round(ach, 1), "model_frac_of_8TBs": round(ach / HBM_PEAK_GBS, 4)}
+        log(f"lanczos tfim {sites}: {dt:.3f} s for {steps} steps, E0 = {e:.12f}, {ach:.0f} GB/s (model bytes)")
+    if csr_sites > 0:
+        op = SparseOperator.tfim(csr_sites)
+        t0 = time.perf_counter()
+        e, a, b, size = be.sparse_lanczos(op, 100, 1e-10, details=True)
+        dt = time.perf_counter() - t0
+        res["csr"] = {"workload": f"TFIM chain as CSR, {csr_sites} sites, {op.nonzero_count()} stored values, "
+                                  f"{size} Lanczos steps", "steps": size, "seconds": round(dt, 4),
+                      "ms_per_step": round(dt / size * 1e3, 3), "lowest": e}
+        log(f"lanczos csr {csr_sites}: {dt:.3f} s for {size} steps, E0 = {e:.12f}")
+    if with_cpu and sites > 0:
+        from oracle import oracle as O
+
+        n = min(sites, 18)
+        t0 = time.perf_counter()
+        O.tfim_lanczos(n)
+        t = time.perf_counter() - t0
+        m_cpu = min(100, 1 << n)
+        # Work of the restated loops: per step (n + 1) operator terms and 2 (j + 1) + 4 vector passes per entry.
+        work = lambda nn, mm: sum((1 << nn) * ((nn + 1) + 2 * (j + 1) + 4) for j in range(mm))
+        scale = work(sites, res["tfim"]["steps"]) / work(n, m_cpu)
+        res["cpu_baseline"] = {"value": round(1.0 / (t * scale), 6), "unit": "TFIM Lanczos runs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the oracle's restatement (sequential, as DMRG::run_sparse_lanczos) at "
+                                         f"{n} sites, {m_cpu} steps ({t:.1f} s), scaled by the loop work to "
+                                         f"{sites} sites"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +438,8 @@ def main() -> int:
     ap.add_argument("--density-steps", type=int, default=2)
     ap.add_argument("--traj-qubits", type=int, default=16, help="noisy-trajectory section size (0 = skip)")
     ap.add_argument("--traj-shots", type=int, default=4096)
+    ap.add_argument("--lanczos-sites", type=int, default=26, help="matrix-free TFIM Lanczos size (0 = skip)")
+    ap.add_argument("--lanczos-csr-sites", type=int, default=22, help="CSR sparse Lanczos size (0 = skip)")
     ap.add_argument("--uneven-instances", type=int, default=0,
                     help="uneven QAOA adjoint batch size (0 = skip; the full workload is 512)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -520,6 +577,8 @@ def main() -> int:
                                        not args.no_cpu)
     if args.traj_qubits > 0:
         out["trajectories"] = trajectory_bench(be, args.traj_qubits, args.traj_shots, 2, not args.no_cpu)
+    if args.lanczos_sites > 0 or args.lanczos_csr_sites > 0:
+        out["lanczos"] = lanczos_bench(be, args.lanczos_sites, args.lanczos_csr_sites, not args.no_cpu)
     if args.uneven_instances > 0:
         out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512)
     if not args.no_cpu:

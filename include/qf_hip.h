@@ -489,6 +489,34 @@ qf_status qf_density_plan_build(const qf_op* ops, size_t nops, uint32_t qubits, 
 qf_status qf_density_run_plan(qf_density* density, const qf_plan* plan);
 
 /* ------------------------------------------------------------------------ */
+/* Sparse Lanczos (sparse_lanczos_gpu, src/quantum.rs:1853-2023; the TFIM     */
+/* Lanczos of DMRG, src/tensor_network.rs:1535-1689 and 1692-2025)           */
+/* ------------------------------------------------------------------------ */
+/* Lowest Ritz value of a real symmetric operator in CSR form (the reference
+ * keeps the real part of every value): the start vector of both reference
+ * routines (seed 42, seed = seed * 6364136223846793005 + 1, x = seed / 2^64 -
+ * 0.5, normalized), m = min(max_iterations, dim, 100) steps with full
+ * reorthogonalization, stop when beta < tolerance, the tridiagonal model cut at
+ * its first alpha equal to 0.0, and its lowest eigenvalue.  The Krylov basis
+ * stays in device memory; reorthogonalization is one multi-dot and one fused
+ * update-and-norm pass over the basis per step (a second pass when the norm
+ * drops below 1/sqrt(2) of the three-term remainder's).  alpha / beta (capacity 100 each) and
+ * size may be NULL.  Errors: InvalidArgument (dim or max_iterations zero,
+ * inconsistent CSR arrays, a first alpha of exactly 0.0), StateTooLarge
+ * (dim >= 2^31 or nnz >= 2^32), MemoryAllocationFailed. */
+qf_status qf_sparse_lanczos(qf_backend* backend, const uint64_t* row_offsets, const uint64_t* column_indices,
+                            const qf_c128* values, uint64_t dim, uint64_t max_iterations, double tolerance,
+                            double* lowest, double* alpha, double* beta, uint64_t* size);
+/* The TFIM chain H = -sum_i Z_i Z_{i+1} - sum_i X_i of `sites` sites (site i is
+ * index bit sites-1-i), matrix-free: m = min(sites >= 24 ? 30 : 100, 2^sites)
+ * steps, reorthogonalization against every stored vector, stop when beta <
+ * 1e-12, a tridiagonal model of m rows (zero beyond the last step), its lowest
+ * eigenvalue.  2 <= sites <= 30 (InvalidArgument below, MemoryAllocationFailed
+ * when the basis does not fit). */
+qf_status qf_tfim_lanczos(qf_backend* backend, uint32_t sites, double* lowest, double* alpha, double* beta,
+                          uint64_t* size);
+
+/* ------------------------------------------------------------------------ */
 /* Planning (no reference counterpart; see qf_plan_options)                  */
 /* ------------------------------------------------------------------------ */
 /* Builds the fused tile-pass schedule of a circuit (host only, no device use).

@@ -112,6 +112,12 @@ def lib():
         L.or_density_probability_one.restype = i32
         L.or_density_run.argtypes = [P, P, sz]
         L.or_density_run.restype = i32
+        L.or_lanczos_start.argtypes = [P, u64]
+        L.or_lanczos_start.restype = None
+        L.or_sparse_lanczos.argtypes = [P, P, P, u64, u64, dbl, P, P, C.POINTER(u64)]
+        L.or_sparse_lanczos.restype = i32
+        L.or_tfim_lanczos.argtypes = [u32, P, P, C.POINTER(u64)]
+        L.or_tfim_lanczos.restype = i32
         _lib = L
     return _lib
 
@@ -384,3 +390,43 @@ class Density:
         ops = as_ops(ops)
         _chk(lib().or_density_run(self.h, _p(ops), len(ops)))
 
+
+
+
+def tridiagonal_lowest(alpha: np.ndarray, beta: np.ndarray, size: int) -> float:
+    """Lowest eigenvalue of the symmetric tridiagonal model (alpha on the
+    diagonal, beta[i-1] beside it), with LAPACK's symmetric eigensolver in
+    place of the reference's faer self_adjoint_evd."""
+    T = np.diag(np.asarray(alpha[:size], dtype=np.float64))
+    if size > 1:
+        off = np.asarray(beta[: size - 1], dtype=np.float64)
+        T += np.diag(off, 1) + np.diag(off, -1)
+    return float(np.linalg.eigvalsh(T)[0]) if size else float("nan")
+
+
+def lanczos_start(dim: int) -> np.ndarray:
+    out = np.empty(dim, dtype=np.float64)
+    lib().or_lanczos_start(_p(out), dim)
+    return out
+
+
+def sparse_lanczos(rows, cols, vals, dim: int, max_iterations: int, tolerance: float):
+    """sparse_lanczos_gpu restated: (lowest, alpha, beta, size)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    cols = np.ascontiguousarray(cols, dtype=np.uint64)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    m = max(1, min(max_iterations, dim, 100))
+    a, b = np.zeros(m), np.zeros(m)
+    size = C.c_uint64()
+    _chk(lib().or_sparse_lanczos(_p(rows), _p(cols), _p(vals), dim, max_iterations, tolerance, _p(a), _p(b),
+                                 C.byref(size)))
+    return tridiagonal_lowest(a, b, size.value), a, b, size.value
+
+
+def tfim_lanczos(n: int):
+    """DMRG::run_sparse_lanczos on the TFIM chain restated: (lowest, alpha, beta, m)."""
+    m = min(30 if n >= 24 else 100, 1 << n)
+    a, b = np.zeros(m), np.zeros(m)
+    size = C.c_uint64()
+    _chk(lib().or_tfim_lanczos(n, _p(a), _p(b), C.byref(size)))
+    return tridiagonal_lowest(a, b, size.value), a, b, size.value

@@ -1189,3 +1189,139 @@ int or_fast_run(double* amps_d, uint32_t qubits, const or_op* ops, size_t nops) 
 }
 
 }  // extern "C"
+
+// ---- Lanczos (sparse_lanczos_gpu, src/quantum.rs:1853-2023; DMRG::run_sparse_lanczos,
+// src/tensor_network.rs:1535-1689, whose GPU form run_sparse_lanczos_gpu,
+// src/tensor_network.rs:1692-2025, runs the same iteration)
+namespace {
+
+// Start vector of both: seed = seed * 6364136223846793005 + 1 (wrapping, seed
+// 42 first), x = seed / u64::MAX - 0.5, then divided by its 2-norm.
+void lanczos_start(std::vector<double>& v) {
+    uint64_t seed = 42;
+    for (double& x : v) {
+        seed = seed * 6364136223846793005ull + 1ull;
+        x = double(seed) / double(UINT64_MAX) - 0.5;
+    }
+    double acc = 0.0;
+    for (double x : v) acc += x * x;
+    const double norm = std::sqrt(acc);
+    for (double& x : v) x /= norm;
+}
+
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double acc = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) acc += a[i] * b[i];
+    return acc;
+}
+
+// tfim_matvec of src/tensor_network.rs:1702-1718 (H = -sum Z_i Z_{i+1} - sum
+// X_i on an open chain, site i = bit n-1-i of the index).
+void tfim_apply(uint32_t n, const std::vector<double>& v, std::vector<double>& w) {
+    const size_t dim = v.size();
+    for (size_t a = 0; a < dim; ++a) {
+        double r = 0.0;
+        for (uint32_t i = 0; i + 1 < n; ++i) {
+            const double zi = ((a >> (n - 1 - i)) & 1) ? -1.0 : 1.0;
+            const double zj = ((a >> (n - 2 - i)) & 1) ? -1.0 : 1.0;
+            r += -zi * zj * v[a];
+        }
+        for (uint32_t i = 0; i < n; ++i) r += -v[a ^ (size_t(1) << (n - 1 - i))];
+        w[a] = r;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// The Lanczos start vector (for pinning the LCG against exact integer arithmetic).
+void or_lanczos_start(double* out, uint64_t dim) {
+    std::vector<double> v(dim);
+    lanczos_start(v);
+    std::memcpy(out, v.data(), dim * sizeof(double));
+}
+
+// sparse_lanczos_gpu on a CSR operator with real values (the reference keeps
+// the real part of each entry): m = min(max_iterations, dim, 100); per step
+// w = A v, w -= beta_{j-1} v_prev, modified Gram-Schmidt against v_0..v_{j-1},
+// alpha_j = v_j . w, w -= alpha_j v_j, beta_j = |w|; stop when beta_j <
+// tolerance or after m steps.  Writes alpha[0..m), beta[0..m) and the size of
+// the tridiagonal model (the first index with alpha == 0.0, else m).
+int or_sparse_lanczos(const uint64_t* rows, const uint64_t* cols, const double* vals, uint64_t dim,
+                      uint64_t max_iterations, double tolerance, double* alpha, double* beta, uint64_t* size) {
+    const size_t m = std::min<uint64_t>(std::min<uint64_t>(max_iterations, dim), 100);
+    std::vector<double> v(dim), vprev(dim, 0.0), w(dim);
+    lanczos_start(v);
+    std::vector<std::vector<double>> vall{v};
+    std::fill(alpha, alpha + m, 0.0);
+    std::fill(beta, beta + m, 0.0);
+    for (size_t j = 0; j < m; ++j) {
+        for (size_t i = 0; i < dim; ++i) {
+            double s = 0.0;
+            for (uint64_t k = rows[i]; k < rows[i + 1]; ++k) s += vals[k] * v[cols[k]];
+            w[i] = s;
+        }
+        if (j > 0)
+            for (size_t i = 0; i < dim; ++i) w[i] -= beta[j - 1] * vprev[i];
+        for (size_t k = 0; k < j; ++k) {
+            const double p = dot(vall[k], w);
+            for (size_t i = 0; i < dim; ++i) w[i] -= p * vall[k][i];
+        }
+        const double a = dot(v, w);
+        for (size_t i = 0; i < dim; ++i) w[i] -= a * v[i];
+        const double b = std::sqrt(dot(w, w));
+        alpha[j] = a;
+        if (j + 1 < m) beta[j] = b;
+        if (b < tolerance || j + 1 == m) break;
+        vprev = v;
+        for (size_t i = 0; i < dim; ++i) v[i] = w[i] / b;
+        vall.push_back(v);
+    }
+    size_t actual = m;
+    for (size_t j = 0; j < m; ++j)
+        if (alpha[j] == 0.0) {
+            actual = j;
+            break;
+        }
+    *size = actual;
+    return OK;
+}
+
+// DMRG::run_sparse_lanczos on the TFIM chain of n sites: m = min(n >= 24 ? 30
+// : 100, 2^n); per step w = H v, alpha_j = v . w, w -= alpha_j v + beta_{j-1}
+// v_prev, modified Gram-Schmidt against every stored vector (the current one
+// included), beta_j = |w|, stop when beta_j < 1e-12.  The tridiagonal model
+// always has m rows (zero beyond the last step).  Writes alpha, beta (m
+// entries) and m.
+int or_tfim_lanczos(uint32_t n, double* alpha, double* beta, uint64_t* size) {
+    if (n < 2 || n > 26) return INVALID_ARGUMENT;
+    const size_t dim = size_t(1) << n;
+    const size_t m = std::min<size_t>(n >= 24 ? 30 : 100, dim);
+    std::vector<double> v(dim), vprev(dim, 0.0), w(dim);
+    lanczos_start(v);
+    std::vector<std::vector<double>> vall{v};
+    std::fill(alpha, alpha + m, 0.0);
+    std::fill(beta, beta + m, 0.0);
+    for (size_t j = 0; j < m; ++j) {
+        tfim_apply(n, v, w);
+        alpha[j] = dot(v, w);
+        for (size_t i = 0; i < dim; ++i) {
+            w[i] -= alpha[j] * v[i];
+            if (j > 0) w[i] -= beta[j - 1] * vprev[i];
+        }
+        for (const auto& u : vall) {
+            const double p = dot(w, u);
+            for (size_t i = 0; i < dim; ++i) w[i] -= p * u[i];
+        }
+        beta[j] = std::sqrt(dot(w, w));
+        if (beta[j] < 1e-12) break;
+        vprev = v;
+        for (size_t i = 0; i < dim; ++i) v[i] = w[i] / beta[j];
+        vall.push_back(v);
+    }
+    *size = m;
+    return OK;
+}
+
+}  // extern "C"

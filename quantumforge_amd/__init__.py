@@ -7,11 +7,11 @@ include/qf_hip.h (backend), gradients (gradients), workload builders
 """
 from ._native import QuantumError
 from .core import (Circuit, DensityMatrix, Gate, NoiseChannel, ParameterizedCircuit, PauliSum, RotationAxis,
-                   ShotResult, SingleQubitObservable, StateVector, rotation_gate)
+                   ShotResult, SingleQubitObservable, SparseOperator, StateVector, rotation_gate)
 
 __all__ = [
     "QuantumError", "Circuit", "DensityMatrix", "Gate", "NoiseChannel", "ParameterizedCircuit", "PauliSum",
-    "RotationAxis", "ShotResult", "SingleQubitObservable", "StateVector", "rotation_gate", "HipBackend",
+    "RotationAxis", "ShotResult", "SingleQubitObservable", "SparseOperator", "StateVector", "rotation_gate", "HipBackend",
     "HipDeviceState", "HipDeviceStateBatch", "HipDeviceDensityMatrix", "Plan",
 ]
 

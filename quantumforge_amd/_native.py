@@ -133,6 +133,8 @@ SIGNATURES = {
     "qf_run_density_circuit": (None, [P, P, sz, u32, P, sz, P, sz]),
     "qf_density_plan_build": (None, [P, sz, u32, C.POINTER(PlanOptions), PP]),
     "qf_density_run_plan": (None, [P, P]),
+    "qf_sparse_lanczos": (None, [P, P, P, P, u64, u64, dbl, C.POINTER(dbl), P, P, C.POINTER(u64)]),
+    "qf_tfim_lanczos": (None, [P, u32, C.POINTER(dbl), P, P, C.POINTER(u64)]),
     "qf_plan_build_parameterized": (None, [P, sz, u32, u32, P, sz, u32, i32, C.POINTER(PlanOptions), PP]),
     "qf_plan_source": (None, [P, i32, C.c_char_p, sz, C.POINTER(sz)]),
     "qf_plan_compile": (None, [P, i32, C.POINTER(dbl), C.POINTER(i32)]),

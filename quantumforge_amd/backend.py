@@ -12,7 +12,7 @@ import numpy as np
 from . import _native as N
 from ._native import QuantumError, check, lib, ptr
 from .core import (Circuit, DensityMatrix, Gate, NoiseChannel, PauliSum, RotationAxis, ShotResult,
-                   SingleQubitObservable, StateVector)
+                   SingleQubitObservable, SparseOperator, StateVector)
 
 C128, C64 = 0, 1
 _PRECISION = {"complex128": C128, "c128": C128, C128: C128, "complex64": C64, "c64": C64, C64: C64}
@@ -239,6 +239,31 @@ class HipBackend:
         check(lib().qf_run_density_circuit(self._h, ptr(e), e.size, initial.qubits, ptr(ops), len(ops), ptr(out),
                                            out.size))
         return DensityMatrix(out, initial.qubits)
+
+    # --- sparse Lanczos (src/quantum.rs:1853-2023, src/tensor_network.rs:1692-2025)
+    def sparse_lanczos(self, op: SparseOperator, max_iterations: int, tolerance: float, details: bool = False):
+        """sparse_lanczos_gpu: the lowest Ritz value of a real symmetric CSR
+        operator; with details, (lowest, alpha, beta, model size)."""
+        if op.rows != op.cols:
+            raise QuantumError(15, "the operator must be square")
+        rows = np.ascontiguousarray(op.row_offsets, dtype=np.uint64)
+        cols = np.ascontiguousarray(op.column_indices, dtype=np.uint64)
+        vals = np.ascontiguousarray(op.values, dtype=np.complex128)
+        lo = C.c_double()
+        a, b = np.zeros(100), np.zeros(100)
+        size = C.c_uint64()
+        check(lib().qf_sparse_lanczos(self._h, ptr(rows), ptr(cols), ptr(vals), op.rows, int(max_iterations),
+                                      float(tolerance), C.byref(lo), ptr(a), ptr(b), C.byref(size)))
+        return (lo.value, a, b, size.value) if details else lo.value
+
+    def tfim_lanczos(self, sites: int, details: bool = False):
+        """The DMRG TFIM Lanczos (H = -sum Z_i Z_{i+1} - sum X_i, open chain),
+        matrix-free on the device; with details, (lowest, alpha, beta, m)."""
+        lo = C.c_double()
+        a, b = np.zeros(100), np.zeros(100)
+        size = C.c_uint64()
+        check(lib().qf_tfim_lanczos(self._h, int(sites), C.byref(lo), ptr(a), ptr(b), C.byref(size)))
+        return (lo.value, a, b, size.value) if details else lo.value
 
     def apply_rotation_chain(self, state: StateVector, gates: Sequence[Tuple[RotationAxis, int, float]]) -> None:
         """Updates `state` in place (CudaBackend::apply_rotation_chain)."""

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libqfhip.so")
 ARCH = os.environ.get("QF_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_plan.cpp", "qf_jit.cpp", "qf_shots.hip",
-           "qf_density.hip"]
+           "qf_density.hip", "qf_lanczos.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
