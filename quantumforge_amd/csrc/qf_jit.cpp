@@ -95,7 +95,8 @@ public:
         const char* R = dbl_ ? "double" : "float";
         o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << ") void " << name_
            << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots) {\n";
-        if (phases > 1) o_ << "  __shared__ C xb[" << (1u << M_) << "];\n";
+        plan_half_exchanges();
+        if (phases > 1) o_ << "  __shared__ C xb[" << (1u << (half_ ? M_ - 1 : M_)) << "];\n";
         o_ << "  typedef " << R << " R;\n";
         o_ << "  const unsigned t = threadIdx.x;\n";
         o_ << "  const unsigned long long b = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * "
@@ -163,6 +164,9 @@ private:
     int rb_[8] = {0};
     std::vector<int> reg_of_tile_, thr_of_tile_;
     std::vector<int> var_;
+    bool half_ = false;           // exchanges through a half-tile buffer
+    std::vector<int> half_bit_;   // the round bit of each exchange
+    size_t xi_ = 0;               // exchanges emitted so far
     std::ostringstream o_;
     int opn_ = 0;
     // Diagonal factor accumulators.  acc_[0]: the product of the diagonal
@@ -214,17 +218,41 @@ private:
     void set_phase(const TileOp& hdr) {
         const uint32_t packed = uint32_t(hdr.q0);
         reg_of_tile_.assign(M_, -1);
-        thr_of_tile_.assign(M_, -1);
         for (uint32_t i = 0; i < RB_; ++i) {
             rb_[i] = int((packed >> (4 * i)) & 15u);
             reg_of_tile_[rb_[i]] = int(i);
         }
-        int u = 0;
-        for (uint32_t tb = 0; tb < M_; ++tb)
-            if (reg_of_tile_[tb] < 0) thr_of_tile_[tb] = u++;
+        thr_of_tile_ = assign_threads(reg_of_tile_);
     }
 
-    // Thread bits go, in order, to the tile bits that are not register bits.
+    // Thread bits of a phase: the top tile positions M-1, M-2, M-3 keep the
+    // wave-level thread bits T-1, T-2, T-3 (T = M - RB thread bits, those
+    // above 6) whenever they are not register bits, so that an exchange finds
+    // a wave bit with the same meaning before and after it; the other tile
+    // positions take the remaining thread bits in increasing order, so the
+    // lane bits stay on the lowest (contiguous) positions.
+    std::vector<int> assign_threads(const std::vector<int>& reg) const {
+        const int T = int(M_ - RB_);
+        const int nres = std::max(0, std::min(3, T - 6));
+        std::vector<int> thr(M_, -1);
+        std::vector<char> taken(T, 0);
+        for (int i = 0; i < nres; ++i) {
+            const uint32_t pos = M_ - 1 - uint32_t(i);
+            if (reg[pos] < 0) {
+                thr[pos] = T - 1 - i;
+                taken[T - 1 - i] = 1;
+            }
+        }
+        int u = 0;
+        for (uint32_t tb = 0; tb < M_; ++tb) {
+            if (reg[tb] >= 0 || thr[tb] >= 0) continue;
+            while (taken[u]) ++u;
+            thr[tb] = u++;
+        }
+        return thr;
+    }
+
+    // Thread t's tile index from the phase's thread-bit map (assign_threads).
     // Runs of consecutive thread bits landing on consecutive positions are one
     // shift-and-mask each.  `global` selects global qubit positions instead of
     // tile bit positions.
@@ -317,7 +345,56 @@ private:
         for (uint32_t b = 0; b < RB_; ++b) acc_flush_bit(b);
     }
 
+    // Thread-bit maps of the phases, for the exchange planning below.
+    std::vector<int> thread_map(const TileOp& hdr) const {
+        std::vector<int> reg(M_, -1);
+        for (uint32_t i = 0; i < RB_; ++i) reg[(uint32_t(hdr.q0) >> (4 * i)) & 15u] = int(i);
+        return assign_threads(reg);
+    }
+
+    // Half-size exchanges: an exchange can go through a buffer of half the
+    // tile in two rounds when some tile bit c is the same thread bit in the
+    // phase before and the phase after: in round h the threads whose bit c is
+    // h write their amplitudes (all with bit c = h) and read back theirs, each
+    // thread reading only what it will hold, so no register is overwritten
+    // before it is written out.  A wave-level bit (thread bit >= 6) keeps
+    // whole waves together.  The kernel then needs half the LDS, and more
+    // workgroups fit on a CU.  Used when every exchange of the pass has such
+    // a bit.
+    void plan_half_exchanges() {
+        half_ = false;
+        half_bit_.clear();
+        if (M_ - RB_ <= 6 || M_ < 11) return;
+        const char* env = std::getenv("QF_JIT_HALF_LDS");
+        if (env && env[0] == '0') return;
+        const TileOp* prev = nullptr;
+        for (const TileOp& op : p_.ops) {
+            if (op.kind != TOP_PHASE) continue;
+            if (prev) {
+                const std::vector<int> a = thread_map(*prev), b = thread_map(op);
+                // The highest thread bit with the same meaning on both sides; a
+                // lane bit (< 6) splits every wave into two rounds, which costs
+                // twice the LDS instructions of that exchange but keeps the
+                // half buffer.
+                int c = -1, best = -1;
+                for (int tb = int(M_) - 1; tb >= 0; --tb)
+                    if (a[tb] >= 0 && a[tb] == b[tb] && a[tb] > best) {
+                        best = a[tb];
+                        c = tb;
+                    }
+                if (c < 0) return;
+                half_bit_.push_back(c);
+            }
+            prev = &op;
+        }
+        half_ = !half_bit_.empty();
+    }
+
     void exchange(const TileOp& hdr, bool lds_used) {
+        if (half_) {
+            exchange_half(hdr, lds_used, half_bit_.at(xi_++));
+            return;
+        }
         acc_flush_all();
         if (lds_used) o_ << "  __syncthreads();\n";
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
@@ -332,6 +409,43 @@ private:
         for (uint32_t k = 0; k < NR_; ++k)
             o_ << "    { const C v = xb[st ^ " << slot(reg_tile(k)) << "u]; r" << k << " = v.re; i" << k
                << " = v.im; }\n";
+        o_ << "  }\n";
+    }
+
+    // Tile index with bit c taken out (the half-buffer index).
+    static uint32_t drop_bit(uint32_t j, uint32_t c) { return (j & ((1u << c) - 1)) | ((j >> (c + 1)) << c); }
+    std::string drop_expr(const std::string& j, uint32_t c) const {
+        return "((" + j + " & " + std::to_string((1u << c) - 1) + "u) | ((" + j + " >> " + std::to_string(c + 1) +
+               ") << " + std::to_string(c) + "))";
+    }
+
+    void exchange_half(const TileOp& hdr, bool lds_used, int c) {
+        acc_flush_all();
+        const uint32_t cb = uint32_t(c);
+        std::vector<std::string> writes;
+        for (uint32_t k = 0; k < NR_; ++k)
+            writes.push_back("xb[sw ^ " + std::to_string(slot(drop_bit(reg_tile(k), cb))) + "u] = C{" + rv(k) + ", " +
+                             iv(k) + "};");
+        o_ << "  {\n    const unsigned jw = " << thread_scatter(false) << ";\n";
+        o_ << "    const unsigned dw = " << drop_expr("jw", cb) << ";\n";
+        o_ << "    const unsigned sw = " << slot_expr("dw") << ";\n";
+        set_phase(hdr);
+        for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
+        o_ << "    const unsigned jr = " << thread_scatter(false) << ";\n";
+        o_ << "    const unsigned dr = " << drop_expr("jr", cb) << ";\n";
+        o_ << "    const unsigned sr = " << slot_expr("dr") << ";\n";
+        o_ << "    const unsigned h = (jw >> " << cb << ") & 1u;\n";
+        for (int round = 0; round < 2; ++round) {
+            if (lds_used || round > 0) o_ << "    __syncthreads();\n";
+            o_ << "    if (h == " << round << "u) {\n";
+            for (const std::string& w : writes) o_ << "      " << w << "\n";
+            o_ << "    }\n    __syncthreads();\n";
+            o_ << "    if (h == " << round << "u) {\n";
+            for (uint32_t k = 0; k < NR_; ++k)
+                o_ << "      { const C v = xb[sr ^ " << slot(drop_bit(reg_tile(k), cb)) << "u]; r" << k << " = v.re; i"
+                   << k << " = v.im; }\n";
+            o_ << "    }\n";
+        }
         o_ << "  }\n";
     }
 

@@ -34,7 +34,8 @@ def random_state(n, seed, count=1):
 
 # (qubits, depth, seed, tile_qubits, reg_bits): register sets of 2..5 bits,
 # 64..512 threads, tiles from 8 qubits to the whole state.
-SHAPES = [(12, 10, 1, 9, 3), (13, 12, 2, 10, 4), (12, 8, 5, 8, 2), (11, 12, 7, 11, 5)]
+SHAPES = [(12, 10, 1, 9, 3), (13, 12, 2, 10, 4), (12, 8, 5, 8, 2), (11, 12, 7, 11, 5), (13, 8, 11, 12, 3),
+          (14, 10, 12, 11, 3)]
 
 
 @pytest.mark.parametrize("n,depth,seed,M,RB", SHAPES)
@@ -43,6 +44,21 @@ def test_generated_kernels_match_oracle_on_random_circuits(n, depth, seed, M, RB
     plan = Plan(c, n, kernel=3, tile_qubits=M, reg_bits=RB)
     assert plan.compiled
     a0 = random_state(n, seed)[0]
+    st = O.State(n, a0)
+    st.run(c.to_ops())
+    close(emulate(plan, a0, n), st.amplitudes(), 1e-13)
+
+
+def test_half_buffer_exchanges_are_generated_and_exact():
+    # Tiles of 11+ qubits with more than 64 threads exchange through half the
+    # tile in two rounds; the emulated kernels must still equal the oracle,
+    # and switching the half buffer off must give the same amplitudes.
+    n = 14
+    c = random_layer_circuit(n, 12, seed=9, pairs_per_layer=7)
+    plan = Plan(c, n, kernel=3, tile_qubits=12, reg_bits=3)
+    src = plan.source()
+    assert "const unsigned h = " in src and "xb[2048]" in src
+    a0 = random_state(n, 9)[0]
     st = O.State(n, a0)
     st.run(c.to_ops())
     close(emulate(plan, a0, n), st.amplitudes(), 1e-13)
