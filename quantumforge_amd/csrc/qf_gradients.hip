@@ -321,6 +321,7 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
         po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
+        po.relabel = 0;  // the reverse sweep runs through the interpreting adjoint kernel
         e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
         if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     }
@@ -847,6 +848,7 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
                                                 std::to_string(inputs_len));
     qf::PlanOptions po = plan_options_from(options);
     po.fuse = 1;
+    if (reverse) po.relabel = 0;
     qf_plan* p = new qf_plan;
     const int precision = options ? options->precision : QF_C128;
     if (precision != QF_C128 && precision != QF_C64) {

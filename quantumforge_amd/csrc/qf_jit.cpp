@@ -95,8 +95,10 @@ public:
         const char* R = dbl_ ? "double" : "float";
         o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << ") void " << name_
            << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots) {\n";
+        relabel_ = !p_.store_phys.empty();
+        if (relabel_ && !check_store_layout()) return fail(err, "store layout is not a permutation of the tile bits");
         plan_half_exchanges();
-        if (phases > 1) o_ << "  __shared__ C xb[" << (1u << (half_ ? M_ - 1 : M_)) << "];\n";
+        if (phases > 1 || relabel_) o_ << "  __shared__ C xb[" << (1u << (half_ ? M_ - 1 : M_)) << "];\n";
         o_ << "  typedef " << R << " R;\n";
         o_ << "  const unsigned t = threadIdx.x;\n";
         o_ << "  const unsigned long long b = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * "
@@ -131,7 +133,7 @@ public:
             const TileOp& op = p_.ops[i];
             ++opn_;
             switch (op.kind) {
-                case TOP_PHASE: exchange(op, lds_used); lds_used = true; break;
+                case TOP_PHASE: exchange(layout_of(op), lds_used); lds_used = true; break;
                 case TOP_MAT1: mat1(op); break;
                 case TOP_DIAG: diag(op); break;
                 case TOP_CNOT: cnot(op); break;
@@ -143,9 +145,13 @@ public:
             }
         }
         acc_flush_all();
-        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_global() << ";\n";
+        // A relabeling pass first moves the tile into the store layout, whose
+        // lanes 0..low-1 carry the tile bits stored to index bits 0..low-1.
+        if (relabel_) exchange(store_layout(), lds_used);
+        const std::vector<uint32_t>& pos = relabel_ ? p_.store_phys : q_of_tile_;
+        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_scatter(true, pos) << ";\n";
         for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    sp[" << reg_global(k) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
+            o_ << "    sp[" << reg_global(k, pos) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
         o_ << "  }\n}\n";
         src = o_.str();
         return true;
@@ -154,6 +160,12 @@ public:
     std::string src;
 
 private:
+    // A phase layout: register index of each tile bit (or -1) and thread bit
+    // of each tile bit (or -1).
+    struct Layout {
+        std::vector<int> reg, thr;
+    };
+
     const Pass& p_;
     uint32_t n_;
     bool dbl_;
@@ -164,8 +176,18 @@ private:
     int rb_[8] = {0};
     std::vector<int> reg_of_tile_, thr_of_tile_;
     std::vector<int> var_;
+    bool relabel_ = false;        // the store permutes the tile's bits (Pass::store_phys)
     bool half_ = false;           // exchanges through a half-tile buffer
-    std::vector<int> half_bit_;   // the round bit of each exchange
+    // Half-buffer exchanges in emission order: the round bit, and for an
+    // exchange whose two layouts share no thread bit, an intermediate layout
+    // (and its round bit) it passes through.
+    struct HalfX {
+        int c = -1;
+        bool hop = false;
+        Layout mid;
+        int c_mid = -1;
+    };
+    std::vector<HalfX> half_x_;
     size_t xi_ = 0;               // exchanges emitted so far
     std::ostringstream o_;
     int opn_ = 0;
@@ -215,14 +237,81 @@ private:
         return s;
     }
 
-    void set_phase(const TileOp& hdr) {
-        const uint32_t packed = uint32_t(hdr.q0);
-        reg_of_tile_.assign(M_, -1);
-        for (uint32_t i = 0; i < RB_; ++i) {
-            rb_[i] = int((packed >> (4 * i)) & 15u);
-            reg_of_tile_[rb_[i]] = int(i);
+    Layout layout_of(const TileOp& hdr) const {
+        Layout L;
+        L.reg.assign(M_, -1);
+        for (uint32_t i = 0; i < RB_; ++i) L.reg[(uint32_t(hdr.q0) >> (4 * i)) & 15u] = int(i);
+        L.thr = assign_threads(L.reg);
+        return L;
+    }
+    void set_layout(const Layout& L) {
+        reg_of_tile_ = L.reg;
+        thr_of_tile_ = L.thr;
+        for (uint32_t tb = 0; tb < M_; ++tb)
+            if (L.reg[tb] >= 0) rb_[L.reg[tb]] = int(tb);
+    }
+    void set_phase(const TileOp& hdr) { set_layout(layout_of(hdr)); }
+
+    // The store layout of a relabeling pass: thread bits 0..low-1 on the tile
+    // bits stored to index bits 0..low-1 (so each run of 2^low lanes writes
+    // one contiguous run), the register and thread bits of the other tile
+    // bits those of the last phase where possible.
+    bool check_store_layout() const {
+        if (p_.store_phys.size() != M_) return false;
+        std::vector<uint32_t> a(p_.store_phys), b(q_of_tile_);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        if (a != b) return false;
+        for (uint32_t i = 0; i < p_.low; ++i)
+            if (std::find(p_.store_phys.begin(), p_.store_phys.end(), i) == p_.store_phys.end()) return false;
+        return true;
+    }
+    Layout store_layout() const {
+        const uint32_t low = p_.low;
+        std::vector<int> dest(low, -1);
+        std::vector<char> is_dest(M_, 0);
+        for (uint32_t t = 0; t < M_; ++t)
+            if (p_.store_phys[t] < low) {
+                dest[p_.store_phys[t]] = int(t);
+                is_dest[t] = 1;
+            }
+        const TileOp* last = nullptr;
+        for (const TileOp& op : p_.ops)
+            if (op.kind == TOP_PHASE) last = &op;
+        Layout L;
+        L.reg.assign(M_, -1);
+        uint32_t nreg = 0;
+        const Layout prev = layout_of(*last);
+        for (uint32_t i = 0; i < RB_; ++i)
+            for (uint32_t t = 0; t < M_; ++t)
+                if (prev.reg[t] == int(i) && !is_dest[t]) L.reg[t] = int(nreg++);
+        // A register bit given up to the store's lanes is replaced by a tile
+        // bit whose thread bit the lanes take anyway (else the highest free).
+        for (uint32_t t = 0; t < M_ && nreg < RB_; ++t)
+            if (L.reg[t] < 0 && !is_dest[t] && prev.thr[t] >= 0 && prev.thr[t] < int(low)) L.reg[t] = int(nreg++);
+        for (uint32_t t = M_; t-- > 0 && nreg < RB_;)
+            if (L.reg[t] < 0 && !is_dest[t]) L.reg[t] = int(nreg++);
+        const int T = int(M_ - RB_);
+        L.thr.assign(M_, -1);
+        std::vector<char> taken(T, 0);
+        for (uint32_t i = 0; i < low; ++i) {
+            L.thr[dest[i]] = int(i);
+            taken[i] = 1;
         }
-        thr_of_tile_ = assign_threads(reg_of_tile_);
+        // Every other tile bit keeps its thread bit from the last phase where
+        // it can, so the store exchange finds a common bit for the half buffer.
+        for (uint32_t t = 0; t < M_; ++t)
+            if (L.reg[t] < 0 && L.thr[t] < 0 && prev.thr[t] >= int(low) && !taken[prev.thr[t]]) {
+                L.thr[t] = prev.thr[t];
+                taken[prev.thr[t]] = 1;
+            }
+        int u = 0;
+        for (uint32_t t = 0; t < M_; ++t) {
+            if (L.reg[t] >= 0 || L.thr[t] >= 0) continue;
+            while (taken[u]) ++u;
+            L.thr[t] = u++;
+        }
+        return L;
     }
 
     // Thread bits of a phase: the top tile positions M-1, M-2, M-3 keep the
@@ -256,10 +345,12 @@ private:
     // Runs of consecutive thread bits landing on consecutive positions are one
     // shift-and-mask each.  `global` selects global qubit positions instead of
     // tile bit positions.
-    std::string thread_scatter(bool global) const {
+    std::string thread_scatter(bool global) const { return thread_scatter(global, q_of_tile_); }
+    std::string thread_scatter(bool global, const std::vector<uint32_t>& qpos) const {
         std::vector<std::pair<int, uint32_t>> pos;  // (thread bit, position)
         for (uint32_t tb = 0; tb < M_; ++tb)
-            if (thr_of_tile_[tb] >= 0) pos.push_back({thr_of_tile_[tb], global ? q_of_tile_[tb] : tb});
+            if (thr_of_tile_[tb] >= 0) pos.push_back({thr_of_tile_[tb], global ? qpos[tb] : tb});
+        std::sort(pos.begin(), pos.end());
         std::string s;
         size_t i = 0;
         while (i < pos.size()) {
@@ -277,10 +368,11 @@ private:
     }
     std::string thread_global() const { return thread_scatter(true); }
 
-    uint64_t reg_global(uint32_t k) const {
+    uint64_t reg_global(uint32_t k) const { return reg_global(k, q_of_tile_); }
+    uint64_t reg_global(uint32_t k, const std::vector<uint32_t>& qpos) const {
         uint64_t g = 0;
         for (uint32_t i = 0; i < RB_; ++i)
-            if ((k >> i) & 1) g |= uint64_t(1) << q_of_tile_[rb_[i]];
+            if ((k >> i) & 1) g |= uint64_t(1) << qpos[rb_[i]];
         return g;
     }
     uint32_t reg_tile(uint32_t k) const {
@@ -345,13 +437,6 @@ private:
         for (uint32_t b = 0; b < RB_; ++b) acc_flush_bit(b);
     }
 
-    // Thread-bit maps of the phases, for the exchange planning below.
-    std::vector<int> thread_map(const TileOp& hdr) const {
-        std::vector<int> reg(M_, -1);
-        for (uint32_t i = 0; i < RB_; ++i) reg[(uint32_t(hdr.q0) >> (4 * i)) & 15u] = int(i);
-        return assign_threads(reg);
-    }
-
     // Half-size exchanges: an exchange can go through a buffer of half the
     // tile in two rounds when some tile bit c is the same thread bit in the
     // phase before and the phase after: in round h the threads whose bit c is
@@ -359,40 +444,70 @@ private:
     // thread reading only what it will hold, so no register is overwritten
     // before it is written out.  A wave-level bit (thread bit >= 6) keeps
     // whole waves together.  The kernel then needs half the LDS, and more
-    // workgroups fit on a CU.  Used when every exchange of the pass has such
-    // a bit.
+    // workgroups fit on a CU.  An exchange whose two layouts share no thread
+    // bit goes through an intermediate layout that shares one with each.
     void plan_half_exchanges() {
         half_ = false;
-        half_bit_.clear();
+        half_x_.clear();
         if (M_ - RB_ <= 6 || M_ < 11) return;
         const char* env = std::getenv("QF_JIT_HALF_LDS");
         if (env && env[0] == '0') return;
-        const TileOp* prev = nullptr;
-        for (const TileOp& op : p_.ops) {
-            if (op.kind != TOP_PHASE) continue;
-            if (prev) {
-                const std::vector<int> a = thread_map(*prev), b = thread_map(op);
-                // The highest thread bit with the same meaning on both sides; a
-                // lane bit (< 6) splits every wave into two rounds, which costs
-                // twice the LDS instructions of that exchange but keeps the
-                // half buffer.
-                int c = -1, best = -1;
-                for (int tb = int(M_) - 1; tb >= 0; --tb)
-                    if (a[tb] >= 0 && a[tb] == b[tb] && a[tb] > best) {
-                        best = a[tb];
-                        c = tb;
-                    }
+        std::vector<Layout> seq;
+        for (const TileOp& op : p_.ops)
+            if (op.kind == TOP_PHASE) seq.push_back(layout_of(op));
+        if (relabel_) seq.push_back(store_layout());
+        for (size_t x = 1; x < seq.size(); ++x) {
+            const Layout& a = seq[x - 1];
+            const Layout& b = seq[x];
+            HalfX h;
+            h.c = common_thread_bit(a.thr, b.thr);
+            if (h.c < 0) {
+                // Through an intermediate layout: b's, with the thread bits of
+                // b at tile bits c and d exchanged so that c (a thread bit in
+                // both a and b, the highest in a) keeps its thread bit from a.
+                int c = -1;
+                for (uint32_t t = 0; t < M_; ++t)
+                    if (a.thr[t] >= 0 && b.thr[t] >= 0 && (c < 0 || a.thr[t] > a.thr[c])) c = int(t);
                 if (c < 0) return;
-                half_bit_.push_back(c);
+                int d = -1;
+                for (uint32_t t = 0; t < M_; ++t)
+                    if (b.thr[t] == a.thr[c]) d = int(t);
+                if (d < 0) return;
+                h.mid = b;
+                h.mid.thr[c] = a.thr[c];
+                h.mid.thr[d] = b.thr[c];
+                h.hop = true;
+                h.c_mid = c;
+                h.c = common_thread_bit(h.mid.thr, b.thr);
+                if (h.c < 0) return;
             }
-            prev = &op;
+            half_x_.push_back(h);
         }
-        half_ = !half_bit_.empty();
+        half_ = !half_x_.empty();
     }
 
-    void exchange(const TileOp& hdr, bool lds_used) {
+    // The tile bit with the highest thread bit that is the same thread bit in
+    // both maps (-1: none).  A lane bit (< 6) splits every wave into two
+    // rounds, which costs twice the LDS instructions of that exchange but
+    // keeps the half buffer.
+    int common_thread_bit(const std::vector<int>& a, const std::vector<int>& b) const {
+        int c = -1, best = -1;
+        for (uint32_t tb = 0; tb < M_; ++tb)
+            if (a[tb] >= 0 && a[tb] == b[tb] && a[tb] > best) {
+                best = a[tb];
+                c = int(tb);
+            }
+        return c;
+    }
+
+    void exchange(const Layout& next, bool lds_used) {
         if (half_) {
-            exchange_half(hdr, lds_used, half_bit_.at(xi_++));
+            const HalfX& h = half_x_.at(xi_++);
+            if (h.hop) {
+                exchange_half(h.mid, lds_used, h.c_mid);
+                lds_used = true;
+            }
+            exchange_half(next, lds_used, h.c);
             return;
         }
         acc_flush_all();
@@ -402,7 +517,7 @@ private:
         for (uint32_t k = 0; k < NR_; ++k)
             o_ << "    xb[st ^ " << slot(reg_tile(k)) << "u] = C{" << rv(k) << ", " << iv(k) << "};\n";
         o_ << "  }\n  __syncthreads();\n";
-        set_phase(hdr);
+        set_layout(next);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
@@ -419,7 +534,7 @@ private:
                ") << " + std::to_string(c) + "))";
     }
 
-    void exchange_half(const TileOp& hdr, bool lds_used, int c) {
+    void exchange_half(const Layout& next, bool lds_used, int c) {
         acc_flush_all();
         const uint32_t cb = uint32_t(c);
         std::vector<std::string> writes;
@@ -429,7 +544,7 @@ private:
         o_ << "  {\n    const unsigned jw = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned dw = " << drop_expr("jw", cb) << ";\n";
         o_ << "    const unsigned sw = " << slot_expr("dw") << ";\n";
-        set_phase(hdr);
+        set_layout(next);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         o_ << "    const unsigned jr = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned dr = " << drop_expr("jr", cb) << ";\n";

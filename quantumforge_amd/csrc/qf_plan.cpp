@@ -14,6 +14,7 @@
 #include "qf_plan.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -532,7 +533,7 @@ std::string describe_plan(const Plan& plan) {
         s += "]";
     };
     s += "{\"qubits\":" + std::to_string(plan.qubits) + ",\"compiled\":" + std::to_string(plan.compiled) +
-         ",\"passes\":[";
+         ",\"relabeled\":" + std::to_string(plan.relabeled) + ",\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const Pass& p = plan.passes[i];
         if (i) s += ",";
@@ -540,6 +541,8 @@ std::string describe_plan(const Plan& plan) {
             s += "{\"kind\":\"tile\",\"tile_bits\":" + std::to_string(p.tile_bits) + ",\"reg_bits\":" +
                  std::to_string(p.reg_bits) + ",\"low\":" + std::to_string(p.low) + ",\"high\":[";
             for (size_t k = 0; k < p.high.size(); ++k) s += (k ? "," : "") + std::to_string(p.high[k]);
+            s += "],\"store\":[";
+            for (size_t k = 0; k < p.store_phys.size(); ++k) s += (k ? "," : "") + std::to_string(p.store_phys[k]);
             s += "],\"ops\":[";
             for (size_t k = 0; k < p.ops.size(); ++k) {
                 const TileOp& o = p.ops[k];
@@ -566,6 +569,145 @@ std::string describe_plan(const Plan& plan) {
          ",\"phases\":" + std::to_string(plan.stats.phases) + "}}";
     return s;
 }
+
+namespace {
+
+// Relabeling of compiled passes: set QF_PLAN_RELABEL=0 to keep the identity layout.
+bool relabel_enabled() {
+    const char* e = std::getenv("QF_PLAN_RELABEL");
+    return !(e && e[0] == '0');
+}
+
+// Low bits and their relabeling.  The low index bits 0..low-1 must be in every
+// tile (they make the 128-byte runs of a tile), so whichever qubits hold them
+// take tile slots in every pass.  A compiled pass may permute the qubits of its
+// tile among the tile's bits for free at its store, so the low bits ("slots")
+// are handed to qubits with pending work.  Invariant: slot i holds its own
+// qubit i, or a visitor v while qubit i waits on v's own bit; every other qubit
+// is on its own bit.  So at most 2 * low qubits are displaced, and one tile
+// always suffices to restore the layout.
+//
+// After a pass, a lookahead picks the next pass's tile without the slot
+// constraint; a slot whose holder the next tile does not want takes a wanted
+// qubit v' of this tile that sits on its own bit (most pending work first):
+//   slot i holds qubit i:          i -> bit v', v' -> slot i;
+//   slot i holds v, i on bit v:    v' -> slot i, v -> bit v, i -> bit v'
+//                                  (needs qubit i in this tile);
+//   v' = i (qubit i comes back):   i -> slot i, v -> bit v.
+// On the last pass the store restores every qubit it can reach to its own
+// bit.  Records the store layout in p.store_phys.
+void relabel_low(const Scheduler& sch, const std::vector<Node>& nodes, const std::vector<char>& in, uint32_t low,
+                 uint32_t M, const std::vector<uint32_t>& all_q, std::vector<uint32_t>& phys,
+                 std::vector<uint32_t>& at, Pass& p, bool last) {
+    const uint32_t n = uint32_t(phys.size());
+    std::vector<uint32_t> nphys(phys);
+    std::vector<char> tile(n, 0);
+    for (uint32_t b = 0; b < low; ++b) tile[b] = 1;
+    for (uint32_t b : p.high) tile[b] = 1;
+    if (last) {
+        // Move every displaced qubit whose slot-chain lies in this tile home.
+        for (uint32_t i = 0; i < low; ++i) {
+            const uint32_t v = at[i];
+            if (v != i && tile[phys[i]]) {
+                nphys[v] = v;
+                nphys[i] = i;
+            }
+        }
+    } else {
+        std::vector<char> next(n, 0);
+        if (!choose_set(sch, nodes, next, 0, M, all_q)) return;
+        std::vector<char> taken(n, 0);
+        for (uint32_t i = 0; i < low; ++i) {
+            const uint32_t v = at[i];
+            if (next[v]) continue;  // the holder is wanted: keep it
+            if (v != i && next[i] && in[i]) {  // bring qubit i back
+                nphys[i] = i;
+                nphys[v] = v;
+                continue;
+            }
+            if (v != i && !in[i]) continue;  // qubit i must move along: not reachable
+            uint32_t best = n;
+            size_t best_work = 0;
+            for (uint32_t q = low; q < n; ++q) {
+                if (!in[q] || !next[q] || taken[q] || phys[q] != q) continue;
+                const size_t work = sch.queue[q].size() - sch.head[q];
+                if (best == n || work > best_work) {
+                    best = q;
+                    best_work = work;
+                }
+            }
+            if (best == n) continue;
+            taken[best] = 1;
+            nphys[best] = i;
+            nphys[i] = best;  // qubit i waits on the visitor's own bit
+            if (v != i) nphys[v] = v;
+        }
+    }
+    bool same = true;
+    for (uint32_t q = 0; q < n; ++q) same = same && nphys[q] == phys[q];
+    if (same) return;
+    // Store layout of the pass: tile bit -> index bit after the store.
+    std::vector<uint32_t> tile_phys(M);
+    for (uint32_t b = 0; b < low; ++b) tile_phys[b] = b;
+    for (size_t k = 0; k < p.high.size(); ++k) tile_phys[low + k] = p.high[k];
+    p.store_phys.resize(M);
+    for (uint32_t t = 0; t < M; ++t) p.store_phys[t] = nphys[at[tile_phys[t]]];
+    phys = nphys;
+    for (uint32_t q = 0; q < n; ++q) at[phys[q]] = q;
+}
+
+// Appends a pass with no gates that moves every qubit back to its own bit
+// (when the relabels left some displaced): its tile is the low bits, the
+// displaced qubits' bits and, to make up M bits, further bits.
+std::string restore_layout(uint32_t qubits, uint32_t low, uint32_t M, uint32_t RB, std::vector<uint32_t>& phys,
+                           std::vector<uint32_t>& at, Plan& out) {
+    std::vector<char> pin(qubits, 0);
+    uint32_t count = low;
+    for (uint32_t b = 0; b < low; ++b) pin[b] = 1;
+    bool any = false;
+    for (uint32_t q = 0; q < qubits; ++q)
+        if (phys[q] != q) {
+            any = true;
+            for (uint32_t b : {phys[q], q})
+                if (!pin[b]) {
+                    pin[b] = 1;
+                    ++count;
+                }
+        }
+    if (!any) return "";
+    if (count > M) return "layout restore needs more than one tile";
+    for (uint32_t b = qubits; b-- > low && count < M;)
+        if (!pin[b]) {
+            pin[b] = 1;
+            ++count;
+        }
+    Pass p{};
+    p.kind = Pass::TILE;
+    p.tile_bits = M;
+    p.low = low;
+    p.reg_bits = RB;
+    for (uint32_t b = low; b < qubits; ++b)
+        if (pin[b]) p.high.push_back(b);
+    std::vector<uint32_t> tile_phys(M);
+    for (uint32_t b = 0; b < low; ++b) tile_phys[b] = b;
+    for (size_t k = 0; k < p.high.size(); ++k) tile_phys[low + k] = p.high[k];
+    p.store_phys.resize(M);
+    for (uint32_t t = 0; t < M; ++t) p.store_phys[t] = at[tile_phys[t]];  // each qubit to its own bit
+    TileOp hdr;
+    std::memset(&hdr, 0, sizeof(hdr));
+    hdr.kind = TOP_PHASE;
+    uint32_t packed = 0;
+    for (uint32_t i = 0; i < RB; ++i) packed |= (low + i) << (4 * i);
+    hdr.q0 = int32_t(packed);
+    hdr.q1 = int32_t(RB);
+    p.ops.push_back(hdr);
+    ++out.stats.phases;
+    out.passes.push_back(std::move(p));
+    for (uint32_t q = 0; q < qubits; ++q) phys[q] = at[q] = q;
+    return "";
+}
+
+}  // namespace
 
 uint32_t default_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 uint32_t default_low_bits(int precision) { return precision == 0 ? 5 : 6; }
@@ -641,10 +783,16 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     Scheduler sch(nodes, qubits, regk ? 1 : 0);
     std::vector<uint32_t> all_q(qubits);
     for (uint32_t q = 0; q < qubits; ++q) all_q[q] = q;
+    // Layout: phys[q] = the index bit that holds logical qubit q.  Compiled
+    // passes may relabel (see relabel_low below); other plans keep the
+    // identity.
+    const bool relabel = out.compiled && opt.relabel && low > 0 && M < qubits && relabel_enabled();
+    std::vector<uint32_t> phys(qubits), at(qubits);
+    for (uint32_t q = 0; q < qubits; ++q) phys[q] = at[q] = q;
     size_t done = 0;
     while (done < nodes.size()) {
         std::vector<char> in(qubits, 0);
-        for (uint32_t q = 0; q < low; ++q) in[q] = 1;
+        for (uint32_t b = 0; b < low; ++b) in[at[b]] = 1;
         if (!choose_set(sch, nodes, in, low, M, all_q)) return "planner made no progress";
         std::vector<int> order;
         sch.simulate(sch.head, in, &order);
@@ -654,24 +802,45 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
         p.kind = Pass::TILE;
         p.tile_bits = M;
         p.low = low;
+        // The pass in index-bit terms: tile bits low..M-1 are the held bits of
+        // the chosen qubits in increasing order.
+        std::vector<char> pin(qubits, 0);
+        for (uint32_t q = 0; q < qubits; ++q)
+            if (in[q]) pin[phys[q]] = 1;
         std::vector<int> tile_bit(qubits, -1);
-        for (uint32_t q = 0; q < low; ++q) tile_bit[q] = int(q);
+        for (uint32_t b = 0; b < low; ++b) tile_bit[b] = int(b);
         uint32_t k = low;
-        for (uint32_t q = low; q < qubits; ++q)
-            if (in[q]) {
-                p.high.push_back(q);
-                tile_bit[q] = int(k++);
+        for (uint32_t b = low; b < qubits; ++b)
+            if (pin[b]) {
+                p.high.push_back(b);
+                tile_bit[b] = int(k++);
             }
+        std::vector<Node> pn;
+        pn.reserve(order.size());
+        for (int id : order) {
+            Node x = nodes[size_t(id)];
+            for (int j = 0; j < x.nq; ++j) x.q[j] = phys[x.q[j]];
+            pn.push_back(x);
+        }
+        std::vector<int> porder(pn.size());
+        for (size_t i = 0; i < pn.size(); ++i) porder[i] = int(i);
         if (!regk) {
-            for (int id : order) p.ops.push_back(make_tile_op(nodes[id], tile_bit));
+            for (const Node& x : pn) p.ops.push_back(make_tile_op(x, tile_bit));
         } else {
             p.reg_bits = RB;
-            std::string e = build_phases(nodes, order, qubits, tile_bit, RB, p.ops, out.stats.phases);
+            std::string e = build_phases(pn, porder, qubits, tile_bit, RB, p.ops, out.stats.phases);
             if (!e.empty()) return e;
         }
+        if (relabel) relabel_low(sch, nodes, in, low, M, all_q, phys, at, p, done == nodes.size());
         out.stats.max_ops_per_pass = std::max<uint64_t>(out.stats.max_ops_per_pass, order.size());
         out.passes.push_back(std::move(p));
     }
+    if (relabel) {
+        std::string e = restore_layout(qubits, low, M, RB, phys, at, out);
+        if (!e.empty()) return e;
+    }
+    for (const Pass& p : out.passes)
+        if (!p.store_phys.empty()) ++out.relabeled;
     out.stats.passes = out.passes.size();
     return "";
 }

@@ -85,6 +85,10 @@ struct Pass {
     uint32_t low = 0;                  // contiguous low qubits
     std::vector<uint32_t> high;        // global qubit of tile bit low+k (ascending)
     std::vector<TileOp> ops;
+    // Compiled passes only: the index bit each tile bit is stored to (empty:
+    // the bit it was loaded from).  A permutation of the tile's bits; the
+    // qubits stored to bits 0..low-1 are the next pass's contiguous ones.
+    std::vector<uint32_t> store_phys;
 };
 
 struct PlanOptions {
@@ -95,6 +99,11 @@ struct PlanOptions {
     int kernel = 0;            // 0 = default, 1 = LDS op-by-op tile kernel, 2 = register kernel,
                                // 3 = compiled passes (register-kernel plan, one generated kernel per pass)
     uint32_t reg_bits = 0;     // register kernel: amplitudes per thread = 2^reg_bits (0 = default)
+    // Compiled passes may store qubits to other index bits than they were
+    // loaded from (the plan's last pass restores the layout); such a plan
+    // runs only as compiled kernels.  Off for plans that also run through
+    // the interpreting kernels (the adjoint sweep's reverse plans).
+    int relabel = 1;
 };
 
 // Circuits of at least this many qubits run compiled passes by default
@@ -124,6 +133,7 @@ struct Plan {
     std::vector<Pass> passes;
     PlanStats stats;
     int compiled = 0;          // 1: register passes run as compiled kernels (2: explicitly requested)
+    int relabeled = 0;         // passes with a store permutation (Pass::store_phys): compiled only
     // Compiled programs per precision, attached on first launch (jit_state:
     // 0 not tried, 1 compiled, -1 the compile failed and the register kernel runs).
     mutable std::shared_ptr<JitProgram> jit[2];

@@ -408,7 +408,7 @@ qf_status plan_program(const qf::Plan& plan, int precision, const qf::JitProgram
         plan.jit[pr] = qf::jit_compile_plan(plan, pr, &err);
         plan.jit_state[pr] = plan.jit[pr] ? 1 : -1;
         if (!plan.jit[pr]) {
-            if (plan.compiled == 2) {
+            if (plan.compiled == 2 || plan.relabeled) {
                 plan.jit_state[pr] = 0;
                 return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
             }
@@ -464,6 +464,8 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
         if (s) return s;
         if (prog && (s = jit_functions(*prog, &fns))) return s;
     }
+    if (plan.relabeled && !prog)
+        return fail(QF_INVALID_ARGUMENT, "a plan that relabels qubits between passes runs only as compiled kernels");
     const qf::TileOp* cur = d_ops;
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const qf::Pass& p = plan.passes[i];

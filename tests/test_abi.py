@@ -167,7 +167,23 @@ def replay(desc, psi):
                     _mat2(psi, idx, bits[op["q0"]], bits[op["q1"]], m)
                 else:
                     raise AssertionError(f"unexpected op kind {k}")
+        if p.get("store"):
+            psi = _store_permute(psi, idx, bits, p["store"])
     return psi
+
+
+def _store_permute(psi, idx, bits, store):
+    """A relabeling pass's store: tile bit t (loaded from index bit bits[t])
+    is written to index bit store[t]; bits outside the tile stay put."""
+    assert sorted(store) == sorted(bits), "a store permutes the tile's own bits"
+    dst = idx.copy()
+    for b in bits:
+        dst &= ~(np.int64(1) << b)
+    for src, to in zip(bits, store):
+        dst |= ((idx >> src) & 1) << to
+    out = np.empty_like(psi)
+    out[dst] = psi
+    return out
 
 
 def check_tile_invariants(desc):
