@@ -604,6 +604,17 @@ void relabel_low(const Scheduler& sch, const std::vector<Node>& nodes, const std
     std::vector<char> tile(n, 0);
     for (uint32_t b = 0; b < low; ++b) tile[b] = 1;
     for (uint32_t b : p.high) tile[b] = 1;
+    std::vector<char> next(n, 0);
+    if (!last) {
+        if (!choose_set(sch, nodes, next, 0, M, all_q)) return;
+        // When the next pass can finish the circuit, restore instead of
+        // displacing more qubits, so that it needs no restore pass after it.
+        std::vector<size_t> h = sch.head;
+        sch.simulate(h, next, nullptr);
+        bool all = true;
+        for (uint32_t q = 0; q < n; ++q) all = all && h[q] == sch.queue[q].size();
+        last = all;
+    }
     if (last) {
         // Move every displaced qubit whose slot-chain lies in this tile home.
         for (uint32_t i = 0; i < low; ++i) {
@@ -614,8 +625,6 @@ void relabel_low(const Scheduler& sch, const std::vector<Node>& nodes, const std
             }
         }
     } else {
-        std::vector<char> next(n, 0);
-        if (!choose_set(sch, nodes, next, 0, M, all_q)) return;
         std::vector<char> taken(n, 0);
         for (uint32_t i = 0; i < low; ++i) {
             const uint32_t v = at[i];
@@ -793,7 +802,30 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     while (done < nodes.size()) {
         std::vector<char> in(qubits, 0);
         for (uint32_t b = 0; b < low; ++b) in[at[b]] = 1;
-        if (!choose_set(sch, nodes, in, low, M, all_q)) return "planner made no progress";
+        bool chosen = false;
+        if (relabel) {
+            // When the rest of the circuit fits one pass that also holds the
+            // displaced qubits, that pass restores the layout at its store
+            // and no restore pass is needed.
+            std::vector<char> fin(in);
+            uint32_t fsize = low;
+            for (uint32_t q = 0; q < qubits; ++q)
+                if (phys[q] != q && !fin[q]) {
+                    fin[q] = 1;
+                    ++fsize;
+                }
+            if (fsize > low && fsize <= M && choose_set(sch, nodes, fin, fsize, M, all_q)) {
+                std::vector<size_t> h = sch.head;
+                sch.simulate(h, fin, nullptr);
+                bool all = true;
+                for (uint32_t q = 0; q < qubits; ++q) all = all && h[q] == sch.queue[q].size();
+                if (all) {
+                    in = fin;
+                    chosen = true;
+                }
+            }
+        }
+        if (!chosen && !choose_set(sch, nodes, in, low, M, all_q)) return "planner made no progress";
         std::vector<int> order;
         sch.simulate(sch.head, in, &order);
         if (order.empty()) return "planner made no progress";

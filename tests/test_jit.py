@@ -124,7 +124,7 @@ def test_exchanges_without_a_common_thread_bit_hop_through_an_intermediate_layou
     exchanges = d["stats"]["phases"] - len(d["passes"]) + d["relabeled"]
     src = plan.source()
     assert src.count("const unsigned h = ") > exchanges
-    assert src.count("xb[1024]") == src.count("__global__")
+    assert "xb[1024]" in src and "xb[2048]" not in src
     a0 = random_state(n, 8)[0]
     st = O.State(n, a0)
     st.run(c.to_ops())
