@@ -144,6 +144,8 @@ public:
                 default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
             }
         }
+        if (g_ != 1.0) acc_mul_lit(acc_[0], g_, 0.0);
+        g_ = 1.0;
         acc_flush_all();
         // A relabeling pass first moves the tile into the store layout, whose
         // lanes 0..low-1 carry the tile bits stored to index bits 0..low-1.
@@ -176,6 +178,7 @@ private:
     int rb_[8] = {0};
     std::vector<int> reg_of_tile_, thr_of_tile_;
     std::vector<int> var_;
+    double g_ = 1.0;              // the tile holds g_ times its amplitudes (take_scale)
     bool relabel_ = false;        // the store permutes the tile's bits (Pass::store_phys)
     bool half_ = false;           // exchanges through a half-tile buffer
     // Half-buffer exchanges in emission order: the round bit, and for an
@@ -605,9 +608,44 @@ private:
 
     void swap_vars(uint32_t a, uint32_t b) { std::swap(var_[a], var_[b]); }
 
+    // Literal dense ops carry a real scale out of their matrix: the matrix
+    // divided by one of its entries' magnitudes sigma has entries of
+    // magnitude 1, which need no multiply (Rx and Ry become one FMA per
+    // output component instead of a multiply and an FMA).  The tile then
+    // holds g times the true amplitudes; g is global to the tile (every
+    // dense op touches every amplitude once, everything else is linear), so
+    // the next literal dense op takes it into its matrix and the store
+    // applies what is left.
+    double take_scale(double* m, int count) {
+        if (g_ != 1.0) {
+            for (int x = 0; x < count; ++x) m[x] *= g_;
+            g_ = 1.0;
+        }
+        double best = 0;
+        int best_n = 0;
+        for (int x = 0; x < count; ++x) {
+            const double a = std::fabs(m[x]);
+            if (a == 0 || !std::isfinite(a)) continue;
+            int n = 0;
+            for (int y = 0; y < count; ++y) n += std::fabs(m[y]) == a;
+            if (n > best_n || (n == best_n && a > best)) {
+                best = a;
+                best_n = n;
+            }
+        }
+        if (best_n == 0 || best == 1.0 || !std::isnormal(best)) return 1.0;
+        for (int x = 0; x < count; ++x) m[x] /= best;
+        return best;
+    }
+
     void mat1(const TileOp& op) {
         const uint32_t i = uint32_t(op.q0);
-        const double* m = op.m;
+        double m[8];
+        std::memcpy(m, op.m, sizeof m);
+        if (g_ != 1.0) {
+            for (double& x : m) x *= g_;
+            g_ = 1.0;
+        }
         const bool offz = m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0;
         const bool diagz = m[0] == 0 && m[1] == 0 && m[6] == 0 && m[7] == 0;
         if (offz) {
@@ -625,6 +663,7 @@ private:
                 else mul_lit(k, m[2], m[3]);
             return;
         }
+        g_ = take_scale(m, 8);
         for (uint32_t k = 0; k < NR_; ++k) {
             if ((k >> i) & 1) continue;
             const uint32_t k1 = k | (1u << i);
@@ -652,7 +691,9 @@ private:
         const uint32_t i = uint32_t(op.q0), j = uint32_t(op.q1);
         acc_flush_bit(i);
         acc_flush_bit(j);
-        const double* m = op.m;
+        double m[32];
+        std::memcpy(m, op.m, sizeof m);
+        g_ = take_scale(m, 32);
         for (uint32_t k = 0; k < NR_; ++k) {
             if (((k >> i) & 1) || ((k >> j) & 1)) continue;
             const uint32_t idx[4] = {k, k | (1u << i), k | (1u << j), k | (1u << i) | (1u << j)};
