@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <sstream>
 #include <thread>
@@ -384,18 +385,107 @@ private:
             if ((k >> i) & 1) j |= 1u << rb_[i];
         return j;
     }
-    // LDS slot swizzle: XOR-folds the higher index bits into the bits that
-    // select the bank row (256 bytes), linear over XOR so slot(a ^ b) =
-    // slot(a) ^ slot(b).
-    uint32_t swz_bits() const { return dbl_ ? 4 : 5; }
-    uint32_t slot(uint32_t j) const {
-        const uint32_t B = swz_bits(), m = (1u << B) - 1;
-        return j ^ (((j >> B) ^ (j >> (2 * B))) & m);
+    // Exchange addressing.  The shared array holds the tile (in a half
+    // exchange, the half with the round bit taken out) at element address
+    // a(j) = j ^ (sum over index bits p >= nb of bit p of j times h[p]).  The
+    // low nb bits of the address pick the bank slot (nb = 4 for 16-byte
+    // elements, 5 for 8-byte ones: a 256-byte row of banks), and a() is a
+    // bijection, as only low bits change, by a function of high bits.  Each
+    // exchange picks its own h so that the lanes the LDS serves together
+    // (lane bits 0..nb-1) hit distinct bank slots on both sides: the index
+    // bits those lanes carry before and after the exchange must map to
+    // independent vectors.  a() is linear, so a thread's base address and
+    // each register's offset are computed apart and combined with one XOR.
+    struct Swz {
+        std::vector<uint32_t> h;
+    };
+    uint32_t nb() const { return dbl_ ? 4 : 5; }
+    static uint32_t swz_apply(const Swz& z, uint32_t j) {
+        uint32_t a = j;
+        for (size_t p = 0; p < z.h.size(); ++p)
+            if ((j >> p) & 1) a ^= z.h[p];
+        return a;
     }
-    std::string slot_expr(const std::string& j) const {
-        const uint32_t B = swz_bits();
-        return j + " ^ (((" + j + " >> " + std::to_string(B) + ") ^ (" + j + " >> " + std::to_string(2 * B) +
-               ")) & " + std::to_string((1u << B) - 1) + "u)";
+    static std::string swz_expr(const Swz& z, const std::string& j) {
+        std::string s = j;
+        for (size_t p = 0; p < z.h.size(); ++p) {
+            const uint32_t v = z.h[p];
+            if (!v) continue;
+            if ((v & (v - 1)) == 0) {  // one bit b < p: move bit p of j down to b
+                uint32_t b = 0;
+                while (!((v >> b) & 1)) ++b;
+                s += " ^ ((" + j + " >> " + std::to_string(p - b) + ") & " + std::to_string(v) + "u)";
+            } else {
+                s += " ^ ((0u - ((" + j + " >> " + std::to_string(p) + ") & 1u)) & " + std::to_string(v) + "u)";
+            }
+        }
+        return "(" + s + ")";
+    }
+    // Index bits (after taking out bit c; c < 0: none) carried by lane bits
+    // 0..nb-1 of a thread map.  A lane bit on the round bit c is constant
+    // among the lanes active in a round and is left out.
+    std::vector<int> lane_bits(const std::vector<int>& thr, int c) const {
+        std::vector<int> pos;
+        for (uint32_t u = 0; u < nb(); ++u)
+            for (uint32_t t = 0; t < M_; ++t)
+                if (thr[t] == int(u) && int(t) != c) pos.push_back(c >= 0 && int(t) > c ? int(t) - 1 : int(t));
+        return pos;
+    }
+    // Rank over GF(2) of the bank-slot vectors of index bits `ps` (low `bits`
+    // bits of each).
+    static uint32_t rank_of(const Swz& z, const std::vector<int>& ps, uint32_t n, uint32_t bits) {
+        std::vector<uint32_t> basis;
+        for (int p : ps) {
+            uint32_t v = (p < int(n) ? (1u << p) : z.h[size_t(p)]) & ((1u << bits) - 1);
+            for (uint32_t b : basis)
+                if ((v ^ b) < v) v ^= b;
+            if (v) {
+                basis.push_back(v);
+                std::sort(basis.begin(), basis.end(), [](uint32_t x, uint32_t y) { return x > y; });
+            }
+        }
+        return uint32_t(basis.size());
+    }
+    // h for one exchange between lane bit sets w (write side) and r (read
+    // side) of a buffer of `bits` index bits.  Full rank on both sides is
+    // required; also asking it of the first nb-1 lanes over nb-1 bank bits
+    // (LDS that serves half as many lanes at once) is preferred; single-bit
+    // vectors are tried first (two instructions each).  No solution: h = 0.
+    Swz choose_swizzle(const std::vector<int>& w, const std::vector<int>& r, uint32_t bits) const {
+        const uint32_t n = nb();
+        Swz z;
+        z.h.assign(bits, 0);
+        std::vector<int> free;
+        for (const std::vector<int>* side : {&w, &r})
+            for (int p : *side)
+                if (p >= int(n) && std::find(free.begin(), free.end(), p) == free.end()) free.push_back(p);
+        std::vector<uint32_t> cand;
+        for (uint32_t b = 0; b < n; ++b) cand.push_back(1u << b);
+        for (uint32_t v = 1; v < (1u << n); ++v)
+            if (v & (v - 1)) cand.push_back(v);
+        auto ok = [&](bool strict) {
+            for (const std::vector<int>* side : {&w, &r}) {
+                if (rank_of(z, *side, n, n) < side->size()) return false;
+                if (strict) {
+                    std::vector<int> sub;
+                    for (int p : *side)
+                        if (sub.size() + 1 < n) sub.push_back(p);
+                    if (rank_of(z, sub, n, n - 1) < sub.size()) return false;
+                }
+            }
+            return true;
+        };
+        std::function<bool(size_t, bool)> search = [&](size_t i, bool strict) {
+            if (i == free.size()) return ok(strict);
+            for (uint32_t v : cand) {
+                z.h[size_t(free[i])] = v;
+                if (search(i + 1, strict)) return true;
+            }
+            z.h[size_t(free[i])] = 0;
+            return false;
+        };
+        if (!search(0, true) && !search(0, false)) z.h.assign(bits, 0);
+        return z;
     }
 
     // a *= (fr + i fi) (fi empty: real factor).
@@ -514,20 +604,29 @@ private:
             return;
         }
         acc_flush_all();
+        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, -1), lane_bits(next.thr, -1), M_);
         if (lds_used) o_ << "  __syncthreads();\n";
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
-        o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
+        o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
         for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    xb[st ^ " << slot(reg_tile(k)) << "u] = C{" << rv(k) << ", " << iv(k) << "};\n";
+            o_ << "    " << lds_at("st", swz_apply(z, reg_tile(k))) << " = C{" << rv(k) << ", " << iv(k) << "};\n";
         o_ << "  }\n  __syncthreads();\n";
         set_layout(next);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
-        o_ << "    const unsigned st = " << slot_expr("jt") << ";\n";
+        o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
         for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    { const C v = xb[st ^ " << slot(reg_tile(k)) << "u]; r" << k << " = v.re; i" << k
+            o_ << "    { const C v = " << lds_at("st", swz_apply(z, reg_tile(k))) << "; r" << k << " = v.re; i" << k
                << " = v.im; }\n";
         o_ << "  }\n";
+    }
+
+    // The shared array's element at byte offset `base` ^ (element address
+    // a * element size): one XOR per access on a byte offset computed once
+    // per exchange.
+    uint32_t csz_log() const { return dbl_ ? 4 : 3; }
+    std::string lds_at(const std::string& base, uint32_t j) const {
+        return "(*(C*)((char*)xb + (" + base + " ^ " + std::to_string(j << csz_log()) + "u)))";
     }
 
     // Tile index with bit c taken out (the half-buffer index).
@@ -540,18 +639,19 @@ private:
     void exchange_half(const Layout& next, bool lds_used, int c) {
         acc_flush_all();
         const uint32_t cb = uint32_t(c);
+        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, c), lane_bits(next.thr, c), M_ - 1);
         std::vector<std::string> writes;
         for (uint32_t k = 0; k < NR_; ++k)
-            writes.push_back("xb[sw ^ " + std::to_string(slot(drop_bit(reg_tile(k), cb))) + "u] = C{" + rv(k) + ", " +
-                             iv(k) + "};");
+            writes.push_back(lds_at("sw", swz_apply(z, drop_bit(reg_tile(k), cb))) + " = C{" + rv(k) + ", " + iv(k) +
+                             "};");
         o_ << "  {\n    const unsigned jw = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned dw = " << drop_expr("jw", cb) << ";\n";
-        o_ << "    const unsigned sw = " << slot_expr("dw") << ";\n";
+        o_ << "    const unsigned sw = " << swz_expr(z, "dw") << " << " << csz_log() << ";\n";
         set_layout(next);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         o_ << "    const unsigned jr = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned dr = " << drop_expr("jr", cb) << ";\n";
-        o_ << "    const unsigned sr = " << slot_expr("dr") << ";\n";
+        o_ << "    const unsigned sr = " << swz_expr(z, "dr") << " << " << csz_log() << ";\n";
         o_ << "    const unsigned h = (jw >> " << cb << ") & 1u;\n";
         for (int round = 0; round < 2; ++round) {
             if (lds_used || round > 0) o_ << "    __syncthreads();\n";
@@ -560,8 +660,8 @@ private:
             o_ << "    }\n    __syncthreads();\n";
             o_ << "    if (h == " << round << "u) {\n";
             for (uint32_t k = 0; k < NR_; ++k)
-                o_ << "      { const C v = xb[sr ^ " << slot(drop_bit(reg_tile(k), cb)) << "u]; r" << k << " = v.re; i"
-                   << k << " = v.im; }\n";
+                o_ << "      { const C v = " << lds_at("sr", swz_apply(z, drop_bit(reg_tile(k), cb))) << "; r" << k
+                   << " = v.re; i" << k << " = v.im; }\n";
             o_ << "    }\n";
         }
         o_ << "  }\n";
