@@ -190,6 +190,11 @@ struct Scheduler {
     // 0: every qubit of a non-diagonal gate must be in the set;
     // 1: a CNOT needs only its target (the control bit is read from the index).
     int mode = 0;
+    // Tie break among qubits that let equally many gates run (choose_set):
+    // 0: most pending gates; 1: earliest pending gate; 2: earliest pending
+    // gate counted in blocks of `block` gates, then most pending gates.
+    int tie = 0;
+    int block = 1;
 
     Scheduler(const std::vector<Node>& n, uint32_t nq, int m = 0)
         : nodes(n), qubits(nq), queue(nq), head(nq, 0), mode(m) {
@@ -249,15 +254,20 @@ bool choose_set(const Scheduler& sch, const std::vector<Node>& nodes, std::vecto
     std::vector<size_t> h_base = sch.head;
     int base_score = sch.simulate(h_base, in, nullptr);
     while (size < target) {
-        int best = -1, best_score = -1, best_tie = -1;
+        int best = -1, best_score = -1, best_tie = std::numeric_limits<int>::min();
         for (uint32_t q : cand) {
             if (in[q]) continue;
             in[q] = 1;
             std::vector<size_t> h = sch.head;
             const int sc = sch.simulate(h, in, nullptr);
             in[q] = 0;
-            // Tie break: pending work on q, then the lowest index.
-            const int tie = int(sch.queue[q].size() - sch.head[q]);
+            // Tie break (Scheduler::tie), then the lowest index.
+            const bool pend = sch.head[q] < sch.queue[q].size();
+            const int work = int(sch.queue[q].size() - sch.head[q]);
+            const int first = pend ? sch.queue[q][sch.head[q]] : int(nodes.size());
+            int tie = work;
+            if (sch.tie == 1) tie = -first;
+            if (sch.tie == 2) tie = (int(nodes.size()) - first / sch.block) * 64 + std::min(63, work);
             if (sc > best_score || (sc == best_score && tie > best_tie)) {
                 best = int(q);
                 best_score = sc;
@@ -718,6 +728,11 @@ std::string restore_layout(uint32_t qubits, uint32_t low, uint32_t M, uint32_t R
 
 }  // namespace
 
+namespace {
+std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_t M, uint32_t low, uint32_t RB,
+                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out);
+}  // namespace
+
 uint32_t default_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 uint32_t default_low_bits(int precision) { return precision == 0 ? 5 : 6; }
 
@@ -789,7 +804,41 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     // gate outside the contiguous low block.
     if (M < qubits && low + 2 > M) low = M >= 2 ? M - 2 : 0;
 
+    // Compiled plans are built under several tie-break policies of the set
+    // choice and the one with the fewest passes (then phases) is kept:
+    // planning takes milliseconds, a pass over a large state takes as long.
+    const Plan base = out;
+    std::vector<std::pair<int, int>> policies = {{0, 1}};
+    if (out.compiled) policies = {{0, 1}, {1, 1}, {2, int(qubits)}, {2, int(qubits + qubits / 3)}};
+    std::string first_err;
+    bool have = false;
+    Plan best;
+    for (const auto& pol : policies) {
+        Plan trial = base;
+        std::string e = plan_passes(nodes, qubits, M, low, RB, regk, opt, pol.first, pol.second, trial);
+        if (!e.empty()) {
+            if (first_err.empty()) first_err = e;
+            continue;
+        }
+        if (!have || trial.passes.size() < best.passes.size() ||
+            (trial.passes.size() == best.passes.size() && trial.stats.phases < best.stats.phases)) {
+            best = std::move(trial);
+            have = true;
+        }
+    }
+    if (!have) return first_err;
+    out = std::move(best);
+    return "";
+}
+
+namespace {
+
+// The pass loop of build_plan under one tie-break policy of the set choice.
+std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_t M, uint32_t low, uint32_t RB,
+                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out) {
     Scheduler sch(nodes, qubits, regk ? 1 : 0);
+    sch.tie = tie;
+    sch.block = block > 0 ? block : 1;
     std::vector<uint32_t> all_q(qubits);
     for (uint32_t q = 0; q < qubits; ++q) all_q[q] = q;
     // Layout: phys[q] = the index bit that holds logical qubit q.  Compiled
@@ -876,5 +925,7 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     out.stats.passes = out.passes.size();
     return "";
 }
+
+}  // namespace
 
 }  // namespace qf
