@@ -539,6 +539,13 @@ qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops, uint32_t n
  * of 18+ qubits): one generated kernel per register-kernel tile pass, in pass
  * order.  Same buffer convention as qf_plan_describe. */
 qf_status qf_plan_source(const qf_plan* plan, int32_t precision, char* buf, size_t cap, size_t* needed);
+/* The same in the adjoint form the gradient engine launches for a reverse
+ * plan: each kernel also carries the costate lambda and writes, per tile, Im
+ * <lambda| G |psi> of every recorded rotation into the partial table (row =
+ * instance * 2^(n - tile) + tile block, column = occurrence slot).  For
+ * inspection; no counterpart in the reference, whose adjoint runs on the CPU
+ * (src/gradients.rs). */
+qf_status qf_plan_adjoint_source(const qf_plan* plan, int32_t precision, char* buf, size_t cap, size_t* needed);
 /* Compiles the plan's passes for gfx950 now (host only; otherwise the first
  * run compiles them).  compile_ms and from_cache may be NULL; from_cache is 1
  * when the code objects came from the on-disk cache (QF_JIT_CACHE, default

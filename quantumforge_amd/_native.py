@@ -137,6 +137,7 @@ SIGNATURES = {
     "qf_tfim_lanczos": (None, [P, u32, C.POINTER(dbl), P, P, C.POINTER(u64)]),
     "qf_plan_build_parameterized": (None, [P, sz, u32, u32, P, sz, u32, i32, C.POINTER(PlanOptions), PP]),
     "qf_plan_source": (None, [P, i32, C.c_char_p, sz, C.POINTER(sz)]),
+    "qf_plan_adjoint_source": (None, [P, i32, C.c_char_p, sz, C.POINTER(sz)]),
     "qf_plan_compile": (None, [P, i32, C.POINTER(dbl), C.POINTER(i32)]),
     "qf_plan_destroy": (None, [P]),
     "qf_gradient": (None, [P, P, sz, u32, u32, P, sz, P, sz, u32, P, sz, C.POINTER(GradOptions), P,

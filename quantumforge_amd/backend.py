@@ -88,13 +88,16 @@ class Plan:
         """Whether the register passes run as compiled kernels."""
         return bool(self.describe().get("compiled", 0))
 
-    def source(self, precision: str = "complex128") -> str:
-        """Generated source of the compiled passes (qf_plan_source)."""
+    def source(self, precision: str = "complex128", adjoint: bool = False) -> str:
+        """Generated source of the compiled passes (qf_plan_source); adjoint:
+        the two-state form the gradient engine launches for a reverse plan
+        (qf_plan_adjoint_source)."""
         pr = 0 if precision == "complex128" else 1
+        fn = lib().qf_plan_adjoint_source if adjoint else lib().qf_plan_source
         need = C.c_size_t()
-        check(lib().qf_plan_source(self._h, pr, None, 0, C.byref(need)))
+        check(fn(self._h, pr, None, 0, C.byref(need)))
         buf = C.create_string_buffer(need.value)
-        check(lib().qf_plan_source(self._h, pr, buf, need.value, C.byref(need)))
+        check(fn(self._h, pr, buf, need.value, C.byref(need)))
         return buf.value.decode()
 
     def compile(self, precision: str = "complex128") -> dict:

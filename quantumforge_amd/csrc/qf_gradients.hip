@@ -313,10 +313,14 @@ struct GradRun {
     uint32_t fwd_tile = 0, rev_tile = 0;
 };
 
-qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_rev) {
+qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_rev, uint64_t sets) {
     qf::PlanOptions po;
     po.fuse = 1;
     po.tile_qubits = tile ? tile : qf::default_tile_bits(R.precision);
+    // A batch of many small states streams as much memory per launch as one
+    // large state: compiled passes (the reverse sweep in their adjoint form)
+    // pay for their compile there too.
+    if ((sets << R.qubits) >= (uint64_t(1) << 22)) po.compile = 1;
     std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
@@ -678,11 +682,11 @@ qf_status run_strategy(GradRun& R, int32_t strategy, const double* inputs, const
     qf_status s;
     switch (strategy) {
         case QF_GRAD_ADJOINT:
-            if ((s = plan_for(R, inputs, o.tile_qubits, true))) return s;
+            if ((s = plan_for(R, inputs, o.tile_qubits, true, sets))) return s;
             return adjoint_sets(R, params, sets, o.chunk, energies, grads);
         case QF_GRAD_PARAMETER_SHIFT:
         case QF_GRAD_FINITE_DIFFERENCE:
-            if ((s = plan_for(R, inputs, o.tile_qubits, false))) return s;
+            if ((s = plan_for(R, inputs, o.tile_qubits, false, sets))) return s;
             return shift_sets(R, params, sets, o.chunk, strategy == QF_GRAD_FINITE_DIFFERENCE, o.fd_step, energies,
                               grads);
         case QF_GRAD_BACKPROP:
@@ -772,7 +776,7 @@ extern "C" qf_status qf_batch_expectation(qf_backend* be, const qf_pop* ops, siz
     for (size_t i = 0; i < nops; ++i)
         if (ops[i].kind == 6)
             return fail(QF_REQUIRES_DENSITY_MATRIX, "this circuit operation requires density-matrix simulation");
-    if ((s = plan_for(R, inputs, o.tile_qubits, false))) return s;
+    if ((s = plan_for(R, inputs, o.tile_qubits, false, sets))) return s;
     return adjoint_sets(R, params, sets, o.chunk, energies, nullptr);
 }
 
@@ -801,7 +805,7 @@ extern "C" qf_status qf_state_run_parameterized(qf_state* st, const qf_pop* ops,
     if (s) return s;
     if ((s = validate_ops(R.pc, qubits))) return s;
     if ((s = activate(be))) return s;
-    if ((s = plan_for(R, inputs, 0, false))) return s;
+    if ((s = plan_for(R, inputs, 0, false, 1))) return s;
     const uint32_t nslots = uint32_t(R.pc.slot_param.size());
     std::vector<double> row(std::max<size_t>(size_t(nslots) * 2, 2));
     fill_row(R.pc, params, -1, 0.0, row.data());

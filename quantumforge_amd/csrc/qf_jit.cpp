@@ -65,8 +65,11 @@ struct Bit {
 
 class PassGen {
 public:
-    PassGen(const Pass& p, uint32_t n, int precision, const std::string& name)
-        : p_(p), n_(n), dbl_(precision == 0), name_(name) {}
+    // ns = 2: the adjoint form, which carries a second state (the costate
+    // lambda) through the same ops and records Im <lambda| G |psi> for the
+    // TF_RECORD rotations before applying them.
+    PassGen(const Pass& p, uint32_t n, int precision, const std::string& name, int ns = 1)
+        : p_(p), n_(n), dbl_(precision == 0), name_(name), ns_(ns) {}
 
     bool run(std::string* err) {
         if (p_.kind != Pass::TILE || p_.reg_bits == 0 || p_.ops.empty() || p_.ops[0].kind != TOP_PHASE)
@@ -89,17 +92,23 @@ public:
         }
         bool any_rot = false;
         size_t phases = 0;
+        std::vector<int> rec_slots;
         for (const TileOp& op : p_.ops) {
             if (op.kind == TOP_ROT || op.kind == TOP_ROTZZ) any_rot = true;
             if (op.kind == TOP_PHASE) ++phases;
+            if (ns_ == 2 && (op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD))
+                rec_slots.push_back(op.slot);
         }
+        const uint32_t W = NT_ >= 64 ? NT_ / 64 : 1;
         const char* R = dbl_ ? "double" : "float";
         o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << ") void " << name_
-           << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots) {\n";
+           << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots"
+           << (ns_ == 2 ? ", C* __restrict__ lam, double* __restrict__ partial" : "") << ") {\n";
         relabel_ = !p_.store_phys.empty();
         if (relabel_ && !check_store_layout()) return fail(err, "store layout is not a permutation of the tile bits");
         plan_half_exchanges();
-        if (phases > 1 || relabel_) o_ << "  __shared__ C xb[" << (1u << (half_ ? M_ - 1 : M_)) << "];\n";
+        if (phases > 1 || relabel_) o_ << "  __shared__ C xb[" << ns_ * (1u << (half_ ? M_ - 1 : M_)) << "];\n";
+        if (!rec_slots.empty()) o_ << "  __shared__ double qred[" << rec_slots.size() * W << "];\n";
         o_ << "  typedef " << R << " R;\n";
         o_ << "  const unsigned t = threadIdx.x;\n";
         o_ << "  const unsigned long long b = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * "
@@ -110,6 +119,8 @@ public:
         if (any_rot) o_ << "  const double* __restrict__ csr = cs + inst * 2ull * nslots;\n";
         else o_ << "  (void)cs; (void)nslots;\n";
         for (uint32_t k = 0; k < NR_; ++k) o_ << "  R r" << k << ", i" << k << ";\n";
+        if (ns_ == 2)
+            for (uint32_t k = 0; k < NR_; ++k) o_ << "  R lr" << k << ", li" << k << ";\n";
         acc_.assign(1 + 2 * NR_, Acc{});
         acc_[0].r = "ar";
         acc_[0].i = "ai";
@@ -125,11 +136,15 @@ public:
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         set_phase(p_.ops[0]);
         // Load with the first phase's mapping.
-        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_global() << ";\n";
-        for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    { const C v = sp[" << reg_global(k) << "ull]; r" << k << " = v.re; i" << k << " = v.im; }\n";
-        o_ << "  }\n";
+        for (int st = 0; st < ns_; ++st) {
+            o_ << "  {\n    C* __restrict__ sp = " << (st ? "lam" : "s") << " + base + " << thread_global() << ";\n";
+            for (uint32_t k = 0; k < NR_; ++k)
+                o_ << "    { const C v = sp[" << reg_global(k) << "ull]; " << reg_name(st, 'r', k) << " = v.re; "
+                   << reg_name(st, 'i', k) << " = v.im; }\n";
+            o_ << "  }\n";
+        }
         bool lds_used = false;
+        uint32_t rec_index = 0;
         for (size_t i = 1; i < p_.ops.size(); ++i) {
             const TileOp& op = p_.ops[i];
             ++opn_;
@@ -140,8 +155,14 @@ public:
                 case TOP_CNOT: cnot(op); break;
                 case TOP_SWAP: swap(op); break;
                 case TOP_MAT2: mat2(op); break;
-                case TOP_ROT: rot(op); break;
-                case TOP_ROTZZ: rotzz(op); break;
+                case TOP_ROT:
+                    if (ns_ == 2 && (op.flags & TF_RECORD)) record(op, rec_index++, W);
+                    rot(op);
+                    break;
+                case TOP_ROTZZ:
+                    if (ns_ == 2 && (op.flags & TF_RECORD)) record(op, rec_index++, W);
+                    rotzz(op);
+                    break;
                 default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
             }
         }
@@ -152,10 +173,28 @@ public:
         // lanes 0..low-1 carry the tile bits stored to index bits 0..low-1.
         if (relabel_) exchange(store_layout(), lds_used);
         const std::vector<uint32_t>& pos = relabel_ ? p_.store_phys : q_of_tile_;
-        o_ << "  {\n    C* __restrict__ sp = s + base + " << thread_scatter(true, pos) << ";\n";
-        for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    sp[" << reg_global(k, pos) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
-        o_ << "  }\n}\n";
+        for (cur_ = 0; cur_ < ns_; ++cur_) {
+            o_ << "  {\n    C* __restrict__ sp = " << (cur_ ? "lam" : "s") << " + base + " << thread_scatter(true, pos)
+               << ";\n";
+            for (uint32_t k = 0; k < NR_; ++k)
+                o_ << "    sp[" << reg_global(k, pos) << "ull] = C{" << rv(k) << ", " << iv(k) << "};\n";
+            o_ << "  }\n";
+        }
+        cur_ = 0;
+        if (!rec_slots.empty()) {
+            // The tile's overlap for each recorded rotation: its waves' sums
+            // added in wave order, into the block's row of the partial table.
+            o_ << "  __syncthreads();\n  for (unsigned r = t; r < " << rec_slots.size() << "u; r += " << NT_
+               << "u) {\n    double q = 0.0;\n";
+            o_ << "    for (unsigned w = 0; w < " << W << "u; ++w) q += qred[r * " << W << "u + w];\n";
+            std::string sl = "0u";
+            for (size_t r = rec_slots.size(); r-- > 0;)
+                sl = r + 1 == rec_slots.size() ? std::to_string(rec_slots[r]) + "u"
+                                               : "(r == " + std::to_string(r) + "u ? " + std::to_string(rec_slots[r]) +
+                                                     "u : " + sl + ")";
+            o_ << "    partial[b * nslots + " << sl << "] = q;\n  }\n";
+        }
+        o_ << "}\n";
         src = o_.str();
         return true;
     }
@@ -173,6 +212,8 @@ private:
     uint32_t n_;
     bool dbl_;
     std::string name_;
+    int ns_ = 1;                  // states carried (2: psi and the costate lambda)
+    int cur_ = 0;                 // the state rv()/iv() name
     uint32_t M_ = 0, RB_ = 0, NR_ = 0, NT_ = 0, LOGB_ = 0;
     std::vector<int> tile_of_q_;
     std::vector<uint32_t> q_of_tile_;
@@ -219,8 +260,12 @@ private:
         s.pop_back();  // the caller appends the suffix
         return s;
     }
-    std::string rv(uint32_t k) const { return "r" + std::to_string(var_[k]); }
-    std::string iv(uint32_t k) const { return "i" + std::to_string(var_[k]); }
+    // Register variables: state 0 r<k>/i<k>, state 1 (lambda) lr<k>/li<k>.
+    static std::string reg_name(int st, char part, uint32_t k) {
+        return std::string(st ? "l" : "") + part + std::to_string(k);
+    }
+    std::string rv(uint32_t k) const { return reg_name(cur_, 'r', uint32_t(var_[k])); }
+    std::string iv(uint32_t k) const { return reg_name(cur_, 'i', uint32_t(var_[k])); }
 
     // Bits of the block index go, in order, to the qubits outside the tile.
     std::string block_scatter() const {
@@ -608,16 +653,21 @@ private:
         if (lds_used) o_ << "  __syncthreads();\n";
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
-        for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    " << lds_at("st", swz_apply(z, reg_tile(k))) << " = C{" << rv(k) << ", " << iv(k) << "};\n";
+        // Lambda's tile follows psi's in the shared array.
+        each_state([&] {
+            for (uint32_t k = 0; k < NR_; ++k)
+                o_ << "    " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(cur_) << M_)) << " = C{" << rv(k)
+                   << ", " << iv(k) << "};\n";
+        });
         o_ << "  }\n  __syncthreads();\n";
         set_layout(next);
         for (uint32_t k = 0; k < NR_; ++k) var_[k] = int(k);
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
-        for (uint32_t k = 0; k < NR_; ++k)
-            o_ << "    { const C v = " << lds_at("st", swz_apply(z, reg_tile(k))) << "; r" << k << " = v.re; i" << k
-               << " = v.im; }\n";
+        for (int sn = 0; sn < ns_; ++sn)
+            for (uint32_t k = 0; k < NR_; ++k)
+                o_ << "    { const C v = " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(sn) << M_)) << "; "
+                   << reg_name(sn, 'r', k) << " = v.re; " << reg_name(sn, 'i', k) << " = v.im; }\n";
         o_ << "  }\n";
     }
 
@@ -641,9 +691,11 @@ private:
         const uint32_t cb = uint32_t(c);
         const Swz z = choose_swizzle(lane_bits(thr_of_tile_, c), lane_bits(next.thr, c), M_ - 1);
         std::vector<std::string> writes;
-        for (uint32_t k = 0; k < NR_; ++k)
-            writes.push_back(lds_at("sw", swz_apply(z, drop_bit(reg_tile(k), cb))) + " = C{" + rv(k) + ", " + iv(k) +
-                             "};");
+        each_state([&] {
+            for (uint32_t k = 0; k < NR_; ++k)
+                writes.push_back(lds_at("sw", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(cur_) << (M_ - 1))) +
+                                 " = C{" + rv(k) + ", " + iv(k) + "};");
+        });
         o_ << "  {\n    const unsigned jw = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned dw = " << drop_expr("jw", cb) << ";\n";
         o_ << "    const unsigned sw = " << swz_expr(z, "dw") << " << " << csz_log() << ";\n";
@@ -659,9 +711,11 @@ private:
             for (const std::string& w : writes) o_ << "      " << w << "\n";
             o_ << "    }\n    __syncthreads();\n";
             o_ << "    if (h == " << round << "u) {\n";
-            for (uint32_t k = 0; k < NR_; ++k)
-                o_ << "      { const C v = " << lds_at("sr", swz_apply(z, drop_bit(reg_tile(k), cb))) << "; r" << k
-                   << " = v.re; i" << k << " = v.im; }\n";
+            for (int sn = 0; sn < ns_; ++sn)
+                for (uint32_t k = 0; k < NR_; ++k)
+                    o_ << "      { const C v = "
+                       << lds_at("sr", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(sn) << (M_ - 1))) << "; "
+                       << reg_name(sn, 'r', k) << " = v.re; " << reg_name(sn, 'i', k) << " = v.im; }\n";
             o_ << "    }\n";
         }
         o_ << "  }\n";
@@ -678,8 +732,19 @@ private:
         return "((unsigned)(base >> " + std::to_string(b.idx) + ") & 1u)";
     }
 
-    // a_k *= (dr + i di), literal.
+    // Emits f() once per carried state (rv()/iv() name that state's
+    // registers).
+    template <class F>
+    void each_state(F f) {
+        for (cur_ = 0; cur_ < ns_; ++cur_) f();
+        cur_ = 0;
+    }
+
+    // a_k *= (dr + i di), literal, in every state.
     void mul_lit(uint32_t k, double dr, double di) {
+        each_state([&] { mul_lit1(k, dr, di); });
+    }
+    void mul_lit1(uint32_t k, double dr, double di) {
         const std::string r = rv(k), i = iv(k);
         if (dr == 1.0 && di == 0.0) return;
         if (di == 0.0) {
@@ -695,8 +760,12 @@ private:
         o_ << "  { const R x = " << r << "; " << r << " = " << lincomb({lit_term(dr, r), lit_term(-di, i)}) << "; "
            << i << " = " << lincomb({lit_term(dr, i), lit_term(di, "x")}) << "; }\n";
     }
-    // a_k *= (DR + i DI), run-time values (DI empty: real factor).
+    // a_k *= (DR + i DI), run-time values (DI empty: real factor), in every
+    // state.
     void mul_rt(uint32_t k, const std::string& DR, const std::string& DI, bool neg_im = false) {
+        each_state([&] { mul_rt1(k, DR, DI, neg_im); });
+    }
+    void mul_rt1(uint32_t k, const std::string& DR, const std::string& DI, bool neg_im) {
         const std::string r = rv(k), i = iv(k);
         if (DI.empty()) {
             o_ << "  " << r << " = " << DR << " * " << r << "; " << i << " = " << DR << " * " << i << ";\n";
@@ -764,24 +833,30 @@ private:
             return;
         }
         g_ = take_scale(m, 8);
-        for (uint32_t k = 0; k < NR_; ++k) {
-            if ((k >> i) & 1) continue;
-            const uint32_t k1 = k | (1u << i);
-            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
-            o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
-            o_ << "    " << rA << " = "
-               << lincomb({lit_term(m[0], "a0r"), lit_term(-m[1], "a0i"), lit_term(m[2], "a1r"), lit_term(-m[3], "a1i")})
-               << ";\n";
-            o_ << "    " << iA << " = "
-               << lincomb({lit_term(m[0], "a0i"), lit_term(m[1], "a0r"), lit_term(m[2], "a1i"), lit_term(m[3], "a1r")})
-               << ";\n";
-            o_ << "    " << rB << " = "
-               << lincomb({lit_term(m[4], "a0r"), lit_term(-m[5], "a0i"), lit_term(m[6], "a1r"), lit_term(-m[7], "a1i")})
-               << ";\n";
-            o_ << "    " << iB << " = "
-               << lincomb({lit_term(m[4], "a0i"), lit_term(m[5], "a0r"), lit_term(m[6], "a1i"), lit_term(m[7], "a1r")})
-               << "; }\n";
-        }
+        each_state([&] {
+            for (uint32_t k = 0; k < NR_; ++k) {
+                if ((k >> i) & 1) continue;
+                const uint32_t k1 = k | (1u << i);
+                const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+                o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
+                o_ << "    " << rA << " = "
+                   << lincomb({lit_term(m[0], "a0r"), lit_term(-m[1], "a0i"), lit_term(m[2], "a1r"),
+                               lit_term(-m[3], "a1i")})
+                   << ";\n";
+                o_ << "    " << iA << " = "
+                   << lincomb({lit_term(m[0], "a0i"), lit_term(m[1], "a0r"), lit_term(m[2], "a1i"),
+                               lit_term(m[3], "a1r")})
+                   << ";\n";
+                o_ << "    " << rB << " = "
+                   << lincomb({lit_term(m[4], "a0r"), lit_term(-m[5], "a0i"), lit_term(m[6], "a1r"),
+                               lit_term(-m[7], "a1i")})
+                   << ";\n";
+                o_ << "    " << iB << " = "
+                   << lincomb({lit_term(m[4], "a0i"), lit_term(m[5], "a0r"), lit_term(m[6], "a1i"),
+                               lit_term(m[7], "a1r")})
+                   << "; }\n";
+            }
+        });
     }
 
     // Dense 4x4 on register bits q0 < q1 over the index bit(q0) + 2 bit(q1),
@@ -794,6 +869,9 @@ private:
         double m[32];
         std::memcpy(m, op.m, sizeof m);
         g_ = take_scale(m, 32);
+        each_state([&] { mat2_regs(i, j, m); });
+    }
+    void mat2_regs(uint32_t i, uint32_t j, const double* m) {
         for (uint32_t k = 0; k < NR_; ++k) {
             if (((k >> i) & 1) || ((k >> j) & 1)) continue;
             const uint32_t idx[4] = {k, k | (1u << i), k | (1u << j), k | (1u << i) | (1u << j)};
@@ -937,14 +1015,16 @@ private:
         }
         const std::string cn = "x" + std::to_string(opn_);
         o_ << "  const bool " << cn << " = " << bit_expr(c) << ";\n";
-        for (uint32_t k = 0; k < NR_; ++k) {
-            if ((k >> ti) & 1) continue;
-            const uint32_t k1 = k | (1u << ti);
-            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
-            o_ << "  { const R ur = " << rA << ", ui = " << iA << "; " << rA << " = " << cn << " ? " << rB << " : "
-               << rA << "; " << iA << " = " << cn << " ? " << iB << " : " << iA << "; " << rB << " = " << cn
-               << " ? ur : " << rB << "; " << iB << " = " << cn << " ? ui : " << iB << "; }\n";
-        }
+        each_state([&] {
+            for (uint32_t k = 0; k < NR_; ++k) {
+                if ((k >> ti) & 1) continue;
+                const uint32_t k1 = k | (1u << ti);
+                const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+                o_ << "  { const R ur = " << rA << ", ui = " << iA << "; " << rA << " = " << cn << " ? " << rB
+                   << " : " << rA << "; " << iA << " = " << cn << " ? " << iB << " : " << iA << "; " << rB << " = "
+                   << cn << " ? ur : " << rB << "; " << iB << " = " << cn << " ? ui : " << iB << "; }\n";
+            }
+        });
     }
 
     void swap(const TileOp& op) {
@@ -964,6 +1044,51 @@ private:
            << "]);\n";
     }
 
+    // Im <lambda| G |psi> for a recorded rotation's generator G, before the
+    // rotation: each thread's share over its registers in FP64, summed over
+    // the wave with lane shuffles into the shared table qred (row r, one
+    // value per wave); the tile's sum goes out after the store.  Pending
+    // diagonal factors are phases common to both states and cancel in
+    // conj(lambda) psi, except across the pair of an X or Y generator's bit,
+    // whose factors are applied first.  The tile-wide scale g (take_scale)
+    // is in both states, so the share is divided by g^2.
+    void record(const TileOp& op, uint32_t r, uint32_t W) {
+        const std::string v = "q" + std::to_string(opn_);
+        auto val = [&](int st, char part, uint32_t k) { return "double(" + reg_name(st, part, uint32_t(var_[k])) + ")"; };
+        o_ << "  double " << v << " = 0.0;\n";
+        if (op.kind == TOP_ROT && op.q1 != 2) {
+            const uint32_t B = uint32_t(op.q0);
+            acc_flush_bit(B);
+            for (uint32_t k = 0; k < NR_; ++k) {
+                const uint32_t kp = k ^ (1u << B);
+                const std::string lr = val(1, 'r', k), li = val(1, 'i', k), pr = val(0, 'r', kp), pi = val(0, 'i', kp);
+                if (op.q1 == 0)  // X: Im(conj(l) p)
+                    o_ << "  " << v << " += " << lr << " * " << pi << " - " << li << " * " << pr << ";\n";
+                else  // Y: -+Re(conj(l) p) by the bit of l
+                    o_ << "  " << v << ((k >> B) & 1 ? " += " : " -= ") << "(" << lr << " * " << pr << " + " << li
+                       << " * " << pi << ");\n";
+            }
+        } else {
+            // Z (Z Z): Im(conj(l) p) with the sign of the generator's parity.
+            std::vector<Bit> bits{bit_of(op.q0)};
+            if (op.kind == TOP_ROTZZ) bits.push_back(bit_of(op.q1));
+            std::string rt;
+            for (const Bit& b : bits)
+                if (b.kind != Bit::REG) rt += (rt.empty() ? "" : " ^ ") + bit_expr(b);
+            for (uint32_t k = 0; k < NR_; ++k) {
+                int par = 0;
+                for (const Bit& b : bits)
+                    if (b.kind == Bit::REG) par ^= int((k >> b.idx) & 1);
+                o_ << "  " << v << (par ? " -= " : " += ") << "(" << val(1, 'r', k) << " * " << val(0, 'i', k) << " - "
+                   << val(1, 'i', k) << " * " << val(0, 'r', k) << ");\n";
+            }
+            if (!rt.empty()) o_ << "  if (" << rt << ") " << v << " = -" << v << ";\n";
+        }
+        if (g_ != 1.0) o_ << "  " << v << " *= " << hexlit(1.0 / (g_ * g_)).substr(1) << ";\n";
+        o_ << "  for (int m = 32; m >= 1; m >>= 1) " << v << " += __shfl_xor(" << v << ", m, 64);\n";
+        o_ << "  if ((t & 63u) == 0u) qred[" << r * W << "u + (t >> 6)] = " << v << ";\n";
+    }
+
     void rot(const TileOp& op) {
         std::string c, s;
         load_cs(op, c, s);
@@ -980,6 +1105,9 @@ private:
         }
         const uint32_t i = uint32_t(op.q0);
         acc_flush_bit(i);
+        each_state([&] { rot_xy_regs(op, i, c, s); });
+    }
+    void rot_xy_regs(const TileOp& op, uint32_t i, const std::string& c, const std::string& s) {
         for (uint32_t k = 0; k < NR_; ++k) {
             if ((k >> i) & 1) continue;
             const uint32_t k1 = k | (1u << i);
@@ -1120,24 +1248,26 @@ std::vector<std::string> jit_compile_options() {
 }
 
 std::string jit_pass_source(const Pass& p, uint32_t qubits, int precision, const std::string& name,
-                            std::string* err) {
-    PassGen g(p, qubits, precision, name);
+                            std::string* err, int ns) {
+    PassGen g(p, qubits, precision, name, ns);
     if (!g.run(err)) return "";
     return g.src;
 }
 
-std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, std::string* err) {
+std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, std::string* err, int ns) {
     // Generate every compiled pass with placeholder names; the source text
     // (with the compile options) is the cache key, and the final kernel names
     // carry the key.
     std::vector<std::string> bodies;
     prog->kernel_of_pass.assign(plan.passes.size(), -1);
     prog->threads.clear();
+    prog->states = ns;
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const Pass& p = plan.passes[i];
         if (p.kind != Pass::TILE || p.reg_bits == 0) continue;
         std::string e;
-        std::string body = jit_pass_source(p, plan.qubits, precision, "qf_pass_@" + std::to_string(bodies.size()), &e);
+        std::string body =
+            jit_pass_source(p, plan.qubits, precision, "qf_pass_@" + std::to_string(bodies.size()), &e, ns);
         if (body.empty()) {
             if (err) *err = "pass " + std::to_string(i) + ": " + e;
             return "";
@@ -1169,10 +1299,10 @@ std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, s
     return out;
 }
 
-std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err) {
+std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err, int ns) {
     auto prog = std::make_shared<JitProgram>();
     std::string gen_err;
-    if (jit_plan_source(plan, precision, prog.get(), &gen_err).empty()) {
+    if (jit_plan_source(plan, precision, prog.get(), &gen_err, ns).empty()) {
         if (err) *err = gen_err;
         return nullptr;
     }

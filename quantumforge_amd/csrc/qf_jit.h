@@ -27,8 +27,10 @@ namespace qf {
 // Source of one compiled pass kernel `name` (precision 0 = complex128,
 // 1 = complex64).  Returns an empty string and sets *err when the pass is not
 // a register-kernel tile pass the compiler handles.
+// ns = 2: the adjoint form (kernel arguments add the costate lambda and the
+// partial table; TF_RECORD rotations record Im <lambda| G |psi> per tile).
 std::string jit_pass_source(const Pass& p, uint32_t qubits, int precision, const std::string& name,
-                            std::string* err);
+                            std::string* err, int ns = 1);
 
 // The compiled kernels of one plan at one precision.
 struct JitProgram {
@@ -38,6 +40,7 @@ struct JitProgram {
     std::vector<int> kernel_of_pass;       // index into names, -1 for passes run by the generic kernels
     std::vector<std::string> names;
     std::vector<uint32_t> threads;         // workgroup size per kernel
+    int states = 1;                        // 2: adjoint form (psi and lambda)
     std::string prelude;                   // shared declarations of the source
     std::vector<std::string> bodies;       // source of each kernel
     double compile_ms = 0;
@@ -46,11 +49,11 @@ struct JitProgram {
 
 // Generates the program source of a plan (kernels named, key set) into *prog
 // without compiling; returns the full source, or "" with *err set.
-std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, std::string* err);
+std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, std::string* err, int ns = 1);
 
 // Generates, compiles (or fetches from the caches) the program of a plan.
 // Returns nullptr and sets *err on failure.
-std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err);
+std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err, int ns = 1);
 
 // Default compile options (exposed for tests and the describe output).
 std::vector<std::string> jit_compile_options();

@@ -784,7 +784,7 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     // Register kernel: 2^RB amplitudes per thread, 2^(M-RB) <= 256 threads.
     // Compiled passes take the same plan with 2 <= RB <= 5 and up to 1024 threads.
     uint32_t RB = opt.reg_bits ? opt.reg_bits : (precision == 0 ? 3 : 4);
-    const bool want_compiled = opt.kernel == 3 || (opt.kernel == 0 && qubits >= kCompileMinQubits);
+    const bool want_compiled = opt.kernel == 3 || (opt.kernel == 0 && (qubits >= kCompileMinQubits || opt.compile));
     bool regk = opt.kernel != 1 && M <= qubits && RB >= 3 && RB <= 4 && M >= RB + 6 && M - RB <= 8;
     if (want_compiled) regk = M <= qubits && RB >= 2 && RB <= 5 && M >= RB + 6 && M - RB <= 10;
     if (opt.kernel == 2 && !regk) return "register kernel needs tile_qubits <= qubits and 6 <= tile - reg_bits <= 8";

@@ -104,6 +104,9 @@ struct PlanOptions {
     // runs only as compiled kernels.  Off for plans that also run through
     // the interpreting kernels (the adjoint sweep's reverse plans).
     int relabel = 1;
+    // 1: compiled passes below kCompileMinQubits as well (a large batch of
+    // small states, where one launch still streams much memory).
+    int compile = 0;
 };
 
 // Circuits of at least this many qubits run compiled passes by default
@@ -134,10 +137,11 @@ struct Plan {
     PlanStats stats;
     int compiled = 0;          // 1: register passes run as compiled kernels (2: explicitly requested)
     int relabeled = 0;         // passes with a store permutation (Pass::store_phys): compiled only
-    // Compiled programs per precision, attached on first launch (jit_state:
-    // 0 not tried, 1 compiled, -1 the compile failed and the register kernel runs).
-    mutable std::shared_ptr<JitProgram> jit[2];
-    mutable int jit_state[2] = {0, 0};
+    // Compiled programs per form (0: one state, 1: adjoint, psi and lambda)
+    // and precision, attached on first launch (jit_state: 0 not tried, 1
+    // compiled, -1 the compile failed and the register kernel runs).
+    mutable std::shared_ptr<JitProgram> jit[2][2];
+    mutable int jit_state[2][2] = {{0, 0}, {0, 0}};
 };
 
 // Builds a plan; returns an empty string on success or an error message.

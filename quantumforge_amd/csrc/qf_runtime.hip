@@ -398,25 +398,25 @@ std::map<std::pair<std::string, int>, std::vector<hipFunction_t>> g_jit_fns;
 // nullptr when the plan runs the register kernel.  A failed compile is an
 // error when compiled passes were requested explicitly; otherwise it is
 // reported once and the register kernel runs the same plan.
-qf_status plan_program(const qf::Plan& plan, int precision, const qf::JitProgram** out) {
+qf_status plan_program(const qf::Plan& plan, int precision, int ns, const qf::JitProgram** out) {
     *out = nullptr;
     if (!plan.compiled) return QF_OK;
     std::lock_guard<std::mutex> lk(g_jit_mu);
-    const int pr = precision == QF_C128 ? 0 : 1;
-    if (plan.jit_state[pr] == 0) {
+    const int pr = precision == QF_C128 ? 0 : 1, f = ns == 2 ? 1 : 0;
+    if (plan.jit_state[f][pr] == 0) {
         std::string err;
-        plan.jit[pr] = qf::jit_compile_plan(plan, pr, &err);
-        plan.jit_state[pr] = plan.jit[pr] ? 1 : -1;
-        if (!plan.jit[pr]) {
+        plan.jit[f][pr] = qf::jit_compile_plan(plan, pr, &err, ns);
+        plan.jit_state[f][pr] = plan.jit[f][pr] ? 1 : -1;
+        if (!plan.jit[f][pr]) {
             if (plan.compiled == 2 || plan.relabeled) {
-                plan.jit_state[pr] = 0;
+                plan.jit_state[f][pr] = 0;
                 return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
             }
             std::fprintf(stderr, "quantumforge_amd: pass compiler failed, running the register kernel: %s\n",
                          err.c_str());
         }
     }
-    *out = plan.jit[pr].get();
+    *out = plan.jit[f][pr].get();
     return QF_OK;
 }
 
@@ -439,7 +439,7 @@ qf_status jit_functions(const qf::JitProgram& prog, const std::vector<hipFunctio
 }
 
 qf_status launch_compiled(qf_backend* be, void* d, uint32_t qubits, uint32_t count, const qf::Pass& p,
-                          hipFunction_t fn, uint32_t threads, const LaunchCtx& ctx) {
+                          hipFunction_t fn, uint32_t threads, int ns, const LaunchCtx& ctx) {
     const uint32_t log_blocks = qubits - p.tile_bits;
     const uint32_t gx_log = log_blocks < 16 ? log_blocks : 16;
     const uint64_t gy = uint64_t(count) << (log_blocks - gx_log);
@@ -447,8 +447,12 @@ qf_status launch_compiled(qf_backend* be, void* d, uint32_t qubits, uint32_t cou
     void* sp = d;
     const double* cs = ctx.cs;
     unsigned nslots = ctx.nslots;
-    void* args[] = {&sp, &cs, &nslots};
-    HIP_TRY(hipModuleLaunchKernel(fn, 1u << gx_log, uint32_t(gy), 1, threads, 1, 1, 0, be->stream, args, nullptr));
+    void* lam = ctx.lam;
+    double* partial = ctx.partial;
+    void* args1[] = {&sp, &cs, &nslots};
+    void* args2[] = {&sp, &cs, &nslots, &lam, &partial};  // the adjoint form
+    HIP_TRY(hipModuleLaunchKernel(fn, 1u << gx_log, uint32_t(gy), 1, threads, 1, 1, 0, be->stream,
+                                  ns == 2 ? args2 : args1, nullptr));
     return QF_OK;
 }
 
@@ -459,8 +463,9 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
     if (plan.qubits != qubits) return fail(QF_INCOMPATIBLE_DEVICE_STATE, "plan was built for a different qubit count");
     const qf::JitProgram* prog = nullptr;
     const std::vector<hipFunction_t>* fns = nullptr;
-    if (!ctx.lam && plan.compiled) {
-        qf_status s = plan_program(plan, precision, &prog);
+    const int ns = ctx.lam ? 2 : 1;
+    if (plan.compiled) {
+        qf_status s = plan_program(plan, precision, ns, &prog);
         if (s) return s;
         if (prog && (s = jit_functions(*prog, &fns))) return s;
     }
@@ -472,7 +477,7 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
         qf_status s;
         const int kern = prog ? prog->kernel_of_pass[i] : -1;
         if (kern >= 0) {
-            s = launch_compiled(be, d, qubits, count, p, (*fns)[kern], prog->threads[kern], ctx);
+            s = launch_compiled(be, d, qubits, count, p, (*fns)[kern], prog->threads[kern], ns, ctx);
         } else {
             if (p.kind == qf::Pass::TILE && !cur) return fail(QF_INVALID_ARGUMENT, "tile pass without uploaded ops");
             s = precision == QF_C128 ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, p, cur, ctx)
@@ -488,7 +493,7 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
 qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
     if (plan.compiled) {
         const qf::JitProgram* prog = nullptr;
-        qf_status s = plan_program(plan, precision, &prog);
+        qf_status s = plan_program(plan, precision, 1, &prog);
         if (s) return s;
         if (prog) return launch_plan(be, d, precision, qubits, count, plan, nullptr, LaunchCtx{});
     }
@@ -1181,12 +1186,13 @@ extern "C" qf_status qf_plan_describe(const qf_plan* p, char* buf, size_t cap, s
     return QF_OK;
 }
 
-extern "C" qf_status qf_plan_source(const qf_plan* p, int32_t precision, char* buf, size_t cap, size_t* needed) {
+namespace {
+qf_status plan_source(const qf_plan* p, int32_t precision, int ns, char* buf, size_t cap, size_t* needed) {
     if (!p) return fail(QF_INVALID_ARGUMENT, "null plan");
     if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
     qf::JitProgram prog;
     std::string err;
-    const std::string s = qf::jit_plan_source(p->plan, precision == QF_C128 ? 0 : 1, &prog, &err);
+    const std::string s = qf::jit_plan_source(p->plan, precision == QF_C128 ? 0 : 1, &prog, &err, ns);
     if (s.empty()) return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
     if (needed) *needed = s.size() + 1;
     if (buf && cap) {
@@ -1195,6 +1201,16 @@ extern "C" qf_status qf_plan_source(const qf_plan* p, int32_t precision, char* b
         buf[n] = '\0';
     }
     return QF_OK;
+}
+}  // namespace
+
+extern "C" qf_status qf_plan_source(const qf_plan* p, int32_t precision, char* buf, size_t cap, size_t* needed) {
+    return plan_source(p, precision, 1, buf, cap, needed);
+}
+
+extern "C" qf_status qf_plan_adjoint_source(const qf_plan* p, int32_t precision, char* buf, size_t cap,
+                                            size_t* needed) {
+    return plan_source(p, precision, 2, buf, cap, needed);
 }
 
 extern "C" qf_status qf_plan_compile(const qf_plan* p, int32_t precision, double* compile_ms, int32_t* from_cache) {

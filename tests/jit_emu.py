@@ -35,12 +35,23 @@ static std::barrier<>* qf_bar;
 #define __launch_bounds__(n)
 #define __shared__ static
 #define __syncthreads() qf_bar->arrive_and_wait()
+// Lane exchange of the wave reductions: every work item of the workgroup
+// calls it at the same point.
+static double qf_shfl[1024];
+static inline double __shfl_xor(double v, int m, int) {
+    qf_shfl[qf_tid.x] = v;
+    qf_bar->arrive_and_wait();
+    const double r = qf_shfl[qf_tid.x ^ unsigned(m)];
+    qf_bar->arrive_and_wait();
+    return r;
+}
 """
 
 LAUNCHER = r"""
 typedef void (*qf_kfn)(C*, const double*, unsigned);
-extern "C" void qf_emu_launch(qf_kfn fn, C* s, const double* cs, unsigned nslots, unsigned gx, unsigned gy,
-                              unsigned nt) {
+typedef void (*qf_kfn2)(C*, const double*, unsigned, C*, double*);
+extern "C" void qf_emu_launch(void* fn, C* s, const double* cs, unsigned nslots, C* lam, double* partial,
+                              unsigned gx, unsigned gy, unsigned nt) {
     qf_grid = {gx, gy, 1};
     for (unsigned by = 0; by < gy; ++by)
         for (unsigned bx = 0; bx < gx; ++bx) {
@@ -52,7 +63,8 @@ extern "C" void qf_emu_launch(qf_kfn fn, C* s, const double* cs, unsigned nslots
                 th.emplace_back([=] {
                     qf_tid = {t, 0, 0};
                     qf_bid = {bx, by, 0};
-                    fn(s, cs, nslots);
+                    if (lam) reinterpret_cast<qf_kfn2>(fn)(s, cs, nslots, lam, partial);
+                    else reinterpret_cast<qf_kfn>(fn)(s, cs, nslots);
                 });
             for (auto& x : th) x.join();
         }
@@ -81,19 +93,29 @@ class EmulatedProgram:
         self.lib = C.CDLL(so)
         self.launch = self.lib.qf_emu_launch
         self.launch.restype = None
-        self.launch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint]
+        self.launch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p, C.c_uint,
+                                C.c_uint, C.c_uint]
+        self.adjoint = "double* __restrict__ partial" in source
 
     def run(self, amps: np.ndarray, qubits: int, tile_bits: list[int], count: int = 1,
-            cs: np.ndarray | None = None, nslots: int = 0) -> np.ndarray:
+            cs: np.ndarray | None = None, nslots: int = 0, lam: np.ndarray | None = None):
         """Applies the passes in order to `count` contiguous states (a copy);
-        tile_bits[i] is the tile size of compiled kernel i."""
+        tile_bits[i] is the tile size of compiled kernel i.  Adjoint-form
+        source: lam is the costate (a copy is carried along) and the call
+        returns (psi, lambda, partial table [count * 2^(n - tile)][nslots])."""
         dt = np.complex64 if self.complex64 else np.complex128
         s = np.ascontiguousarray(amps.astype(dt, copy=True))
         csp = C.c_void_p(cs.ctypes.data) if cs is not None and cs.size else C.c_void_p(0)
+        lp = pp = C.c_void_p(0)
+        if self.adjoint:
+            lam = np.ascontiguousarray(lam.astype(dt, copy=True))
+            rows = count << (qubits - (tile_bits[0] if tile_bits else qubits))
+            partial = np.zeros((rows, max(nslots, 1)))
+            lp, pp = C.c_void_p(lam.ctypes.data), C.c_void_p(partial.ctypes.data)
         for (name, nt), M in zip(self.kernels, tile_bits):
             fn = C.cast(getattr(self.lib, name), C.c_void_p)
             logb = qubits - M
             gx_log = min(logb, 16)
             gy = count << (logb - gx_log)
-            self.launch(fn, C.c_void_p(s.ctypes.data), csp, nslots, 1 << gx_log, gy, nt)
-        return s
+            self.launch(fn, C.c_void_p(s.ctypes.data), csp, nslots, lp, pp, 1 << gx_log, gy, nt)
+        return (s, lam, partial) if self.adjoint else s

@@ -139,6 +139,39 @@ def test_batched_adjoint_tile_shapes(backend):
             assert np.max(np.abs(e - e0)) < 1e-13 and np.max(np.abs(g - g0)) < 1e-12, (tile, chunk)
 
 
+@pytest.mark.parametrize("precision,tol", [("complex128", 1e-11), ("complex64", 2e-4)])
+def test_large_batch_runs_the_compiled_adjoint_sweep(backend, precision, tol):
+    # 256 sets of 14 qubits (2^22 amplitudes per launch): the forward and
+    # reverse plans run as compiled passes, the reverse sweep in the
+    # two-state adjoint form that records the rotation overlaps per tile.
+    # The first sets must equal a 4-set batch (interpreting kernels) and the
+    # oracle's adjoint gradient.
+    n, sets = 14, 256
+    pc = hea_circuit(n, 2)
+    H = heisenberg_hamiltonian(n, seed=6)
+    params = uniform_parameters(pc.parameter_count, sets, seed=17)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H, precision=precision)
+    e4, g4 = G.batch_adjoint_gradient(backend, pc, n, params[:4], H, precision=precision)
+    assert np.max(np.abs(e[:4] - e4)) < tol and np.max(np.abs(g[:4] - g4)) < tol
+    t = H.to_terms()
+    for s in (0, 1, 255):
+        want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        assert np.max(np.abs(g[s] - want)) < tol, s
+
+
+def test_adjoint_at_18_qubits_runs_compiled(backend):
+    # From 18 qubits every gradient plan is compiled, one set included.
+    n = 18
+    pc = hea_circuit(n, 1)
+    H = heisenberg_hamiltonian(n, seed=2)
+    params = uniform_parameters(pc.parameter_count, 1, seed=3)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    t = H.to_terms()
+    assert abs(e[0] - O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[0], [], n, t)) < 1e-11
+    want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[0], [], n, t)
+    assert np.max(np.abs(g[0] - want)) < 1e-10
+
+
 def test_complex64_adjoint_error_bound(backend):
     n = 10
     pc = hea_circuit(n, 3)

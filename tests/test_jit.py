@@ -10,8 +10,9 @@ import pytest
 from oracle import oracle as O
 from quantumforge_amd import ParameterizedCircuit, QuantumError, RotationAxis
 from quantumforge_amd.backend import Plan
-from quantumforge_amd.workloads import (MaxCutProblem, hea_circuit, qaoa_parameterized, qft_like_circuit,
-                                        random_layer_circuit, uniform_parameters)
+from oracle import np_oracle as NP
+from quantumforge_amd.workloads import (MaxCutProblem, hea_circuit, heisenberg_hamiltonian, qaoa_parameterized,
+                                        qft_like_circuit, random_layer_circuit, uniform_parameters)
 
 from .conftest import ROOT, close
 from .jit_emu import EmulatedProgram
@@ -223,6 +224,42 @@ def test_generated_kernels_read_rotation_angles_per_instance(case):
     out = emulate(plan, a0, n, count=2, cs=cs, nslots=len(slots)).reshape(2, -1)
     for i in range(2):
         close(out[i], O.pc_run(pops, P, 0, sets[i], [], n), 1e-13)
+
+
+@pytest.mark.parametrize("n,M,RB,precision", [(11, 9, 3, "complex128"), (12, 10, 3, "complex128"),
+                                                (12, 10, 4, "complex64")])
+def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision):
+    # The compiled reverse sweep of the adjoint gradient: the forward plan's
+    # kernels make psi, lambda = H psi, then the reverse plan's kernels in
+    # their two-state form carry psi and lambda back and record
+    # Im <lambda| G |psi> per tile for every rotation.  Summing the partial
+    # table per parameter gives the oracle's adjoint gradient, and psi
+    # returns to |0>.
+    pc = hea_circuit(n, 2)
+    P = pc.parameter_count
+    params = np.array(uniform_parameters(P, 1, seed=n + M)[0])
+    H = heisenberg_hamiltonian(n, seed=3)
+    pops = pc.to_pops()
+    slots = [int(r["index"]) for r in pops if r["kind"] == 1]
+    cs = np.stack([np.cos(params[slots] / 2), np.sin(params[slots] / 2)], axis=1).ravel()
+    fwd = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
+    rev = Plan.parameterized(pc, n, reverse=True, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
+    src = rev.source(precision, adjoint=True)
+    assert "__shfl_xor" in src and "partial[b * nslots" in src
+    a0 = np.zeros(1 << n, complex)
+    a0[0] = 1
+    psi = emulate(fwd, a0, n, cs=cs, nslots=len(slots), precision=precision)
+    lam = NP.pauli_sum_matrix(H.terms) @ psi.astype(complex)
+    tb = [p["tile_bits"] for p in rev.describe()["passes"]]
+    psi2, _, part = EmulatedProgram(src, EMU_DIR).run(psi, n, tb, cs=cs, nslots=len(slots), lam=lam)
+    per_slot = part.sum(axis=0)
+    grad = np.zeros(P)
+    for k, p in enumerate(slots):
+        grad[p] += per_slot[k]
+    want = O.pc_adjoint_gradient(pops, P, 0, params, [], n, H.to_terms())
+    tol = 1e-12 if precision == "complex128" else 2e-4
+    close(grad, want, tol)
+    close(psi2, a0, 1e-13 if precision == "complex128" else 1e-5)
 
 
 def test_compiled_flag_defaults_and_errors():
