@@ -64,6 +64,8 @@ struct Bit {
 };
 
 class PassGen {
+    using zc = std::complex<double>;
+
 public:
     // ns = 2: the adjoint form, which carries a second state (the costate
     // lambda) through the same ops and records Im <lambda| G |psi> for the
@@ -166,7 +168,7 @@ public:
                 default: return fail(err, "op kind " + std::to_string(op.kind) + " is not compiled");
             }
         }
-        if (g_ != 1.0) acc_mul_lit(acc_[0], g_, 0.0);
+        if (g_ != zc(1.0, 0.0)) acc_mul_lit(acc_[0], g_.real(), g_.imag());
         g_ = 1.0;
         acc_flush_all();
         // A relabeling pass first moves the tile into the store layout, whose
@@ -220,7 +222,7 @@ private:
     int rb_[8] = {0};
     std::vector<int> reg_of_tile_, thr_of_tile_;
     std::vector<int> var_;
-    double g_ = 1.0;              // the tile holds g_ times its amplitudes (take_scale)
+    zc g_{1.0, 0.0};              // the tile holds g_ times its amplitudes (take_scale)
     bool relabel_ = false;        // the store permutes the tile's bits (Pass::store_phys)
     bool half_ = false;           // exchanges through a half-tile buffer
     // Half-buffer exchanges in emission order: the round bit, and for an
@@ -777,19 +779,34 @@ private:
 
     void swap_vars(uint32_t a, uint32_t b) { std::swap(var_[a], var_[b]); }
 
-    // Literal dense ops carry a real scale out of their matrix: the matrix
-    // divided by one of its entries' magnitudes sigma has entries of
-    // magnitude 1, which need no multiply (Rx and Ry become one FMA per
-    // output component instead of a multiply and an FMA).  The tile then
-    // holds g times the true amplitudes; g is global to the tile (every
-    // dense op touches every amplitude once, everything else is linear), so
-    // the next literal dense op takes it into its matrix and the store
-    // applies what is left.
-    double take_scale(double* m, int count) {
-        if (g_ != 1.0) {
-            for (int x = 0; x < count; ++x) m[x] *= g_;
-            g_ = 1.0;
+    // Literal ops carry a scale out of their matrix.  A dense one divided by
+    // one of its entries' magnitudes sigma has entries of magnitude 1, which
+    // need no multiply (Rx and Ry become one FMA per output component
+    // instead of a multiply and an FMA); a diagonal one divided by its first
+    // entry leaves that entry's registers untouched (Rz multiplies half the
+    // registers).  The tile then holds g times the true amplitudes; g is
+    // global to the tile (every dense op touches every amplitude once,
+    // everything else is linear), so the next literal dense op takes it into
+    // its matrix and the store applies what is left.
+    void fold_scale(double* m, int count) {
+        if (g_ == zc(1.0, 0.0)) return;
+        for (int x = 0; x + 1 < count; x += 2) {
+            const zc v = zc(m[x], m[x + 1]) * g_;
+            m[x] = v.real();
+            m[x + 1] = v.imag();
         }
+        g_ = 1.0;
+    }
+    // Divides the complex values d[0..n) by d[0] and multiplies g by it.
+    void factor_first(zc* d, int n) {
+        if (!std::isnormal(std::abs(d[0])) || d[0] == zc(1.0, 0.0)) return;
+        const zc f = d[0];
+        for (int x = 1; x < n; ++x) d[x] /= f;
+        d[0] = 1.0;
+        g_ *= f;
+    }
+    double take_scale(double* m, int count) {
+        fold_scale(m, count);
         double best = 0;
         int best_n = 0;
         for (int x = 0; x < count; ++x) {
@@ -811,25 +828,27 @@ private:
         const uint32_t i = uint32_t(op.q0);
         double m[8];
         std::memcpy(m, op.m, sizeof m);
-        if (g_ != 1.0) {
-            for (double& x : m) x *= g_;
-            g_ = 1.0;
-        }
         const bool offz = m[2] == 0 && m[3] == 0 && m[4] == 0 && m[5] == 0;
         const bool diagz = m[0] == 0 && m[1] == 0 && m[6] == 0 && m[7] == 0;
         if (offz) {
-            for (uint32_t k = 0; k < NR_; ++k)
-                if ((k >> i) & 1) mul_lit(k, m[6], m[7]);
-                else mul_lit(k, m[0], m[1]);
+            zc d[2] = {zc(m[0], m[1]), zc(m[6], m[7])};
+            factor_first(d, 2);
+            for (uint32_t k = 0; k < NR_; ++k) {
+                const zc v = d[(k >> i) & 1];
+                mul_lit(k, v.real(), v.imag());
+            }
             return;
         }
         acc_flush_bit(i);
         if (diagz) {
+            zc d[2] = {zc(m[2], m[3]), zc(m[4], m[5])};
+            factor_first(d, 2);
             for (uint32_t k = 0; k < NR_; ++k)
                 if (!((k >> i) & 1)) swap_vars(k, k | (1u << i));
-            for (uint32_t k = 0; k < NR_; ++k)
-                if ((k >> i) & 1) mul_lit(k, m[4], m[5]);
-                else mul_lit(k, m[2], m[3]);
+            for (uint32_t k = 0; k < NR_; ++k) {
+                const zc v = d[(k >> i) & 1];
+                mul_lit(k, v.real(), v.imag());
+            }
             return;
         }
         g_ = take_scale(m, 8);
@@ -906,10 +925,15 @@ private:
         const bool rt0 = b0.kind != Bit::REG, rt1 = two && b1.kind != Bit::REG;
         const int nreg = (rt0 ? 0 : 1) + (two && !rt1 ? 1 : 0);
         if (nreg == 2 || (nreg == 1 && !rt0 && !rt1)) {
-            // Only register bits: a literal factor per register.
+            // Only register bits: a literal factor per register, the first
+            // value taken out into the tile's scale.
+            zc dv[4];
+            for (int x = 0; x < (two ? 4 : 2); ++x) dv[x] = zc(d(x, 0), d(x, 1));
+            factor_first(dv, two ? 4 : 2);
             for (uint32_t k = 0; k < NR_; ++k) {
                 const int x0 = int((k >> b0.idx) & 1), x1 = two ? int((k >> b1.idx) & 1) : 0;
-                mul_lit(k, d(x1 * 2 + x0, 0), d(x1 * 2 + x0, 1));
+                const zc v = dv[x1 * 2 + x0];
+                mul_lit(k, v.real(), v.imag());
             }
             return;
         }
@@ -1084,7 +1108,7 @@ private:
             }
             if (!rt.empty()) o_ << "  if (" << rt << ") " << v << " = -" << v << ";\n";
         }
-        if (g_ != 1.0) o_ << "  " << v << " *= " << hexlit(1.0 / (g_ * g_)).substr(1) << ";\n";
+        if (g_ != zc(1.0, 0.0)) o_ << "  " << v << " *= " << hexlit(1.0 / std::norm(g_)).substr(1) << ";\n";
         o_ << "  for (int m = 32; m >= 1; m >>= 1) " << v << " += __shfl_xor(" << v << ", m, 64);\n";
         o_ << "  if ((t & 63u) == 0u) qred[" << r * W << "u + (t >> 6)] = " << v << ";\n";
     }
