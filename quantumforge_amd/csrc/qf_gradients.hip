@@ -686,7 +686,10 @@ qf_status run_strategy(GradRun& R, int32_t strategy, const double* inputs, const
             return adjoint_sets(R, params, sets, o.chunk, energies, grads);
         case QF_GRAD_PARAMETER_SHIFT:
         case QF_GRAD_FINITE_DIFFERENCE:
-            if ((s = plan_for(R, inputs, o.tile_qubits, false, sets))) return s;
+            // The shifted circuits run as one batch of 2 x occurrences per set.
+            if ((s = plan_for(R, inputs, o.tile_qubits, false,
+                              uint64_t(sets) * 2 * std::max<size_t>(R.pc.slot_param.size(), 1))))
+                return s;
             return shift_sets(R, params, sets, o.chunk, strategy == QF_GRAD_FINITE_DIFFERENCE, o.fd_step, energies,
                               grads);
         case QF_GRAD_BACKPROP:
