@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "qf_runtime.h"
@@ -174,6 +175,24 @@ void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double 
 }
 
 
+// Runs f(b) for b in [0, count) on up to 16 host threads (the angle tables
+// of large batches: one cos and one sin per occurrence and row).
+template <class F>
+void parallel_rows(uint64_t count, F f) {
+    const uint64_t hw = std::max<uint64_t>(1, std::min<uint64_t>(16, std::thread::hardware_concurrency()));
+    const uint64_t nt = std::min<uint64_t>(hw, count / 256 + 1);
+    if (nt <= 1) {
+        for (uint64_t b = 0; b < count; ++b) f(b);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (uint64_t w = 0; w < nt; ++w)
+        pool.emplace_back([&, w] {
+            for (uint64_t b = count * w / nt; b < count * (w + 1) / nt; ++b) f(b);
+        });
+    for (auto& t : pool) t.join();
+}
+
 // Observable terms sorted by x mask (k_pauli_apply reloads psi[i ^ x] only when x changes).
 std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t) {
     std::vector<qf::PauliDev> s(t);
@@ -203,19 +222,20 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __res
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
         double ar = 0.0, ai = 0.0;
-        uint64_t cur = ~uint64_t(0);
-        double pr = 0.0, pi = 0.0;
-        for (uint32_t t = 0; t < nterms; ++t) {
-            const qf::PauliDev& T = terms[t];
-            if (T.x != cur) {
-                cur = T.x;
-                const cx<R> v = p[i ^ cur];
-                pr = v.re;
-                pi = v.im;
+        // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x: the
+        // terms of one x mask (adjacent, sorted) share psi[col], so their
+        // signed coefficients are summed first and multiply it once.
+        for (uint32_t t = 0; t < nterms;) {
+            const uint64_t x = terms[t].x, col = i ^ x;
+            double cr = 0.0, ci = 0.0;
+            for (; t < nterms && terms[t].x == x; ++t) {
+                const qf::PauliDev& T = terms[t];
+                const bool neg = (__popcll(col & T.z) & 1) != 0;
+                cr += neg ? -T.cr : T.cr;
+                if (T.ci != 0.0) ci += neg ? -T.ci : T.ci;
             }
-            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
-            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
-            const double cr = sg * T.cr, ci = sg * T.ci;
+            const cx<R> v = p[col];
+            const double pr = v.re, pi = v.im;
             ar += cr * pr - ci * pi;
             ai += cr * pi + ci * pr;
         }
@@ -474,8 +494,9 @@ qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t
                        double* grads) {
     const uint32_t nslots = uint32_t(R.pc.slot_param.size());
     std::vector<double> table(std::max<size_t>(size_t(sets) * nslots * 2, 1));
-    for (uint32_t b = 0; b < sets; ++b)
+    parallel_rows(sets, [&](uint64_t b) {
         fill_row(R.pc, params + size_t(b) * R.pc.nparams, -1, 0.0, table.data() + size_t(b) * nslots * 2);
+    });
     return table_run(R, table.data(), sets, chunk_opt, energies, grads);
 }
 
@@ -500,13 +521,11 @@ qf_status shift_sets(GradRun& R, const double* params, uint32_t sets, uint32_t c
         const uint32_t gs = std::min(per_group, sets - g0);
         const uint32_t cnt = gs * nslots * 2;
         table.assign(size_t(cnt) * nslots * 2, 0.0);
-        for (uint32_t b = 0; b < gs; ++b)
-            for (uint32_t k = 0; k < nslots; ++k)
-                for (int sg = 0; sg < 2; ++sg) {
-                    const size_t row = (size_t(b) * nslots + k) * 2 + sg;
-                    fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), sg == 0 ? delta : -delta,
-                             table.data() + row * nslots * 2);
-                }
+        parallel_rows(uint64_t(gs) * nslots * 2, [&](uint64_t row) {
+            const uint64_t b = row / (uint64_t(nslots) * 2), k = (row / 2) % nslots;
+            fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), row % 2 == 0 ? delta : -delta,
+                     table.data() + row * nslots * 2);
+        });
         eg.assign(cnt, 0.0);
         if ((s = table_run(R, table.data(), cnt, chunk_opt, eg.data(), nullptr))) return s;
         for (uint32_t b = 0; b < gs; ++b) {

@@ -1263,6 +1263,57 @@ std::string compile_one(const std::string& src, const std::string& fname, std::v
 
 std::mutex g_mu;
 std::map<std::string, std::shared_ptr<JitProgram>> g_cache;
+// Programs by plan fingerprint: a plan rebuilt for every call (the gradient
+// engine's) finds its program without generating and hashing the source.
+std::map<std::string, std::shared_ptr<JitProgram>> g_fp_cache;
+
+// 128 bits over everything the generated source depends on: qubit count,
+// precision, state count, compile options, the half-buffer switch, and every
+// pass's shape, tile bits, store layout and ops.
+std::string plan_fingerprint(const Plan& plan, int precision, int ns) {
+    uint64_t h1 = 1469598103934665603ull, h2 = 0x9e3779b97f4a7c15ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h1 = (h1 ^ b[i]) * 1099511628211ull;
+            h2 = (h2 + b[i] + 1) * 0xff51afd7ed558ccdull;
+            h2 ^= h2 >> 31;
+        }
+    };
+    auto val = [&](int64_t v) { mix(&v, sizeof v); };
+    val(plan.qubits);
+    val(precision);
+    val(ns);
+    const char* half = std::getenv("QF_JIT_HALF_LDS");
+    val(half && half[0] == '0' ? 0 : 1);
+    for (const auto& o : jit_compile_options()) {
+        mix(o.data(), o.size());
+        val(-1);
+    }
+    for (const Pass& p : plan.passes) {
+        val(p.kind);
+        val(p.tile_bits);
+        val(p.low);
+        val(p.reg_bits);
+        val(int64_t(p.high.size()));
+        for (uint32_t h : p.high) val(h);
+        val(int64_t(p.store_phys.size()));
+        for (uint32_t h : p.store_phys) val(h);
+        val(int64_t(p.ops.size()));
+        for (const TileOp& o : p.ops) {
+            val(o.kind);
+            val(o.q0);
+            val(o.q1);
+            val(o.flags);
+            val(o.param);
+            val(o.slot);
+            mix(o.m, sizeof o.m);
+        }
+    }
+    char b[40];
+    std::snprintf(b, sizeof b, "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
+    return b;
+}
 
 }  // namespace
 
@@ -1324,6 +1375,12 @@ std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, s
 }
 
 std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, std::string* err, int ns) {
+    const std::string fp = plan_fingerprint(plan, precision, ns);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_fp_cache.find(fp);
+        if (it != g_fp_cache.end()) return it->second;
+    }
     auto prog = std::make_shared<JitProgram>();
     std::string gen_err;
     if (jit_plan_source(plan, precision, prog.get(), &gen_err, ns).empty()) {
@@ -1334,7 +1391,10 @@ std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, st
     {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_cache.find(key);
-        if (it != g_cache.end()) return it->second;
+        if (it != g_cache.end()) {
+            g_fp_cache.emplace(fp, it->second);
+            return it->second;
+        }
     }
     const auto t0 = std::chrono::steady_clock::now();
     // Chunks of 8 kernels, compiled by a few host threads; each chunk is its
@@ -1376,6 +1436,7 @@ std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, st
     prog->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::lock_guard<std::mutex> lk(g_mu);
     auto ins = g_cache.emplace(key, prog);
+    g_fp_cache.emplace(fp, ins.first->second);
     return ins.first->second;
 }
 
