@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -345,7 +346,9 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
         po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
-        po.relabel = 0;  // the reverse sweep runs through the interpreting adjoint kernel
+        // A reverse plan that runs compiled (its adjoint form) may relabel;
+        // one that runs through the interpreting adjoint kernel may not.
+        po.relabel = po.compile || R.qubits >= qf::kCompileMinQubits ? 1 : 0;
         e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
         if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     }
@@ -874,7 +877,9 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
                                                 std::to_string(inputs_len));
     qf::PlanOptions po = plan_options_from(options);
     po.fuse = 1;
-    if (reverse) po.relabel = 0;
+    // A reverse plan relabels only when it runs compiled (its adjoint form).
+    if (reverse && !(po.kernel == 3 || (po.kernel == 0 && (po.compile || qubits >= qf::kCompileMinQubits))))
+        po.relabel = 0;
     qf_plan* p = new qf_plan;
     const int precision = options ? options->precision : QF_C128;
     if (precision != QF_C128 && precision != QF_C64) {

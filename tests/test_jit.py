@@ -107,11 +107,13 @@ def test_relabeling_saves_passes_and_can_be_switched_off(monkeypatch):
     assert src.count("xb[2048]") == src.count("__global__") == len(on["passes"])
 
 
-def test_reverse_plans_keep_the_identity_layout():
-    # The adjoint sweep runs reverse plans through the interpreting kernel,
-    # which loads and stores every qubit at its own bit.
+def test_reverse_plans_relabel_only_when_compiled():
+    # A reverse plan that runs through the interpreting adjoint kernel keeps
+    # every qubit on its own bit; one that runs compiled (its adjoint form)
+    # may relabel like any compiled plan.
     pc = hea_circuit(20, 3)
-    assert Plan.parameterized(pc, 20, reverse=True).describe()["relabeled"] == 0
+    assert Plan.parameterized(pc, 20, reverse=True, kernel=2).describe()["relabeled"] == 0
+    assert Plan.parameterized(pc, 20, reverse=True).describe()["relabeled"] > 0
 
 
 def test_exchanges_without_a_common_thread_bit_hop_through_an_intermediate_layout():
