@@ -223,20 +223,19 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __res
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
         double ar = 0.0, ai = 0.0;
-        // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x: the
-        // terms of one x mask (adjacent, sorted) share psi[col], so their
-        // signed coefficients are summed first and multiply it once.
-        for (uint32_t t = 0; t < nterms;) {
-            const uint64_t x = terms[t].x, col = i ^ x;
-            double cr = 0.0, ci = 0.0;
-            for (; t < nterms && terms[t].x == x; ++t) {
-                const qf::PauliDev& T = terms[t];
-                const bool neg = (__popcll(col & T.z) & 1) != 0;
-                cr += neg ? -T.cr : T.cr;
-                if (T.ci != 0.0) ci += neg ? -T.ci : T.ci;
+        uint64_t cur = ~uint64_t(0);
+        double pr = 0.0, pi = 0.0;
+        for (uint32_t t = 0; t < nterms; ++t) {
+            const qf::PauliDev& T = terms[t];
+            if (T.x != cur) {
+                cur = T.x;
+                const cx<R> v = p[i ^ cur];
+                pr = v.re;
+                pi = v.im;
             }
-            const cx<R> v = p[col];
-            const double pr = v.re, pi = v.im;
+            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
+            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
+            const double cr = sg * T.cr, ci = sg * T.ci;
             ar += cr * pr - ci * pi;
             ai += cr * pi + ci * pr;
         }
