@@ -344,10 +344,14 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
-        po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
-        // A reverse plan that runs compiled (its adjoint form) may relabel;
-        // one that runs through the interpreting adjoint kernel may not.
-        po.relabel = po.compile || R.qubits >= qf::kCompileMinQubits ? 1 : 0;
+        // A reverse plan that runs compiled (its adjoint form) may relabel
+        // and keeps the forward tile (its half-buffer exchanges fit both
+        // states); one that runs through the interpreting adjoint kernel may
+        // not relabel and takes a tile one qubit smaller (two states in LDS).
+        const bool compiled = po.compile || R.qubits >= qf::kCompileMinQubits;
+        if (!compiled || tile)
+            po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
+        po.relabel = compiled ? 1 : 0;
         e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
         if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     }

@@ -589,7 +589,9 @@ private:
     void plan_half_exchanges() {
         half_ = false;
         half_x_.clear();
-        if (M_ - RB_ <= 6 || M_ < 11) return;
+        // Two-state (adjoint) tiles hold twice the data: half buffers from 10
+        // qubits.
+        if (M_ - RB_ <= 6 || M_ < (ns_ == 2 ? 10u : 11u)) return;
         const char* env = std::getenv("QF_JIT_HALF_LDS");
         if (env && env[0] == '0') return;
         std::vector<Layout> seq;
