@@ -194,11 +194,30 @@ void parallel_rows(uint64_t count, F f) {
     for (auto& t : pool) t.join();
 }
 
-// Observable terms sorted by x mask (k_pauli_apply reloads psi[i ^ x] only when x changes).
+// Observable terms sorted by x mask (k_pauli_apply reads psi[i ^ x] once per x).
 std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t) {
     std::vector<qf::PauliDev> s(t);
     std::stable_sort(s.begin(), s.end(), [](const qf::PauliDev& a, const qf::PauliDev& b) { return a.x < b.x; });
     return s;
+}
+
+// The sorted terms' runs of one x mask: terms [t0, t1) share psi[i ^ x].
+struct PauliGroup {
+    uint64_t x;
+    uint32_t t0, t1;
+};
+
+// Device image of a sorted term list: the terms, then their x-mask groups.
+std::vector<unsigned char> pack_terms(const std::vector<qf::PauliDev>& sorted, uint32_t* ngroups) {
+    std::vector<PauliGroup> g;
+    for (uint32_t t = 0; t < sorted.size(); ++t)
+        if (g.empty() || g.back().x != sorted[t].x) g.push_back(PauliGroup{sorted[t].x, t, t + 1});
+        else g.back().t1 = t + 1;
+    *ngroups = uint32_t(g.size());
+    std::vector<unsigned char> out(std::max<size_t>(sorted.size() * sizeof(qf::PauliDev) + g.size() * sizeof(PauliGroup), 16));
+    if (!sorted.empty()) std::memcpy(out.data(), sorted.data(), sorted.size() * sizeof(qf::PauliDev));
+    if (!g.empty()) std::memcpy(out.data() + sorted.size() * sizeof(qf::PauliDev), g.data(), g.size() * sizeof(PauliGroup));
+    return out;
 }
 
 }  // namespace qfrt
@@ -209,12 +228,16 @@ namespace {
 // Kernels
 // ---------------------------------------------------------------------------
 
-// lambda = O psi (Pauli sum, terms sorted by x mask) and the energy partials
-// conj(psi_i) lambda_i, FP64 compensated, one row per instance (grid.y).
+// lambda = O psi (Pauli sum, terms sorted by x mask, groups after them) and
+// the energy partials conj(psi_i) lambda_i, FP64 compensated, one row per
+// instance (grid.y).  The gathers psi[i ^ x] of up to 8 groups are issued
+// together before their terms are added, so their latencies overlap.
 template <class R>
 __global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
                                                               uint32_t n, const qf::PauliDev* __restrict__ terms,
-                                                              uint32_t nterms, qf::dd* __restrict__ partial) {
+                                                              uint32_t nterms, uint32_t ngroups,
+                                                              qf::dd* __restrict__ partial) {
+    const PauliGroup* __restrict__ groups = reinterpret_cast<const PauliGroup*>(terms + nterms);
     const uint64_t dim = uint64_t(1) << n;
     const uint64_t inst = blockIdx.y;
     const cx<R>* __restrict__ p = psi + inst * dim;
@@ -223,21 +246,25 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_apply(const cx<R>* __res
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride) {
         double ar = 0.0, ai = 0.0;
-        uint64_t cur = ~uint64_t(0);
-        double pr = 0.0, pi = 0.0;
-        for (uint32_t t = 0; t < nterms; ++t) {
-            const qf::PauliDev& T = terms[t];
-            if (T.x != cur) {
-                cur = T.x;
-                const cx<R> v = p[i ^ cur];
-                pr = v.re;
-                pi = v.im;
+        for (uint32_t g0 = 0; g0 < ngroups; g0 += 8) {
+            cx<R> v[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u)
+                if (g0 + u < ngroups) v[u] = p[i ^ groups[g0 + u].x];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                if (g0 + u >= ngroups) break;
+                const PauliGroup G = groups[g0 + u];
+                const double pr = v[u].re, pi = v[u].im;
+                for (uint32_t t = G.t0; t < G.t1; ++t) {
+                    const qf::PauliDev& T = terms[t];
+                    // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
+                    const double sg = (__popcll((i ^ G.x) & T.z) & 1) ? -1.0 : 1.0;
+                    const double cr = sg * T.cr, ci = sg * T.ci;
+                    ar += cr * pr - ci * pi;
+                    ai += cr * pi + ci * pr;
+                }
             }
-            // P|col> = c' (-1)^{popcount(col & z)} |col ^ x>, col = i ^ x.
-            const double sg = (__popcll((i ^ cur) & T.z) & 1) ? -1.0 : 1.0;
-            const double cr = sg * T.cr, ci = sg * T.ci;
-            ar += cr * pr - ci * pi;
-            ai += cr * pi + ci * pr;
         }
         if (l) l[i] = cx<R>{R(ar), R(ai)};
         const cx<R> v = p[i];
@@ -424,13 +451,13 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
     }
     if ((s = pstart.alloc(ps.size() * 4))) return s;
     if ((s = pslots.alloc(std::max<size_t>(pl.size(), 1) * 4))) return s;
-    if ((s = terms.alloc(std::max<size_t>(R.terms.size(), 1) * sizeof(qf::PauliDev)))) return s;
+    const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
+    uint32_t ngroups = 0;
+    const std::vector<unsigned char> tpack = pack_terms(tsorted, &ngroups);
+    if ((s = terms.alloc(tpack.size()))) return s;
     QF_HIP_TRY(hipMemcpyAsync(pstart.p, ps.data(), ps.size() * 4, hipMemcpyHostToDevice, st));
     if (!pl.empty()) QF_HIP_TRY(hipMemcpyAsync(pslots.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, st));
-    const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
-    if (!tsorted.empty())
-        QF_HIP_TRY(hipMemcpyAsync(terms.p, tsorted.data(), tsorted.size() * sizeof(qf::PauliDev),
-                                  hipMemcpyHostToDevice, st));
+    QF_HIP_TRY(hipMemcpyAsync(terms.p, tpack.data(), tpack.size(), hipMemcpyHostToDevice, st));
     QF_HIP_TRY(hipStreamSynchronize(st));  // host vectors above go out of scope with this function only
 
     std::vector<const qf::TileOp*> bases;
@@ -459,12 +486,12 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         if (R.precision == QF_C128)
             hipLaunchKernelGGL(k_pauli_apply<double>, eg, dim3(qf::kThreads), 0, st,
                                static_cast<const cx<double>*>(psi.p), static_cast<cx<double>*>(lam_out), n,
-                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
                                static_cast<qf::dd*>(epart.p));
         else
             hipLaunchKernelGGL(k_pauli_apply<float>, eg, dim3(qf::kThreads), 0, st,
                                static_cast<const cx<float>*>(psi.p), static_cast<cx<float>*>(lam_out), n,
-                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
                                static_cast<qf::dd*>(epart.p));
         if ((s = launch_check("k_pauli_apply"))) return s;
         hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(cnt), dim3(qf::kThreads), 0, st,
@@ -579,9 +606,10 @@ qf_status backprop_set(GradRun& R, const double* inputs, const double* params, d
     if ((s = part.alloc(size_t(eblk) * 2 * sizeof(qf::dd)))) return s;
     if ((s = eout.alloc(2 * sizeof(double)))) return s;
     const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
-    if ((s = terms.alloc(std::max<size_t>(tsorted.size(), 1) * sizeof(qf::PauliDev)))) return s;
-    if (!tsorted.empty())
-        QF_HIP_TRY(hipMemcpy(terms.p, tsorted.data(), tsorted.size() * sizeof(qf::PauliDev), hipMemcpyHostToDevice));
+    uint32_t ngroups = 0;
+    const std::vector<unsigned char> tpack = pack_terms(tsorted, &ngroups);
+    if ((s = terms.alloc(tpack.size()))) return s;
+    QF_HIP_TRY(hipMemcpy(terms.p, tpack.data(), tpack.size(), hipMemcpyHostToDevice));
     char* base = static_cast<char*>(states.p);
     auto sp_at = [&](size_t k) { return static_cast<void*>(base + k * dim * ab); };
     if (R.precision == QF_C128)
@@ -618,12 +646,12 @@ qf_status backprop_set(GradRun& R, const double* inputs, const double* params, d
     if (R.precision == QF_C128)
         hipLaunchKernelGGL(k_pauli_apply<double>, dim3(eblk, 1), dim3(qf::kThreads), 0, st,
                            static_cast<const cx<double>*>(sp_at(R.pc.nops)), static_cast<cx<double>*>(lam.p), n,
-                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
                            static_cast<qf::dd*>(part.p));
     else
         hipLaunchKernelGGL(k_pauli_apply<float>, dim3(eblk, 1), dim3(qf::kThreads), 0, st,
                            static_cast<const cx<float>*>(sp_at(R.pc.nops)), static_cast<cx<float>*>(lam.p), n,
-                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()),
+                           static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
                            static_cast<qf::dd*>(part.p));
     if ((s = launch_check("k_pauli_apply"))) return s;
     double e2[2];
