@@ -342,10 +342,35 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
     }
 }
 
+// Recorded overlaps wait in LDS in batches of kRecBatch: each wave adds its
+// share (lane shuffles, no barrier) into a row of red, and one pass over the
+// batch with a single barrier writes the tile sums (waves added in order).
+constexpr uint32_t kRecBatch = 32;
+constexpr uint32_t kRedDoubles = kRecBatch * (kThreads / 64) + kRecBatch;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Writes the pending records' tile sums (row r: waves' values, then the
+// slot at red[kRecBatch * W + r]) and empties the batch.
+__device__ __forceinline__ void flush_records(double* red, uint32_t& nrec, double* __restrict__ prow) {
+    const uint32_t W = blockDim.x / 64;
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < nrec; r += blockDim.x) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < W; ++w) t += red[r * W + w];
+        prow[uint32_t(red[kRecBatch * W + r])] = t;
+    }
+    __syncthreads();
+    nrec = 0;
+}
+
 // One workgroup's tile: block index b (instance = b >> log_blocks).  ops[0]
 // is always a TOP_PHASE header.  NS = 1: s0 is the state; NS = 2: s0 = psi,
 // s1 = lambda, partial as k_tile_adjoint.  `xb` is the LDS exchange buffer
-// (NS x 2^M amplitudes), `red` 4 doubles of LDS.
+// (NS x 2^M amplitudes), `red` kRedDoubles doubles of LDS.
 template <class R, int RB, int NS>
 __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __restrict__ s1, const TileArgs& args,
                                               const TileOp* __restrict__ ops, const double* __restrict__ cs,
@@ -373,6 +398,7 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
 
     RegMap<RB> mp;
     decode_map<RB>(uint32_t(ops[0].q0), args, base, mp);
+    uint32_t nrec = 0;
     cx<R> a[NS][NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -397,8 +423,11 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
         }
         if constexpr (NS == 2) {
             if ((op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD)) {
-                const double v = block_sum(reg_overlap<R, RB>(a[0], a[1], mp, op), red);
-                if (threadIdx.x == 0) prow[op.slot] = v;
+                const double v = wave_sum(reg_overlap<R, RB>(a[0], a[1], mp, op));
+                const uint32_t W = blockDim.x / 64;
+                if (threadIdx.x % 64 == 0) red[nrec * W + threadIdx.x / 64] = v;
+                if (threadIdx.x == 0) red[kRecBatch * W + nrec] = double(op.slot);
+                if (++nrec == kRecBatch) flush_records(red, nrec, prow);
             }
         }
 #pragma unroll
@@ -409,6 +438,9 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
         s0[mp.gt | mp.gk[k]] = a[0][k];
         if constexpr (NS == 2) s1[mp.gt | mp.gk[k]] = a[1][k];
     }
+    if constexpr (NS == 2) {
+        if (nrec) flush_records(red, nrec, prow);
+    }
 }
 
 template <class R, int RB, int NS>
@@ -416,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_reg(cx<R>* __restrict__ s0, c
                                                        const TileOp* __restrict__ ops, const double* __restrict__ cs,
                                                        double* __restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[kThreads / 64];
+    __shared__ double red[kRedDoubles];
     const uint64_t b = uint64_t(blockIdx.x) + uint64_t(blockIdx.y) * gridDim.x;
     tile_reg_body<R, RB, NS>(s0, s1, args, ops, cs, partial, b, reinterpret_cast<cx<R>*>(smem_raw), red);
 }

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(qf::kThreads) void k_tile_reg_grouped(cx<R>* __rest
                                                                    const double* __restrict__ cs,
                                                                    double* __restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[qf::kThreads / 64];
+    __shared__ double red[qf::kRedDoubles];
     const GroupHeader* h = reinterpret_cast<const GroupHeader*>(launch);
     const TileEntry* E = reinterpret_cast<const TileEntry*>(launch + sizeof(GroupHeader));
     const uint64_t total = h->total_blocks;
