@@ -317,6 +317,48 @@ bool choose_set(const Scheduler& sch, const std::vector<Node>& nodes, std::vecto
     return sch.simulate(h, in, nullptr) > 0;
 }
 
+// The set of a compiled pass with one pass of lookahead: candidates from the
+// four tie-break policies, each scored by the gates it runs plus the gates
+// the greedy set after it (same low-slot holders) would run.  The first best
+// candidate wins.
+bool choose_set_lookahead(Scheduler& sch, const std::vector<Node>& nodes, std::vector<char>& in, uint32_t low,
+                          uint32_t M, const std::vector<uint32_t>& all_q, const std::vector<uint32_t>& at) {
+    const int tie0 = sch.tie, block0 = sch.block;
+    const int n = int(sch.qubits);
+    const int pols[4][2] = {{0, 1}, {1, 1}, {2, n}, {2, n + n / 3}};
+    const std::vector<size_t> head0 = sch.head;
+    std::vector<char> best_in;
+    long best = -1;
+    for (const auto& pol : pols) {
+        sch.tie = pol[0];
+        sch.block = pol[1] > 0 ? pol[1] : 1;
+        std::vector<char> cand(in);
+        if (!choose_set(sch, nodes, cand, low, M, all_q)) continue;
+        std::vector<size_t> h = head0;
+        const int now = sch.simulate(h, cand, nullptr);
+        sch.tie = tie0;
+        sch.block = block0;
+        sch.head = h;
+        std::vector<char> nx(sch.qubits, 0);
+        for (uint32_t b = 0; b < low; ++b) nx[at[b]] = 1;
+        int next = 0;
+        if (choose_set(sch, nodes, nx, low, M, all_q)) {
+            std::vector<size_t> h2 = h;
+            next = sch.simulate(h2, nx, nullptr);
+        }
+        sch.head = head0;
+        if (long(now) + next > best) {
+            best = long(now) + next;
+            best_in = cand;
+        }
+    }
+    sch.tie = tie0;
+    sch.block = block0;
+    if (best < 0) return false;
+    in = best_in;
+    return true;
+}
+
 TileOp make_reg_op(const Node& n, const std::vector<int>& reg_index);
 
 // Splits one pass's ops into register phases of RB tile bits each.  Emits a
@@ -874,7 +916,9 @@ std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_
                 }
             }
         }
-        if (!chosen && !choose_set(sch, nodes, in, low, M, all_q)) return "planner made no progress";
+        if (!chosen && !(out.compiled ? choose_set_lookahead(sch, nodes, in, low, M, all_q, at)
+                                      : choose_set(sch, nodes, in, low, M, all_q)))
+            return "planner made no progress";
         std::vector<int> order;
         sch.simulate(sch.head, in, &order);
         if (order.empty()) return "planner made no progress";
