@@ -445,6 +445,7 @@ private:
     // each register's offset are computed apart and combined with one XOR.
     struct Swz {
         std::vector<uint32_t> h;
+        bool add = false;  // thread and register parts of every address have disjoint bits
     };
     uint32_t nb() const { return dbl_ ? 4 : 5; }
     static uint32_t swz_apply(const Swz& z, uint32_t j) {
@@ -498,10 +499,31 @@ private:
     // required; also asking it of the first nb-1 lanes over nb-1 bank bits
     // (LDS that serves half as many lanes at once) is preferred; single-bit
     // vectors are tried first (two instructions each).  No solution: h = 0.
-    Swz choose_swizzle(const std::vector<int>& w, const std::vector<int>& r, uint32_t bits) const {
+    // Buffer index bits (bit c taken out; c < 0: none) of the tile bits that
+    // are thread bits (want_reg false) or register bits of a layout.
+    std::vector<int> buffer_bits(const Layout& L, int c, bool want_reg) const {
+        std::vector<int> out;
+        for (uint32_t t = 0; t < M_; ++t)
+            if ((want_reg ? L.reg[t] : L.thr[t]) >= 0 && int(t) != c) out.push_back(c >= 0 && int(t) > c ? int(t) - 1 : int(t));
+        return out;
+    }
+    // Also preferred: on each side, the bits a thread's part of an address
+    // can set and those a register's part can set are disjoint, so the
+    // register offsets add to the thread base (the LDS instruction's
+    // immediate offset) instead of taking an XOR per access.
+    Swz choose_swizzle(const std::vector<int>& w, const std::vector<int>& r, uint32_t bits, const Layout& lw,
+                       const Layout& lr, int c) const {
         const uint32_t n = nb();
         Swz z;
         z.h.assign(bits, 0);
+        const std::vector<int> wt = buffer_bits(lw, c, false), wr = buffer_bits(lw, c, true);
+        const std::vector<int> rt = buffer_bits(lr, c, false), rr = buffer_bits(lr, c, true);
+        auto used = [&](const std::vector<int>& ps) {
+            uint32_t u = 0;
+            for (int p : ps) u |= (1u << p) | z.h[size_t(p)];
+            return u;
+        };
+        auto additive = [&] { return (used(wt) & used(wr)) == 0 && (used(rt) & used(rr)) == 0; };
         std::vector<int> free;
         for (const std::vector<int>* side : {&w, &r})
             for (int p : *side)
@@ -510,9 +532,13 @@ private:
         for (uint32_t b = 0; b < n; ++b) cand.push_back(1u << b);
         for (uint32_t v = 1; v < (1u << n); ++v)
             if (v & (v - 1)) cand.push_back(v);
-        auto ok = [&](bool strict) {
+        cand.push_back(0);  // no fold: admitted only with rank slack
+        // slack: lanes allowed to share a bank slot's row (rank deficit).
+        int slack = 0;
+        auto ok = [&](bool strict, bool add) {
+            if (add && !additive()) return false;
             for (const std::vector<int>* side : {&w, &r}) {
-                if (rank_of(z, *side, n, n) < side->size()) return false;
+                if (int(rank_of(z, *side, n, n)) + slack < int(side->size())) return false;
                 if (strict) {
                     std::vector<int> sub;
                     for (int p : *side)
@@ -522,16 +548,29 @@ private:
             }
             return true;
         };
-        std::function<bool(size_t, bool)> search = [&](size_t i, bool strict) {
-            if (i == free.size()) return ok(strict);
+        std::function<bool(size_t, bool, bool)> search = [&](size_t i, bool strict, bool add) {
+            if (i == free.size()) return ok(strict, add);
             for (uint32_t v : cand) {
                 z.h[size_t(free[i])] = v;
-                if (search(i + 1, strict)) return true;
+                if (search(i + 1, strict, add)) return true;
             }
             z.h[size_t(free[i])] = 0;
             return false;
         };
-        if (!search(0, true) && !search(0, false)) z.h.assign(bits, 0);
+        if (search(0, true, true) || search(0, false, true)) {
+            z.add = true;
+            return z;
+        }
+        // An add per access costs no instruction: it is worth a two-way bank
+        // conflict on one side (measured; more sharing than that is not).
+        slack = 1;
+        if (search(0, false, true)) {
+            z.add = true;
+            return z;
+        }
+        slack = 0;
+        if (!search(0, true, false) && !search(0, false, false)) z.h.assign(bits, 0);
+        z.add = additive();
         return z;
     }
 
@@ -653,14 +692,15 @@ private:
             return;
         }
         acc_flush_all();
-        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, -1), lane_bits(next.thr, -1), M_);
+        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, -1), lane_bits(next.thr, -1), M_,
+                                     Layout{reg_of_tile_, thr_of_tile_}, next, -1);
         if (lds_used) o_ << "  __syncthreads();\n";
         o_ << "  {\n    const unsigned jt = " << thread_scatter(false) << ";\n";
         o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
         // Lambda's tile follows psi's in the shared array.
         each_state([&] {
             for (uint32_t k = 0; k < NR_; ++k)
-                o_ << "    " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(cur_) << M_)) << " = C{" << rv(k)
+                o_ << "    " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(cur_) << M_), z.add) << " = C{" << rv(k)
                    << ", " << iv(k) << "};\n";
         });
         o_ << "  }\n  __syncthreads();\n";
@@ -670,17 +710,18 @@ private:
         o_ << "    const unsigned st = " << swz_expr(z, "jt") << " << " << csz_log() << ";\n";
         for (int sn = 0; sn < ns_; ++sn)
             for (uint32_t k = 0; k < NR_; ++k)
-                o_ << "    { const C v = " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(sn) << M_)) << "; "
+                o_ << "    { const C v = " << lds_at("st", swz_apply(z, reg_tile(k)) | (uint32_t(sn) << M_), z.add) << "; "
                    << reg_name(sn, 'r', k) << " = v.re; " << reg_name(sn, 'i', k) << " = v.im; }\n";
         o_ << "  }\n";
     }
 
     // The shared array's element at byte offset `base` ^ (element address
     // a * element size): one XOR per access on a byte offset computed once
-    // per exchange.
+    // per exchange, or, when the two parts have disjoint bits, an add that
+    // the LDS instruction's immediate offset takes.
     uint32_t csz_log() const { return dbl_ ? 4 : 3; }
-    std::string lds_at(const std::string& base, uint32_t j) const {
-        return "(*(C*)((char*)xb + (" + base + " ^ " + std::to_string(j << csz_log()) + "u)))";
+    std::string lds_at(const std::string& base, uint32_t j, bool add = false) const {
+        return "(*(C*)((char*)xb + (" + base + (add ? " + " : " ^ ") + std::to_string(j << csz_log()) + "u)))";
     }
 
     // Tile index with bit c taken out (the half-buffer index).
@@ -693,11 +734,12 @@ private:
     void exchange_half(const Layout& next, bool lds_used, int c) {
         acc_flush_all();
         const uint32_t cb = uint32_t(c);
-        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, c), lane_bits(next.thr, c), M_ - 1);
+        const Swz z = choose_swizzle(lane_bits(thr_of_tile_, c), lane_bits(next.thr, c), M_ - 1,
+                                     Layout{reg_of_tile_, thr_of_tile_}, next, c);
         std::vector<std::string> writes;
         each_state([&] {
             for (uint32_t k = 0; k < NR_; ++k)
-                writes.push_back(lds_at("sw", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(cur_) << (M_ - 1))) +
+                writes.push_back(lds_at("sw", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(cur_) << (M_ - 1)), z.add) +
                                  " = C{" + rv(k) + ", " + iv(k) + "};");
         });
         o_ << "  {\n    const unsigned jw = " << thread_scatter(false) << ";\n";
@@ -718,7 +760,7 @@ private:
             for (int sn = 0; sn < ns_; ++sn)
                 for (uint32_t k = 0; k < NR_; ++k)
                     o_ << "      { const C v = "
-                       << lds_at("sr", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(sn) << (M_ - 1))) << "; "
+                       << lds_at("sr", swz_apply(z, drop_bit(reg_tile(k), cb)) | (uint32_t(sn) << (M_ - 1)), z.add) << "; "
                        << reg_name(sn, 'r', k) << " = v.re; " << reg_name(sn, 'i', k) << " = v.im; }\n";
             o_ << "    }\n";
         }
