@@ -364,13 +364,46 @@ TileOp make_reg_op(const Node& n, const std::vector<int>& reg_index);
 // Splits one pass's ops into register phases of RB tile bits each.  Emits a
 // TOP_PHASE header (packed tile bits of the register set, register index i
 // <-> 4 bits at 4*i) before each phase's ops.
+std::string build_phases_tie(const std::vector<Node>& nodes, const std::vector<int>& order, uint32_t qubits,
+                             const std::vector<int>& tile_bit, uint32_t RB, std::vector<TileOp>& ops_out,
+                             uint64_t& phase_count, int tie);
+
+// Register phases of one pass: the greedy split under each tie-break policy
+// of the set choice, keeping the one with the fewest phases (each phase
+// change is an exchange of the whole tile through LDS).
 std::string build_phases(const std::vector<Node>& nodes, const std::vector<int>& order, uint32_t qubits,
                          const std::vector<int>& tile_bit, uint32_t RB, std::vector<TileOp>& ops_out,
                          uint64_t& phase_count) {
+    std::vector<TileOp> best;
+    uint64_t best_n = 0;
+    std::string err;
+    for (int tie = 0; tie < 2; ++tie) {
+        std::vector<TileOp> ops;
+        uint64_t n = 0;
+        std::string e = build_phases_tie(nodes, order, qubits, tile_bit, RB, ops, n, tie);
+        if (!e.empty()) {
+            err = e;
+            continue;
+        }
+        if (best.empty() || n < best_n) {
+            best = std::move(ops);
+            best_n = n;
+        }
+    }
+    if (best.empty()) return err;
+    ops_out.insert(ops_out.end(), best.begin(), best.end());
+    phase_count += best_n;
+    return "";
+}
+
+std::string build_phases_tie(const std::vector<Node>& nodes, const std::vector<int>& order, uint32_t qubits,
+                             const std::vector<int>& tile_bit, uint32_t RB, std::vector<TileOp>& ops_out,
+                             uint64_t& phase_count, int tie) {
     std::vector<Node> sub;
     sub.reserve(order.size());
     for (int id : order) sub.push_back(nodes[id]);
     Scheduler ps(sub, qubits, 1);
+    ps.tie = tie;
     std::vector<uint32_t> tq;
     for (uint32_t q = 0; q < qubits; ++q)
         if (tile_bit[q] >= 0) tq.push_back(q);
