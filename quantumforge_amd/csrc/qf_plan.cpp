@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
 
 namespace qf {
 
@@ -811,8 +813,95 @@ std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_
 uint32_t default_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
 uint32_t default_low_bits(int precision) { return precision == 0 ? 5 : 6; }
 
+namespace {
+
+// Plans by content: a circuit run again and again (a training loop, the
+// gradient engine's per-call plans) is planned once.  The key covers every
+// input of build_plan; a copied plan shares its compiled programs.
+std::mutex g_plan_mu;
+std::map<std::string, Plan> g_plans;
+std::deque<std::string> g_plan_order;
+constexpr size_t kPlanCacheEntries = 64;
+
+std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt) {
+    uint64_t h1 = 1469598103934665603ull, h2 = 0x9e3779b97f4a7c15ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) {
+            h1 = (h1 ^ b[i]) * 1099511628211ull;
+            h2 = (h2 + b[i] + 1) * 0xff51afd7ed558ccdull;
+            h2 ^= h2 >> 31;
+        }
+    };
+    auto val = [&](int64_t v) { mix(&v, sizeof v); };
+    auto dbl = [&](double v) { mix(&v, sizeof v); };
+    val(qubits);
+    val(precision);
+    val(opt.fuse);
+    val(opt.tile_qubits);
+    val(opt.low_qubits);
+    val(opt.max_block);
+    val(opt.kernel);
+    val(opt.reg_bits);
+    val(opt.relabel);
+    val(opt.compile);
+    val(relabel_enabled() ? 1 : 0);
+    val(int64_t(gates.size()));
+    for (const GateIn& g : gates) {
+        val(g.kind);
+        val(g.a);
+        val(g.b);
+        for (const cd& c : g.m) {
+            dbl(c.re);
+            dbl(c.im);
+        }
+        if (g.kind == 6)
+            for (const cd& c : g.m4) {
+                dbl(c.re);
+                dbl(c.im);
+            }
+        val(g.axis);
+        val(g.param);
+        val(g.slot);
+        val(g.flags);
+    }
+    char b[40];
+    std::snprintf(b, sizeof b, "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
+    return b;
+}
+
+std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubits, int precision,
+                                const PlanOptions& opt, Plan& out);
+
+}  // namespace
+
 std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt,
                        Plan& out) {
+    const std::string key = plan_key(gates, qubits, precision, opt);
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_plans.find(key);
+        if (it != g_plans.end()) {
+            out = it->second;
+            return "";
+        }
+    }
+    std::string e = build_plan_uncached(gates, qubits, precision, opt, out);
+    if (!e.empty()) return e;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_plans.emplace(key, out).second) {
+        g_plan_order.push_back(key);
+        if (g_plan_order.size() > kPlanCacheEntries) {
+            g_plans.erase(g_plan_order.front());
+            g_plan_order.pop_front();
+        }
+    }
+    return "";
+}
+
+namespace {
+std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubits, int precision,
+                                const PlanOptions& opt, Plan& out) {
     out = Plan{};
     out.qubits = qubits;
     out.precision = precision;
@@ -905,6 +994,8 @@ std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int pr
     out = std::move(best);
     return "";
 }
+
+}  // namespace
 
 namespace {
 
