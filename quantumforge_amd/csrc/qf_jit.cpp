@@ -222,7 +222,7 @@ private:
     int rb_[8] = {0};
     std::vector<int> reg_of_tile_, thr_of_tile_;
     std::vector<int> var_;
-    zc g_{1.0, 0.0};              // the tile holds g_ times its amplitudes (take_scale)
+    zc g_{1.0, 0.0};              // the true amplitudes are g_ times the tile's (take_scale)
     bool relabel_ = false;        // the store permutes the tile's bits (Pass::store_phys)
     bool half_ = false;           // exchanges through a half-tile buffer
     // Half-buffer exchanges in emission order: the round bit, and for an
@@ -828,7 +828,7 @@ private:
     // need no multiply (Rx and Ry become one FMA per output component
     // instead of a multiply and an FMA); a diagonal one divided by its first
     // entry leaves that entry's registers untouched (Rz multiplies half the
-    // registers).  The tile then holds g times the true amplitudes; g is
+    // registers).  The true amplitudes are then g times the tile's; g is
     // global to the tile (every dense op touches every amplitude once,
     // everything else is linear), so the next literal dense op takes it into
     // its matrix and the store applies what is left.
@@ -1114,12 +1114,14 @@ private:
 
     // Im <lambda| G |psi> for a recorded rotation's generator G, before the
     // rotation: each thread's share over its registers in FP64, summed over
-    // the wave with lane shuffles into the shared table qred (row r, one
+    // the wave (DPP row operations) into the shared table qred (row r, one
     // value per wave); the tile's sum goes out after the store.  Pending
-    // diagonal factors are phases common to both states and cancel in
-    // conj(lambda) psi, except across the pair of an X or Y generator's bit,
-    // whose factors are applied first.  The tile-wide scale g (take_scale)
-    // is in both states, so the share is divided by g^2.
+    // per-bit diagonal factors are phases common to both states and cancel
+    // in conj(lambda) psi, except across the pair of an X or Y generator's
+    // bit, whose factors are applied first.  The factors common to all of a
+    // thread's amplitudes (the literal g of take_scale, the thread
+    // accumulator with its rotation scales) are still owed by both states,
+    // so the share is multiplied by their squared magnitudes.
     void record(const TileOp& op, uint32_t r, uint32_t W) {
         const std::string v = "q" + std::to_string(opn_);
         auto val = [&](int st, char part, uint32_t k) { return "double(" + reg_name(st, part, uint32_t(var_[k])) + ")"; };
@@ -1152,9 +1154,12 @@ private:
             }
             if (!rt.empty()) o_ << "  if (" << rt << ") " << v << " = -" << v << ";\n";
         }
-        if (g_ != zc(1.0, 0.0)) o_ << "  " << v << " *= " << hexlit(1.0 / std::norm(g_)).substr(1) << ";\n";
-        o_ << "  for (int m = 32; m >= 1; m >>= 1) " << v << " += __shfl_xor(" << v << ", m, 64);\n";
-        o_ << "  if ((t & 63u) == 0u) qred[" << r * W << "u + (t >> 6)] = " << v << ";\n";
+        if (g_ != zc(1.0, 0.0)) o_ << "  " << v << " *= " << hexlit(std::norm(g_)).substr(1) << ";\n";
+        if (acc_[0].on)
+            o_ << "  " << v << " *= double(" << acc_[0].r << ") * double(" << acc_[0].r << ")"
+               << (acc_[0].real ? "" : " + double(" + acc_[0].i + ") * double(" + acc_[0].i + ")") << ";\n";
+        o_ << "  " << v << " = qf_wave_sum(" << v << ");\n";
+        o_ << "  if ((t & 63u) == 63u) qred[" << r * W << "u + (t >> 6)] = " << v << ";\n";
     }
 
     void rot(const TileOp& op) {
@@ -1163,7 +1168,14 @@ private:
         if (op.q1 == 2) {  // exp(-i theta Z / 2): (c - is) on bit 0, (c + is) on bit 1
             const Bit b = bit_of(op.q0);
             if (b.kind == Bit::REG) {
-                for (uint32_t k = 0; k < NR_; ++k) mul_rt(k, c, s, !((k >> b.idx) & 1));
+                // (c - is) for the whole thread (its scale accumulator), and
+                // (c + is) / (c - is) = (c^2 - s^2) + i 2cs on bit 1 only.
+                const std::string er = "e" + std::to_string(opn_) + "r", ei = "e" + std::to_string(opn_) + "i";
+                o_ << "  const R " << er << " = " << c << " * " << c << " - " << s << " * " << s << ", " << ei
+                   << " = R(2) * " << c << " * " << s << ";\n";
+                acc_mul(acc_[0], c, "(-" + s + ")");
+                for (uint32_t k = 0; k < NR_; ++k)
+                    if ((k >> b.idx) & 1) mul_rt(k, er, ei);
                 return;
             }
             const std::string sz = "z" + std::to_string(opn_);
@@ -1171,26 +1183,45 @@ private:
             acc_mul(acc_[0], c, sz);
             return;
         }
+        // The larger of |c|, |s| goes into the thread's scale accumulator and
+        // the matrix keeps a unit pair and the ratio: one FMA per output
+        // component.  Which form applies is one branch per rotation, uniform
+        // over the workgroup (one instance's angles).
         const uint32_t i = uint32_t(op.q0);
         acc_flush_bit(i);
-        each_state([&] { rot_xy_regs(op, i, c, s); });
+        const std::string w = "w" + std::to_string(opn_), f = "f" + std::to_string(opn_),
+                          u = "u" + std::to_string(opn_);
+        o_ << "  const bool " << w << " = __builtin_fabs(" << c << ") >= __builtin_fabs(" << s << ");\n";
+        o_ << "  const R " << f << " = " << w << " ? " << c << " : " << s << ";\n";
+        o_ << "  const R " << u << " = " << w << " ? " << s << " / " << c << " : " << c << " / " << s << ";\n";
+        acc_mul(acc_[0], f, "");
+        o_ << "  if (" << w << ") {\n";
+        each_state([&] { rot_xy_regs(op, i, u, true); });
+        o_ << "  } else {\n";
+        each_state([&] { rot_xy_regs(op, i, u, false); });
+        o_ << "  }\n";
     }
-    void rot_xy_regs(const TileOp& op, uint32_t i, const std::string& c, const std::string& s) {
+    // Rx = c [[1, -it], [-it, 1]] = s [[u, -i], [-i, u]] and
+    // Ry = c [[1, -t], [t, 1]] = s [[u, -1], [1, u]] (t = s / c, u = c / s),
+    // the scale outside: by_c selects the first form, with ratio t = u.
+    void rot_xy_regs(const TileOp& op, uint32_t i, const std::string& u, bool by_c) {
         for (uint32_t k = 0; k < NR_; ++k) {
             if ((k >> i) & 1) continue;
             const uint32_t k1 = k | (1u << i);
             const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
             o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
-            if (op.q1 == 0) {  // Rx = [[c, -is], [-is, c]]
-                o_ << "    " << rA << " = " << c << " * a0r + " << s << " * a1i; " << iA << " = " << c << " * a0i - "
-                   << s << " * a1r;\n";
-                o_ << "    " << rB << " = " << c << " * a1r + " << s << " * a0i; " << iB << " = " << c << " * a1i - "
-                   << s << " * a0r; }\n";
-            } else {  // Ry = [[c, -s], [s, c]]
-                o_ << "    " << rA << " = " << c << " * a0r - " << s << " * a1r; " << iA << " = " << c << " * a0i - "
-                   << s << " * a1i;\n";
-                o_ << "    " << rB << " = " << s << " * a0r + " << c << " * a1r; " << iB << " = " << s << " * a0i + "
-                   << c << " * a1i; }\n";
+            if (op.q1 == 0 && by_c) {
+                o_ << "    " << rA << " = a0r + " << u << " * a1i; " << iA << " = a0i - " << u << " * a1r;\n";
+                o_ << "    " << rB << " = a1r + " << u << " * a0i; " << iB << " = a1i - " << u << " * a0r; }\n";
+            } else if (op.q1 == 0) {
+                o_ << "    " << rA << " = " << u << " * a0r + a1i; " << iA << " = " << u << " * a0i - a1r;\n";
+                o_ << "    " << rB << " = " << u << " * a1r + a0i; " << iB << " = " << u << " * a1i - a0r; }\n";
+            } else if (by_c) {
+                o_ << "    " << rA << " = a0r - " << u << " * a1r; " << iA << " = a0i - " << u << " * a1i;\n";
+                o_ << "    " << rB << " = " << u << " * a0r + a1r; " << iB << " = " << u << " * a0i + a1i; }\n";
+            } else {
+                o_ << "    " << rA << " = " << u << " * a0r - a1r; " << iA << " = " << u << " * a0i - a1i;\n";
+                o_ << "    " << rB << " = a0r + " << u << " * a1r; " << iB << " = a0i + " << u << " * a1i; }\n";
             }
         }
     }
@@ -1395,8 +1426,25 @@ std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, s
         prog->threads.push_back(1u << (p.tile_bits - p.reg_bits));
         bodies.push_back(std::move(body));
     }
-    const std::string prelude = precision == 0 ? "struct alignas(16) C { double re, im; };\n"
-                                               : "struct alignas(8) C { float re, im; };\n";
+    // qf_wave_sum: a wave's sum in lane 63 by DPP row operations on the two
+    // halves of each double (quad swaps, half-row and row mirrors, then the
+    // row 15 and 31 broadcasts into rows 1, 3 and 2, 3), with no LDS traffic.
+    // Defined only where the adjoint form uses it.
+    std::string prelude = precision == 0 ? "struct alignas(16) C { double re, im; };\n"
+                                         : "struct alignas(8) C { float re, im; };\n";
+    if (ns == 2)
+        prelude +=
+            "#ifndef QF_WAVE_SUM_DEFINED\n"
+            "template <int CTRL, int ROWS> __device__ inline double qf_dpp(double v) {\n"
+            "  const long long b = __builtin_bit_cast(long long, v);\n"
+            "  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xf, false);\n"
+            "  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xf, false);\n"
+            "  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));\n"
+            "}\n"
+            "__device__ inline double qf_wave_sum(double v) {\n"
+            "  v += qf_dpp<0xB1, 0xf>(v);\n  v += qf_dpp<0x4E, 0xf>(v);\n  v += qf_dpp<0x141, 0xf>(v);\n"
+            "  v += qf_dpp<0x140, 0xf>(v);\n  v += qf_dpp<0x142, 0xa>(v);\n  v += qf_dpp<0x143, 0xc>(v);\n"
+            "  return v;\n}\n#endif\n";
     std::string all = prelude;
     for (const auto& b : bodies) all += b;
     std::string optstr;

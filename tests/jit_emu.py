@@ -45,6 +45,17 @@ static inline double __shfl_xor(double v, int m, int) {
     qf_bar->arrive_and_wait();
     return r;
 }
+// The wave sum the generated adjoint code leaves in lane 63 (here in every
+// lane), added in lane order.
+#define QF_WAVE_SUM_DEFINED
+static inline double qf_wave_sum(double v) {
+    qf_shfl[qf_tid.x] = v;
+    qf_bar->arrive_and_wait();
+    double s = 0.0;
+    for (unsigned l = qf_tid.x & ~63u; l < (qf_tid.x & ~63u) + 64u; ++l) s += qf_shfl[l];
+    qf_bar->arrive_and_wait();
+    return s;
+}
 """
 
 LAUNCHER = r"""

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from quantumforge_amd import ParameterizedCircuit, QuantumError, RotationAxis
+from quantumforge_amd import Gate, ParameterizedCircuit, QuantumError, RotationAxis
 from quantumforge_amd.backend import Plan
 from oracle import np_oracle as NP
 from quantumforge_amd.workloads import (MaxCutProblem, hea_circuit, heisenberg_hamiltonian, qaoa_parameterized,
@@ -228,16 +228,27 @@ def test_generated_kernels_read_rotation_angles_per_instance(case):
         close(out[i], O.pc_run(pops, P, 0, sets[i], [], n), 1e-13)
 
 
-@pytest.mark.parametrize("n,M,RB,precision", [(11, 9, 3, "complex128"), (12, 10, 3, "complex128"),
-                                                (12, 10, 4, "complex64")])
-def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision):
+def with_fixed_gates(pc: ParameterizedCircuit, n: int) -> ParameterizedCircuit:
+    # Fixed gates between the rotations: literal dense ops (H, Y) and
+    # diagonal ones (T, S) in the compiled passes, scales carried out of them.
+    for q in range(n):
+        pc.push([Gate.H, Gate.T, Gate.Y, Gate.S][q % 4], q)
+    return pc
+
+
+@pytest.mark.parametrize("n,M,RB,precision,circ", [(11, 9, 3, "complex128", "hea"), (12, 10, 3, "complex128", "hea"),
+                                                     (12, 10, 4, "complex64", "hea"),
+                                                     (11, 9, 3, "complex128", "mixed")])
+def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision, circ):
     # The compiled reverse sweep of the adjoint gradient: the forward plan's
     # kernels make psi, lambda = H psi, then the reverse plan's kernels in
     # their two-state form carry psi and lambda back and record
     # Im <lambda| G |psi> per tile for every rotation.  Summing the partial
     # table per parameter gives the oracle's adjoint gradient, and psi
     # returns to |0>.
-    pc = hea_circuit(n, 2)
+    pc = hea_circuit(n, 2) if circ == "hea" else with_fixed_gates(mixed_rotations(n), n)
+    if circ == "hea":
+        with_fixed_gates(pc, n)
     P = pc.parameter_count
     params = np.array(uniform_parameters(P, 1, seed=n + M)[0])
     H = heisenberg_hamiltonian(n, seed=3)
@@ -247,7 +258,7 @@ def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision):
     fwd = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
     rev = Plan.parameterized(pc, n, reverse=True, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
     src = rev.source(precision, adjoint=True)
-    assert "__shfl_xor" in src and "partial[b * nslots" in src
+    assert "qf_wave_sum" in src and "partial[b * nslots" in src
     a0 = np.zeros(1 << n, complex)
     a0[0] = 1
     psi = emulate(fwd, a0, n, cs=cs, nslots=len(slots), precision=precision)
