@@ -343,13 +343,32 @@ __device__ __forceinline__ void reg_apply(cx<R> (&a)[1 << RB], const RegMap<RB>&
 }
 
 // Recorded overlaps wait in LDS in batches of kRecBatch: each wave adds its
-// share (lane shuffles, no barrier) into a row of red, and one pass over the
-// batch with a single barrier writes the tile sums (waves added in order).
+// share (DPP row operations, no LDS traffic, no barrier) into a row of red,
+// and one pass over the batch with a single barrier writes the tile sums
+// (waves added in order).
 constexpr uint32_t kRecBatch = 32;
 constexpr uint32_t kRedDoubles = kRecBatch * (kThreads / 64) + kRecBatch;
 
+// One DPP move of a double (two 32-bit halves); lanes of rows outside
+// ROWS (and lanes whose source is out of range) read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// The wave's sum, in lane 63: quad permutes (swap neighbours, swap pairs),
+// row half-mirror and mirror (every lane of a row holds its row's sum),
+// then row_bcast:15 and row_bcast:31 carry rows into rows 1 and 3, 2 and 3.
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    v += dpp_move<0xB1, 0xf>(v);
+    v += dpp_move<0x4E, 0xf>(v);
+    v += dpp_move<0x141, 0xf>(v);
+    v += dpp_move<0x140, 0xf>(v);
+    v += dpp_move<0x142, 0xa>(v);
+    v += dpp_move<0x143, 0xc>(v);
     return v;
 }
 
@@ -425,7 +444,7 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
             if ((op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD)) {
                 const double v = wave_sum(reg_overlap<R, RB>(a[0], a[1], mp, op));
                 const uint32_t W = blockDim.x / 64;
-                if (threadIdx.x % 64 == 0) red[nrec * W + threadIdx.x / 64] = v;
+                if (threadIdx.x % 64 == 63) red[nrec * W + threadIdx.x / 64] = v;
                 if (threadIdx.x == 0) red[kRecBatch * W + nrec] = double(op.slot);
                 if (++nrec == kRecBatch) flush_records(red, nrec, prow);
             }

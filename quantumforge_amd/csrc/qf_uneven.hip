@@ -406,7 +406,10 @@ qf_status capture_wave_graph(hipStream_t st, uint32_t F, uint32_t Rv, uint32_t R
                              const qf::TileOp* ops, const double* cs, double* part, qf::dd* epart,
                              const qf::PauliDev* terms, const uint32_t* pcsr, double* energies, double* grads,
                              const uint64_t* total_amps, hipGraphExec_t* exec, uint32_t* nodes) {
-    const dim3 tile_grid(1024), elem_grid(4096), inst_grid(512);
+    // The grouped tile kernels loop over their tiles; 8192 workgroups keep
+    // every CU at its LDS-limited occupancy (the reverse kernel runs one
+    // wave per workgroup).
+    const dim3 tile_grid(8192), elem_grid(4096), inst_grid(512);
     const dim3 fblock(1u << (Mf - RBf)), rblock(1u << (Mr - RBr));
     const size_t flds = (size_t(1) << Mf) * sizeof(cx<R>), rlds = 2 * (size_t(1) << Mr) * sizeof(cx<R>);
     auto fwd_fn = RBf == 3 ? reinterpret_cast<const void*>(&k_tile_reg_grouped<R, 3, 1>)
