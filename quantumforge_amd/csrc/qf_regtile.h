@@ -419,6 +419,8 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
     decode_map<RB>(uint32_t(ops[0].q0), args, base, mp);
     uint32_t nrec = 0;
     cx<R> a[NS][NR];
+    double t[NR];       // NS = 2: Im(conj(lambda) psi) per register while t_ok
+    bool t_ok = false;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         a[0][k] = s0[mp.gt | mp.gk[k]];
@@ -438,9 +440,44 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
             for (int s = 0; s < NS; ++s)
 #pragma unroll
                 for (int k = 0; k < NR; ++k) a[s][k] = xb[s * T + lds_slot<R>(mp.jt | mp.jk[k])];
+            t_ok = false;
             continue;
         }
         if constexpr (NS == 2) {
+            if (op.kind == TOP_ROTZZ || (op.kind == TOP_ROT && op.q1 == 2)) {
+                // Z / ZZ rotation of both states: one parity per register for
+                // the record and both products.  Im(conj(lambda) psi) per
+                // register is unchanged by diagonal unitaries applied to both
+                // states, so a run of recorded Z rotations computes it once.
+                R c, sn;
+                rot_cs<R>(op, cs_row, c, sn);
+                const bool rec = (op.flags & TF_RECORD) != 0;
+                if (rec && !t_ok) {
+#pragma unroll
+                    for (int k = 0; k < NR; ++k)
+                        t[k] = double(a[1][k].re) * double(a[0][k].im) - double(a[1][k].im) * double(a[0][k].re);
+                    t_ok = true;
+                }
+                const int q0 = op.q0, q1 = op.kind == TOP_ROTZZ ? op.q1 : -1;
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    const uint64_t gi = mp.gt | mp.gk[k];
+                    const bool odd = (((gi >> q0) ^ (q1 >= 0 ? (gi >> q1) : 0)) & 1) != 0;
+                    acc += odd ? -t[k] : t[k];
+                    a[0][k] = cm(c, odd ? sn : -sn, a[0][k]);
+                    a[1][k] = cm(c, odd ? sn : -sn, a[1][k]);
+                }
+                if (rec) {
+                    const double v = wave_sum(acc);
+                    const uint32_t W = blockDim.x / 64;
+                    if (threadIdx.x % 64 == 63) red[nrec * W + threadIdx.x / 64] = v;
+                    if (threadIdx.x == 0) red[kRecBatch * W + nrec] = double(op.slot);
+                    if (++nrec == kRecBatch) flush_records(red, nrec, prow);
+                }
+                continue;
+            }
+            t_ok = false;  // any other op changes the registers' overlaps
             if ((op.kind == TOP_ROT || op.kind == TOP_ROTZZ) && (op.flags & TF_RECORD)) {
                 const double v = wave_sum(reg_overlap<R, RB>(a[0], a[1], mp, op));
                 const uint32_t W = blockDim.x / 64;
