@@ -257,13 +257,31 @@ __device__ __forceinline__ void reg_diag(cx<R> (&a)[1 << RB], const RegMap<RB>& 
     }
 }
 
+// Bit k: parity of global qubits q0 (and q1 if >= 0) in register k's index.
+// The register offsets gk[k] are ORs of the single-bit offsets gk[1 << b]
+// and share no bit with the thread's gt, so one parity of gt and one per
+// register bit (uniform across the workgroup) give all 2^RB parities.
+template <int RB>
+__device__ __forceinline__ uint32_t reg_parities(const RegMap<RB>& mp, int q0, int q1) {
+    const uint64_t mask = (uint64_t(1) << q0) | (q1 >= 0 ? uint64_t(1) << q1 : 0);
+    uint32_t p = (__popcll(mp.gt & mask) & 1) ? (1u << (1 << RB)) - 1 : 0u;
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+        uint32_t pattern = 0;  // registers whose index has bit b set
+#pragma unroll
+        for (int k = 0; k < (1 << RB); ++k) pattern |= uint32_t((k >> b) & 1) << k;
+        if (__popcll(mp.gk[1 << b] & mask) & 1) p ^= pattern;
+    }
+    return p;
+}
+
 // exp(-i theta Z/2) or exp(-i theta Z Z/2) from (c, s): c - is on even parity, c + is on odd.
 template <class R, int RB>
 __device__ __forceinline__ void reg_rotz(cx<R> (&a)[1 << RB], const RegMap<RB>& mp, int q0, int q1, R c, R s) {
+    const uint32_t par = reg_parities<RB>(mp, q0, q1);
 #pragma unroll
     for (int k = 0; k < (1 << RB); ++k) {
-        const uint64_t gi = mp.gt | mp.gk[k];
-        const bool odd = (((gi >> q0) ^ (q1 >= 0 ? (gi >> q1) : 0)) & 1) != 0;
+        const bool odd = (par >> k) & 1;
         a[k] = cm(c, odd ? s : -s, a[k]);
     }
 }
@@ -458,12 +476,11 @@ __device__ __forceinline__ void tile_reg_body(cx<R>* __restrict__ s0, cx<R>* __r
                         t[k] = double(a[1][k].re) * double(a[0][k].im) - double(a[1][k].im) * double(a[0][k].re);
                     t_ok = true;
                 }
-                const int q0 = op.q0, q1 = op.kind == TOP_ROTZZ ? op.q1 : -1;
+                const uint32_t par = reg_parities<RB>(mp, op.q0, op.kind == TOP_ROTZZ ? op.q1 : -1);
                 double acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < NR; ++k) {
-                    const uint64_t gi = mp.gt | mp.gk[k];
-                    const bool odd = (((gi >> q0) ^ (q1 >= 0 ? (gi >> q1) : 0)) & 1) != 0;
+                    const bool odd = (par >> k) & 1;
                     acc += odd ? -t[k] : t[k];
                     a[0][k] = cm(c, odd ? sn : -sn, a[0][k]);
                     a[1][k] = cm(c, odd ? sn : -sn, a[1][k]);
