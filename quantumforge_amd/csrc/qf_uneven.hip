@@ -190,14 +190,17 @@ __global__ __launch_bounds__(qf::kThreads) void k_finish_grouped(const unsigned 
         if (threadIdx.x == 0) energies[e.out_index] = re.s + re.c;
         const uint32_t* pstart = pcsr + e.pcsr_off;
         const uint32_t* pslots = pstart + e.nparams + 1;
-        for (uint32_t p = threadIdx.x; p < e.nparams; p += blockDim.x) {
-            double g = 0.0;
+        // One parameter at a time, its occurrences' tile partials spread over
+        // the workgroup (an instance has few parameters and up to 2^16 tiles).
+        for (uint32_t p = 0; p < e.nparams; ++p) {
+            qf::dd acc{0, 0}, unused{0, 0};
             for (uint32_t q = pstart[p]; q < pstart[p + 1]; ++q) {
-                qf::dd acc{0, 0};
-                for (uint32_t b = 0; b < e.nblk; ++b) qf::neumaier(acc, partial[e.part_off + uint64_t(b) * e.nslots + pslots[q]]);
-                g += acc.s + acc.c;
+                const double* col = partial + e.part_off + pslots[q];
+                for (uint32_t b = threadIdx.x; b < e.nblk; b += blockDim.x) qf::neumaier(acc, col[uint64_t(b) * e.nslots]);
             }
-            grads[e.grad_off + p] = g;
+            __syncthreads();  // thread 0 has read the previous reduction's table
+            qf::block_reduce(acc, unused);
+            if (threadIdx.x == 0) grads[e.grad_off + p] = acc.s + acc.c;
         }
         __syncthreads();
     }
