@@ -510,6 +510,13 @@ def main() -> int:
 
     norm = state.norm_sqr()
     log(f"norm after {args.warmup + args.steps} runs: {norm:.15f}")
+    # The one-read-one-write ceiling of this device over the state's bytes:
+    # hand-written streaming kernels (16-byte words; nontemporal; 64 bytes per
+    # lane; one contiguous 64 KiB chunk per workgroup), same stream, same bytes.
+    stream_hip = {}
+    for v, name in ((0, "word"), (1, "nontemporal"), (2, "four_words"), (3, "chunk_64KiB")):
+        stream_hip[name] = round(state.stream_rate(v, 5)[0], 1)
+    log(f"in-place streaming kernels over the state: {stream_hip} GB/s")
     del state
     ceiling = inplace_stream_rate(torch, (1 << n) * amp_bytes)
     log(f"in-place streaming rate of a plain torch multiply over the same bytes: {ceiling:.0f} GB/s")
@@ -554,6 +561,8 @@ def main() -> int:
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "inplace_stream_GBs": round(ceiling, 1),
+            "stream_kernels_GBs": stream_hip,
+            "frac_of_stream_ceiling": round(achieved / max(max(stream_hip.values()), ceiling), 4),
         },
         "cpu_baseline": None,
     }

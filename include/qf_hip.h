@@ -170,6 +170,15 @@ typedef struct {
     int32_t precision;     /* QF_C128 or QF_C64: the amplitude type the defaults are chosen for */
 } qf_plan_options;
 
+/* Kernel families a plan launch can use (qf_plan_stats.kernels_run bits). */
+enum {
+    QF_KERNEL_SWEEP = 1,     /* one full-state sweep per gate (exact lowering) */
+    QF_KERNEL_LDS_TILE = 2,  /* LDS tile kernel, op by op */
+    QF_KERNEL_REG_TILE = 4,  /* interpreting register-tile kernel */
+    QF_KERNEL_COMPILED = 8,  /* generated per-pass kernels */
+    QF_KERNEL_MFMA = 16      /* dense-block matrix-core passes */
+};
+
 /* Summary of a plan (counts the HBM sweeps the plan streams). */
 typedef struct {
     uint64_t passes;       /* full-state sweeps (kernel launches that stream the state once) */
@@ -178,6 +187,8 @@ typedef struct {
     uint64_t dense_blocks; /* fused dense k-qubit blocks */
     uint64_t max_ops_per_pass;
     uint64_t phases;       /* register-kernel phases (LDS re-distributions + 1 per pass) */
+    uint32_t kernels_run;  /* QF_KERNEL_* mask of what the last run of this plan launched (0: not run) */
+    uint32_t runtime_rotations; /* rotations whose angles come from a per-set table (parameterized plans) */
 } qf_plan_stats;
 
 typedef struct qf_backend qf_backend;
@@ -218,6 +229,13 @@ qf_status qf_backend_stream(const qf_backend* backend, void** out);
 /* StateVector::zero (src/lib.rs:930) allocated directly on the device. */
 qf_status qf_state_zero(qf_backend* backend, uint32_t qubits, int32_t precision, qf_state** out);
 /* CudaBackend::upload, src/cuda.rs:844-867. */
+/* StateVector::zero_with_memory_budget, src/lib.rs:940-966: |0...0> only when
+ * the amplitude payload (2^qubits x 16 bytes complex128, 8 bytes complex64)
+ * fits budget_bytes; errors in the reference's order: ZeroQubits,
+ * SimulationSizeOverflow (64 or more qubits), MemoryBudgetExceeded, then the
+ * device limits of qf_state_zero. */
+qf_status qf_state_zero_with_memory_budget(qf_backend* backend, uint32_t qubits, int32_t precision,
+                                           uint64_t budget_bytes, qf_state** out);
 qf_status qf_state_upload(qf_backend* backend, const qf_c128* amps, uint32_t qubits, qf_state** out);
 /* complex64 upload (no reference counterpart: the reference is complex128 only). */
 qf_status qf_state_upload_c64(qf_backend* backend, const qf_c64* amps, uint32_t qubits, qf_state** out);
@@ -248,10 +266,24 @@ qf_status qf_state_apply_2q(qf_state* state, int32_t op, uint32_t a, uint32_t b)
  * lowers to Rz/CNOT up to the global phase e^{-i phi/4}). */
 qf_status qf_state_apply_cp(qf_state* state, uint32_t control, uint32_t target, double phi);
 /* CudaBackend::run_circuit / Circuit::run on a resident state
- * (src/cuda.rs:1050-1083, src/lib.rs:2786-2802).  plan may be NULL. */
+ * (src/cuda.rs:1050-1083, src/lib.rs:2786-2802).  plan may be NULL; a plan
+ * must come from qf_plan_build of this very op list and width, otherwise
+ * QF_INVALID_ARGUMENT is returned before anything is launched (a
+ * parameterized or density plan, or one built for other gates or angles). */
 qf_status qf_state_run_circuit(qf_state* state, const qf_op* ops, size_t count, const qf_plan* plan);
 
 /* StateVector::norm_sqr, src/lib.rs:1000-1002 (FP64, compensated). */
+/* Amplitudes at `count` basis indices as complex128 (StateVector::amplitudes()[i],
+ * src/lib.rs:992-994) without downloading the whole state; an index >= 2^qubits
+ * is QF_INVALID_ARGUMENT before any launch. */
+qf_status qf_state_gather(const qf_state* state, const uint64_t* indices, size_t count, qf_c128* out);
+/* The device's in-place streaming rate over the state's bytes (no reference
+ * counterpart; a measurement aid): every amplitude is read and written back
+ * unchanged, `launches` timed launches after one warm launch, rate =
+ * 2 x state bytes / average launch time.  variant 0: one 16-byte word per lane
+ * per grid-stride step, 1: nontemporal loads and stores, 2: four words per lane
+ * per step, 3: one contiguous 64 KiB chunk per workgroup. */
+qf_status qf_state_stream_rate(qf_state* state, int32_t variant, uint32_t launches, double* gbs, double* avg_ms);
 qf_status qf_state_norm_sqr(const qf_state* state, double* out);
 /* StateVector::probabilities, src/lib.rs:1144-1146. */
 qf_status qf_state_probabilities(const qf_state* state, double* out);
@@ -362,7 +394,9 @@ typedef struct {
 
 /* Energies <obs_i> and adjoint gradients of `count` parameterized circuits of
  * different widths.  grads is the concatenation of every instance's [nparams].
- * Each instance is validated like qf_gradient.  stats may be NULL. */
+ * Each instance is validated like qf_gradient.  stats may be NULL.  Instances
+ * are packed into device-memory waves of at most 80% of the free memory, or of
+ * the environment variable QF_UNEVEN_WAVE_BYTES bytes when that is smaller. */
 qf_status qf_uneven_adjoint_gradient(qf_backend* backend, const qf_uneven_instance* instances, uint32_t count,
                                      int32_t precision, double* energies, double* grads, qf_uneven_stats* stats);
 
@@ -441,6 +475,10 @@ qf_status qf_density_upload(qf_backend* backend, const qf_c128* entries, size_t 
 qf_status qf_density_from_state(const qf_state* state, qf_density** out);
 /* DensityMatrix::maximally_mixed (src/lib.rs:1659-1704) on the device. */
 qf_status qf_density_maximally_mixed(qf_backend* backend, uint32_t qubits, qf_density** out);
+/* DensityMatrix::maximally_mixed_with_memory_budget, src/lib.rs:1668-1700: the
+ * budget covers the 4^qubits complex128 entries. */
+qf_status qf_density_maximally_mixed_with_memory_budget(qf_backend* backend, uint32_t qubits, uint64_t budget_bytes,
+                                                        qf_density** out);
 qf_status qf_density_destroy(qf_density* density);
 /* CudaDeviceDensityMatrix::qubits / download, src/cuda.rs:2009-2011, 2215-2227. */
 qf_status qf_density_qubits(const qf_density* density, uint32_t* out);

@@ -42,7 +42,12 @@ class PlanOptions(C.Structure):
 
 class PlanStats(C.Structure):
     _fields_ = [("passes", C.c_uint64), ("ops_in", C.c_uint64), ("ops_applied", C.c_uint64),
-                ("dense_blocks", C.c_uint64), ("max_ops_per_pass", C.c_uint64), ("phases", C.c_uint64)]
+                ("dense_blocks", C.c_uint64), ("max_ops_per_pass", C.c_uint64), ("phases", C.c_uint64),
+                ("kernels_run", C.c_uint32), ("runtime_rotations", C.c_uint32)]
+
+
+# qf_plan_stats.kernels_run bits (QF_KERNEL_*)
+KERNEL_BITS = {1: "sweep", 2: "lds_tile", 4: "register_tile", 8: "compiled", 16: "mfma"}
 
 
 class GradOptions(C.Structure):
@@ -76,6 +81,10 @@ SIGNATURES = {
     "qf_backend_set_plan_options": (None, [P, C.POINTER(PlanOptions)]),
     "qf_backend_stream": (None, [P, PP]),
     "qf_state_zero": (None, [P, u32, i32, PP]),
+    "qf_state_zero_with_memory_budget": (None, [P, u32, i32, u64, PP]),
+    "qf_state_gather": (None, [P, P, sz, P]),
+    "qf_state_stream_rate": (None, [P, i32, u32, C.POINTER(dbl), C.POINTER(dbl)]),
+    "qf_density_maximally_mixed_with_memory_budget": (None, [P, u32, u64, PP]),
     "qf_state_upload": (None, [P, P, u32, PP]),
     "qf_state_upload_c64": (None, [P, P, u32, PP]),
     "qf_state_download": (None, [P, P]),

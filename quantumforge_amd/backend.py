@@ -76,6 +76,13 @@ class Plan:
         check(lib().qf_plan_stats_get(self._h, C.byref(s)))
         return {f: int(getattr(s, f)) for f, _ in N.PlanStats._fields_}
 
+    def kernels_run(self) -> list:
+        """Kernel families the last run of this plan launched
+        (qf_plan_stats.kernels_run): "sweep", "lds_tile", "register_tile",
+        "compiled", "mfma"; empty before the first run."""
+        bits = self.stats()["kernels_run"]
+        return [name for b, name in sorted(N.KERNEL_BITS.items()) if bits & b]
+
     def describe(self) -> dict:
         need = C.c_size_t()
         check(lib().qf_plan_describe(self._h, None, 0, C.byref(need)))
@@ -150,6 +157,15 @@ class HipBackend:
     def zero_state(self, qubits: int, precision="complex128") -> "HipDeviceState":
         h = C.c_void_p()
         check(lib().qf_state_zero(self._h, qubits, _precision(precision), C.byref(h)))
+        return HipDeviceState(h, self)
+
+    def zero_state_with_memory_budget(self, qubits: int, budget_bytes: int,
+                                      precision="complex128") -> "HipDeviceState":
+        """StateVector::zero_with_memory_budget, src/lib.rs:940-966, on the
+        device: MemoryBudgetExceeded when 2^qubits amplitudes do not fit."""
+        h = C.c_void_p()
+        check(lib().qf_state_zero_with_memory_budget(self._h, int(qubits), _precision(precision),
+                                                     int(budget_bytes), C.byref(h)))
         return HipDeviceState(h, self)
 
     def upload(self, state: StateVector, precision="complex128") -> "HipDeviceState":
@@ -232,6 +248,13 @@ class HipBackend:
     def maximally_mixed_density(self, qubits: int) -> "HipDeviceDensityMatrix":
         h = C.c_void_p()
         check(lib().qf_density_maximally_mixed(self._h, int(qubits), C.byref(h)))
+        return HipDeviceDensityMatrix(h)
+
+    def maximally_mixed_density_with_memory_budget(self, qubits: int, budget_bytes: int) -> "HipDeviceDensityMatrix":
+        """DensityMatrix::maximally_mixed_with_memory_budget, src/lib.rs:1668-1700."""
+        h = C.c_void_p()
+        check(lib().qf_density_maximally_mixed_with_memory_budget(self._h, int(qubits), int(budget_bytes),
+                                                                  C.byref(h)))
         return HipDeviceDensityMatrix(h)
 
     def run_density_circuit(self, initial: DensityMatrix, circuit: Circuit) -> DensityMatrix:
@@ -349,6 +372,21 @@ class HipDeviceState:
     def download_c64(self) -> np.ndarray:
         out = np.empty(1 << self.qubits_, dtype=np.complex64)
         check(lib().qf_state_download_c64(self._h, ptr(out)))
+        return out
+
+    def stream_rate(self, variant: int = 0, launches: int = 5) -> Tuple[float, float]:
+        """(GB/s, ms per launch) of an in-place read-and-write-back sweep over
+        this state's bytes (qf_state_stream_rate; the state is unchanged)."""
+        g, ms = C.c_double(), C.c_double()
+        check(lib().qf_state_stream_rate(self._h, int(variant), int(launches), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
+
+    def gather(self, indices) -> np.ndarray:
+        """Amplitudes at the given basis indices (complex128), read on the
+        device without downloading the state (qf_state_gather)."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64).reshape(-1))
+        out = np.empty(idx.size, dtype=np.complex128)
+        check(lib().qf_state_gather(self._h, ptr(idx), idx.size, ptr(out)))
         return out
 
     def measure(self, qubit: int, rng) -> int:

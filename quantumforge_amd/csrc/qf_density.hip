@@ -16,6 +16,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -317,6 +318,24 @@ extern "C" qf_status qf_density_maximally_mixed(qf_backend* be, uint32_t qubits,
     return s;
 }
 
+// DensityMatrix::maximally_mixed_with_memory_budget, src/lib.rs:1668-1700:
+// the budget covers the 4^qubits complex128 entries.
+extern "C" qf_status qf_density_maximally_mixed_with_memory_budget(qf_backend* be, uint32_t qubits,
+                                                                   uint64_t budget_bytes, qf_density** out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (qubits == 0) return fail(QF_ZERO_QUBITS, "a state vector must contain at least one qubit");
+    if (qubits >= 32) return fail(QF_SIMULATION_SIZE_OVERFLOW, "dense simulation size exceeds the supported addressable range");
+    const unsigned __int128 required = (unsigned __int128)(uint64_t(1) << (2 * qubits)) * 16;
+    if (required > budget_bytes) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "dense state needs at least %llu bytes, exceeding the %llu-byte memory budget",
+                      (unsigned long long)required, (unsigned long long)budget_bytes);
+        return fail(QF_MEMORY_BUDGET_EXCEEDED, msg);
+    }
+    return qf_density_maximally_mixed(be, qubits, out);
+}
+
 extern "C" qf_status qf_density_destroy(qf_density* d) {
     if (!d) return fail(QF_INVALID_ARGUMENT, "null density matrix");
     qf_backend* be = d->be;
@@ -485,12 +504,15 @@ extern "C" qf_status qf_density_plan_build(const qf_op* ops, size_t nops, uint32
         delete p;
         return fail(QF_INVALID_ARGUMENT, e);
     }
+    p->plan.origin = qf::PLAN_DENSITY;
     *out = p;
     return QF_OK;
 }
 
 extern "C" qf_status qf_density_run_plan(qf_density* d, const qf_plan* plan) {
     if (!d || !plan) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (plan->plan.origin != qf::PLAN_DENSITY || plan->plan.runtime_rotations)
+        return fail(QF_INVALID_ARGUMENT, "plan was not built by qf_density_plan_build");
     if (plan->plan.qubits != 2 * d->qubits || plan->plan.precision != QF_C128)
         return fail(QF_INVALID_ARGUMENT, "plan was built for another density size");
     qf_status s = activate(d->be);

@@ -922,6 +922,7 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
         delete p;
         return fail(QF_INVALID_ARGUMENT, e);
     }
+    p->plan.origin = qf::PLAN_PARAMETERIZED;
     *out = p;
     return QF_OK;
 }

@@ -167,6 +167,72 @@ __global__ __launch_bounds__(kThreads) void k_fill_zero_state(cx<R>* __restrict_
     }
 }
 
+// In-place streaming probe over n 16-byte words (a state's amplitudes viewed as
+// double2): every word is read once and written back multiplied by `scale`,
+// the memory pattern of a tile pass without its arithmetic.  Variant 0: one
+// word per thread per grid-stride step; 1: the same with nontemporal loads and
+// stores; 2: four consecutive words per thread per step (64 bytes in flight
+// per lane); 3: each workgroup streams one contiguous chunk of 2^chunk_log
+// words (the grid covers the buffer once).
+template <int V>
+__global__ __launch_bounds__(kThreads) void k_stream_probe(double2* __restrict__ s, uint64_t n, double scale,
+                                                           uint32_t chunk_log) {
+    if constexpr (V == 3) {
+        const uint64_t base = uint64_t(blockIdx.x) << chunk_log, end = min(base + (uint64_t(1) << chunk_log), n);
+        for (uint64_t i = base + threadIdx.x; i < end; i += blockDim.x) {
+            double2 v = s[i];
+            v.x *= scale;
+            v.y *= scale;
+            s[i] = v;
+        }
+    } else if constexpr (V == 2) {
+        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 4;
+        for (uint64_t i = (uint64_t(blockIdx.x) * blockDim.x) * 4 + threadIdx.x; i < n; i += stride) {
+            double2 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i + uint64_t(k) * blockDim.x < n) v[k] = s[i + uint64_t(k) * blockDim.x];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (i + uint64_t(k) * blockDim.x < n) {
+                    v[k].x *= scale;
+                    v[k].y *= scale;
+                    s[i + uint64_t(k) * blockDim.x] = v[k];
+                }
+        }
+    } else {
+        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+        for (uint64_t i = global_thread(); i < n; i += stride) {
+            double2 v;
+            if constexpr (V == 1) {
+                v.x = __builtin_nontemporal_load(&s[i].x);
+                v.y = __builtin_nontemporal_load(&s[i].y);
+            } else {
+                v = s[i];
+            }
+            v.x *= scale;
+            v.y *= scale;
+            if constexpr (V == 1) {
+                __builtin_nontemporal_store(v.x, &s[i].x);
+                __builtin_nontemporal_store(v.y, &s[i].y);
+            } else {
+                s[i] = v;
+            }
+        }
+    }
+}
+
+// out[k] = s[idx[k]] as complex128 (indices checked on the host).
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_gather(const cx<R>* __restrict__ s, const uint64_t* __restrict__ idx,
+                                                     cx<double>* __restrict__ out, uint64_t n) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t k = global_thread(); k < n; k += stride) {
+        const cx<R> v = s[idx[k]];
+        out[k] = cx<double>{double(v.re), double(v.im)};
+    }
+}
+
 template <class A, class B>
 __global__ __launch_bounds__(kThreads) void k_convert(const cx<A>* __restrict__ in, cx<B>* __restrict__ out,
                                                       uint64_t n) {

@@ -823,57 +823,76 @@ std::map<std::string, Plan> g_plans;
 std::deque<std::string> g_plan_order;
 constexpr size_t kPlanCacheEntries = 64;
 
-std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt) {
+// Two independent 64-bit hashes over a byte stream (FNV-1a and a
+// multiply-xorshift), printed as 32 hex digits.
+struct KeyHash {
     uint64_t h1 = 1469598103934665603ull, h2 = 0x9e3779b97f4a7c15ull;
-    auto mix = [&](const void* p, size_t n) {
+    void mix(const void* p, size_t n) {
         const unsigned char* b = static_cast<const unsigned char*>(p);
         for (size_t i = 0; i < n; ++i) {
             h1 = (h1 ^ b[i]) * 1099511628211ull;
             h2 = (h2 + b[i] + 1) * 0xff51afd7ed558ccdull;
             h2 ^= h2 >> 31;
         }
-    };
-    auto val = [&](int64_t v) { mix(&v, sizeof v); };
-    auto dbl = [&](double v) { mix(&v, sizeof v); };
-    val(qubits);
-    val(precision);
-    val(opt.fuse);
-    val(opt.tile_qubits);
-    val(opt.low_qubits);
-    val(opt.max_block);
-    val(opt.kernel);
-    val(opt.reg_bits);
-    val(opt.relabel);
-    val(opt.compile);
-    val(relabel_enabled() ? 1 : 0);
-    val(int64_t(gates.size()));
+    }
+    void val(int64_t v) { mix(&v, sizeof v); }
+    void dbl(double v) { mix(&v, sizeof v); }
+    std::string hex() const {
+        char b[40];
+        std::snprintf(b, sizeof b, "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
+        return b;
+    }
+};
+
+void mix_gates(KeyHash& k, const std::vector<GateIn>& gates) {
+    k.val(int64_t(gates.size()));
     for (const GateIn& g : gates) {
-        val(g.kind);
-        val(g.a);
-        val(g.b);
+        k.val(g.kind);
+        k.val(g.a);
+        k.val(g.b);
         for (const cd& c : g.m) {
-            dbl(c.re);
-            dbl(c.im);
+            k.dbl(c.re);
+            k.dbl(c.im);
         }
         if (g.kind == 6)
             for (const cd& c : g.m4) {
-                dbl(c.re);
-                dbl(c.im);
+                k.dbl(c.re);
+                k.dbl(c.im);
             }
-        val(g.axis);
-        val(g.param);
-        val(g.slot);
-        val(g.flags);
+        k.val(g.axis);
+        k.val(g.param);
+        k.val(g.slot);
+        k.val(g.flags);
     }
-    char b[40];
-    std::snprintf(b, sizeof b, "%016llx%016llx", (unsigned long long)h1, (unsigned long long)h2);
-    return b;
 }
 
+std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt) {
+    KeyHash k;
+    k.val(qubits);
+    k.val(precision);
+    k.val(opt.fuse);
+    k.val(opt.tile_qubits);
+    k.val(opt.low_qubits);
+    k.val(opt.max_block);
+    k.val(opt.kernel);
+    k.val(opt.reg_bits);
+    k.val(opt.relabel);
+    k.val(opt.compile);
+    k.val(relabel_enabled() ? 1 : 0);
+    mix_gates(k, gates);
+    return k.hex();
+}
 std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubits, int precision,
                                 const PlanOptions& opt, Plan& out);
 
 }  // namespace
+
+std::string circuit_key(const std::vector<GateIn>& gates, uint32_t qubits) {
+    KeyHash k;
+    k.val(qubits);
+    mix_gates(k, gates);
+    return k.hex();
+}
 
 std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt,
                        Plan& out) {
@@ -906,6 +925,9 @@ std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubit
     out.qubits = qubits;
     out.precision = precision;
     out.stats.ops_in = gates.size();
+    out.circuit_key = circuit_key(gates, qubits);
+    for (const GateIn& g : gates) out.runtime_rotations += g.kind == 5 ? 1 : 0;
+    out.origin = out.runtime_rotations ? PLAN_PARAMETERIZED : PLAN_CIRCUIT;
     if (qubits == 0) return "a state vector must contain at least one qubit";
     for (const GateIn& g : gates) {
         const bool two = g.kind != 0 && g.kind != 5;

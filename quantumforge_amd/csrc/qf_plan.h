@@ -117,6 +117,18 @@ struct PlanStats {
     uint64_t passes = 0, ops_in = 0, ops_applied = 0, dense_blocks = 0, max_ops_per_pass = 0, phases = 0;
 };
 
+// Kernel families a plan launch used (bit mask, PlanStats::kernels_run).
+enum : uint32_t {
+    KR_SWEEP = 1,     // one full-state sweep per gate (exact lowering)
+    KR_LDS_TILE = 2,  // k_tile / k_tile_adjoint (LDS, op by op)
+    KR_REG_TILE = 4,  // k_tile_reg (interpreting register kernel)
+    KR_COMPILED = 8,  // generated per-pass kernels
+    KR_MFMA = 16,     // dense-block matrix-core passes
+};
+
+// Where a plan came from; a run entry point accepts only its own kind.
+enum PlanOrigin : int32_t { PLAN_CIRCUIT = 0, PLAN_PARAMETERIZED = 1, PLAN_DENSITY = 2 };
+
 // One circuit gate in planner form.
 struct GateIn {
     int32_t kind;      // 0 1q matrix, 1 CNOT, 2 CZ, 3 SWAP, 4 diag2 (controlled phase etc.), 5 rotation,
@@ -135,6 +147,17 @@ struct Plan {
     int precision = 0;
     std::vector<Pass> passes;
     PlanStats stats;
+    // Hash of the planner gate list the plan was built from (circuit_key):
+    // a run entry point lowers the ops it is given and refuses a plan whose
+    // key differs, so a plan never runs in place of another circuit.
+    std::string circuit_key;
+    // Rotations whose angles come from a per-instance table at launch
+    // (GateIn kind 5): such a plan needs that table and runs only through
+    // the parameterized entry points.
+    uint32_t runtime_rotations = 0;
+    int32_t origin = PLAN_CIRCUIT;
+    // KR_* mask of the kernels the last launch of this plan object used.
+    mutable uint32_t kernels_run = 0;
     int compiled = 0;          // 1: register passes run as compiled kernels (2: explicitly requested)
     int relabeled = 0;         // passes with a store permutation (Pass::store_phys): compiled only
     // Compiled programs per form (0: one state, 1: adjoint, psi and lambda)
@@ -147,6 +170,10 @@ struct Plan {
 // Builds a plan; returns an empty string on success or an error message.
 std::string build_plan(const std::vector<GateIn>& gates, uint32_t qubits, int precision, const PlanOptions& opt,
                        Plan& out);
+
+// Hash of a planner gate list and width (every field that changes what the
+// gates do).
+std::string circuit_key(const std::vector<GateIn>& gates, uint32_t qubits);
 
 // JSON text of a plan (passes, tile sets, ops with exact matrices).
 std::string describe_plan(const Plan& plan);

@@ -88,6 +88,7 @@ double host_sin(double x);
 void gate_matrix(int gate, double theta, qf::cd m[4]);
 bool is_rotation(int gate);
 qf_status check_pair(uint32_t qubits, uint32_t a, uint32_t b);
+qf_status check_circuit_plan(const qf::Plan& plan, const std::vector<qf::GateIn>& g, uint32_t qubits);
 qf_status check_qubits(uint32_t qubits);
 qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector<qf::GateIn>& out);
 qf_status activate(qf_backend* be);

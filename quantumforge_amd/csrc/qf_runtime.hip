@@ -472,13 +472,16 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
     if (plan.relabeled && !prog)
         return fail(QF_INVALID_ARGUMENT, "a plan that relabels qubits between passes runs only as compiled kernels");
     const qf::TileOp* cur = d_ops;
+    uint32_t kinds = 0;
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const qf::Pass& p = plan.passes[i];
         qf_status s;
         const int kern = prog ? prog->kernel_of_pass[i] : -1;
         if (kern >= 0) {
+            kinds |= qf::KR_COMPILED;
             s = launch_compiled(be, d, qubits, count, p, (*fns)[kern], prog->threads[kern], ns, ctx);
         } else {
+            kinds |= p.kind != qf::Pass::TILE ? qf::KR_SWEEP : (p.reg_bits ? qf::KR_REG_TILE : qf::KR_LDS_TILE);
             if (p.kind == qf::Pass::TILE && !cur) return fail(QF_INVALID_ARGUMENT, "tile pass without uploaded ops");
             s = precision == QF_C128 ? launch_pass<double>(be, static_cast<cx<double>*>(d), qubits, count, p, cur, ctx)
                                      : launch_pass<float>(be, static_cast<cx<float>*>(d), qubits, count, p, cur, ctx);
@@ -486,6 +489,7 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
         if (s) return s;
         if (cur) cur += p.ops.size();
     }
+    plan.kernels_run = kinds;
     return QF_OK;
 }
 
@@ -690,6 +694,25 @@ extern "C" qf_status qf_state_zero(qf_backend* be, uint32_t qubits, int32_t prec
     return s;
 }
 
+// StateVector::zero_with_memory_budget, src/lib.rs:940-966: the budget covers
+// the amplitude payload (2^qubits amplitudes of the state's precision).
+extern "C" qf_status qf_state_zero_with_memory_budget(qf_backend* be, uint32_t qubits, int32_t precision,
+                                                      uint64_t budget_bytes, qf_state** out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (qubits == 0) return fail(QF_ZERO_QUBITS, "a state vector must contain at least one qubit");
+    if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
+    if (qubits >= 64) return fail(QF_SIMULATION_SIZE_OVERFLOW, "dense simulation size exceeds the supported addressable range");
+    const unsigned __int128 required = (unsigned __int128)(uint64_t(1) << qubits) * amp_bytes(precision);
+    if (required > budget_bytes) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "dense state needs at least %llu bytes, exceeding the %llu-byte memory budget",
+                      (unsigned long long)required, (unsigned long long)budget_bytes);
+        return fail(QF_MEMORY_BUDGET_EXCEEDED, msg);
+    }
+    return qf_state_zero(be, qubits, precision, out);
+}
+
 extern "C" qf_status qf_state_upload(qf_backend* be, const qf_c128* amps, uint32_t qubits, qf_state** out) {
     if (!amps) return fail(QF_INVALID_ARGUMENT, "null amplitudes");
     qf_status s = new_state(be, qubits, QF_C128, out);
@@ -874,18 +897,124 @@ extern "C" qf_status qf_state_apply_cp(qf_state* st, uint32_t control, uint32_t 
     return apply_gates(st->be, st->d, st->precision, st->qubits, 1, {g}, po);
 }
 
+// A plan passed to a circuit run must have been built from this very
+// circuit (same lowered gates and width) by qf_plan_build: a plan of another
+// circuit, a parameterized plan (its rotations read an angle table the
+// circuit runs do not supply) or a density plan is refused before any launch.
+qf_status qfrt::check_circuit_plan(const qf::Plan& plan, const std::vector<qf::GateIn>& g, uint32_t qubits) {
+    if (plan.origin != qf::PLAN_CIRCUIT || plan.runtime_rotations)
+        return fail(QF_INVALID_ARGUMENT, "plan was not built by qf_plan_build (parameterized or density plan)");
+    if (plan.qubits != qubits) return fail(QF_INVALID_ARGUMENT, "plan was built for a different qubit count");
+    if (plan.circuit_key != qf::circuit_key(g, qubits))
+        return fail(QF_INVALID_ARGUMENT, "plan was built for a different circuit");
+    return QF_OK;
+}
+
 extern "C" qf_status qf_state_run_circuit(qf_state* st, const qf_op* ops, size_t count, const qf_plan* plan) {
     if (!st || (!ops && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
     std::vector<qf::GateIn> g;
     qf_status s = lower_ops(ops, count, st->qubits, g);
     if (s) return s;
+    if (plan && (s = check_circuit_plan(plan->plan, g, st->qubits))) return s;
     if ((s = activate(st->be))) return s;
-    if (plan) {
-        if (plan->plan.stats.ops_in != count)
-            return fail(QF_INVALID_ARGUMENT, "plan was built for a different circuit");
-        return run_plan(st->be, st->d, st->precision, st->qubits, 1, plan->plan);
-    }
+    if (plan) return run_plan(st->be, st->d, st->precision, st->qubits, 1, plan->plan);
     return apply_gates(st->be, st->d, st->precision, st->qubits, 1, g, st->be->opts);
+}
+
+// Amplitudes at `count` basis indices, as complex128 (StateVector::amplitudes()[i],
+// src/lib.rs:992-994, without downloading the state).
+extern "C" qf_status qf_state_gather(const qf_state* st, const uint64_t* indices, size_t count, qf_c128* out) {
+    if (!st || ((!indices || !out) && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    for (size_t k = 0; k < count; ++k)
+        if (indices[k] >= st->dim)
+            return fail(QF_INVALID_ARGUMENT, "basis index " + std::to_string(indices[k]) + " is out of range for " +
+                                                 std::to_string(st->qubits) + " qubits");
+    if (count == 0) return QF_OK;
+    qf_backend* be = st->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    const size_t ib = count * sizeof(uint64_t), ob = count * sizeof(qf_c128);
+    void* tmp = nullptr;
+    if ((s = device_alloc(&tmp, ((ib + 255) / 256) * 256 + ob))) return s;
+    uint64_t* d_idx = static_cast<uint64_t*>(tmp);
+    cx<double>* d_out = reinterpret_cast<cx<double>*>(static_cast<char*>(tmp) + ((ib + 255) / 256) * 256);
+    hipError_t e = hipMemcpyAsync(d_idx, indices, ib, hipMemcpyHostToDevice, be->stream);
+    if (e == hipSuccess) {
+        if (st->precision == QF_C128)
+            hipLaunchKernelGGL(qf::k_gather<double>, dim3(grid_for(count)), dim3(qf::kThreads), 0, be->stream,
+                               static_cast<const cx<double>*>(st->d), d_idx, d_out, uint64_t(count));
+        else
+            hipLaunchKernelGGL(qf::k_gather<float>, dim3(grid_for(count)), dim3(qf::kThreads), 0, be->stream,
+                               static_cast<const cx<float>*>(st->d), d_idx, d_out, uint64_t(count));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, ob, hipMemcpyDeviceToHost, be->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(be->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "qf_state_gather");
+    return QF_OK;
+}
+
+// The device's in-place streaming rate over this state's bytes (the memory
+// pattern of one tile pass: read every amplitude once, write it back).  The
+// amplitudes are multiplied by 1.0, so the state is unchanged.
+extern "C" qf_status qf_state_stream_rate(qf_state* st, int32_t variant, uint32_t launches, double* gbs,
+                                          double* avg_ms) {
+    if (!st || !gbs) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (variant < 0 || variant > 3) return fail(QF_INVALID_ARGUMENT, "unknown streaming variant");
+    if (launches == 0) launches = 1;
+    qf_backend* be = st->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    const uint64_t bytes = st->dim * amp_bytes(st->precision);
+    const uint64_t words = bytes / 16;
+    if (words == 0) return fail(QF_INVALID_ARGUMENT, "state too small to stream");
+    double2* p = static_cast<double2*>(st->d);
+    const uint32_t chunk_log = 12;  // 64 KiB per workgroup, the bytes of a 12-qubit complex128 tile
+    auto launch = [&]() {
+        switch (variant) {
+            case 0:
+                hipLaunchKernelGGL(qf::k_stream_probe<0>, dim3(grid_for(words)), dim3(qf::kThreads), 0, be->stream,
+                                   p, words, 1.0, chunk_log);
+                break;
+            case 1:
+                hipLaunchKernelGGL(qf::k_stream_probe<1>, dim3(grid_for(words)), dim3(qf::kThreads), 0, be->stream,
+                                   p, words, 1.0, chunk_log);
+                break;
+            case 2:
+                hipLaunchKernelGGL(qf::k_stream_probe<2>, dim3(grid_for((words + 3) / 4)), dim3(qf::kThreads), 0,
+                                   be->stream, p, words, 1.0, chunk_log);
+                break;
+            default:
+                hipLaunchKernelGGL(qf::k_stream_probe<3>,
+                                   dim3(unsigned((words + (uint64_t(1) << chunk_log) - 1) >> chunk_log)),
+                                   dim3(qf::kThreads), 0, be->stream, p, words, 1.0, chunk_log);
+        }
+    };
+    launch();  // warm
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_fail(e, "hipEventCreate");
+    }
+    e = hipEventRecord(a, be->stream);
+    for (uint32_t k = 0; k < launches && e == hipSuccess; ++k) {
+        launch();
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, be->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(e, "qf_state_stream_rate");
+    const double per = double(ms) / launches;
+    *gbs = 2.0 * double(bytes) / (per * 1e-3) / 1e9;
+    if (avg_ms) *avg_ms = per;
+    return QF_OK;
 }
 
 extern "C" qf_status qf_state_norm_sqr(const qf_state* st, double* out) {
@@ -1087,12 +1216,9 @@ extern "C" qf_status qf_batch_run_circuit(qf_batch* b, const qf_op* ops, size_t 
     std::vector<qf::GateIn> g;
     qf_status s = lower_ops(ops, count, b->qubits, g);
     if (s) return s;
+    if (plan && (s = check_circuit_plan(plan->plan, g, b->qubits))) return s;
     if ((s = activate(b->be))) return s;
-    if (plan) {
-        if (plan->plan.stats.ops_in != count)
-            return fail(QF_INVALID_ARGUMENT, "plan was built for a different circuit");
-        return run_plan(b->be, b->d, b->precision, b->qubits, b->count, plan->plan);
-    }
+    if (plan) return run_plan(b->be, b->d, b->precision, b->qubits, b->count, plan->plan);
     return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, b->be->opts);
 }
 
@@ -1171,6 +1297,8 @@ extern "C" qf_status qf_plan_stats_get(const qf_plan* p, qf_plan_stats* out) {
     out->dense_blocks = p->plan.stats.dense_blocks;
     out->max_ops_per_pass = p->plan.stats.max_ops_per_pass;
     out->phases = p->plan.stats.phases;
+    out->kernels_run = p->plan.kernels_run;
+    out->runtime_rotations = p->plan.runtime_rotations;
     return QF_OK;
 }
 

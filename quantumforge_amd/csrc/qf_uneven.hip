@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -630,7 +631,13 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
                      [&](uint32_t a, uint32_t b) { return inst[a].qubits > inst[b].qubits; });
     size_t freeb = 0, totalb = 0;
     QF_HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    const uint64_t budget = uint64_t(double(freeb) * 0.8);
+    uint64_t budget = uint64_t(double(freeb) * 0.8);
+    // QF_UNEVEN_WAVE_BYTES caps the bytes of one wave (a process sharing the
+    // device, or a test that wants several waves on small instances).
+    if (const char* cap = std::getenv("QF_UNEVEN_WAVE_BYTES")) {
+        const unsigned long long v = std::strtoull(cap, nullptr, 10);
+        if (v > 0 && v < budget) budget = v;
+    }
     std::vector<std::vector<uint32_t>> waves;
     std::vector<uint64_t> wave_bytes;
     for (uint32_t k : order) {

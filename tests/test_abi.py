@@ -49,6 +49,22 @@ def test_record_layouts():
     assert N.OP_DTYPE.names == ("kind", "gate", "a", "b", "theta", "theta2")
 
 
+def test_plan_stats_layout_and_identity_fields():
+    import ctypes as C
+
+    assert C.sizeof(N.PlanStats) == 56
+    src = open(HEADER).read()
+    assert "uint32_t kernels_run;" in src and "uint32_t runtime_rotations;" in src
+    from quantumforge_amd.workloads import hea_circuit
+
+    pc = hea_circuit(6, 2)
+    ps = Plan.parameterized(pc, 6).stats()
+    assert ps["runtime_rotations"] == pc.parameter_count and ps["kernels_run"] == 0
+    c = random_layer_circuit(6, 2, seed=1, pairs_per_layer=2)
+    s = Plan(c, 6).stats()
+    assert s["runtime_rotations"] == 0 and s["kernels_run"] == 0
+
+
 def test_status_names_match_reference_variants():
     lib = N.lib()
     for code, name in N.STATUS.items():

@@ -255,3 +255,37 @@ def test_uneven_heterogeneous_circuits(backend):
         eb, gb = G.batch_adjoint_gradient(backend, pc, n, p[None, :], H)
         assert abs(e[k] - eb[0]) < 1e-12
         assert np.max(np.abs(g[k] - gb[0])) < 1e-11
+
+
+def test_uneven_qaoa_multi_wave_at_config_shape(backend, monkeypatch):
+    """Config-shape QAOA (p = 6, 3-regular weighted graphs) through several
+    memory waves: QF_UNEVEN_WAVE_BYTES caps a wave below two 18-qubit
+    instances, so the captured per-wave graph is replayed with new tables over
+    reused psi, lambda and partial buffers; a 10-qubit instance narrower than
+    the tile (unless the tuner picks a 10-qubit tile) takes the single-instance
+    path in the same call.  Energies and
+    occurrence gradients against the oracle's store-every-state adjoint
+    (adjoint_gradient_with_inputs, src/lib.rs:3564-3704) at 1e-10."""
+    from quantumforge_amd.workloads import qaoa_occurrence_angles, qaoa_parameterized, qaoa_uneven_instances
+
+    specs = qaoa_uneven_instances(3, seed=21, p=6, qubit_choices=(18,))
+    specs += qaoa_uneven_instances(1, seed=22, p=6, qubit_choices=(16,))
+    specs += qaoa_uneven_instances(1, seed=23, p=6, qubit_choices=(10,))
+    instances = []
+    for s in specs:
+        prob = s["problem"]
+        pc = qaoa_parameterized(prob.num_qubits, prob.edges, s["p"])
+        ang = qaoa_occurrence_angles(prob.edges, prob.num_qubits, s["p"], s["gammas"], s["betas"])
+        instances.append((pc, prob.num_qubits, ang, maxcut_cost_hamiltonian(prob)))
+    # One 18-qubit instance needs 2 x 2^18 x 16 bytes of psi and lambda plus its partials.
+    monkeypatch.setenv("QF_UNEVEN_WAVE_BYTES", str(12 << 20))
+    e, g, stats = G.uneven_adjoint_gradient(backend, instances)
+    print(f"[uneven multi-wave] stats {stats}", flush=True)
+    assert stats["waves"] >= 3 and stats["graph_replays"] == stats["waves"]
+    assert stats["grouped_instances"] >= 4
+    for k, (pc, q, ang, H) in enumerate(instances):
+        t = H.to_terms()
+        want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+        want_g = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+        assert abs(e[k] - want_e) < 1e-10, (k, e[k], want_e)
+        assert np.max(np.abs(g[k] - want_g)) < 1e-10, k

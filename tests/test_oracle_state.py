@@ -248,3 +248,23 @@ def test_controlled_phase_matches_reference_lowering_up_to_global_phase():
     low, nat = qft_like_circuit(6, 2, seed=5, max_distance=3)
     total_phi = sum(op[4] for op in nat.operations if op[0] == 7)
     close(O.run_circuit(6, low.to_ops()), np.exp(-0.25j * total_phi) * O.run_circuit(6, nat.to_ops()), 1e-13)
+
+
+def test_qft_like_layer_closed_form_matches_oracle():
+    """The closed form the >2^32-amplitude GPU tests compare against
+    (tests/test_gpu_large.py) equals the oracle's state after one QFT-like
+    layer, at every index of a 12-qubit state; and the inverse circuit used
+    there undoes the layer."""
+    from quantumforge_amd.workloads import qft_like_circuit
+
+    from .test_gpu_large import inverse, layer_one_phases
+
+    n = 12
+    _, c = qft_like_circuit(n, 1, seed=33)
+    st = O.State(n)
+    st.run(c.to_ops())
+    idx = np.arange(1 << n, dtype=np.uint64)
+    want = 2.0 ** (-n / 2) * np.exp(1j * layer_one_phases(n, 33, idx))
+    assert np.max(np.abs(st.amplitudes() - want)) < 1e-14
+    st.run(inverse(c).to_ops())
+    assert abs(abs(st.amplitudes()[0]) ** 2 - 1) < 1e-13
