@@ -98,11 +98,15 @@ def cpu_baseline(circuit, qubits: int, sample_ops: int):
     }
 
 
-def vqe_bench(be, sets: int, shift_sets: int, precision: str, with_cpu: bool) -> dict:
+def vqe_bench(be, sets: int, shift_sets: int, with_cpu: bool) -> dict:
     """16-qubit hardware-efficient ansatz (8 layers of Ry, Rz on every qubit,
     linear CNOT ladder, 256 parameters ~ U[-pi, pi]) with a Heisenberg + Z-field
     chain: batched adjoint gradients for `sets` parameter sets against the
-    plain parameter-shift GPU form (2 circuits per parameter) on a subset."""
+    plain parameter-shift GPU form (2 circuits per parameter) on a subset, in
+    complex128 and complex64.  The roofline is the whole adjoint call's
+    algorithmic bytes (forward passes 2 x 2^n x s per set, costate 2 x 2^n x s,
+    reverse two-state passes 4 x 2^n x s) over its device time, with each
+    phase's share from HIP events (qf_backend_last_gradient_profile)."""
     import numpy as np
 
     from quantumforge_amd import gradients as G
@@ -113,38 +117,80 @@ def vqe_bench(be, sets: int, shift_sets: int, precision: str, with_cpu: bool) ->
     H = heisenberg_hamiltonian(nq, seed=16)
     params = uniform_parameters(pc.parameter_count, sets, seed=4096)
     P = pc.parameter_count
-    G.batch_adjoint_gradient(be, pc, nq, params[: min(sets, 64)], H, precision)  # warmup
-    t0 = time.perf_counter()
-    e, g = G.batch_adjoint_gradient(be, pc, nq, params, H, precision)
-    t_adj = time.perf_counter() - t0
-    log(f"vqe: adjoint {sets} sets x {P} params in {t_adj * 1e3:.1f} ms")
-    k = max(1, min(shift_sets, sets))
-    G.batch_shift_gradient(be, pc, nq, params[:1], H, precision)  # warmup
-    t0 = time.perf_counter()
-    es, gs = G.batch_shift_gradient(be, pc, nq, params[:k], H, precision)
-    t_shift = time.perf_counter() - t0
-    log(f"vqe: parameter shift {k} sets in {t_shift * 1e3:.1f} ms")
-    adj_gps = sets * P / t_adj
-    shift_gps = k * P / t_shift
-    res = {
-        "workload": f"{nq}q HEA {layers} layers ({P} params), Heisenberg+Z chain, {sets} sets, {precision}",
-        "adjoint_grads_per_s": round(adj_gps, 1),
-        "adjoint_ms": round(t_adj * 1e3, 2),
-        "shift_grads_per_s": round(shift_gps, 1),
-        "shift_sets_timed": k,
-        "ratio_vs_shift": round(adj_gps / shift_gps, 2),
-        "max_abs_adjoint_minus_shift": float(np.max(np.abs(g[:k] - gs))),
-        "max_abs_energy_diff": float(np.max(np.abs(e[:k] - es))),
-    }
+    res = {"workload": f"{nq}q HEA {layers} layers ({P} params), Heisenberg+Z chain, {sets} sets"}
+    ref_e = ref_g = None
+    for precision in ("complex128", "complex64"):
+        G.batch_adjoint_gradient(be, pc, nq, params[: min(sets, 64)], H, precision)  # warmup
+        be.set_profiling(False)
+        t0 = time.perf_counter()
+        e, g = G.batch_adjoint_gradient(be, pc, nq, params, H, precision)
+        t_adj = time.perf_counter() - t0
+        prof_plain = be.last_gradient_profile()
+        be.set_profiling(True)
+        G.batch_adjoint_gradient(be, pc, nq, params, H, precision)
+        prof = be.last_gradient_profile()
+        be.set_profiling(False)
+        log(f"vqe {precision}: adjoint {sets} sets x {P} params in {t_adj * 1e3:.1f} ms; phases {prof}")
+        k = max(1, min(shift_sets, sets))
+        G.batch_shift_gradient(be, pc, nq, params[:1], H, precision)  # warmup
+        t0 = time.perf_counter()
+        es, gs = G.batch_shift_gradient(be, pc, nq, params[:k], H, precision)
+        t_shift = time.perf_counter() - t0
+        log(f"vqe {precision}: parameter shift {k} sets in {t_shift * 1e3:.1f} ms")
+        adj_gps = sets * P / t_adj
+        shift_gps = k * P / t_shift
+        dev_ms = prof["forward_ms"] + prof["costate_ms"] + prof["reverse_ms"] + prof["finish_ms"]
+        alg = prof["forward_bytes"] + prof["costate_bytes"] + prof["reverse_bytes"]
+
+        def gbs(b, ms):
+            return round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None
+
+        r = {
+            "adjoint_grads_per_s": round(adj_gps, 1),
+            "adjoint_ms": round(t_adj * 1e3, 2),
+            "shift_grads_per_s": round(shift_gps, 1),
+            "shift_sets_timed": k,
+            "ratio_vs_shift": round(adj_gps / shift_gps, 2),
+            "max_abs_adjoint_minus_shift": float(np.max(np.abs(g[:k] - gs))),
+            "max_abs_energy_diff": float(np.max(np.abs(e[:k] - es))),
+            "plan": {"forward_passes": prof["forward_passes"], "reverse_passes": prof["reverse_passes"],
+                     "forward_kernels": prof_plain["forward_kernels"],
+                     "reverse_kernels": prof_plain["reverse_kernels"], "costate_kernel": prof["costate_kernel"],
+                     "waves": prof["waves"]},
+            "roofline": {"bound": "hbm", "achieved": gbs(alg, dev_ms), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if dev_ms > 0 else None,
+                         "alg_bytes": alg, "device_ms": round(dev_ms, 3),
+                         "phases": {
+                             "forward": {"ms": round(prof["forward_ms"], 3),
+                                         "GBs": gbs(prof["forward_bytes"], prof["forward_ms"])},
+                             "costate": {"ms": round(prof["costate_ms"], 3),
+                                         "GBs": gbs(prof["costate_bytes"], prof["costate_ms"])},
+                             "reverse": {"ms": round(prof["reverse_ms"], 3),
+                                         "GBs": gbs(prof["reverse_bytes"], prof["reverse_ms"])},
+                             "finish_ms": round(prof["finish_ms"], 3)}},
+        }
+        if precision == "complex128":
+            ref_e, ref_g = e, g
+        else:
+            # complex64 amplitudes, FP64 reductions: error against the complex128
+            # run (itself within 1e-14 of the FP64 oracle, below).
+            r["max_abs_energy_err_vs_c128"] = float(np.max(np.abs(e - ref_e)))
+            r["max_abs_grad_err_vs_c128"] = float(np.max(np.abs(g - ref_g)))
+        res[precision] = r
     if with_cpu:
         from oracle import oracle as O
 
         t0 = time.perf_counter()
         gc = O.pc_adjoint_gradient(pc.to_pops(), P, 0, params[0], [], nq, H.to_terms())
         t_cpu = time.perf_counter() - t0
+        ec = O.pc_expectation(pc.to_pops(), P, 0, params[0], [], nq, H.to_terms())
         res["cpu_adjoint_grads_per_s"] = round(P / t_cpu, 1)
         res["cpu_sample"] = "1 set, oracle restatement of adjoint_gradient_with_inputs (store-every-state)"
-        res["max_abs_gpu_vs_cpu_adjoint"] = float(np.max(np.abs(g[0] - gc)))
+        res["max_abs_gpu_vs_cpu_adjoint"] = float(np.max(np.abs(ref_g[0] - gc)))
+        res["abs_gpu_vs_cpu_energy"] = float(abs(ref_e[0] - ec))
+    # the headline figures of the section (complex128)
+    for key in ("adjoint_grads_per_s", "adjoint_ms", "shift_grads_per_s", "ratio_vs_shift", "roofline"):
+        res[key] = res["complex128"][key]
     return res
 
 
@@ -200,10 +246,12 @@ def qft_like_bench(be, torch, steps: int) -> list:
     return out
 
 
-def uneven_bench(be, count: int, seed: int) -> dict:
+def uneven_bench(be, count: int, seed: int, check_instances: int) -> dict:
     """QAOA MaxCut adjoint gradients of `count` instances of different widths
     (even n in 18..26, random 3-regular graphs with U[0.5, 1) weights, p = 6,
-    gamma, beta ~ U[0, pi]) through the uneven-batch engine."""
+    gamma, beta ~ U[0, pi]) through the uneven-batch engine; energies and
+    gradients of the `check_instances` narrowest instances against the FP64
+    oracle (the reference's store-every-state adjoint)."""
     from quantumforge_amd import gradients as G
     from quantumforge_amd.workloads import (maxcut_cost_hamiltonian, qaoa_occurrence_angles, qaoa_parameterized,
                                             qaoa_uneven_instances)
@@ -220,10 +268,95 @@ def uneven_bench(be, count: int, seed: int) -> dict:
     dt = time.perf_counter() - t0
     occ = sum(pc.parameter_count for pc, _, _, _ in insts)
     log(f"uneven: {count} instances, {occ} rotation occurrences, {dt:.2f} s, stats {stats}")
-    return {"workload": f"{count} QAOA MaxCut instances, even n in 18..26, p=6, complex128, seed {seed}",
-            "seconds": round(dt, 3), "instances_per_s": round(count / dt, 2),
-            "occurrence_grads_per_s": round(occ / dt, 1), "stats": stats,
-            "mean_qubits": float(np.mean([q for _, q, _, _ in insts]))}
+    dev_s = stats["device_ms"] * 1e-3
+    res = {"workload": f"{count} QAOA MaxCut instances, even n in 18..26, p=6, complex128, seed {seed}",
+           "seconds": round(dt, 3), "instances_per_s": round(count / dt, 2),
+           "occurrence_grads_per_s": round(occ / dt, 1), "stats": stats,
+           "mean_qubits": float(np.mean([q for _, q, _, _ in insts])),
+           "roofline": {"bound": "hbm", "achieved": round(stats["alg_bytes"] / dev_s / 1e9, 1) if dev_s else None,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(stats["alg_bytes"] / dev_s / 1e9 / HBM_PEAK_GBS, 4) if dev_s else None,
+                        "alg_bytes": stats["alg_bytes"], "device_s": round(dev_s, 3)}}
+    if check_instances > 0:
+        from oracle import oracle as O
+
+        order = sorted(range(count), key=lambda k: insts[k][1])[:check_instances]
+        errs = []
+        t0 = time.perf_counter()
+        for k in order:
+            pc, q, ang, H = insts[k]
+            t = H.to_terms()
+            we = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+            wg = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
+            errs.append({"instance": k, "qubits": q, "energy_err": float(abs(e[k] - we)),
+                         "max_grad_err": float(np.max(np.abs(g[k] - wg)))})
+        res["fp64_check"] = {"instances": errs, "cpu_s": round(time.perf_counter() - t0, 1),
+                             "max_energy_err": max(x["energy_err"] for x in errs),
+                             "max_grad_err": max(x["max_grad_err"] for x in errs)}
+        log(f"uneven: FP64 check of {len(errs)} instances: {res['fp64_check']}")
+    return res
+
+
+def complex64_section(be, torch, circuit, n: int, steps: int, warmup: int) -> dict:
+    """The headline circuit in complex64 (8 GiB at 30 qubits): layers/s and the
+    pass roofline (2 x 2^n x 8 bytes per launch), and the fidelity of the
+    complex64 state against the complex128 state of the same circuit, both
+    computed on the device (the complex64 state widened exactly to
+    complex128, FP64 compensated inner product)."""
+    r = circuit_section(be, torch, circuit, n, "complex64", steps, warmup,
+                        f"random layered circuit, {n} qubits, depth 40, complex64, fused tile passes")
+    r["layers_per_s"] = round(40 / (r["ms_per_circuit"] * 1e-3), 3)
+    r["roofline"] = {"bound": "hbm", "achieved": r["GBs_per_pass"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["GBs_per_pass"] / HBM_PEAK_GBS, 4), "bytes_per_launch": 2 * (1 << n) * 8}
+    from quantumforge_amd.backend import Plan
+
+    a = be.zero_state(n)
+    a.run_circuit(circuit, Plan(circuit, n))
+    b = be.zero_state(n, precision="complex64")
+    b.run_circuit(circuit, Plan(circuit, n, precision="complex64"))
+    wide = be.zero_state(n)
+    wide.copy_from_device(b)
+    del b
+    ov = a.inner(wide)
+    nb = wide.norm_sqr()
+    del a, wide
+    r["fidelity_vs_complex128"] = abs(ov) ** 2 / nb
+    r["one_minus_fidelity"] = 1.0 - r["fidelity_vs_complex128"]
+    r["norm_sqr_complex64"] = nb
+    log(f"complex64 {n}q: {r['ms_per_circuit']:.1f} ms/circuit, fidelity vs complex128 1 - {r['one_minus_fidelity']:.3e}")
+    return r
+
+
+def fidelity_oracle_section(be, n: int, seed: int) -> dict:
+    """The headline circuit family at `n` qubits (depth 40) against the FP64
+    CPU oracle (its in-place OpenMP run, the reference's per-pair arithmetic):
+    complex128 and complex64 states from the compiled passes, max amplitude
+    error and fidelity |<oracle|gpu>|^2 over the whole state."""
+    from oracle import oracle as O
+    from quantumforge_amd.backend import Plan
+    from quantumforge_amd.workloads import random_layer_circuit
+
+    c = random_layer_circuit(n, 40, seed=seed)
+    st = be.zero_state(n)
+    plan = Plan(c, n)
+    st.run_circuit(c, plan)
+    got128 = st.download().amplitudes
+    del st
+    st = be.zero_state(n, precision="complex64")
+    st.run_circuit(c, Plan(c, n, precision="complex64"))
+    got64 = st.download_c64().astype(np.complex128)
+    del st
+    t0 = time.perf_counter()
+    want = O.fast_run(n, c.to_ops())
+    t = time.perf_counter() - t0
+    res = {"workload": f"random layered circuit, {n} qubits, depth 40, seed {seed}, vs the FP64 CPU oracle",
+           "oracle_s": round(t, 1), "oracle_threads": O.max_threads()}
+    for name, got in (("complex128", got128), ("complex64", got64)):
+        err = float(np.max(np.abs(got - want)))
+        fid = float(abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real)
+        res[name] = {"max_abs_amp_err": err, "fidelity": fid, "one_minus_fidelity": 1.0 - fid}
+    log(f"fidelity {n}q vs oracle: {res}")
+    return res
 
 
 def pcie_bench(be, qubits: int) -> dict:
@@ -440,8 +573,13 @@ def main() -> int:
     ap.add_argument("--traj-shots", type=int, default=4096)
     ap.add_argument("--lanczos-sites", type=int, default=26, help="matrix-free TFIM Lanczos size (0 = skip)")
     ap.add_argument("--lanczos-csr-sites", type=int, default=22, help="CSR sparse Lanczos size (0 = skip)")
-    ap.add_argument("--uneven-instances", type=int, default=0,
+    ap.add_argument("--uneven-instances", type=int, default=512,
                     help="uneven QAOA adjoint batch size (0 = skip; the full workload is 512)")
+    ap.add_argument("--uneven-check", type=int, default=2,
+                    help="uneven instances (the narrowest) checked against the FP64 oracle")
+    ap.add_argument("--c64-steps", type=int, default=3, help="timed runs of the complex64 headline circuit (0 = skip)")
+    ap.add_argument("--fidelity-qubits", type=int, default=28,
+                    help="width of the headline-family check against the FP64 CPU oracle (0 = skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (3 launches) to scale HBM counters")
     args = ap.parse_args()
@@ -507,6 +645,9 @@ def main() -> int:
     wall = time.perf_counter() - t0
     gpu_ms = start.elapsed_time(end)
     log(f"timed {args.steps} steps: {gpu_ms:.1f} ms GPU, {wall * 1e3:.1f} ms wall")
+    ran = plan.kernels_run()
+    if args.kernel in (0, 3) and compiled and ran != ["compiled"]:
+        raise RuntimeError(f"the headline plan was to run compiled passes but launched {ran}")
 
     norm = state.norm_sqr()
     log(f"norm after {args.warmup + args.steps} runs: {norm:.15f}")
@@ -547,6 +688,7 @@ def main() -> int:
             "qubits": n, "depth": args.depth, "ops": stats["ops_in"], "passes_per_step": stats["passes"],
             "tile_qubits": tile_bits, "register_bits": reg_bits, "phases_per_step": stats["phases"],
             "pass_kernels": "compiled" if compiled else ("register" if reg_bits else "lds"),
+            "kernels_run": ran,
             "compile_ms": round(compile_info["compile_ms"], 1) if compile_info else None,
         },
         "roofline": {
@@ -568,8 +710,10 @@ def main() -> int:
     }
     out["roofline"]["traffic"] = measured_traffic(n, args, tile_bits, desc["passes"][0].get("low", 0), reg_bits,
                                                   stats["passes"], compiled)
+    if args.c64_steps > 0 and args.precision == "complex128":
+        out["complex64"] = complex64_section(be, torch, circuit, n, args.c64_steps, 1)
     if args.vqe_sets > 0:
-        out["vqe_adjoint"] = vqe_bench(be, args.vqe_sets, args.vqe_shift_sets, args.precision, not args.no_cpu)
+        out["vqe_adjoint"] = vqe_bench(be, args.vqe_sets, args.vqe_shift_sets, not args.no_cpu)
     if args.small_steps > 0:
         c20 = random_layer_circuit(20, args.depth, seed=args.seed)
         r = circuit_section(be, torch, c20, 20, "complex128", args.small_steps, 2,
@@ -589,10 +733,12 @@ def main() -> int:
     if args.lanczos_sites > 0 or args.lanczos_csr_sites > 0:
         out["lanczos"] = lanczos_bench(be, args.lanczos_sites, args.lanczos_csr_sites, not args.no_cpu)
     if args.uneven_instances > 0:
-        out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512)
+        out["uneven_qaoa"] = uneven_bench(be, args.uneven_instances, 512, 0 if args.no_cpu else args.uneven_check)
     if not args.no_cpu:
         log("timing the CPU baseline (oracle restatement of the reference CPU path)")
         out["cpu_baseline"] = cpu_baseline(circuit, n, args.cpu_sample_ops)
+    if args.fidelity_qubits > 0 and not args.no_cpu:
+        out["fidelity_vs_oracle"] = fidelity_oracle_section(be, args.fidelity_qubits, args.seed)
     print(json.dumps(out), flush=True)
     return 0
 

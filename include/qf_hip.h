@@ -248,7 +248,9 @@ qf_status qf_state_destroy(qf_state* state);
 /* CudaDeviceState::qubits, src/cuda.rs:1279. */
 qf_status qf_state_qubits(const qf_state* state, uint32_t* out);
 qf_status qf_state_precision(const qf_state* state, int32_t* out);
-/* CudaDeviceState::copy_from_device, src/cuda.rs:1444-1456. */
+/* CudaDeviceState::copy_from_device, src/cuda.rs:1444-1456; states of different
+ * precisions convert on the device (complex64 widened exactly, complex128
+ * rounded to nearest float). */
 qf_status qf_state_copy_from(qf_state* dst, const qf_state* src);
 /* Device address of the amplitudes (interop, e.g. wrapping as a torch tensor). */
 qf_status qf_state_device_pointer(const qf_state* state, void** out);
@@ -358,6 +360,29 @@ qf_status qf_batch_gradient(qf_backend* backend, const qf_pop* ops, size_t nops,
                             const qf_pauli_term* obs, size_t nterms, const qf_gradient_options* options,
                             double* energies, double* grads);
 
+/* What the last batched gradient or expectation call of a backend ran (no
+ * reference counterpart; a measurement record).  Pass counts and kernel
+ * families of the forward and reverse plans, the costate kernel, the
+ * algorithmic bytes of each phase (forward pass: 2 x 2^n x s per set, costate
+ * lambda = O psi: 2 x 2^n x s, reverse two-state pass: 4 x 2^n x s; s = 16
+ * complex128, 8 complex64), and with profiling on (qf_backend_set_profiling)
+ * the device time of each phase from HIP events on the backend's stream,
+ * summed over the call's memory waves. */
+typedef struct {
+    uint32_t sets, waves;
+    uint32_t forward_passes, reverse_passes;
+    uint32_t forward_kernels, reverse_kernels;  /* QF_KERNEL_* masks */
+    uint32_t costate_kernel;                    /* 1 windowed coefficient tables, 2 per-term signs */
+    uint32_t profiled;                          /* 1 when the *_ms fields were measured */
+    double forward_bytes, costate_bytes, reverse_bytes;
+    double forward_ms, costate_ms, reverse_ms, finish_ms;
+} qf_gradient_profile;
+
+/* Profiling of batched gradient calls on this backend (0 off, the default;
+ * 1 on: HIP events around each phase, read back after each wave). */
+qf_status qf_backend_set_profiling(qf_backend* backend, int32_t on);
+qf_status qf_backend_last_gradient_profile(const qf_backend* backend, qf_gradient_profile* out);
+
 /* <obs> at `sets` parameter sets (ParameterizedCircuit::run + expectation). */
 qf_status qf_batch_expectation(qf_backend* backend, const qf_pop* ops, size_t nops, uint32_t nparams,
                                uint32_t ninputs, const double* inputs, uint32_t qubits, const double* params,
@@ -390,6 +415,10 @@ typedef struct {
     uint32_t tile_qubits, low_qubits, reg_bits;  /* autotuned forward tile shape */
     uint32_t reserved;
     double tune_ms;                /* tuned forward time of the widest instance */
+    double alg_bytes;              /* algorithmic HBM bytes of the grouped instances' passes: per instance
+                                      forward passes x 2 x 2^n x s, costate 2 x 2^n x s, reverse passes
+                                      x 4 x 2^n x s (s = 16 complex128, 8 complex64) */
+    double device_ms;              /* device time of the graph replays (HIP events), summed over waves */
 } qf_uneven_stats;
 
 /* Energies <obs_i> and adjoint gradients of `count` parameterized circuits of

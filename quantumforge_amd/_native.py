@@ -55,6 +55,14 @@ class GradOptions(C.Structure):
                 ("shots", C.c_uint64), ("tile_qubits", C.c_uint32), ("chunk", C.c_uint32)]
 
 
+class GradientProfile(C.Structure):
+    _fields_ = [("sets", C.c_uint32), ("waves", C.c_uint32), ("forward_passes", C.c_uint32),
+                ("reverse_passes", C.c_uint32), ("forward_kernels", C.c_uint32), ("reverse_kernels", C.c_uint32),
+                ("costate_kernel", C.c_uint32), ("profiled", C.c_uint32), ("forward_bytes", C.c_double),
+                ("costate_bytes", C.c_double), ("reverse_bytes", C.c_double), ("forward_ms", C.c_double),
+                ("costate_ms", C.c_double), ("reverse_ms", C.c_double), ("finish_ms", C.c_double)]
+
+
 class UnevenInstance(C.Structure):
     _fields_ = [("ops", C.c_void_p), ("nops", C.c_size_t), ("nparams", C.c_uint32), ("qubits", C.c_uint32),
                 ("params", C.c_void_p), ("obs", C.c_void_p), ("nterms", C.c_size_t)]
@@ -63,7 +71,8 @@ class UnevenInstance(C.Structure):
 class UnevenStats(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("graph_nodes", C.c_uint32), ("graph_replays", C.c_uint32),
                 ("grouped_instances", C.c_uint32), ("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32),
-                ("reg_bits", C.c_uint32), ("reserved", C.c_uint32), ("tune_ms", C.c_double)]
+                ("reg_bits", C.c_uint32), ("reserved", C.c_uint32), ("tune_ms", C.c_double),
+                ("alg_bytes", C.c_double), ("device_ms", C.c_double)]
 
 
 P = C.c_void_p
@@ -80,6 +89,8 @@ SIGNATURES = {
     "qf_backend_device_ordinal": (None, [P, C.POINTER(i32)]),
     "qf_backend_set_plan_options": (None, [P, C.POINTER(PlanOptions)]),
     "qf_backend_stream": (None, [P, PP]),
+    "qf_backend_set_profiling": (None, [P, i32]),
+    "qf_backend_last_gradient_profile": (None, [P, C.POINTER(GradientProfile)]),
     "qf_state_zero": (None, [P, u32, i32, PP]),
     "qf_state_zero_with_memory_budget": (None, [P, u32, i32, u64, PP]),
     "qf_state_gather": (None, [P, P, sz, P]),

@@ -143,6 +143,22 @@ class HipBackend:
     def synchronize(self) -> None:
         check(lib().qf_backend_synchronize(self._h))
 
+    def set_profiling(self, on: bool) -> None:
+        """HIP-event timing of each phase of batched gradient calls."""
+        check(lib().qf_backend_set_profiling(self._h, int(bool(on))))
+
+    def last_gradient_profile(self) -> dict:
+        """What the last batched gradient / expectation call ran: pass
+        counts, kernel families, algorithmic bytes and (profiling on) device
+        milliseconds per phase (qf_backend_last_gradient_profile)."""
+        p = N.GradientProfile()
+        check(lib().qf_backend_last_gradient_profile(self._h, C.byref(p)))
+        d = {f: getattr(p, f) for f, _ in N.GradientProfile._fields_}
+        for k in ("forward_kernels", "reverse_kernels"):
+            d[k] = [name for b, name in sorted(N.KERNEL_BITS.items()) if d[k] & b]
+        d["costate_kernel"] = {1: "windowed", 2: "per_term"}.get(d["costate_kernel"], "none")
+        return d
+
     def set_plan_options(self, fuse: bool = True, tile_qubits: int = 0, low_qubits: int = 0, kernel: int = 0,
                          reg_bits: int = 0) -> None:
         o = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits)

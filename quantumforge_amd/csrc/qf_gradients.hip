@@ -220,6 +220,109 @@ std::vector<unsigned char> pack_terms(const std::vector<qf::PauliDev>& sorted, u
     return out;
 }
 
+// Windowed form of a sorted term list for k_pauli_windows.  The terms of one
+// x mask act on psi[i ^ x] with the coefficient
+//   C_x(i) = sum_t c'_t (-1)^popcount((i ^ x) & z_t),
+// which depends only on the bits of i in the union of the z masks.  Terms are
+// packed into windows of at most kPauliWindowBits consecutive index bits that
+// hold their whole z mask; a window's 2^w coefficients (every sign pattern,
+// x folded in) are tabulated on the host in FP64, so the device takes one
+// table lookup per window instead of one popcount and sign per term.  Terms
+// whose z mask spans more bits stay per-term ("wide").
+struct PWGroup {
+    uint64_t x;
+    uint32_t w0, w1;  // windows [w0, w1)
+    uint32_t t0, t1;  // wide terms [t0, t1)
+};
+struct PWWin {
+    uint32_t lo, mask, off, pad;  // key = (i >> lo) & mask, coefficient at off + key
+};
+struct PWHeader {
+    uint32_t ngroups, nwin, ncoef, nwide;
+    uint64_t groups_off, wins_off, coef_off, wide_off;  // byte offsets in the pack
+};
+constexpr uint32_t kPauliWindowBits = 6;
+constexpr uint32_t kPauliMaxCoef = 2048;  // 32 KiB of LDS
+
+// Returns an empty vector when the windowed tables would exceed kPauliMaxCoef
+// entries (k_pauli_apply takes such term lists).
+std::vector<unsigned char> pack_pauli_windows(const std::vector<qf::PauliDev>& sorted) {
+    std::vector<PWGroup> groups;
+    std::vector<PWWin> wins;
+    std::vector<double> coef;  // (re, im) pairs
+    std::vector<qf::PauliDev> wide;
+    size_t t = 0;
+    while (t < sorted.size()) {
+        size_t e = t;
+        while (e < sorted.size() && sorted[e].x == sorted[t].x) ++e;
+        PWGroup g{sorted[t].x, uint32_t(wins.size()), 0, uint32_t(wide.size()), 0};
+        // terms of this x mask by the lowest bit of their z mask (z = 0 first)
+        std::vector<size_t> idx;
+        for (size_t k = t; k < e; ++k) idx.push_back(k);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+            const uint64_t za = sorted[a].z, zb = sorted[b].z;
+            const int la = za ? __builtin_ctzll(za) : -1, lb = zb ? __builtin_ctzll(zb) : -1;
+            return la < lb;
+        });
+        std::vector<bool> used(idx.size(), false);
+        for (size_t a = 0; a < idx.size(); ++a) {
+            if (used[a]) continue;
+            const uint64_t za = sorted[idx[a]].z;
+            if (za && 64 - __builtin_clzll(za) - __builtin_ctzll(za) > int(kPauliWindowBits)) {
+                wide.push_back(sorted[idx[a]]);
+                used[a] = true;
+                continue;
+            }
+            const uint32_t lo = za ? uint32_t(__builtin_ctzll(za)) : 0;
+            const uint64_t wmask = ((uint64_t(1) << kPauliWindowBits) - 1) << lo;
+            std::vector<size_t> members;
+            uint64_t span = 0;
+            for (size_t b = a; b < idx.size(); ++b) {
+                const uint64_t zb = sorted[idx[b]].z;
+                if (!used[b] && (zb & ~wmask) == 0) {
+                    used[b] = true;
+                    members.push_back(idx[b]);
+                    span |= zb;
+                }
+            }
+            const uint32_t width = span ? uint32_t(64 - __builtin_clzll(span)) - lo : 0;
+            const uint32_t size = 1u << width;
+            PWWin w{lo, size - 1, uint32_t(coef.size() / 2), 0};
+            for (uint32_t key = 0; key < size; ++key) {
+                const uint64_t bits = (uint64_t(key) << lo) ^ g.x;
+                double cr = 0.0, ci = 0.0;
+                for (size_t m : members) {
+                    const double sg = (__builtin_popcountll(bits & sorted[m].z) & 1) ? -1.0 : 1.0;
+                    cr += sg * sorted[m].cr;
+                    ci += sg * sorted[m].ci;
+                }
+                coef.push_back(cr);
+                coef.push_back(ci);
+            }
+            wins.push_back(w);
+        }
+        g.w1 = uint32_t(wins.size());
+        g.t1 = uint32_t(wide.size());
+        groups.push_back(g);
+        t = e;
+    }
+    if (coef.size() / 2 > kPauliMaxCoef) return {};
+    auto al = [](size_t v) { return (v + 15) / 16 * 16; };
+    PWHeader h{uint32_t(groups.size()), uint32_t(wins.size()), uint32_t(coef.size() / 2), uint32_t(wide.size()),
+               0, 0, 0, 0};
+    h.groups_off = al(sizeof(PWHeader));
+    h.wins_off = al(h.groups_off + groups.size() * sizeof(PWGroup));
+    h.coef_off = al(h.wins_off + wins.size() * sizeof(PWWin));
+    h.wide_off = al(h.coef_off + coef.size() * sizeof(double));
+    std::vector<unsigned char> out(al(h.wide_off + wide.size() * sizeof(qf::PauliDev)) + 16, 0);
+    std::memcpy(out.data(), &h, sizeof h);
+    if (!groups.empty()) std::memcpy(out.data() + h.groups_off, groups.data(), groups.size() * sizeof(PWGroup));
+    if (!wins.empty()) std::memcpy(out.data() + h.wins_off, wins.data(), wins.size() * sizeof(PWWin));
+    if (!coef.empty()) std::memcpy(out.data() + h.coef_off, coef.data(), coef.size() * sizeof(double));
+    if (!wide.empty()) std::memcpy(out.data() + h.wide_off, wide.data(), wide.size() * sizeof(qf::PauliDev));
+    return out;
+}
+
 }  // namespace qfrt
 
 namespace {
@@ -227,6 +330,81 @@ namespace {
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
+
+// lambda = O psi and the energy partials conj(psi_i) lambda_i (FP64,
+// compensated) from the windowed term tables (pack_pauli_windows).  The grid
+// is 8 x (blocks / 8): workgroup b runs on XCD b mod 8 (round-robin dispatch)
+// and takes logical block (b mod 8) * (blocks / 8) + b / 8, so every XCD walks
+// its own contiguous run of instances and the partner amplitudes
+// psi[i ^ x] of an instance are read from that XCD's L2 rather than from HBM
+// once per x mask.  Logical block L is (instance L / eblk, chunk L % eblk);
+// its partial goes to the same slot as k_pauli_apply's, so k_reduce_finish is
+// shared.
+template <class R>
+__global__ __launch_bounds__(qf::kThreads) void k_pauli_windows(const cx<R>* __restrict__ psi,
+                                                                cx<R>* __restrict__ lam, uint32_t n,
+                                                                const unsigned char* __restrict__ pack,
+                                                                uint32_t cnt, uint32_t eblk,
+                                                                qf::dd* __restrict__ partial) {
+    extern __shared__ double2 coef_s[];
+    const PWHeader* __restrict__ h = reinterpret_cast<const PWHeader*>(pack);
+    const PWGroup* __restrict__ groups = reinterpret_cast<const PWGroup*>(pack + h->groups_off);
+    const PWWin* __restrict__ wins = reinterpret_cast<const PWWin*>(pack + h->wins_off);
+    const double2* __restrict__ coef = reinterpret_cast<const double2*>(pack + h->coef_off);
+    const qf::PauliDev* __restrict__ wide = reinterpret_cast<const qf::PauliDev*>(pack + h->wide_off);
+    const uint32_t per = gridDim.x >> 3;
+    const uint64_t L = uint64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (L >= uint64_t(cnt) * eblk) return;  // the whole workgroup leaves together
+    for (uint32_t k = threadIdx.x; k < h->ncoef; k += blockDim.x) coef_s[k] = coef[k];
+    __syncthreads();
+    const uint64_t inst = L / eblk, chunk = L % eblk;
+    const uint64_t dim = uint64_t(1) << n;
+    const cx<R>* __restrict__ p = psi + inst * dim;
+    cx<R>* __restrict__ l = lam ? lam + inst * dim : nullptr;
+    const uint32_t ngroups = h->ngroups;
+    qf::dd er{0, 0}, ei{0, 0};
+    const uint64_t stride = uint64_t(eblk) * blockDim.x;
+    for (uint64_t i = chunk * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        double ar = 0.0, ai = 0.0;
+        for (uint32_t g0 = 0; g0 < ngroups; g0 += 8) {
+            cx<R> v[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u)
+                if (g0 + u < ngroups) v[u] = p[i ^ groups[g0 + u].x];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                if (g0 + u >= ngroups) break;
+                const PWGroup G = groups[g0 + u];
+                double cr = 0.0, ci = 0.0;
+                for (uint32_t w = G.w0; w < G.w1; ++w) {
+                    const PWWin W = wins[w];
+                    const double2 c = coef_s[W.off + (uint32_t(i >> W.lo) & W.mask)];
+                    cr += c.x;
+                    ci += c.y;
+                }
+                for (uint32_t t = G.t0; t < G.t1; ++t) {
+                    const qf::PauliDev& T = wide[t];
+                    const double sg = (__popcll((i ^ G.x) & T.z) & 1) ? -1.0 : 1.0;
+                    cr += sg * T.cr;
+                    ci += sg * T.ci;
+                }
+                const double pr = v[u].re, pi = v[u].im;
+                ar += cr * pr - ci * pi;
+                ai += cr * pi + ci * pr;
+            }
+        }
+        if (l) l[i] = cx<R>{R(ar), R(ai)};
+        const cx<R> v = p[i];
+        const double vr = v.re, vi = v.im;
+        qf::neumaier(er, vr * ar + vi * ai);  // Re(conj(psi) lambda)
+        qf::neumaier(ei, vr * ai - vi * ar);  // Im(conj(psi) lambda)
+    }
+    qf::block_reduce(er, ei);
+    if (threadIdx.x == 0) {
+        partial[2 * (inst * eblk + chunk)] = er;
+        partial[2 * (inst * eblk + chunk) + 1] = ei;
+    }
+}
 
 // lambda = O psi (Pauli sum, terms sorted by x mask, groups after them) and
 // the energy partials conj(psi_i) lambda_i, FP64 compensated, one row per
@@ -432,7 +610,7 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
     const uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax);
     if (wave == 0) return fail(QF_OUT_OF_MEMORY, "device memory cannot hold one parameter set's states");
 
-    DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms;
+    DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms, wterms;
     qf_status s;
     if ((s = psi.alloc(size_t(wave) * dim * ab))) return s;
     if ((s = lam.alloc(want_grad ? size_t(wave) * dim * ab : 16))) return s;
@@ -455,6 +633,13 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
     uint32_t ngroups = 0;
     const std::vector<unsigned char> tpack = pack_terms(tsorted, &ngroups);
     if ((s = terms.alloc(tpack.size()))) return s;
+    const std::vector<unsigned char> wpack = pack_pauli_windows(tsorted);
+    uint32_t wcoef = 0;
+    if (!wpack.empty()) {
+        wcoef = reinterpret_cast<const PWHeader*>(wpack.data())->ncoef;
+        if ((s = wterms.alloc(wpack.size()))) return s;
+        QF_HIP_TRY(hipMemcpyAsync(wterms.p, wpack.data(), wpack.size(), hipMemcpyHostToDevice, st));
+    }
     QF_HIP_TRY(hipMemcpyAsync(pstart.p, ps.data(), ps.size() * 4, hipMemcpyHostToDevice, st));
     if (!pl.empty()) QF_HIP_TRY(hipMemcpyAsync(pslots.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice, st));
     QF_HIP_TRY(hipMemcpyAsync(terms.p, tpack.data(), tpack.size(), hipMemcpyHostToDevice, st));
@@ -465,9 +650,32 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
     if (want_grad) plans.push_back(&R.rev);
     if ((s = upload_plans(be, plans, bases))) return s;
 
+    qf_gradient_profile& P = be->grad_prof;
+    P = qf_gradient_profile{};
+    P.sets = cnt_total;
+    P.forward_passes = uint32_t(R.fwd.passes.size());
+    P.reverse_passes = want_grad ? uint32_t(R.rev.passes.size()) : 0;
+    P.costate_kernel = wpack.empty() ? 2 : 1;
+    P.profiled = be->profile ? 1 : 0;
+    const double sweep = 2.0 * double(dim) * double(ab);
+    P.forward_bytes = sweep * P.forward_passes * cnt_total;
+    P.costate_bytes = sweep * cnt_total;
+    P.reverse_bytes = 2.0 * sweep * P.reverse_passes * cnt_total;
+    struct Ev {
+        hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        ~Ev() {
+            for (auto& x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    if (be->profile)
+        for (auto& x : ev.e) QF_HIP_TRY(hipEventCreate(&x));
+    auto mark = [&](int k) { return be->profile ? hipEventRecord(ev.e[k], st) : hipSuccess; };
+
     std::vector<double> eo(size_t(wave) * 2);
     for (uint32_t off = 0; off < cnt_total; off += wave) {
         const uint32_t cnt = std::min(wave, cnt_total - off);
+        ++P.waves;
         // Synchronous copy: the previous wave finished at its download below.
         if (nslots)
             QF_HIP_TRY(hipMemcpy(cs.p, table + size_t(off) * nslots * 2, size_t(cnt) * nslots * 16,
@@ -480,23 +688,44 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         LaunchCtx fctx;
         fctx.cs = static_cast<const double*>(cs.p);
         fctx.nslots = nslots;
+        QF_HIP_TRY(mark(0));
         if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.fwd, bases[0], fctx))) return s;
-        dim3 eg(eblk, cnt);
+        P.forward_kernels |= R.fwd.kernels_run;
+        QF_HIP_TRY(mark(1));
         void* lam_out = want_grad ? lam.p : nullptr;
-        if (R.precision == QF_C128)
-            hipLaunchKernelGGL(k_pauli_apply<double>, eg, dim3(qf::kThreads), 0, st,
-                               static_cast<const cx<double>*>(psi.p), static_cast<cx<double>*>(lam_out), n,
-                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
-                               static_cast<qf::dd*>(epart.p));
-        else
-            hipLaunchKernelGGL(k_pauli_apply<float>, eg, dim3(qf::kThreads), 0, st,
-                               static_cast<const cx<float>*>(psi.p), static_cast<cx<float>*>(lam_out), n,
-                               static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
-                               static_cast<qf::dd*>(epart.p));
-        if ((s = launch_check("k_pauli_apply"))) return s;
+        if (!wpack.empty()) {
+            const uint64_t blocks = (uint64_t(cnt) * eblk + 7) / 8 * 8;
+            const size_t lds = size_t(std::max<uint32_t>(wcoef, 1)) * 16;
+            if (R.precision == QF_C128)
+                hipLaunchKernelGGL(k_pauli_windows<double>, dim3(unsigned(blocks)), dim3(qf::kThreads), lds, st,
+                                   static_cast<const cx<double>*>(psi.p), static_cast<cx<double>*>(lam_out), n,
+                                   static_cast<const unsigned char*>(wterms.p), cnt, eblk,
+                                   static_cast<qf::dd*>(epart.p));
+            else
+                hipLaunchKernelGGL(k_pauli_windows<float>, dim3(unsigned(blocks)), dim3(qf::kThreads), lds, st,
+                                   static_cast<const cx<float>*>(psi.p), static_cast<cx<float>*>(lam_out), n,
+                                   static_cast<const unsigned char*>(wterms.p), cnt, eblk,
+                                   static_cast<qf::dd*>(epart.p));
+            if ((s = launch_check("k_pauli_windows"))) return s;
+        } else {
+            dim3 eg(eblk, cnt);
+            if (R.precision == QF_C128)
+                hipLaunchKernelGGL(k_pauli_apply<double>, eg, dim3(qf::kThreads), 0, st,
+                                   static_cast<const cx<double>*>(psi.p), static_cast<cx<double>*>(lam_out), n,
+                                   static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
+                                   static_cast<qf::dd*>(epart.p));
+            else
+                hipLaunchKernelGGL(k_pauli_apply<float>, eg, dim3(qf::kThreads), 0, st,
+                                   static_cast<const cx<float>*>(psi.p), static_cast<cx<float>*>(lam_out), n,
+                                   static_cast<const qf::PauliDev*>(terms.p), uint32_t(tsorted.size()), ngroups,
+                                   static_cast<qf::dd*>(epart.p));
+            if ((s = launch_check("k_pauli_apply"))) return s;
+        }
+        QF_HIP_TRY(mark(2));
         hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(cnt), dim3(qf::kThreads), 0, st,
                            static_cast<const qf::dd*>(epart.p), eblk, static_cast<double*>(eout.p));
         if ((s = launch_check("k_reduce_finish"))) return s;
+        QF_HIP_TRY(mark(3));
         if (want_grad) {
             LaunchCtx rctx;
             rctx.cs = static_cast<const double*>(cs.p);
@@ -504,6 +733,7 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
             rctx.lam = lam.p;
             rctx.partial = static_cast<double*>(part.p);
             if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.rev, bases[1], rctx))) return s;
+            P.reverse_kernels |= R.rev.kernels_run;
             const uint64_t work = uint64_t(cnt) * R.pc.nparams;
             if (work) {
                 hipLaunchKernelGGL(k_grad_finish, dim3(unsigned((work + 255) / 256)), dim3(256), 0, st,
@@ -515,9 +745,19 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
                                           hipMemcpyDeviceToHost, st));
             }
         }
+        QF_HIP_TRY(mark(4));
         QF_HIP_TRY(hipMemcpyAsync(eo.data(), eout.p, size_t(cnt) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         QF_HIP_TRY(hipStreamSynchronize(st));
         for (uint32_t b = 0; b < cnt; ++b) energies[off + b] = eo[2 * b];
+        if (be->profile) {
+            // ev 0..1 forward, 1..2 costate, 2..3 energy finish, 3..4 reverse + gradient finish
+            float t[4] = {0, 0, 0, 0};
+            for (int k = 0; k < 4; ++k) QF_HIP_TRY(hipEventElapsedTime(&t[k], ev.e[k], ev.e[k + 1]));
+            P.forward_ms += t[0];
+            P.costate_ms += t[1];
+            P.finish_ms += t[2];
+            P.reverse_ms += t[3];
+        }
     }
     return QF_OK;
 }
@@ -924,5 +1164,17 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
     }
     p->plan.origin = qf::PLAN_PARAMETERIZED;
     *out = p;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_set_profiling(qf_backend* be, int32_t on) {
+    if (!be) return fail(QF_INVALID_ARGUMENT, "null backend");
+    be->profile = on ? 1 : 0;
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_last_gradient_profile(const qf_backend* be, qf_gradient_profile* out) {
+    if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    *out = be->grad_prof;
     return QF_OK;
 }

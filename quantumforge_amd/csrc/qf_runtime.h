@@ -37,6 +37,9 @@ struct qf_backend {
     size_t meas_a_bytes = 0;
     void* meas_b = nullptr;
     size_t meas_b_bytes = 0;
+    // Measurement record of the last batched gradient / expectation call.
+    int32_t profile = 0;
+    qf_gradient_profile grad_prof{};
 };
 
 struct qf_state {

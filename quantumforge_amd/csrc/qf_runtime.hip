@@ -797,13 +797,25 @@ extern "C" qf_status qf_state_precision(const qf_state* st, int32_t* out) {
 
 extern "C" qf_status qf_state_copy_from(qf_state* dst, const qf_state* src) {
     if (!dst || !src) return fail(QF_INVALID_ARGUMENT, "null argument");
-    if (dst->dim != src->dim || dst->precision != src->precision || dst->be != src->be)
+    if (dst->dim != src->dim || dst->be != src->be)
         return fail(QF_INCOMPATIBLE_DEVICE_STATE, "device states must share a backend and have equal dimensions");
     qf_status s = activate(dst->be);
     if (s) return s;
-    HIP_TRY(hipMemcpyAsync(dst->d, src->d, dst->dim * amp_bytes(dst->precision), hipMemcpyDeviceToDevice,
-                           dst->be->stream));
-    return QF_OK;
+    if (dst->precision == src->precision) {
+        HIP_TRY(hipMemcpyAsync(dst->d, src->d, dst->dim * amp_bytes(dst->precision), hipMemcpyDeviceToDevice,
+                               dst->be->stream));
+        return QF_OK;
+    }
+    // Precision conversion on the device (complex64 <-> complex128).
+    if (src->precision == QF_C64)
+        hipLaunchKernelGGL((qf::k_convert<float, double>), dim3(grid_for(dst->dim)), dim3(qf::kThreads), 0,
+                           dst->be->stream, static_cast<const cx<float>*>(src->d), static_cast<cx<double>*>(dst->d),
+                           dst->dim);
+    else
+        hipLaunchKernelGGL((qf::k_convert<double, float>), dim3(grid_for(dst->dim)), dim3(qf::kThreads), 0,
+                           dst->be->stream, static_cast<const cx<double>*>(src->d), static_cast<cx<float>*>(dst->d),
+                           dst->dim);
+    return launch_check("k_convert");
 }
 
 extern "C" qf_status qf_state_device_pointer(const qf_state* st, void** out) {

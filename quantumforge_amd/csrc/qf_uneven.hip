@@ -707,6 +707,17 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
     if (s) return s;
     S.graph_nodes = nodes;
 
+    for (uint32_t k : grouped) {
+        const double sweep = 2.0 * double(uint64_t(1) << inst[k].qubits) * double(ab);
+        S.alg_bytes += sweep * (double(I[k].fwd.passes.size()) + 1.0 + 2.0 * double(I[k].rev.passes.size()));
+    }
+    hipEvent_t wave_a = nullptr, wave_b = nullptr;
+    QF_HIP_TRY(hipEventCreate(&wave_a));
+    if (hipEventCreate(&wave_b) != hipSuccess) {
+        (void)hipEventDestroy(wave_a);
+        return fail(QF_DEVICE_ERROR, "hipEventCreate");
+    }
+
     // 8. per wave: build and upload the tables, replay the graph
     std::vector<unsigned char> host(ntables * table_stride);
     qf_status result = QF_OK;
@@ -782,14 +793,21 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
             result = fail(QF_DEVICE_ERROR, "uploading wave tables failed");
             break;
         }
-        hipError_t e = hipGraphLaunch(exec, st);
+        hipError_t e = hipEventRecord(wave_a, st);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+        if (e == hipSuccess) e = hipEventRecord(wave_b, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, wave_a, wave_b);
         if (e != hipSuccess) {
             result = hip_fail(e, "hipGraphLaunch (uneven wave)");
             break;
         }
+        S.device_ms += ms;
         ++S.graph_replays;
     }
+    (void)hipEventDestroy(wave_a);
+    (void)hipEventDestroy(wave_b);
     (void)hipGraphExecDestroy(exec);
     if (result) return result;
     std::vector<double> e_all(count);

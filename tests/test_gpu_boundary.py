@@ -161,3 +161,22 @@ def test_stream_rate_leaves_the_state_unchanged(backend, variant):
     assert np.array_equal(dev.download().amplitudes, a0)
     with pytest.raises(QuantumError):
         dev.stream_rate(4, 1)
+
+
+def test_copy_from_device_same_and_converted_precision(backend):
+    # CudaDeviceState::copy_from_device, src/cuda.rs:1444-1456, plus conversion.
+    n = 12
+    a0 = random_state(n, 12)
+    src = backend.upload(StateVector(a0, n))
+    dst = backend.zero_state(n)
+    dst.copy_from_device(src)
+    assert np.array_equal(dst.download().amplitudes, a0)
+    narrow = backend.zero_state(n, precision="complex64")
+    narrow.copy_from_device(src)
+    assert np.array_equal(narrow.download_c64(), a0.astype(np.complex64))
+    wide = backend.zero_state(n)
+    wide.copy_from_device(narrow)
+    assert np.array_equal(wide.download().amplitudes, a0.astype(np.complex64).astype(np.complex128))
+    with pytest.raises(QuantumError) as e:
+        backend.zero_state(n + 1).copy_from_device(src)
+    assert e.value.kind == "IncompatibleDeviceState"

@@ -289,3 +289,33 @@ def test_uneven_qaoa_multi_wave_at_config_shape(backend, monkeypatch):
         want_g = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, ang, [], q, t)
         assert abs(e[k] - want_e) < 1e-10, (k, e[k], want_e)
         assert np.max(np.abs(g[k] - want_g)) < 1e-10, k
+
+
+def test_windowed_pauli_tables_on_general_sums(backend):
+    """The costate kernel's windowed coefficient tables on a Pauli sum with
+    X, Y and Z mixed in one word, complex coefficients, several words per x
+    mask, z masks wider than one table window (per-term signs) and a constant
+    term: energies and adjoint gradients against the oracle (PauliSum
+    expectation, src/analytic.rs:2751-2778)."""
+    n = 12
+    rng = np.random.default_rng(5)
+    H = PauliSum(n)
+    H.add(0.7, "I" * n)
+    for _ in range(40):
+        w = ["I"] * n
+        for q in rng.choice(n, size=int(rng.integers(1, 5)), replace=False):
+            w[q] = "XYZ"[int(rng.integers(0, 3))]
+        H.add(complex(rng.normal(), 0.0), "".join(w))
+    # the same x mask with many z patterns, and a z span of 12 bits
+    H.add(0.3, "X" + "I" * (n - 2) + "Z")
+    H.add(-0.4, "Y" + "Z" * (n - 2) + "Z")
+    H.add(0.25, "Z" * n)
+    pc = hea_circuit(n, 2)
+    params = uniform_parameters(pc.parameter_count, 5, seed=3)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    t = H.to_terms()
+    for s in range(params.shape[0]):
+        want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        want_g = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        assert abs(e[s] - want_e) < 1e-12, s
+        assert np.max(np.abs(g[s] - want_g)) < 1e-11, s
