@@ -294,6 +294,18 @@ qf_status qf_state_expectation(const qf_state* state, int32_t observable, uint32
 /* PauliSum::expectation_ket, src/analytic.rs:2775-2778 (FP64, compensated). */
 qf_status qf_state_expectation_pauli_sum(const qf_state* state, const qf_pauli_term* terms, size_t count,
                                          double* out_re, double* out_im);
+/* qf_state_expectation_pauli_sum without allocating: no reference
+ * counterpart.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the reduction partials, the term table and the result live in a fixed
+ * device array of 164096 bytes per GPU (held until the process ends), so the
+ * call makes no hipMalloc / hipFree.  At most 4096 terms (QF_INVALID_ARGUMENT
+ * beyond); same kernels and summation order as
+ * qf_state_expectation_pauli_sum, so the results are identical.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (states of
+ * different backends used from several threads) race. */
+qf_status qf_state_expectation_pauli_sum_noalloc(const qf_state* state, const qf_pauli_term* terms, size_t count,
+                                                 double* out_re, double* out_im);
 /* <a|b> (Bra::inner, src/quantum.rs:327-337), FP64 compensated. */
 qf_status qf_state_inner(const qf_state* a, const qf_state* b, double* out_re, double* out_im);
 

@@ -116,6 +116,7 @@ SIGNATURES = {
     "qf_state_expectation": (None, [P, i32, u32, C.POINTER(dbl)]),
     "qf_state_expectation_pauli_sum": (None, [P, P, sz, C.POINTER(dbl), C.POINTER(dbl)]),
     "qf_state_inner": (None, [P, P, C.POINTER(dbl), C.POINTER(dbl)]),
+    "qf_state_expectation_pauli_sum_noalloc": (None, [P, P, sz, C.POINTER(dbl), C.POINTER(dbl)]),
     "qf_batch_upload": (None, [P, P, P, u32, PP]),
     "qf_batch_zero": (None, [P, u32, u32, i32, PP]),
     "qf_batch_destroy": (None, [P]),

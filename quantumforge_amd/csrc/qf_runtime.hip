@@ -582,6 +582,62 @@ qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits,
 }  // namespace qfrt
 
 // ---------------------------------------------------------------------------
+// Fixed device scratch for the entry points that allocate nothing
+// (qf_state_expectation_pauli_sum_noalloc).  One array per device, held until
+// the process ends; every call on a device shares it, so calls that overlap on
+// one device (two backends' streams at once) race.  Layout: the reduction's
+// per-block partials, then the two result doubles, then the term table.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchPartialBytes = 1024 * 2 * sizeof(qf::dd);  // k_reduce: at most 1024 blocks
+constexpr size_t kScratchOutOff = kScratchPartialBytes;
+constexpr size_t kScratchTermsOff = kScratchOutOff + 256;
+constexpr size_t kNoAllocMaxTerms = 4096;
+constexpr size_t kScratchBytes = kScratchTermsOff + kNoAllocMaxTerms * sizeof(qf::PauliDev);
+static_assert(sizeof(qf::PauliDev) == sizeof(qf_pauli_term), "term records are converted in place");
+static_assert(kScratchBytes % 256 == 0, "scratch size");
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// The device address of `scratch` on device `ordinal` (which must be
+// current): hipGetSymbolAddress once per device, then the cached address.
+qf_status scratch_address(int ordinal, void** out) {
+    if (ordinal < 0 || ordinal >= 64) return fail(QF_DEVICE_ERROR, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ordinal]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        }
+        g_scratch_ptr[ordinal] = p;
+    }
+    *out = g_scratch_ptr[ordinal];
+    return QF_OK;
+}
+
+// qf_pauli_term -> qf::PauliDev in place: c' = c * i^(#Y), the same exact
+// swaps and negations as pauli_terms on the host.
+__global__ void k_pauli_terms_in_place(qf::PauliDev* __restrict__ t, uint32_t count) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double cr = t[k].cr, ci = t[k].ci;
+    const int ny = __popcll(t[k].x & t[k].z) & 3;
+    for (int r = 0; r < ny; ++r) {
+        const double nr = -ci, ni = cr;
+        cr = nr;
+        ci = ni;
+    }
+    t[k].cr = cr;
+    t[k].ci = ci;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Backend
 // ---------------------------------------------------------------------------
 extern "C" qf_status qf_backend_create(int32_t ordinal, qf_backend** out) {
@@ -593,6 +649,12 @@ extern "C" qf_status qf_backend_create(int32_t ordinal, qf_backend** out) {
         return fail(QF_DEVICE_ERROR, "HIP device ordinal " + std::to_string(ordinal) + " not present (" +
                                          std::to_string(n) + " devices)");
     HIP_TRY(hipSetDevice(ordinal));
+    // The no-allocation entry points' scratch address, once per device with
+    // that device current (a failure here is reported by those entry points).
+    {
+        void* sp = nullptr;
+        if (scratch_address(ordinal, &sp) != QF_OK) g_last_error.clear();
+    }
     qf_backend* be = new qf_backend;
     be->ordinal = ordinal;
     hipError_t e = hipStreamCreateWithFlags(&be->stream, hipStreamNonBlocking);
@@ -1093,6 +1155,65 @@ extern "C" qf_status qf_state_expectation_pauli_sum(const qf_state* st, const qf
     std::vector<double> r;
     s = reduce(st->be, st->d, nullptr, st->precision, st->dim, 1, qf::RED_PAULI, 0, t, r);
     if (s) return s;
+    *out_re = r[0];
+    *out_im = r[1];
+    return QF_OK;
+}
+
+// qf_state_expectation_pauli_sum without any allocation: the reduction
+// partials, the term table and the result live in the device's fixed scratch
+// array (kScratchBytes); the same kernels, grid and summation order as the
+// allocating entry point, so the results are identical.  Nothing in the
+// scratch needs clearing: every partial, term and result slot a call reads
+// was written by that call.
+extern "C" qf_status qf_state_expectation_pauli_sum_noalloc(const qf_state* st, const qf_pauli_term* terms,
+                                                            size_t count, double* out_re, double* out_im) {
+    if (!st || (!terms && count) || !out_re || !out_im) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (count > kNoAllocMaxTerms)
+        return fail(QF_INVALID_ARGUMENT, "the no-allocation expectation takes at most " +
+                                             std::to_string(kNoAllocMaxTerms) + " Pauli terms, got " +
+                                             std::to_string(count));
+    for (size_t t = 0; t < count; ++t) {
+        const uint64_t all = terms[t].x_mask | terms[t].z_mask;
+        if (st->qubits < 64 && (all >> st->qubits))
+            return fail(QF_QUBIT_OUT_OF_RANGE, "Pauli term acts on a qubit out of range for " +
+                                                   std::to_string(st->qubits) + " qubits");
+    }
+    if (count == 0) {
+        *out_re = *out_im = 0.0;
+        return QF_OK;
+    }
+    qf_backend* be = st->be;
+    qf_status s = activate(be);
+    if (s) return s;
+    void* base = nullptr;
+    if ((s = scratch_address(be->ordinal, &base))) return s;
+    char* sc = static_cast<char*>(base);
+    qf::dd* partial = reinterpret_cast<qf::dd*>(sc);
+    double* d_out = reinterpret_cast<double*>(sc + kScratchOutOff);
+    qf::PauliDev* d_terms = reinterpret_cast<qf::PauliDev*>(sc + kScratchTermsOff);
+    uint64_t blocks = (st->dim + qf::kThreads * 4 - 1) / (qf::kThreads * 4);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    HIP_TRY(hipMemcpyAsync(d_terms, terms, count * sizeof(qf_pauli_term), hipMemcpyHostToDevice, be->stream));
+    hipLaunchKernelGGL(k_pauli_terms_in_place, dim3(unsigned((count + 255) / 256)), dim3(256), 0, be->stream, d_terms,
+                       uint32_t(count));
+    if ((s = launch_check("k_pauli_terms_in_place"))) return s;
+    if (st->precision == QF_C128)
+        hipLaunchKernelGGL(qf::k_reduce<double>, dim3(unsigned(blocks), 1), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<double>*>(st->d), static_cast<const cx<double>*>(nullptr), st->dim,
+                           int(qf::RED_PAULI), 0u, d_terms, uint32_t(count), partial);
+    else
+        hipLaunchKernelGGL(qf::k_reduce<float>, dim3(unsigned(blocks), 1), dim3(qf::kThreads), 0, be->stream,
+                           static_cast<const cx<float>*>(st->d), static_cast<const cx<float>*>(nullptr), st->dim,
+                           int(qf::RED_PAULI), 0u, d_terms, uint32_t(count), partial);
+    if ((s = launch_check("k_reduce"))) return s;
+    hipLaunchKernelGGL(qf::k_reduce_finish<>, dim3(1), dim3(qf::kThreads), 0, be->stream, partial, uint32_t(blocks),
+                       d_out);
+    if ((s = launch_check("k_reduce_finish"))) return s;
+    double r[2] = {0.0, 0.0};
+    HIP_TRY(hipMemcpyAsync(r, d_out, sizeof r, hipMemcpyDeviceToHost, be->stream));
+    HIP_TRY(hipStreamSynchronize(be->stream));
     *out_re = r[0];
     *out_im = r[1];
     return QF_OK;
