@@ -194,12 +194,13 @@ def vqe_bench(be, sets: int, shift_sets: int, with_cpu: bool) -> dict:
     return res
 
 
-def circuit_section(be, torch, circuit, n: int, precision: str, steps: int, warmup: int, label: str) -> dict:
+def circuit_section(be, torch, circuit, n: int, precision: str, steps: int, warmup: int, label: str,
+                    kernel: int = 0, max_block: int = 0, stream_probe: bool = False) -> dict:
     """Times `steps` runs of `circuit` on a resident |0..0> state with its
-    default plan (HIP events on the engine's stream)."""
+    default plan (or the plan of `kernel`; HIP events on the engine's stream)."""
     from quantumforge_amd.backend import Plan
 
-    plan = Plan(circuit, n, precision=precision)
+    plan = Plan(circuit, n, precision=precision, kernel=kernel, max_block=max_block)
     stats = plan.stats()
     compiled = bool(plan.describe().get("compiled", 0))
     if compiled:
@@ -221,27 +222,51 @@ def circuit_section(be, torch, circuit, n: int, precision: str, steps: int, warm
     per_pass = ms / max(1, stats["passes"])
     res = {"workload": label, "qubits": n, "precision": precision, "ops": stats["ops_in"],
            "passes": stats["passes"], "pass_kernels": "compiled" if compiled else "generic",
+           "kernels_run": plan.kernels_run(), "dense_blocks": stats["dense_blocks"],
            "ms_per_circuit": round(ms, 3), "ms_per_pass": round(per_pass, 4),
            "GBs_per_pass": round(2 * (1 << n) * amp / (per_pass * 1e-3) / 1e9, 1),
            "norm_after": st.norm_sqr(), "steps": steps, "warmup": warmup}
+    if stream_probe:
+        # the device's one-read-one-write rate over this state's bytes
+        res["stream_kernels_GBs"] = {name: round(st.stream_rate(v, 3)[0], 1)
+                                     for v, name in ((1, "nontemporal"), (2, "four_words"), (4, "eight_words"))}
     del st
     return res
+
+
+MFMA_PEAK_TFLOPS = {"complex128": 78.6, "complex64": 157.3}  # dense FP64 / FP32 matrix peaks
 
 
 def qft_like_bench(be, torch, steps: int) -> list:
     """33 qubits complex128 and 34 qubits complex64 (128 GiB each): 2 layers of
     H on every qubit, exact controlled phases CP(pi/2^d) at distances d <= 8,
-    RZ(2 pi u) on every qubit (SmallRng seed 33)."""
+    RZ(2 pi u) on every qubit (SmallRng seed 33).  Each circuit runs through
+    the compiled tile passes (the default) and, for comparison, through dense
+    5-qubit blocks on the matrix cores (plan kernel = 4), whose MFMA
+    utilisation is the block products' flops (8 x 2^5 per amplitude and pass)
+    over the pass time and the dense matrix peak of the precision."""
     from quantumforge_amd.workloads import qft_like_circuit
 
     out = []
     for n, prec in ((33, "complex128"), (34, "complex64")):
         _, native = qft_like_circuit(n, 2, seed=33)
         r = circuit_section(be, torch, native, n, prec, steps, 1,
-                            f"QFT-like, {n} qubits, 2 layers, exact controlled phases, {prec}")
+                            f"QFT-like, {n} qubits, 2 layers, exact controlled phases, {prec}", stream_probe=True)
         r["layers_per_s"] = round(2 / (r["ms_per_circuit"] * 1e-3), 3)
         log(f"qft-like {n}q {prec}: {r['ms_per_circuit']:.1f} ms/circuit, {r['passes']} passes, "
             f"{r['GBs_per_pass']:.0f} GB/s per pass")
+        d = circuit_section(be, torch, native, n, prec, steps, 1,
+                            f"QFT-like, {n} qubits, 2 layers, dense 5-qubit blocks on the matrix cores, {prec}",
+                            kernel=4, max_block=5)
+        flops = d["passes"] * 8 * 32 * float(1 << n)
+        tflops = flops / (d["ms_per_circuit"] * 1e-3) / 1e12
+        d["layers_per_s"] = round(2 / (d["ms_per_circuit"] * 1e-3), 3)
+        d["mfma"] = {"block_qubits": 5, "tflops": round(tflops, 2), "peak_tflops": MFMA_PEAK_TFLOPS[prec],
+                     "utilisation": round(tflops / MFMA_PEAK_TFLOPS[prec], 4)}
+        log(f"qft-like {n}q {prec} dense blocks: {d['ms_per_circuit']:.1f} ms/circuit, {d['passes']} passes, "
+            f"{d['GBs_per_pass']:.0f} GB/s per pass, MFMA {tflops:.1f} TFLOP/s")
+        r["dense_blocks_ab"] = d
+        r["faster"] = "compiled tile passes" if r["ms_per_circuit"] <= d["ms_per_circuit"] else "dense MFMA blocks"
         out.append(r)
     return out
 
@@ -652,10 +677,11 @@ def main() -> int:
     norm = state.norm_sqr()
     log(f"norm after {args.warmup + args.steps} runs: {norm:.15f}")
     # The one-read-one-write ceiling of this device over the state's bytes:
-    # hand-written streaming kernels (16-byte words; nontemporal; 64 bytes per
-    # lane; one contiguous 64 KiB chunk per workgroup), same stream, same bytes.
+    # hand-written streaming kernels (16-byte words; nontemporal; 64 and 128
+    # bytes in flight per lane; one contiguous 64 KiB chunk per workgroup),
+    # same stream, same bytes.
     stream_hip = {}
-    for v, name in ((0, "word"), (1, "nontemporal"), (2, "four_words"), (3, "chunk_64KiB")):
+    for v, name in ((0, "word"), (1, "nontemporal"), (2, "four_words"), (3, "chunk_64KiB"), (4, "eight_words")):
         stream_hip[name] = round(state.stream_rate(v, 5)[0], 1)
     log(f"in-place streaming kernels over the state: {stream_hip} GB/s")
     del state

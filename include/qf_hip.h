@@ -163,9 +163,12 @@ typedef struct {
     int32_t fuse;          /* 0: one sweep per gate (reference order, bit-exact arithmetic); 1: tile passes */
     uint32_t tile_qubits;  /* qubits held per workgroup tile (0 = default) */
     uint32_t low_qubits;   /* contiguous low qubits always in a tile (0 = default) */
-    uint32_t max_block;    /* widest dense fused block k (2..5; 0 = default) */
+    uint32_t max_block;    /* kernel 4: qubits of a dense block, 4 or 5 (0 = 5) */
     int32_t kernel;        /* 0 default, 1 LDS op-by-op tile kernel, 2 register-resident tile kernel,
-                              3 compiled passes (one generated kernel per pass; default from 18 qubits) */
+                              3 compiled passes (one generated kernel per pass; default from 18 qubits),
+                              4 dense max_block-qubit blocks on the matrix cores (one block matrix per
+                              pass, composed in FP64 on the host; fixed gates, at least max_block + 4
+                              qubits) */
     uint32_t reg_bits;     /* register kernel: 2^reg_bits amplitudes per thread (0 = default) */
     int32_t precision;     /* QF_C128 or QF_C64: the amplitude type the defaults are chosen for */
 } qf_plan_options;
@@ -184,7 +187,7 @@ typedef struct {
     uint64_t passes;       /* full-state sweeps (kernel launches that stream the state once) */
     uint64_t ops_in;       /* operations in the circuit */
     uint64_t ops_applied;  /* operations after 1q merging */
-    uint64_t dense_blocks; /* fused dense k-qubit blocks */
+    uint64_t dense_blocks; /* fused dense k-qubit blocks (kernel 4: one per pass) */
     uint64_t max_ops_per_pass;
     uint64_t phases;       /* register-kernel phases (LDS re-distributions + 1 per pass) */
     uint32_t kernels_run;  /* QF_KERNEL_* mask of what the last run of this plan launched (0: not run) */
@@ -284,7 +287,8 @@ qf_status qf_state_gather(const qf_state* state, const uint64_t* indices, size_t
  * unchanged, `launches` timed launches after one warm launch, rate =
  * 2 x state bytes / average launch time.  variant 0: one 16-byte word per lane
  * per grid-stride step, 1: nontemporal loads and stores, 2: four words per lane
- * per step, 3: one contiguous 64 KiB chunk per workgroup. */
+ * per step, 3: one contiguous 64 KiB chunk per workgroup, 4: eight words per
+ * lane per step. */
 qf_status qf_state_stream_rate(qf_state* state, int32_t variant, uint32_t launches, double* gbs, double* avg_ms);
 qf_status qf_state_norm_sqr(const qf_state* state, double* out);
 /* StateVector::probabilities, src/lib.rs:1144-1146. */

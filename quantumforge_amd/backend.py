@@ -29,11 +29,14 @@ class Plan:
     """Fused execution plan of a circuit (qf_plan_build)."""
 
     def __init__(self, circuit: Circuit, qubits: int, fuse: bool = True, tile_qubits: int = 0,
-                 low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0, precision="complex128") -> None:
+                 low_qubits: int = 0, kernel: int = 0, reg_bits: int = 0, precision="complex128",
+                 max_block: int = 0) -> None:
         """precision picks the default tile shape (complex64 tiles hold twice
-        the amplitudes in the same LDS); the plan runs on either precision."""
+        the amplitudes in the same LDS); the plan runs on either precision.
+        kernel=4 plans dense max_block-qubit blocks (4 or 5, default 5) for the
+        matrix cores."""
         self._ops = circuit.to_ops()
-        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, 0, kernel, reg_bits, _precision(precision))
+        opts = N.PlanOptions(int(fuse), tile_qubits, low_qubits, max_block, kernel, reg_bits, _precision(precision))
         h = C.c_void_p()
         check(lib().qf_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
         self._h = h

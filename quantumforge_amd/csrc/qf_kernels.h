@@ -171,9 +171,10 @@ __global__ __launch_bounds__(kThreads) void k_fill_zero_state(cx<R>* __restrict_
 // double2): every word is read once and written back multiplied by `scale`,
 // the memory pattern of a tile pass without its arithmetic.  Variant 0: one
 // word per thread per grid-stride step; 1: the same with nontemporal loads and
-// stores; 2: four consecutive words per thread per step (64 bytes in flight
-// per lane); 3: each workgroup streams one contiguous chunk of 2^chunk_log
-// words (the grid covers the buffer once).
+// stores; 2: four words per thread per step (64 bytes in flight per lane);
+// 3: each workgroup streams one contiguous chunk of 2^chunk_log words (the
+// grid covers the buffer once); 4: eight words per thread per step (128 bytes
+// in flight per lane, the load depth of a 5-qubit dense-block pass).
 template <int V>
 __global__ __launch_bounds__(kThreads) void k_stream_probe(double2* __restrict__ s, uint64_t n, double scale,
                                                            uint32_t chunk_log) {
@@ -185,15 +186,16 @@ __global__ __launch_bounds__(kThreads) void k_stream_probe(double2* __restrict__
             v.y *= scale;
             s[i] = v;
         }
-    } else if constexpr (V == 2) {
-        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x * 4;
-        for (uint64_t i = (uint64_t(blockIdx.x) * blockDim.x) * 4 + threadIdx.x; i < n; i += stride) {
-            double2 v[4];
+    } else if constexpr (V == 2 || V == 4) {
+        constexpr int W = V == 2 ? 4 : 8;
+        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x * W;
+        for (uint64_t i = (uint64_t(blockIdx.x) * blockDim.x) * W + threadIdx.x; i < n; i += stride) {
+            double2 v[W];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < W; ++k)
                 if (i + uint64_t(k) * blockDim.x < n) v[k] = s[i + uint64_t(k) * blockDim.x];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < W; ++k)
                 if (i + uint64_t(k) * blockDim.x < n) {
                     v[k].x *= scale;
                     v[k].y *= scale;

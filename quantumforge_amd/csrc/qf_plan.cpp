@@ -641,6 +641,23 @@ std::string describe_plan(const Plan& plan) {
                 s += "}";
             }
             s += "]}";
+        } else if (p.kind == Pass::DENSE) {
+            s += "{\"kind\":\"dense\",\"block\":[";
+            for (size_t k = 0; k < p.block.size(); ++k) s += (k ? "," : "") + std::to_string(p.block[k]);
+            s += "],\"U\":";
+            cplx(&p.U[0].re, int(p.U.size()));
+            for (int list = 0; list < 2; ++list) {
+                const auto& L = list ? p.pre_blk : p.pre_col;
+                s += list ? ",\"pre_blk\":[" : ",\"pre_col\":[";
+                for (size_t k = 0; k < L.size(); ++k) {
+                    if (k) s += ",";
+                    s += "{\"a\":" + std::to_string(L[k].a) + ",\"b\":" + std::to_string(L[k].b) + ",\"d\":";
+                    cplx(&L[k].d[0].re, 4);
+                    s += "}";
+                }
+                s += "]";
+            }
+            s += "}";
         } else {
             static const char* names[] = {"sweep_1q", "sweep_2q", "tile", "sweep_diag"};
             s += std::string("{\"kind\":\"") + names[p.kind] + "\",\"op2\":" + std::to_string(p.op2) +
@@ -653,6 +670,7 @@ std::string describe_plan(const Plan& plan) {
          ",\"ops_in\":" + std::to_string(plan.stats.ops_in) +
          ",\"ops_applied\":" + std::to_string(plan.stats.ops_applied) +
          ",\"max_ops_per_pass\":" + std::to_string(plan.stats.max_ops_per_pass) +
+         ",\"dense_blocks\":" + std::to_string(plan.stats.dense_blocks) +
          ",\"phases\":" + std::to_string(plan.stats.phases) + "}}";
     return s;
 }
@@ -885,6 +903,246 @@ std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int prec
 std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubits, int precision,
                                 const PlanOptions& opt, Plan& out);
 
+// ---------------------------------------------------------------------------
+// Dense-block plans (kernel = 4): each pass applies one k-qubit block matrix
+// (2^k x 2^k, composed in FP64 on the host) on the matrix cores, after the
+// diagonal gates that are ready at the start of the pass (elementwise
+// factors).  Greedy over the gate DAG: a gate waits for the earlier gates on
+// its qubits that do not commute with it (diagonal gates commute with each
+// other); a pass first takes every ready diagonal gate, then starts its block
+// at the first ready non-diagonal gate and takes, in circuit order, every
+// further ready gate whose qubits fit in k.
+// ---------------------------------------------------------------------------
+inline cd c_mul(cd a, cd b) { return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline cd c_add(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+
+bool gate_is_diag(const GateIn& g) {
+    if (g.kind == 2 || g.kind == 4) return true;
+    return g.kind == 0 && g.m[1].re == 0.0 && g.m[1].im == 0.0 && g.m[2].re == 0.0 && g.m[2].im == 0.0;
+}
+
+// Diagonal of a diagonal gate as d[bit(b) * 2 + bit(a)] (a == b for one qubit).
+void gate_diag(const GateIn& g, uint32_t* a, uint32_t* b, cd d[4]) {
+    *a = g.a;
+    if (g.kind == 0) {
+        *b = g.a;
+        d[0] = g.m[0];
+        d[1] = d[2] = cd{0, 0};
+        d[3] = g.m[3];
+    } else if (g.kind == 2) {
+        *b = g.b;
+        d[0] = d[1] = d[2] = cd{1, 0};
+        d[3] = cd{-1, 0};
+    } else {
+        *b = g.b;
+        for (int k = 0; k < 4; ++k) d[k] = g.m[k];
+    }
+}
+
+struct DenseBlock {
+    std::vector<uint32_t> q;  // bit i of a block index = qubit q[i]
+    std::vector<cd> U{cd{1, 0}};
+    int pos(uint32_t qubit) const {
+        for (size_t i = 0; i < q.size(); ++i)
+            if (q[i] == qubit) return int(i);
+        return -1;
+    }
+    void add_qubit(uint32_t qb) {
+        const size_t d = size_t(1) << q.size(), d2 = 2 * d;
+        std::vector<cd> V(d2 * d2, cd{0, 0});
+        for (size_t r = 0; r < d2; ++r)
+            for (size_t c = 0; c < d2; ++c)
+                if ((r >> q.size()) == (c >> q.size())) V[r * d2 + c] = U[(r & (d - 1)) * d + (c & (d - 1))];
+        U.swap(V);
+        q.push_back(qb);
+    }
+    // U <- G U for a gate on block positions.
+    void apply(const GateIn& g) {
+        const size_t d = size_t(1) << q.size();
+        const int pa = pos(g.a), pb = g.kind == 0 ? -1 : pos(g.b);
+        const size_t ma = size_t(1) << pa, mb = pb >= 0 ? size_t(1) << pb : 0;
+        std::vector<cd> v(d);
+        for (size_t c = 0; c < d; ++c) {
+            for (size_t r = 0; r < d; ++r) v[r] = U[r * d + c];
+            switch (g.kind) {
+                case 0:
+                    for (size_t i = 0; i < d; ++i)
+                        if (!(i & ma)) {
+                            const cd x0 = v[i], x1 = v[i | ma];
+                            v[i] = c_add(c_mul(g.m[0], x0), c_mul(g.m[1], x1));
+                            v[i | ma] = c_add(c_mul(g.m[2], x0), c_mul(g.m[3], x1));
+                        }
+                    break;
+                case 1:  // CNOT a -> b
+                    for (size_t i = 0; i < d; ++i)
+                        if ((i & ma) && !(i & mb)) std::swap(v[i], v[i | mb]);
+                    break;
+                case 2:
+                    for (size_t i = 0; i < d; ++i)
+                        if ((i & ma) && (i & mb)) v[i] = cd{-v[i].re, -v[i].im};
+                    break;
+                case 3:
+                    for (size_t i = 0; i < d; ++i)
+                        if ((i & ma) && !(i & mb)) std::swap(v[i], v[(i ^ ma) | mb]);
+                    break;
+                case 4:
+                    for (size_t i = 0; i < d; ++i) v[i] = c_mul(g.m[((i & mb) ? 2 : 0) + ((i & ma) ? 1 : 0)], v[i]);
+                    break;
+                default: {  // 6: dense 4x4, basis index bit(b) * 2 + bit(a)
+                    for (size_t i = 0; i < d; ++i)
+                        if (!(i & ma) && !(i & mb)) {
+                            const size_t idx[4] = {i, i | ma, i | mb, i | ma | mb};
+                            cd x[4], y[4];
+                            for (int k = 0; k < 4; ++k) x[k] = v[idx[k]];
+                            for (int r = 0; r < 4; ++r) {
+                                y[r] = cd{0, 0};
+                                for (int k = 0; k < 4; ++k) y[r] = c_add(y[r], c_mul(g.m4[r * 4 + k], x[k]));
+                            }
+                            for (int k = 0; k < 4; ++k) v[idx[k]] = y[k];
+                        }
+                }
+            }
+            for (size_t r = 0; r < d; ++r) U[r * d + c] = v[r];
+        }
+    }
+    // U <- U D for a diagonal D over block positions pa, pb (d[bit(pb) * 2 + bit(pa)]).
+    void right_diag(int pa, int pb, const cd d[4]) {
+        const size_t n = size_t(1) << q.size();
+        for (size_t c = 0; c < n; ++c) {
+            const cd f = d[(((c >> pb) & 1) << 1) | ((c >> pa) & 1)];
+            for (size_t r = 0; r < n; ++r) U[r * n + c] = c_mul(U[r * n + c], f);
+        }
+    }
+};
+
+std::string plan_dense(const std::vector<GateIn>& gates, uint32_t qubits, uint32_t k, Plan& out) {
+    if (k < 4 || k > 5) return "dense-block plans take max_block 4 or 5";
+    if (qubits < k + 4) return "dense-block passes need at least max_block + 4 qubits";
+    const size_t G = gates.size();
+    std::vector<char> diag(G);
+    for (size_t i = 0; i < G; ++i) {
+        if (gates[i].kind == 5) return "dense-block plans take fixed gates only (no run-time rotations)";
+        diag[i] = gate_is_diag(gates[i]) ? 1 : 0;
+    }
+    // Direct predecessors: per qubit, a non-diagonal gate waits for the run of
+    // gates back to the last non-diagonal one; a diagonal gate for the last
+    // non-diagonal gate only.
+    std::vector<std::vector<size_t>> on(qubits), succ(G);
+    std::vector<uint32_t> npred(G, 0);
+    for (size_t i = 0; i < G; ++i) {
+        const GateIn& g = gates[i];
+        const uint32_t qs[2] = {g.a, g.b};
+        const int nq = g.kind == 0 ? 1 : 2;
+        std::vector<size_t> pred;
+        for (int t = 0; t < nq; ++t) {
+            const auto& L = on[qs[t]];
+            for (size_t j = L.size(); j-- > 0;) {
+                const size_t h = L[j];
+                if (!diag[i] || !diag[h]) pred.push_back(h);
+                if (!diag[h]) break;
+            }
+        }
+        std::sort(pred.begin(), pred.end());
+        pred.erase(std::unique(pred.begin(), pred.end()), pred.end());
+        for (size_t h : pred) succ[h].push_back(i);
+        npred[i] = uint32_t(pred.size());
+        for (int t = 0; t < nq; ++t) on[qs[t]].push_back(i);
+    }
+    std::vector<char> done(G, 0);
+    size_t left = G;
+    auto finish = [&](size_t i) {
+        done[i] = 1;
+        --left;
+        for (size_t s2 : succ[i]) --npred[s2];
+    };
+    while (left) {
+        Pass p{};
+        p.kind = Pass::DENSE;
+        std::vector<size_t> pre;
+        for (size_t i = 0; i < G; ++i)
+            if (!done[i] && diag[i] && npred[i] == 0) {
+                pre.push_back(i);
+                finish(i);
+            }
+        DenseBlock B;
+        size_t ops = pre.size();
+        for (size_t i = 0; i < G; ++i) {
+            if (done[i] || npred[i] != 0) continue;
+            const GateIn& g = gates[i];
+            const uint32_t qs[2] = {g.a, g.b};
+            const int nq = g.kind == 0 ? 1 : 2;
+            if (B.q.empty() && diag[i]) continue;  // a block starts at a non-diagonal gate
+            int fresh = 0;
+            for (int t = 0; t < nq; ++t) fresh += B.pos(qs[t]) < 0 ? 1 : 0;
+            if (B.q.size() + fresh > k) continue;
+            for (int t = 0; t < nq; ++t)
+                if (B.pos(qs[t]) < 0) B.add_qubit(qs[t]);
+            B.apply(g);
+            finish(i);
+            ++ops;
+        }
+        // Pad the block to k qubits with qubits it does not touch, highest first
+        // (the low index bits stay column bits: contiguous columns).
+        for (uint32_t qb = qubits; qb-- > 0 && B.q.size() < k;)
+            if (B.pos(qb) < 0) B.add_qubit(qb);
+        // Diagonal gates ready at the start of the pass: inside the block into
+        // U (applied before it), otherwise per column or per block bit.
+        for (size_t i : pre) {
+            uint32_t a, b;
+            cd d[4];
+            gate_diag(gates[i], &a, &b, d);
+            const int pa = B.pos(a), pb = B.pos(b);
+            if (pa >= 0 && pb >= 0) {
+                B.right_diag(pa, pb, d);
+            } else if (pa >= 0 || pb >= 0) {
+                DiagPre e{};
+                if (pa >= 0) {
+                    e.a = uint32_t(pa);
+                    e.b = b;
+                    for (int t = 0; t < 4; ++t) e.d[t] = d[t];
+                } else {  // swap the roles: block position first
+                    e.a = uint32_t(pb);
+                    e.b = a;
+                    e.d[0] = d[0];
+                    e.d[1] = d[2];
+                    e.d[2] = d[1];
+                    e.d[3] = d[3];
+                }
+                p.pre_blk.push_back(e);
+            } else {
+                DiagPre e{a, b, {d[0], d[1], d[2], d[3]}};
+                p.pre_col.push_back(e);
+            }
+        }
+        // Block qubits in ascending order.
+        std::vector<uint32_t> order(B.q.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = uint32_t(i);
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return B.q[x] < B.q[y]; });
+        std::vector<uint32_t> newpos(B.q.size());
+        for (size_t i = 0; i < order.size(); ++i) {
+            p.block.push_back(B.q[order[i]]);
+            newpos[order[i]] = uint32_t(i);
+        }
+        const size_t d = size_t(1) << k;
+        auto remap = [&](size_t x) {
+            size_t y = 0;
+            for (size_t i = 0; i < k; ++i)
+                if ((x >> i) & 1) y |= size_t(1) << newpos[i];
+            return y;
+        };
+        p.U.assign(d * d, cd{0, 0});
+        for (size_t r = 0; r < d; ++r)
+            for (size_t c = 0; c < d; ++c) p.U[remap(r) * d + remap(c)] = B.U[r * d + c];
+        for (DiagPre& e : p.pre_blk) e.a = newpos[e.a];
+        out.stats.max_ops_per_pass = std::max<uint64_t>(out.stats.max_ops_per_pass, ops);
+        out.passes.push_back(std::move(p));
+        ++out.stats.dense_blocks;
+    }
+    out.stats.passes = out.passes.size();
+    out.stats.ops_applied = G;
+    return "";
+}
+
 }  // namespace
 
 std::string circuit_key(const std::vector<GateIn>& gates, uint32_t qubits) {
@@ -934,6 +1192,7 @@ std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubit
         if (g.a >= qubits || (two && g.b >= qubits)) return "qubit out of range";
         if (two && g.a == g.b) return "two-qubit operation requires distinct qubits";
     }
+    if (opt.fuse && opt.kernel == 4) return plan_dense(gates, qubits, opt.max_block ? opt.max_block : 5, out);
     if (!opt.fuse) {
         for (const GateIn& g : gates)
             if (g.kind == 5 || g.kind == 6) return "per-instance rotations and dense two-qubit ops need the tile lowering";

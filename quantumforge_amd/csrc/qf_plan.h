@@ -72,9 +72,18 @@ struct TileOp {
     double m[32];     // MAT1: 4 complex; DIAG: 4 complex (index g1bit*2+g0bit); MAT2: 16 complex row major
 };
 
+// A diagonal gate applied before a dense block (DENSE passes): on the index
+// bits a and b (a == b for a one-qubit diagonal) with factor d[bit(b) * 2 +
+// bit(a)].  In `col` lists both qubits lie outside the block (one factor per
+// column); in `blk` lists a is a position in the block and b a qubit outside it.
+struct DiagPre {
+    uint32_t a, b;
+    cd d[4];
+};
+
 // One full-state pass.
 struct Pass {
-    enum Kind : int32_t { SWEEP_1Q = 0, SWEEP_2Q = 1, TILE = 2, SWEEP_DIAG = 3 } kind;
+    enum Kind : int32_t { SWEEP_1Q = 0, SWEEP_2Q = 1, TILE = 2, SWEEP_DIAG = 3, DENSE = 4 } kind;
     // SWEEP_1Q: q0 target, m; SWEEP_2Q: op2 (0 CNOT, 1 CZ, 2 SWAP), q0, q1; SWEEP_DIAG: q0, q1, m as diag
     int32_t op2 = 0;
     uint32_t q0 = 0, q1 = 0;
@@ -89,6 +98,13 @@ struct Pass {
     // the bit it was loaded from).  A permutation of the tile's bits; the
     // qubits stored to bits 0..low-1 are the next pass's contiguous ones.
     std::vector<uint32_t> store_phys;
+    // DENSE: a fused k-qubit block (matrix-core pass).  block: its qubits in
+    // ascending order (bit i of a block index is qubit block[i]); U: the
+    // 2^k x 2^k block matrix, row major, composed in FP64 on the host;
+    // pre_col / pre_blk: diagonal gates applied before U (see DiagPre).
+    std::vector<uint32_t> block;
+    std::vector<cd> U;
+    std::vector<DiagPre> pre_col, pre_blk;
 };
 
 struct PlanOptions {
@@ -97,7 +113,8 @@ struct PlanOptions {
     uint32_t low_qubits = 0;   // 0 = default
     uint32_t max_block = 0;
     int kernel = 0;            // 0 = default, 1 = LDS op-by-op tile kernel, 2 = register kernel,
-                               // 3 = compiled passes (register-kernel plan, one generated kernel per pass)
+                               // 3 = compiled passes (register-kernel plan, one generated kernel per pass),
+                               // 4 = dense k-qubit blocks on the matrix cores (k = max_block, 4 or 5)
     uint32_t reg_bits = 0;     // register kernel: amplitudes per thread = 2^reg_bits (0 = default)
     // Compiled passes may store qubits to other index bits than they were
     // loaded from (the plan's last pass restores the layout); such a plan

@@ -32,6 +32,8 @@ struct qf_backend {
     size_t d_out_count = 0;
     void* d_ops = nullptr;
     size_t d_ops_bytes = 0;
+    void* d_dense = nullptr;
+    size_t d_dense_bytes = 0;
     // Per-instance measurement outcomes / masks and collapse scales.
     void* meas_a = nullptr;
     size_t meas_a_bytes = 0;
@@ -78,6 +80,8 @@ struct LaunchCtx {
     uint32_t nslots = 0;
     void* lam = nullptr;
     double* partial = nullptr;
+    // Dense-block passes: the plan's uploaded operand blob (dense_blob).
+    const unsigned char* dense = nullptr;
 };
 
 qf_status fail(qf_status st, const std::string& msg);

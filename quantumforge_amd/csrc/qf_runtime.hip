@@ -3,6 +3,7 @@
 // 64-bit indexing throughout, validation on the host before any launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "../../include/qf_hip.h"
+#include "qf_dense.h"
 #include "qf_kernels.h"
 #include "qf_regtile.h"
 #include "qf_jit.h"
@@ -158,6 +160,7 @@ void backend_release(qf_backend* be) {
         if (be->scratch) (void)hipFree(be->scratch);
         if (be->d_out) (void)hipFree(be->d_out);
         if (be->d_ops) (void)hipFree(be->d_ops);
+        if (be->d_dense) (void)hipFree(be->d_dense);
         if (be->meas_a) (void)hipFree(be->meas_a);
         if (be->meas_b) (void)hipFree(be->meas_b);
         (void)hipStreamDestroy(be->stream);
@@ -297,6 +300,7 @@ qf_status launch_pass(qf_backend* be, cx<R>* d, uint32_t qubits, uint32_t count,
     const uint64_t total = dim * count;
     hipStream_t st = be->stream;
     switch (p.kind) {
+        case qf::Pass::DENSE: return fail(QF_INVALID_ARGUMENT, "dense pass without uploaded operands");
         case qf::Pass::SWEEP_1Q:
             hipLaunchKernelGGL(qf::k_sweep_1q<R>, dim3(grid_for(total / 2)), dim3(qf::kThreads), 0, st, d, total / 2,
                                p.q0, p.m[0], p.m[1], p.m[2], p.m[3]);
@@ -456,6 +460,109 @@ qf_status launch_compiled(qf_backend* be, void* d, uint32_t qubits, uint32_t cou
     return QF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Dense-block passes: per pass, the A operand of the real embedding of U in
+// lane order ([tile][k-step][lane], the amplitude precision) and the two
+// diagonal lists, each section 256-byte aligned.
+// ---------------------------------------------------------------------------
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+size_t dense_pass_bytes(const qf::Pass& p, int precision) {
+    const size_t k = p.block.size(), nt = size_t(1) << (k - 3), nks = size_t(1) << (k - 1);
+    const size_t rb = precision == QF_C128 ? 8 : 4;
+    return align256(nt * nks * 64 * rb) + align256(p.pre_col.size() * sizeof(qf::DenseDiag)) +
+           align256(p.pre_blk.size() * sizeof(qf::DenseDiag));
+}
+
+template <class R>
+void fill_dense_pass(const qf::Pass& p, unsigned char* out) {
+    const size_t k = p.block.size(), D = size_t(1) << k, nt = size_t(1) << (k - 3), nks = size_t(1) << (k - 1);
+    R* aop = reinterpret_cast<R*>(out);
+    for (size_t t = 0; t < nt; ++t)
+        for (size_t ks = 0; ks < nks; ++ks)
+            for (size_t l = 0; l < 64; ++l) {
+                const size_t i = 16 * t + (l % 16), kk = 4 * ks + l / 16;
+                const size_t row = i % D, colx = kk % D;
+                const qf::cd u = p.U[row * D + colx];
+                double v;
+                if (i < D) v = kk < D ? u.re : -u.im;  // [Ur -Ui]
+                else v = kk < D ? u.im : u.re;          // [Ui  Ur]
+                aop[(t * nks + ks) * 64 + l] = R(v);
+            }
+    unsigned char* q = out + align256(nt * nks * 64 * sizeof(R));
+    for (int list = 0; list < 2; ++list) {
+        const auto& L = list ? p.pre_blk : p.pre_col;
+        qf::DenseDiag* d = reinterpret_cast<qf::DenseDiag*>(q);
+        for (size_t e = 0; e < L.size(); ++e) {
+            d[e] = qf::DenseDiag{};
+            d[e].a = L[e].a;
+            d[e].b = L[e].b;
+            for (int c = 0; c < 4; ++c) {
+                d[e].d[2 * c] = L[e].d[c].re;
+                d[e].d[2 * c + 1] = L[e].d[c].im;
+            }
+        }
+        q += align256(L.size() * sizeof(qf::DenseDiag));
+    }
+}
+
+bool plan_has_dense(const qf::Plan& plan) {
+    for (const auto& p : plan.passes)
+        if (p.kind == qf::Pass::DENSE) return true;
+    return false;
+}
+
+// Uploads the dense passes' operands of a plan into the backend's buffer.
+qf_status upload_dense(qf_backend* be, const qf::Plan& plan, int precision, const unsigned char** out) {
+    *out = nullptr;
+    size_t total = 0;
+    for (const auto& p : plan.passes)
+        if (p.kind == qf::Pass::DENSE) total += dense_pass_bytes(p, precision);
+    if (!total) return QF_OK;
+    std::vector<unsigned char> host(total, 0);
+    size_t off = 0;
+    for (const auto& p : plan.passes)
+        if (p.kind == qf::Pass::DENSE) {
+            if (precision == QF_C128) fill_dense_pass<double>(p, host.data() + off);
+            else fill_dense_pass<float>(p, host.data() + off);
+            off += dense_pass_bytes(p, precision);
+        }
+    HIP_TRY(hipStreamSynchronize(be->stream));  // earlier launches may still read the buffer
+    qf_status s = ensure(be, &be->d_dense, &be->d_dense_bytes, total);
+    if (s) return s;
+    HIP_TRY(hipMemcpy(be->d_dense, host.data(), total, hipMemcpyHostToDevice));
+    *out = static_cast<const unsigned char*>(be->d_dense);
+    return QF_OK;
+}
+
+template <class R>
+qf_status launch_dense(qf_backend* be, void* d, uint32_t qubits, uint32_t count, const qf::Pass& p,
+                       const unsigned char* blob) {
+    const uint32_t k = uint32_t(p.block.size());
+    if ((k != 4 && k != 5) || qubits < k + 4) return fail(QF_INVALID_ARGUMENT, "dense pass shape");
+    qf::DenseArgs a{};
+    a.n = qubits;
+    a.ncol = uint32_t(p.pre_col.size());
+    a.nblk = uint32_t(p.pre_blk.size());
+    for (uint32_t i = 0; i < k; ++i) a.S[i] = p.block[i];
+    a.groups = uint64_t(1) << (qubits - k - 4);
+    a.states = count;
+    const size_t nt = size_t(1) << (k - 3), nks = size_t(1) << (k - 1);
+    const R* aop = reinterpret_cast<const R*>(blob);
+    const qf::DenseDiag* dcol = reinterpret_cast<const qf::DenseDiag*>(blob + align256(nt * nks * 64 * sizeof(R)));
+    const qf::DenseDiag* dblk = reinterpret_cast<const qf::DenseDiag*>(
+        blob + align256(nt * nks * 64 * sizeof(R)) + align256(p.pre_col.size() * sizeof(qf::DenseDiag)));
+    const uint64_t waves = a.groups * a.states;
+    const unsigned blocks = unsigned(std::min<uint64_t>((waves + 3) / 4, 65536));
+    if (k == 5)
+        hipLaunchKernelGGL((qf::k_dense_block<R, 5>), dim3(blocks), dim3(256), 0, be->stream, static_cast<cx<R>*>(d), a,
+                           aop, dcol, dblk);
+    else
+        hipLaunchKernelGGL((qf::k_dense_block<R, 4>), dim3(blocks), dim3(256), 0, be->stream, static_cast<cx<R>*>(d), a,
+                           aop, dcol, dblk);
+    return launch_check("k_dense_block");
+}
+
 // Launches every pass of a plan whose ops were uploaded at d_ops (d_ops may
 // be null when every tile pass runs compiled).
 qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan,
@@ -473,11 +580,18 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
         return fail(QF_INVALID_ARGUMENT, "a plan that relabels qubits between passes runs only as compiled kernels");
     const qf::TileOp* cur = d_ops;
     uint32_t kinds = 0;
+    size_t dense_off = 0;
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const qf::Pass& p = plan.passes[i];
         qf_status s;
         const int kern = prog ? prog->kernel_of_pass[i] : -1;
-        if (kern >= 0) {
+        if (p.kind == qf::Pass::DENSE) {
+            if (!ctx.dense || ctx.lam) return fail(QF_INVALID_ARGUMENT, "dense pass without uploaded operands");
+            kinds |= qf::KR_MFMA;
+            s = precision == QF_C128 ? launch_dense<double>(be, d, qubits, count, p, ctx.dense + dense_off)
+                                     : launch_dense<float>(be, d, qubits, count, p, ctx.dense + dense_off);
+            dense_off += dense_pass_bytes(p, precision);
+        } else if (kern >= 0) {
             kinds |= qf::KR_COMPILED;
             s = launch_compiled(be, d, qubits, count, p, (*fns)[kern], prog->threads[kern], ns, ctx);
         } else {
@@ -495,6 +609,12 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
 
 // Runs a plan on `count` contiguous states of `qubits` qubits.
 qf_status run_plan(qf_backend* be, void* d, int precision, uint32_t qubits, uint32_t count, const qf::Plan& plan) {
+    if (plan_has_dense(plan)) {
+        LaunchCtx ctx;
+        qf_status s = upload_dense(be, plan, precision, &ctx.dense);
+        if (s) return s;
+        return launch_plan(be, d, precision, qubits, count, plan, nullptr, ctx);
+    }
     if (plan.compiled) {
         const qf::JitProgram* prog = nullptr;
         qf_status s = plan_program(plan, precision, 1, &prog);
@@ -1035,7 +1155,7 @@ extern "C" qf_status qf_state_gather(const qf_state* st, const uint64_t* indices
 extern "C" qf_status qf_state_stream_rate(qf_state* st, int32_t variant, uint32_t launches, double* gbs,
                                           double* avg_ms) {
     if (!st || !gbs) return fail(QF_INVALID_ARGUMENT, "null argument");
-    if (variant < 0 || variant > 3) return fail(QF_INVALID_ARGUMENT, "unknown streaming variant");
+    if (variant < 0 || variant > 4) return fail(QF_INVALID_ARGUMENT, "unknown streaming variant");
     if (launches == 0) launches = 1;
     qf_backend* be = st->be;
     qf_status s = activate(be);
@@ -1057,6 +1177,10 @@ extern "C" qf_status qf_state_stream_rate(qf_state* st, int32_t variant, uint32_
                 break;
             case 2:
                 hipLaunchKernelGGL(qf::k_stream_probe<2>, dim3(grid_for((words + 3) / 4)), dim3(qf::kThreads), 0,
+                                   be->stream, p, words, 1.0, chunk_log);
+                break;
+            case 4:
+                hipLaunchKernelGGL(qf::k_stream_probe<4>, dim3(grid_for((words + 7) / 8)), dim3(qf::kThreads), 0,
                                    be->stream, p, words, 1.0, chunk_log);
                 break;
             default:

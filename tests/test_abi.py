@@ -274,3 +274,76 @@ def test_30q_plan_is_fused():
     assert st["passes"] <= 60, st
     exact = Plan(c, 30, False).stats()
     assert exact["passes"] == 1600
+
+
+# ---------------------------------------------------------------------------
+# Dense-block plans (kernel = 4): replayed from their description
+# ---------------------------------------------------------------------------
+def replay_dense(desc, psi):
+    """Applies a described dense-block plan: per pass the column and block-bit
+    diagonal factors, then the block matrix U on the block's qubits (bit i of a
+    block index is qubit block[i])."""
+    n = desc["qubits"]
+    idx = np.arange(1 << n, dtype=np.int64)
+    psi = psi.astype(complex).copy()
+    for p in desc["passes"]:
+        assert p["kind"] == "dense"
+        blk = p["block"]
+        k = len(blk)
+        assert blk == sorted(blk) and len(set(blk)) == k
+        for e in p["pre_col"]:
+            assert e["a"] not in blk and e["b"] not in blk
+            psi = _diag(psi, idx, e["a"], e["b"], e["d"])
+        for e in p["pre_blk"]:
+            assert e["b"] not in blk
+            psi = _diag(psi, idx, blk[e["a"]], e["b"], e["d"])
+        U = np.array([complex(p["U"][2 * i], p["U"][2 * i + 1]) for i in range(1 << (2 * k))]).reshape(1 << k, 1 << k)
+        # block index of every amplitude, and the column base
+        b = np.zeros_like(idx)
+        for i, q in enumerate(blk):
+            b |= ((idx >> q) & 1) << i
+        mask = sum(1 << q for q in blk)
+        base = idx & ~mask
+        cols = np.unique(base)
+        off = np.array([sum(((bb >> i) & 1) << q for i, q in enumerate(blk)) for bb in range(1 << k)])
+        A = psi[cols[:, None] + off[None, :]]  # [column][block index]
+        psi[cols[:, None] + off[None, :]] = A @ U.T
+    return psi
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_dense_block_plans_replay_to_the_oracle(k):
+    from quantumforge_amd.workloads import qft_like_circuit
+
+    n = 11
+    cases = [random_layer_circuit(n, 6, seed=3, pairs_per_layer=4), qft_like_circuit(n, 2, seed=4)[1],
+             qft_like_circuit(n, 1, seed=5)[0], qft_circuit(n)]
+    for c in cases:
+        plan = Plan(c, n, kernel=4, max_block=k)
+        d = plan.describe()
+        st = plan.stats()
+        assert st["dense_blocks"] == st["passes"] == len(d["passes"]) > 0
+        assert all(len(p["block"]) == k for p in d["passes"])
+        rng = np.random.default_rng(k)
+        a0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        a0 /= np.linalg.norm(a0)
+        o = O.State(n, a0)
+        o.run(c.to_ops())
+        close(replay_dense(d, a0), o.amplitudes(), 1e-12)
+
+
+def test_dense_block_plan_counts_and_errors():
+    from quantumforge_amd.workloads import hea_circuit, qft_like_circuit
+
+    # QFT-like layers: one block per 5 qubits and layer, cross-block phases as
+    # diagonal factors of the next pass.
+    n = 33
+    _, c = qft_like_circuit(n, 2, seed=33)
+    st = Plan(c, n, kernel=4, max_block=5).stats()
+    assert st["dense_blocks"] == st["passes"] <= 2 * 7 + 2, st
+    with pytest.raises(QuantumError):
+        Plan(c, n, kernel=4, max_block=3)
+    with pytest.raises(QuantumError):
+        Plan(random_layer_circuit(8, 2, seed=1, pairs_per_layer=2), 8, kernel=4, max_block=5)
+    with pytest.raises(QuantumError):
+        Plan.parameterized(hea_circuit(10, 1), 10, kernel=4)

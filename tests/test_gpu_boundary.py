@@ -151,7 +151,7 @@ def test_gather_reads_what_download_reads(backend, precision):
     assert e.value.kind == "InvalidArgument"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_stream_rate_leaves_the_state_unchanged(backend, variant):
     n = 20
     a0 = random_state(n, 4)
@@ -160,7 +160,7 @@ def test_stream_rate_leaves_the_state_unchanged(backend, variant):
     assert gbs > 0 and ms > 0
     assert np.array_equal(dev.download().amplitudes, a0)
     with pytest.raises(QuantumError):
-        dev.stream_rate(4, 1)
+        dev.stream_rate(5, 1)
 
 
 def test_copy_from_device_same_and_converted_precision(backend):
