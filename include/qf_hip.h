@@ -316,21 +316,36 @@ qf_status qf_state_inner(const qf_state* a, const qf_state* b, double* out_re, d
 /* ------------------------------------------------------------------------ */
 /* Batches (CudaDeviceStateBatch, src/cuda.rs:1486-2005)                     */
 /* ------------------------------------------------------------------------ */
-/* CudaBackend::upload_batch, src/cuda.rs:871-954: equal widths, keeps the
- * reference's EmptyBatch / IncompatibleBatch errors.  amps is [count][2^qubits]. */
+/* A batch of `count` states with widths qubits[0..count), stored back to back
+ * (amps: the states' 2^qubits[i] amplitudes one after the other).  Widths may
+ * differ (no reference counterpart: the reference batches equal widths only);
+ * every batch operation then runs over each run of consecutive equal-width
+ * states, and validates a gate against every width before anything launches.
+ * EmptyBatch for count 0. */
 qf_status qf_batch_upload(qf_backend* backend, const qf_c128* amps, const uint32_t* qubits, uint32_t count,
                           qf_batch** out);
-/* |0..0> batch of `count` equal-width states. */
+/* CudaBackend::upload_batch, src/cuda.rs:871-954: equal widths only, with the
+ * reference's EmptyBatch / IncompatibleBatch (src/cuda.rs:875-877) errors. */
+qf_status qf_batch_upload_equal(qf_backend* backend, const qf_c128* amps, const uint32_t* qubits, uint32_t count,
+                                qf_batch** out);
+/* |0..0> batch of `count` equal-width states, and of states of the given widths. */
 qf_status qf_batch_zero(qf_backend* backend, uint32_t qubits, uint32_t count, int32_t precision, qf_batch** out);
+qf_status qf_batch_zero_uneven(qf_backend* backend, const uint32_t* qubits, uint32_t count, int32_t precision,
+                               qf_batch** out);
 qf_status qf_batch_destroy(qf_batch* batch);
-/* CudaDeviceStateBatch::qubits / batch_size, src/cuda.rs:1488-1496. */
+/* CudaDeviceStateBatch::qubits / batch_size, src/cuda.rs:1488-1496 (qubits: the
+ * widest state of an uneven batch). */
 qf_status qf_batch_shape(const qf_batch* batch, uint32_t* qubits, uint32_t* count);
+/* Width of every state (out is [count]). */
+qf_status qf_batch_widths(const qf_batch* batch, uint32_t* out);
 /* apply_gate / apply_cnot / apply_cz / apply_swap on every state, src/cuda.rs:1819-1936. */
 qf_status qf_batch_apply_gate(qf_batch* batch, int32_t gate, uint32_t target, double theta);
 qf_status qf_batch_apply_2q(qf_batch* batch, int32_t op, uint32_t a, uint32_t b);
-/* CudaDeviceStateBatch::run_circuit, src/cuda.rs:1949-1970. */
+/* CudaDeviceStateBatch::run_circuit, src/cuda.rs:1949-1970 (a plan only for an
+ * equal-width batch: QF_INCOMPATIBLE_BATCH otherwise). */
 qf_status qf_batch_run_circuit(qf_batch* batch, const qf_op* ops, size_t count, const qf_plan* plan);
-/* CudaDeviceStateBatch::download, src/cuda.rs:1973-1992; out is [count][2^qubits]. */
+/* CudaDeviceStateBatch::download, src/cuda.rs:1973-1992; out holds every state's
+ * amplitudes back to back. */
 qf_status qf_batch_download(const qf_batch* batch, qf_c128* out);
 /* Per-state Pauli-sum energies (FP64 compensated). out is [count]. */
 qf_status qf_batch_expectation_pauli_sum(const qf_batch* batch, const qf_pauli_term* terms, size_t nterms,
@@ -444,6 +459,13 @@ typedef struct {
  * the environment variable QF_UNEVEN_WAVE_BYTES bytes when that is smaller. */
 qf_status qf_uneven_adjoint_gradient(qf_backend* backend, const qf_uneven_instance* instances, uint32_t count,
                                      int32_t precision, double* energies, double* grads, qf_uneven_stats* stats);
+
+/* Energies <obs_i> of `count` parameterized circuits of different widths (no
+ * gradients): the uneven engine's forward passes and energy reductions only,
+ * one psi per instance (no costate), so each wave holds twice the instances of
+ * qf_uneven_adjoint_gradient.  Validation and stats as there. */
+qf_status qf_uneven_expectation(qf_backend* backend, const qf_uneven_instance* instances, uint32_t count,
+                                int32_t precision, double* energies, qf_uneven_stats* stats);
 
 /* Times the candidate register-kernel tile shapes of a circuit on the device
  * and returns the fastest as plan options (cached per backend process). */

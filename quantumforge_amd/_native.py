@@ -118,7 +118,10 @@ SIGNATURES = {
     "qf_state_inner": (None, [P, P, C.POINTER(dbl), C.POINTER(dbl)]),
     "qf_state_expectation_pauli_sum_noalloc": (None, [P, P, sz, C.POINTER(dbl), C.POINTER(dbl)]),
     "qf_batch_upload": (None, [P, P, P, u32, PP]),
+    "qf_batch_upload_equal": (None, [P, P, P, u32, PP]),
     "qf_batch_zero": (None, [P, u32, u32, i32, PP]),
+    "qf_batch_zero_uneven": (None, [P, P, u32, i32, PP]),
+    "qf_batch_widths": (None, [P, P]),
     "qf_batch_destroy": (None, [P]),
     "qf_batch_shape": (None, [P, C.POINTER(u32), C.POINTER(u32)]),
     "qf_batch_apply_gate": (None, [P, i32, u32, dbl]),
@@ -167,6 +170,7 @@ SIGNATURES = {
     "qf_batch_expectation": (None, [P, P, sz, u32, u32, P, u32, P, u32, P, sz, C.POINTER(GradOptions), P]),
     "qf_state_run_parameterized": (None, [P, P, sz, u32, u32, P, sz, P, sz]),
     "qf_uneven_adjoint_gradient": (None, [P, C.POINTER(UnevenInstance), u32, i32, P, P, C.POINTER(UnevenStats)]),
+    "qf_uneven_expectation": (None, [P, C.POINTER(UnevenInstance), u32, i32, P, C.POINTER(UnevenStats)]),
     "qf_autotune": (None, [P, P, sz, u32, i32, C.POINTER(PlanOptions), C.POINTER(dbl)]),
 }
 

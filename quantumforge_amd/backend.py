@@ -207,12 +207,28 @@ class HipBackend:
             raise QuantumError(7, "all states in a HIP batch must have equal dimensions")
         amps = np.ascontiguousarray(np.stack([s.amplitudes for s in states]), dtype=np.complex128)
         h = C.c_void_p()
+        check(lib().qf_batch_upload_equal(self._h, ptr(amps), ptr(qubits), len(states), C.byref(h)))
+        return HipDeviceStateBatch(h, self)
+
+    def upload_uneven_batch(self, states: Sequence[StateVector]) -> "HipDeviceStateBatch":
+        """A resident batch of states of different widths (qf_batch_upload):
+        gates, circuits and energies apply to every state."""
+        qubits = np.array([s.qubits for s in states], dtype=np.uint32)
+        amps = np.ascontiguousarray(np.concatenate([np.asarray(s.amplitudes, dtype=np.complex128) for s in states])
+                                    if len(states) else np.zeros(0, dtype=np.complex128))
+        h = C.c_void_p()
         check(lib().qf_batch_upload(self._h, ptr(amps), ptr(qubits), len(states), C.byref(h)))
         return HipDeviceStateBatch(h, self)
 
     def zero_batch(self, qubits: int, count: int, precision="complex128") -> "HipDeviceStateBatch":
         h = C.c_void_p()
         check(lib().qf_batch_zero(self._h, qubits, count, _precision(precision), C.byref(h)))
+        return HipDeviceStateBatch(h, self)
+
+    def zero_uneven_batch(self, widths: Sequence[int], precision="complex128") -> "HipDeviceStateBatch":
+        w = np.ascontiguousarray(np.asarray(widths, dtype=np.uint32))
+        h = C.c_void_p()
+        check(lib().qf_batch_zero_uneven(self._h, ptr(w), w.size, _precision(precision), C.byref(h)))
         return HipDeviceStateBatch(h, self)
 
     # --- convenience (src/cuda.rs:958-1083) --------------------------------
@@ -443,7 +459,8 @@ class HipDeviceState:
 
 
 class HipDeviceStateBatch:
-    """CudaDeviceStateBatch, src/cuda.rs:1486-2005: equal-width resident batch."""
+    """CudaDeviceStateBatch, src/cuda.rs:1486-2005: a resident batch (equal
+    widths, or different widths from upload_uneven_batch / zero_uneven_batch)."""
 
     def __init__(self, handle: C.c_void_p, backend: HipBackend) -> None:
         self._h = handle
@@ -451,6 +468,9 @@ class HipDeviceStateBatch:
         q, n = C.c_uint32(), C.c_uint32()
         check(lib().qf_batch_shape(handle, C.byref(q), C.byref(n)))
         self.qubits_, self.count = int(q.value), int(n.value)
+        w = np.zeros(max(1, self.count), dtype=np.uint32)
+        check(lib().qf_batch_widths(handle, ptr(w)))
+        self.widths = [int(x) for x in w[: self.count]]
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
@@ -511,9 +531,13 @@ class HipDeviceStateBatch:
         self._backend.synchronize()
 
     def download(self) -> List[StateVector]:
-        out = np.empty((self.count, 1 << self.qubits_), dtype=np.complex128)
+        out = np.empty(sum(1 << q for q in self.widths), dtype=np.complex128)
         check(lib().qf_batch_download(self._h, ptr(out)))
-        return [StateVector(out[i].copy(), self.qubits_) for i in range(self.count)]
+        res, off = [], 0
+        for q in self.widths:
+            res.append(StateVector(out[off: off + (1 << q)].copy(), q))
+            off += 1 << q
+        return res
 
     def expectation_pauli_sum(self, h: PauliSum) -> np.ndarray:
         t = h.to_terms()

@@ -61,6 +61,11 @@ struct qf_density {
     uint64_t dim = 0;
 };
 
+// A batch of `count` states stored back to back.  Equal-width batches have
+// one segment; a batch of different widths (qf_batch_upload /
+// qf_batch_zero_uneven) is cut into segments of consecutive states of one
+// width, and every batch operation runs segment by segment.  qubits is the
+// widest state, dim its dimension (equal-width batches: every state's).
 struct qf_batch {
     qf_backend* be = nullptr;
     void* d = nullptr;
@@ -68,6 +73,13 @@ struct qf_batch {
     uint32_t count = 0;
     int32_t precision = QF_C128;
     uint64_t dim = 0;
+    struct Seg {
+        uint32_t qubits, first, count;
+        uint64_t off;  // amplitude offset of the segment's first state
+    };
+    std::vector<Seg> segs;
+    uint64_t total = 0;  // amplitudes of all states
+    bool uneven() const { return segs.size() > 1; }
 };
 
 namespace qfrt {

@@ -1361,35 +1361,89 @@ extern "C" qf_status qf_state_inner(const qf_state* a, const qf_state* b, double
 // Batches
 // ---------------------------------------------------------------------------
 namespace {
-qf_status new_batch(qf_backend* be, uint32_t qubits, uint32_t count, int32_t precision, qf_batch** out) {
+// A batch of count states of widths qubits[0..count) (uniform: all equal to
+// qubits[0]); segments of consecutive equal widths.
+qf_status new_batch_widths(qf_backend* be, const uint32_t* qubits, uint32_t count, int32_t precision,
+                           qf_batch** out) {
     if (!be || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (count == 0) return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
-    qf_status s = check_qubits(qubits);
-    if (s) return s;
     if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
-    const uint64_t dim = uint64_t(1) << qubits;
-    if (qubits + 32 > 63 && (dim * count) / count != dim) return fail(QF_STATE_TOO_LARGE, "batch too large");
-    s = activate(be);
+    std::vector<qf_batch::Seg> segs;
+    uint64_t total = 0;
+    uint32_t widest = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        qf_status s = check_qubits(qubits[i]);
+        if (s) return s;
+        if (segs.empty() || segs.back().qubits != qubits[i]) segs.push_back(qf_batch::Seg{qubits[i], i, 0, total});
+        ++segs.back().count;
+        const uint64_t dim = uint64_t(1) << qubits[i];
+        if (total + dim < total || (total + dim) > (uint64_t(1) << 40))
+            return fail(QF_STATE_TOO_LARGE, "batch too large");
+        total += dim;
+        widest = std::max(widest, qubits[i]);
+    }
+    qf_status s = activate(be);
     if (s) return s;
     void* d = nullptr;
-    s = device_alloc(&d, dim * count * amp_bytes(precision));
+    s = device_alloc(&d, total * amp_bytes(precision));
     if (s) return s;
     qf_batch* b = new qf_batch;
     b->be = be;
     be->refs.fetch_add(1);
     b->d = d;
-    b->qubits = qubits;
+    b->qubits = widest;
     b->count = count;
     b->precision = precision;
-    b->dim = dim;
+    b->dim = uint64_t(1) << widest;
+    b->segs = std::move(segs);
+    b->total = total;
     *out = b;
     return QF_OK;
 }
+
+qf_status new_batch(qf_backend* be, uint32_t qubits, uint32_t count, int32_t precision, qf_batch** out) {
+    if (count == 0) {
+        if (out) *out = nullptr;
+        return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
+    }
+    std::vector<uint32_t> w(count, qubits);
+    return new_batch_widths(be, w.data(), count, precision, out);
+}
+
+qf_status upload_into(qf_batch** out, const qf_c128* amps) {
+    hipError_t e = hipMemcpy((*out)->d, amps, (*out)->total * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        qf_batch_destroy(*out);
+        *out = nullptr;
+        return hip_fail(e, "hipMemcpy batch upload");
+    }
+    return QF_OK;
+}
+
+void* seg_base(const qf_batch* b, const qf_batch::Seg& g) {
+    return static_cast<char*>(b->d) + g.off * amp_bytes(b->precision);
+}
 }  // namespace
 
+// Uneven widths allowed: the states are stored back to back, each with its
+// own width (no reference counterpart; CudaBackend::upload_batch accepts
+// equal widths only, src/cuda.rs:871-954, kept as qf_batch_upload_equal).
 extern "C" qf_status qf_batch_upload(qf_backend* be, const qf_c128* amps, const uint32_t* qubits, uint32_t count,
                                      qf_batch** out) {
+    if (!out) return fail(QF_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (count == 0) return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
+    if (!amps || !qubits) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_status s = new_batch_widths(be, qubits, count, QF_C128, out);
+    if (s) return s;
+    return upload_into(out, amps);
+}
+
+// CudaBackend::upload_batch, src/cuda.rs:871-954: EmptyBatch, IncompatibleBatch
+// (src/cuda.rs:875-877) for states of different widths.
+extern "C" qf_status qf_batch_upload_equal(qf_backend* be, const qf_c128* amps, const uint32_t* qubits,
+                                           uint32_t count, qf_batch** out) {
     if (!out) return fail(QF_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (count == 0) return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
@@ -1399,30 +1453,45 @@ extern "C" qf_status qf_batch_upload(qf_backend* be, const qf_c128* amps, const 
             return fail(QF_INCOMPATIBLE_BATCH, "all states in a HIP batch must have equal dimensions");
     qf_status s = new_batch(be, qubits[0], count, QF_C128, out);
     if (s) return s;
-    hipError_t e = hipMemcpy((*out)->d, amps, (*out)->dim * count * 16, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        qf_batch_destroy(*out);
-        *out = nullptr;
-        return hip_fail(e, "hipMemcpy batch upload");
-    }
-    return QF_OK;
+    return upload_into(out, amps);
 }
 
 extern "C" qf_status qf_batch_zero(qf_backend* be, uint32_t qubits, uint32_t count, int32_t precision,
                                    qf_batch** out) {
-    qf_status s = new_batch(be, qubits, count, precision, out);
+    if (count == 0) {
+        if (out) *out = nullptr;
+        return fail(QF_EMPTY_BATCH, "a HIP state batch cannot be empty");
+    }
+    std::vector<uint32_t> w(count, qubits);
+    return qf_batch_zero_uneven(be, w.data(), count, precision, out);
+}
+
+extern "C" qf_status qf_batch_zero_uneven(qf_backend* be, const uint32_t* qubits, uint32_t count, int32_t precision,
+                                          qf_batch** out) {
+    if (!qubits && count) return fail(QF_INVALID_ARGUMENT, "null argument");
+    qf_status s = new_batch_widths(be, qubits, count, precision, out);
     if (s) return s;
     qf_batch* b = *out;
-    if (precision == QF_C128)
-        fill_zero<double>(be, b->d, b->dim * count, qubits);
-    else
-        fill_zero<float>(be, b->d, b->dim * count, qubits);
+    for (const auto& g : b->segs) {
+        const uint64_t n = (uint64_t(1) << g.qubits) * g.count;
+        if (precision == QF_C128)
+            fill_zero<double>(be, seg_base(b, g), n, g.qubits);
+        else
+            fill_zero<float>(be, seg_base(b, g), n, g.qubits);
+    }
     s = launch_check("k_fill_zero_state");
     if (s) {
         qf_batch_destroy(b);
         *out = nullptr;
     }
     return s;
+}
+
+extern "C" qf_status qf_batch_widths(const qf_batch* b, uint32_t* out) {
+    if (!b || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
+    for (const auto& g : b->segs)
+        for (uint32_t i = 0; i < g.count; ++i) out[g.first + i] = g.qubits;
+    return QF_OK;
 }
 
 extern "C" qf_status qf_batch_destroy(qf_batch* b) {
@@ -1443,40 +1512,54 @@ extern "C" qf_status qf_batch_shape(const qf_batch* b, uint32_t* qubits, uint32_
     return QF_OK;
 }
 
+namespace {
+// Lowers `ops` for every segment width first (errors follow the narrowest
+// state that rejects an op; nothing is launched before all are valid), then
+// runs them segment by segment.
+qf_status batch_ops(qf_batch* b, const qf_op* ops, size_t count, const qf::PlanOptions& po) {
+    std::vector<std::vector<qf::GateIn>> g(b->segs.size());
+    for (size_t i = 0; i < b->segs.size(); ++i) {
+        qf_status s = lower_ops(ops, count, b->segs[i].qubits, g[i]);
+        if (s) return s;
+    }
+    qf_status s = activate(b->be);
+    if (s) return s;
+    for (size_t i = 0; i < b->segs.size(); ++i) {
+        const auto& sg = b->segs[i];
+        if ((s = apply_gates(b->be, seg_base(b, sg), b->precision, sg.qubits, sg.count, g[i], po))) return s;
+    }
+    return QF_OK;
+}
+}  // namespace
+
 extern "C" qf_status qf_batch_apply_gate(qf_batch* b, int32_t gate, uint32_t target, double theta) {
     if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
     qf_op op{0, gate, target, 0, theta};
-    std::vector<qf::GateIn> g;
-    qf_status s = lower_ops(&op, 1, b->qubits, g);
-    if (s) return s;
-    if ((s = activate(b->be))) return s;
     qf::PlanOptions o;
     exact_opts(o);
-    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, o);
+    return batch_ops(b, &op, 1, o);
 }
 
 extern "C" qf_status qf_batch_apply_2q(qf_batch* b, int32_t op, uint32_t a, uint32_t c) {
     if (!b) return fail(QF_INVALID_ARGUMENT, "null batch");
     if (op < 0 || op > 2) return fail(QF_INVALID_ARGUMENT, "unknown two-qubit op");
     qf_op o{op + 1, 0, a, c, 0.0};
-    std::vector<qf::GateIn> g;
-    qf_status s = lower_ops(&o, 1, b->qubits, g);
-    if (s) return s;
-    if ((s = activate(b->be))) return s;
     qf::PlanOptions po;
     exact_opts(po);
-    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, po);
+    return batch_ops(b, &o, 1, po);
 }
 
 extern "C" qf_status qf_batch_run_circuit(qf_batch* b, const qf_op* ops, size_t count, const qf_plan* plan) {
     if (!b || (!ops && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (!plan) return batch_ops(b, ops, count, b->be->opts);
+    if (b->uneven())
+        return fail(QF_INCOMPATIBLE_BATCH, "a plan is built for one width: run a batch of different widths without one");
     std::vector<qf::GateIn> g;
     qf_status s = lower_ops(ops, count, b->qubits, g);
     if (s) return s;
-    if (plan && (s = check_circuit_plan(plan->plan, g, b->qubits))) return s;
+    if ((s = check_circuit_plan(plan->plan, g, b->qubits))) return s;
     if ((s = activate(b->be))) return s;
-    if (plan) return run_plan(b->be, b->d, b->precision, b->qubits, b->count, plan->plan);
-    return apply_gates(b->be, b->d, b->precision, b->qubits, b->count, g, b->be->opts);
+    return run_plan(b->be, b->d, b->precision, b->qubits, b->count, plan->plan);
 }
 
 extern "C" qf_status qf_batch_download(const qf_batch* b, qf_c128* out) {
@@ -1484,7 +1567,7 @@ extern "C" qf_status qf_batch_download(const qf_batch* b, qf_c128* out) {
     qf_backend* be = b->be;
     qf_status s = activate(be);
     if (s) return s;
-    const uint64_t total = b->dim * b->count;
+    const uint64_t total = b->total;
     if (b->precision == QF_C128) {
         HIP_TRY(hipMemcpyAsync(out, b->d, total * 16, hipMemcpyDeviceToHost, be->stream));
         HIP_TRY(hipStreamSynchronize(be->stream));
@@ -1506,17 +1589,24 @@ extern "C" qf_status qf_batch_download(const qf_batch* b, qf_c128* out) {
 extern "C" qf_status qf_batch_expectation_pauli_sum(const qf_batch* b, const qf_pauli_term* terms, size_t nterms,
                                                     double* out) {
     if (!b || (!terms && nterms) || !out) return fail(QF_INVALID_ARGUMENT, "null argument");
-    std::vector<qf::PauliDev> t;
-    qf_status s = pauli_terms(terms, nterms, b->qubits, t);
-    if (s) return s;
-    if ((s = activate(b->be))) return s;
-    std::vector<double> r;
-    for (uint32_t off = 0; off < b->count; off += 65535) {
-        const uint32_t n = std::min<uint32_t>(65535, b->count - off);
-        const char* base = static_cast<const char*>(b->d) + size_t(off) * b->dim * amp_bytes(b->precision);
-        s = reduce(b->be, base, nullptr, b->precision, b->dim, n, qf::RED_PAULI, 0, t, r);
+    std::vector<std::vector<qf::PauliDev>> t(b->segs.size());
+    for (size_t k = 0; k < b->segs.size(); ++k) {
+        qf_status s = pauli_terms(terms, nterms, b->segs[k].qubits, t[k]);
         if (s) return s;
-        for (uint32_t i = 0; i < n; ++i) out[off + i] = r[2 * i];
+    }
+    qf_status s = activate(b->be);
+    if (s) return s;
+    std::vector<double> r;
+    for (size_t k = 0; k < b->segs.size(); ++k) {
+        const auto& sg = b->segs[k];
+        const uint64_t dim = uint64_t(1) << sg.qubits;
+        for (uint32_t off = 0; off < sg.count; off += 65535) {
+            const uint32_t n = std::min<uint32_t>(65535, sg.count - off);
+            const char* base = static_cast<const char*>(seg_base(b, sg)) + size_t(off) * dim * amp_bytes(b->precision);
+            s = reduce(b->be, base, nullptr, b->precision, dim, n, qf::RED_PAULI, 0, t[k], r);
+            if (s) return s;
+            for (uint32_t i = 0; i < n; ++i) out[sg.first + off + i] = r[2 * i];
+        }
     }
     return QF_OK;
 }

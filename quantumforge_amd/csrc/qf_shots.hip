@@ -517,6 +517,7 @@ extern "C" qf_status qf_state_measure(qf_state* st, uint32_t qubit, double unifo
 extern "C" qf_status qf_batch_measure_with_uniforms(qf_batch* b, uint32_t qubit, const double* uniforms,
                                                    size_t count, uint8_t* outcomes) {
     if (!b || !outcomes || (!uniforms && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (b->uneven()) return fail(QF_INCOMPATIBLE_BATCH, "this operation needs a batch of equal widths");
     if (qubit >= b->qubits)
         return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(qubit) + " is out of range for " +
                                                std::to_string(b->qubits) + " qubits");
@@ -532,6 +533,7 @@ extern "C" qf_status qf_batch_measure_with_uniforms(qf_batch* b, uint32_t qubit,
 extern "C" qf_status qf_batch_apply_gate_masked(qf_batch* b, int32_t gate, uint32_t target, double theta,
                                                const uint8_t* enabled, size_t count) {
     if (!b || (!enabled && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (b->uneven()) return fail(QF_INCOMPATIBLE_BATCH, "this operation needs a batch of equal widths");
     qf_status s = check_gate(gate, target, theta, b->qubits);
     if (s) return s;
     if (count != b->count) return fail(QF_INVALID_BATCH_SAMPLES, "the mask needs one entry per state");
@@ -559,6 +561,7 @@ extern "C" qf_status qf_run_circuit_shots(qf_backend* be, const qf_c128* initial
 extern "C" qf_status qf_batch_apply_channel_trajectory(qf_batch* b, int32_t channel, uint32_t target, double p0,
                                                       double p1, const double* uniforms, size_t count) {
     if (!b || (!uniforms && count)) return fail(QF_INVALID_ARGUMENT, "null argument");
+    if (b->uneven()) return fail(QF_INCOMPATIBLE_BATCH, "this operation needs a batch of equal widths");
     if (target >= b->qubits)
         return fail(QF_QUBIT_OUT_OF_RANGE, "qubit index " + std::to_string(target) + " is out of range for " +
                                                std::to_string(b->qubits) + " qubits");

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "qf_regtile.h"
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_grouped(const cx<R>* __r
         const uint64_t lb = b - e.block_start;
         const uint64_t dim = uint64_t(1) << e.n;
         const cx<R>* __restrict__ p = psi + e.state_off;
-        cx<R>* __restrict__ l = lam + e.state_off;
+        cx<R>* __restrict__ l = lam ? lam + e.state_off : nullptr;  // null: energies only
         const qf::PauliDev* T = terms + e.terms_off;
         qf::dd er{0, 0}, ei{0, 0};
         const uint64_t stride = uint64_t(e.nblocks) * blockDim.x;
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_grouped(const cx<R>* __r
                 ar += cr * pr - ci * pi;
                 ai += cr * pi + ci * pr;
             }
-            l[i] = cx<R>{R(ar), R(ai)};
+            if (lam) l[i] = cx<R>{R(ar), R(ai)};
             const cx<R> v = p[i];
             const double vr = v.re, vi = v.im;
             qf::neumaier(er, vr * ar + vi * ai);
@@ -465,9 +466,13 @@ qf_status capture_wave_graph(hipStream_t st, uint32_t F, uint32_t Rv, uint32_t R
 
 }  // namespace
 
-extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_instance* inst, uint32_t count,
-                                                int32_t precision, double* energies, double* grads,
-                                                qf_uneven_stats* stats) {
+namespace {
+// Energies, and with want_grad the adjoint gradients, of an uneven batch.
+// Without gradients the instances have no reverse plans, no costate buffer
+// and no partial table: the per-wave graph is the forward passes, the energy
+// reduction and the finishing kernel.
+qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t count, int32_t precision,
+                     double* energies, double* grads, qf_uneven_stats* stats, bool want_grad) {
     if (!be || !inst || !energies) return fail(QF_INVALID_ARGUMENT, "null argument");
     if (count == 0) return fail(QF_EMPTY_BATCH, "an uneven batch cannot be empty");
     if (precision != QF_C128 && precision != QF_C64) return fail(QF_INVALID_ARGUMENT, "unknown precision");
@@ -501,7 +506,7 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
         goff += q.nparams;
         if (q.qubits > inst[widest].qubits) widest = k;
     }
-    if (goff && !grads) return fail(QF_INVALID_ARGUMENT, "null gradient output");
+    if (want_grad && goff && !grads) return fail(QF_INVALID_ARGUMENT, "null gradient output");
     if ((s = activate(be))) return s;
     void* sp = nullptr;
     qf_backend_stream(be, &sp);
@@ -525,36 +530,54 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
     S.tune_ms = tune.ms;
     const uint32_t Mf = tune.tile, RBf = tune.reg, Mr = tune.tile ? tune.tile - 1 : 0, RBr = 3;
 
-    // 3. plans; instances too narrow for the shared register tile run on their own
+    // 3. plans (on host threads: one forward and one reverse plan per
+    // instance); instances too narrow for the shared register tile run on
+    // their own
     std::vector<uint32_t> grouped, single;
     for (uint32_t k = 0; k < count; ++k) {
-        if (Mf == 0 || inst[k].qubits < Mf) {
-            single.push_back(k);
-            continue;
-        }
-        qf::PlanOptions po;
-        po.fuse = 1;
-        po.kernel = 2;
-        po.tile_qubits = Mf;
-        po.low_qubits = tune.low;
-        po.reg_bits = RBf;
-        std::string e = qf::build_plan(param_gates(pcs[k], nullptr, false), inst[k].qubits, precision, po, I[k].fwd);
-        if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
-        po.tile_qubits = Mr;
-        po.reg_bits = RBr;
-        e = qf::build_plan(param_gates(pcs[k], nullptr, true), inst[k].qubits, precision, po, I[k].rev);
-        if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
-        grouped.push_back(k);
+        if (Mf == 0 || inst[k].qubits < Mf) single.push_back(k);
+        else grouped.push_back(k);
+    }
+    {
+        std::vector<std::string> errs(grouped.size());
+        auto plan_one = [&](size_t g) {
+            const uint32_t k = grouped[g];
+            qf::PlanOptions po;
+            po.fuse = 1;
+            po.kernel = 2;
+            po.tile_qubits = Mf;
+            po.low_qubits = tune.low;
+            po.reg_bits = RBf;
+            errs[g] = qf::build_plan(param_gates(pcs[k], nullptr, false), inst[k].qubits, precision, po, I[k].fwd);
+            if (!errs[g].empty() || !want_grad) return;
+            po.tile_qubits = Mr;
+            po.reg_bits = RBr;
+            errs[g] = qf::build_plan(param_gates(pcs[k], nullptr, true), inst[k].qubits, precision, po, I[k].rev);
+        };
+        const size_t nt = std::min<size_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+                                           grouped.size() / 8 + 1);
+        std::vector<std::thread> pool;
+        for (size_t w = 0; w < nt; ++w)
+            pool.emplace_back([&, w] {
+                for (size_t g = w; g < grouped.size(); g += nt) plan_one(g);
+            });
+        for (auto& t : pool) t.join();
+        for (const std::string& e : errs)
+            if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     }
     for (uint32_t k : single) {  // narrow instances: the batched path with one set each
         qf_gradient_options o{};
         o.strategy = QF_GRAD_ADJOINT;
         o.precision = precision;
         o.fd_step = 1e-3;
-        if ((s = qf_batch_gradient(be, inst[k].ops, inst[k].nops, inst[k].nparams, 0, nullptr, inst[k].qubits,
-                                   inst[k].params, 1, inst[k].obs, inst[k].nterms, &o, energies + k,
-                                   inst[k].nparams ? grads + I[k].grad_off : nullptr)))
-            return s;
+        if (want_grad)
+            s = qf_batch_gradient(be, inst[k].ops, inst[k].nops, inst[k].nparams, 0, nullptr, inst[k].qubits,
+                                  inst[k].params, 1, inst[k].obs, inst[k].nterms, &o, energies + k,
+                                  inst[k].nparams ? grads + I[k].grad_off : nullptr);
+        else
+            s = qf_batch_expectation(be, inst[k].ops, inst[k].nops, inst[k].nparams, 0, nullptr, inst[k].qubits,
+                                     inst[k].params, 1, inst[k].obs, inst[k].nterms, &o, energies + k);
+        if (s) return s;
     }
     S.grouped_instances = uint32_t(grouped.size());
     if (grouped.empty()) return QF_OK;
@@ -624,7 +647,8 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
     auto footprint = [&](uint32_t k) {
         const uint32_t n = inst[k].qubits;
         const uint64_t nblk = uint64_t(1) << (n - Mr);
-        return 2 * (uint64_t(1) << n) * ab + nblk * pcs[k].slot_param.size() * 8 + energy_blocks(n) * 32 + 256;
+        return (want_grad ? 2 : 1) * (uint64_t(1) << n) * ab +
+               (want_grad ? nblk * pcs[k].slot_param.size() * 8 : 0) + energy_blocks(n) * 32 + 256;
     };
     std::vector<uint32_t> order(grouped);
     std::stable_sort(order.begin(), order.end(),
@@ -665,7 +689,7 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
         for (uint32_t k : w) {
             const uint32_t n = inst[k].qubits;
             a += uint64_t(1) << n;
-            p += (uint64_t(1) << (n - Mr)) * pcs[k].slot_param.size();
+            if (want_grad) p += (uint64_t(1) << (n - Mr)) * pcs[k].slot_param.size();
             e += 2 * energy_blocks(n);
         }
         max_amps = std::max(max_amps, a);
@@ -678,7 +702,7 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
     const size_t ntables = size_t(F) + Rv + 2;
     WaveBuffers B;
     if ((s = device_alloc(&B.psi, max_amps * ab))) return s;
-    if ((s = device_alloc(&B.lam, max_amps * ab))) return s;
+    if (want_grad && (s = device_alloc(&B.lam, max_amps * ab))) return s;
     if ((s = device_alloc(&B.part, std::max<uint64_t>(max_part, 1) * 8))) return s;
     if ((s = device_alloc(&B.epart, std::max<uint64_t>(max_epart, 2) * sizeof(qf::dd)))) return s;
     if ((s = device_alloc(&B.tables, ntables * table_stride))) return s;
@@ -733,7 +757,7 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
             poff[i] = part_off;
             eoff[i] = ep_off;
             amp_off += uint64_t(1) << n;
-            part_off += (uint64_t(1) << (n - Mr)) * pcs[w[i]].slot_param.size();
+            if (want_grad) part_off += (uint64_t(1) << (n - Mr)) * pcs[w[i]].slot_param.size();
             ep_off += 2 * energy_blocks(n);
         }
         // instances (table 0)
@@ -742,7 +766,8 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
             for (size_t i = 0; i < w.size(); ++i) {
                 const uint32_t k = w[i], n = inst[k].qubits;
                 E[i] = InstEntry{aoff[i], eoff[i], poff[i], I[k].grad_off, k, I[k].pcsr_off, energy_blocks(n),
-                                 uint32_t(1) << (n - Mr), uint32_t(pcs[k].slot_param.size()), pcs[k].nparams};
+                                 uint32_t(1) << (n - Mr), uint32_t(pcs[k].slot_param.size()),
+                                 want_grad ? pcs[k].nparams : 0u};
             }
             header(0)->nentries = uint32_t(w.size());
             header(0)->total_blocks = w.size();
@@ -813,11 +838,23 @@ extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_
     std::vector<double> e_all(count);
     QF_HIP_TRY(hipMemcpy(e_all.data(), d_energy.p, size_t(count) * sizeof(double), hipMemcpyDeviceToHost));
     for (uint32_t k : grouped) energies[k] = e_all[k];
-    if (goff) {
+    if (want_grad && goff) {
         std::vector<double> g_all(goff);
         QF_HIP_TRY(hipMemcpy(g_all.data(), d_grads.p, goff * sizeof(double), hipMemcpyDeviceToHost));
         for (uint32_t k : grouped)
             std::memcpy(grads + I[k].grad_off, g_all.data() + I[k].grad_off, inst[k].nparams * sizeof(double));
     }
     return QF_OK;
+}
+}  // namespace
+
+extern "C" qf_status qf_uneven_adjoint_gradient(qf_backend* be, const qf_uneven_instance* inst, uint32_t count,
+                                                int32_t precision, double* energies, double* grads,
+                                                qf_uneven_stats* stats) {
+    return uneven_run(be, inst, count, precision, energies, grads, stats, true);
+}
+
+extern "C" qf_status qf_uneven_expectation(qf_backend* be, const qf_uneven_instance* inst, uint32_t count,
+                                           int32_t precision, double* energies, qf_uneven_stats* stats) {
+    return uneven_run(be, inst, count, precision, energies, nullptr, stats, false);
 }

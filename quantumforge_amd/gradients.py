@@ -254,6 +254,27 @@ def uneven_adjoint_gradient(backend, instances: Sequence[Tuple[ParameterizedCirc
     return energies, out, {f: getattr(stats, f) for f, _ in N.UnevenStats._fields_}
 
 
+def uneven_expectation(backend, instances: Sequence[Tuple[ParameterizedCircuit, int, Sequence[float], object]],
+                       precision: str = "complex128"):
+    """Energies of circuits of different widths in one device schedule (the
+    uneven engine without the reverse sweep).  instances: (circuit, qubits,
+    parameters, observable) tuples.  Returns (energies [count], stats dict)."""
+    keep = []
+    recs = (N.UnevenInstance * len(instances))()
+    for k, (pc, q, params, obs) in enumerate(instances):
+        ops = pc.to_pops()
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        t = _terms(obs, q)
+        keep.append((ops, p, t))
+        recs[k] = N.UnevenInstance(ops.ctypes.data if len(ops) else None, len(ops), pc.parameter_count, q,
+                                   p.ctypes.data if p.size else None, t.ctypes.data if len(t) else None, len(t))
+    energies = np.zeros(len(instances), dtype=np.float64)
+    stats = N.UnevenStats()
+    prec = 0 if precision in ("complex128", "c128") else 1
+    check(lib().qf_uneven_expectation(backend._h, recs, len(instances), prec, ptr(energies), C.byref(stats)))
+    return energies, {f: getattr(stats, f) for f, _ in N.UnevenStats._fields_}
+
+
 def autotune(backend, circuit, qubits: int, precision: str = "complex128") -> Tuple[dict, float]:
     """Fastest register-kernel tile shape for a circuit (timed on the device)."""
     ops = circuit.to_ops()
