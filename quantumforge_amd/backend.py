@@ -40,6 +40,7 @@ class Plan:
         h = C.c_void_p()
         check(lib().qf_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
         self._h = h
+        self._lib = lib()  # kept for release at interpreter exit
         self.qubits = qubits
 
     @classmethod
@@ -57,6 +58,7 @@ class Plan:
                                                 pc.input_count, ptr(inp), inp.size, qubits, int(reverse),
                                                 C.byref(opts), C.byref(h)))
         self._h = h
+        self._lib = lib()  # kept for release at interpreter exit
         self.qubits = qubits
         return self
 
@@ -71,6 +73,7 @@ class Plan:
         h = C.c_void_p()
         check(lib().qf_density_plan_build(ptr(self._ops), len(self._ops), qubits, C.byref(opts), C.byref(h)))
         self._h = h
+        self._lib = lib()  # kept for release at interpreter exit
         self.qubits = 2 * qubits
         return self
 
@@ -121,7 +124,7 @@ class Plan:
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
         if h:
-            lib().qf_plan_destroy(h)
+            getattr(self, '_lib', None) and self._lib.qf_plan_destroy(h)
             self._h = None
 
 
@@ -132,12 +135,13 @@ class HipBackend:
         h = C.c_void_p()
         check(lib().qf_backend_create(int(ordinal), C.byref(h)))
         self._h = h
+        self._lib = lib()  # kept for release at interpreter exit
         self.ordinal = int(ordinal)
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
         if h:
-            lib().qf_backend_destroy(h)
+            getattr(self, '_lib', None) and self._lib.qf_backend_destroy(h)
             self._h = None
 
     def device_ordinal(self) -> int:
@@ -343,6 +347,7 @@ class HipDeviceState:
         # The C handle holds a reference on its backend (freed with the last
         # state), so the Python backend object may be dropped first.
         self._h = handle
+        self._lib = lib()  # kept for release at interpreter exit
         q = C.c_uint32()
         check(lib().qf_state_qubits(handle, C.byref(q)))
         self.qubits_ = int(q.value)
@@ -353,7 +358,7 @@ class HipDeviceState:
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
         if h:
-            lib().qf_state_destroy(h)
+            getattr(self, '_lib', None) and self._lib.qf_state_destroy(h)
             self._h = None
 
     def qubits(self) -> int:
@@ -464,6 +469,7 @@ class HipDeviceStateBatch:
 
     def __init__(self, handle: C.c_void_p, backend: HipBackend) -> None:
         self._h = handle
+        self._lib = lib()  # kept for release at interpreter exit
         self._backend = backend
         q, n = C.c_uint32(), C.c_uint32()
         check(lib().qf_batch_shape(handle, C.byref(q), C.byref(n)))
@@ -475,7 +481,7 @@ class HipDeviceStateBatch:
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
         if h:
-            lib().qf_batch_destroy(h)
+            getattr(self, '_lib', None) and self._lib.qf_batch_destroy(h)
             self._h = None
 
     def qubits(self) -> int:
@@ -552,6 +558,7 @@ class HipDeviceDensityMatrix:
 
     def __init__(self, handle: C.c_void_p) -> None:
         self._h = handle
+        self._lib = lib()  # kept for release at interpreter exit
         q = C.c_uint32()
         check(lib().qf_density_qubits(handle, C.byref(q)))
         self.qubits_ = int(q.value)
@@ -559,7 +566,7 @@ class HipDeviceDensityMatrix:
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
         if h:
-            lib().qf_density_destroy(h)
+            getattr(self, '_lib', None) and self._lib.qf_density_destroy(h)
             self._h = None
 
     def qubits(self) -> int:

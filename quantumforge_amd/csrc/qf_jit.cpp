@@ -2,6 +2,7 @@
 // compilation and the in-process / on-disk caches (interface in qf_jit.h).
 #include "qf_jit.h"
 
+#include <dirent.h>
 #include <hip/hiprtc.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -14,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <functional>
 #include <map>
@@ -1310,6 +1312,34 @@ void write_file_atomic(const std::string& dir, const std::string& name, const st
     std::rename(tmp.c_str(), (dir + "/" + name).c_str());
 }
 
+// At most this many code objects stay in the disk cache; older ones (by
+// modification time) are removed when a new program is written.
+constexpr size_t kDiskCacheFiles = 4096;
+
+void prune_disk_cache(const std::string& dir) {
+    DIR* d = ::opendir(dir.c_str());
+    if (!d) return;
+    std::vector<std::pair<long long, std::string>> files;
+    while (dirent* e = ::readdir(d)) {
+        const std::string name = e->d_name;
+        if (name.size() < 4 || name.compare(name.size() - 3, 3, ".co") != 0 || name[0] == '.') continue;
+        struct stat st {};
+        if (::stat((dir + "/" + name).c_str(), &st) == 0) files.emplace_back((long long)st.st_mtime, name);
+    }
+    ::closedir(d);
+    if (files.size() <= kDiskCacheFiles) return;
+    std::sort(files.begin(), files.end());
+    for (size_t i = 0; i + kDiskCacheFiles < files.size(); ++i) ::unlink((dir + "/" + files[i].second).c_str());
+}
+
+// The compiler's version, part of every disk-cache file name: code objects
+// of another hiprtc are never loaded.
+std::string rtc_tag() {
+    int major = 0, minor = 0;
+    if (hiprtcVersion(&major, &minor) != HIPRTC_SUCCESS) return "rtc-unknown";
+    return "rtc" + std::to_string(major) + "." + std::to_string(minor);
+}
+
 std::string compile_one(const std::string& src, const std::string& fname, std::vector<char>& code) {
     hiprtcProgram prog;
     hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), fname.c_str(), 0, nullptr, nullptr);
@@ -1338,6 +1368,10 @@ std::string compile_one(const std::string& src, const std::string& fname, std::v
 
 std::mutex g_mu;
 std::map<std::string, std::shared_ptr<JitProgram>> g_cache;
+// Insertion order of g_cache: beyond kProgramCacheEntries programs the oldest
+// leaves both maps (a plan that still holds it keeps it alive).
+std::deque<std::string> g_cache_order;
+constexpr size_t kProgramCacheEntries = 256;
 // Programs by plan fingerprint: a plan rebuilt for every call (the gradient
 // engine's) finds its program without generating and hashing the source.
 std::map<std::string, std::shared_ptr<JitProgram>> g_fp_cache;
@@ -1502,9 +1536,10 @@ std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, st
         srcs[c] += prog->bodies[i];
     }
     const std::string dir = cache_dir();
+    const std::string file_key = key + "_" + rtc_tag();
     bool cached = !dir.empty() && nchunk > 0;
     for (size_t c = 0; c < nchunk && cached; ++c)
-        cached = read_file(dir + "/" + key + "_" + std::to_string(c) + ".co", prog->codes[c]);
+        cached = read_file(dir + "/" + file_key + "_" + std::to_string(c) + ".co", prog->codes[c]);
     if (cached) {
         prog->from_disk = true;
     } else if (nchunk > 0) {
@@ -1523,11 +1558,25 @@ std::shared_ptr<JitProgram> jit_compile_plan(const Plan& plan, int precision, st
                 if (err) *err = errs[c];
                 return nullptr;
             }
-        for (size_t c = 0; c < nchunk; ++c) write_file_atomic(dir, key + "_" + std::to_string(c) + ".co", prog->codes[c]);
+        for (size_t c = 0; c < nchunk; ++c)
+            write_file_atomic(dir, file_key + "_" + std::to_string(c) + ".co", prog->codes[c]);
+        if (!dir.empty()) prune_disk_cache(dir);
     }
     prog->compile_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::lock_guard<std::mutex> lk(g_mu);
     auto ins = g_cache.emplace(key, prog);
+    if (ins.second) {
+        g_cache_order.push_back(key);
+        while (g_cache_order.size() > kProgramCacheEntries) {
+            auto old = g_cache.find(g_cache_order.front());
+            if (old != g_cache.end()) {
+                for (auto f = g_fp_cache.begin(); f != g_fp_cache.end();)
+                    f = f->second == old->second ? g_fp_cache.erase(f) : std::next(f);
+                g_cache.erase(old);
+            }
+            g_cache_order.pop_front();
+        }
+    }
     g_fp_cache.emplace(fp, ins.first->second);
     return ins.first->second;
 }

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -395,7 +396,16 @@ qf_status upload_plans(qf_backend* be, const std::vector<const qf::Plan*>& plans
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_jit_mu;
-std::map<std::pair<std::string, int>, std::vector<hipFunction_t>> g_jit_fns;
+// Loaded programs per (program key, device).  Beyond kLoadedPrograms the
+// oldest is unloaded (after the device drains, since a launch may still run
+// it); a plan that needs it again reloads its code objects.
+struct Loaded {
+    std::vector<hipModule_t> mods;
+    std::vector<hipFunction_t> fns;
+};
+std::map<std::pair<std::string, int>, Loaded> g_jit_fns;
+std::deque<std::pair<std::string, int>> g_jit_order;
+constexpr size_t kLoadedPrograms = 256;
 }  // namespace
 
 // The plan's compiled program for `precision` (compiled on first use), or
@@ -431,14 +441,38 @@ qf_status jit_functions(const qf::JitProgram& prog, const std::vector<hipFunctio
     const auto key = std::make_pair(prog.key, dev);
     auto it = g_jit_fns.find(key);
     if (it == g_jit_fns.end()) {
-        std::vector<hipModule_t> mods(prog.codes.size(), nullptr);
-        for (size_t c = 0; c < prog.codes.size(); ++c) HIP_TRY(hipModuleLoadData(&mods[c], prog.codes[c].data()));
-        std::vector<hipFunction_t> fns(prog.names.size(), nullptr);
-        for (size_t i = 0; i < prog.names.size(); ++i)
-            HIP_TRY(hipModuleGetFunction(&fns[i], mods[prog.code_of_kernel[i]], prog.names[i].c_str()));
-        it = g_jit_fns.emplace(key, std::move(fns)).first;
+        if (g_jit_order.size() >= kLoadedPrograms) {
+            HIP_TRY(hipDeviceSynchronize());
+            while (g_jit_order.size() >= kLoadedPrograms) {
+                auto old = g_jit_fns.find(g_jit_order.front());
+                if (old != g_jit_fns.end()) {
+                    for (hipModule_t m : old->second.mods) (void)hipModuleUnload(m);
+                    g_jit_fns.erase(old);
+                }
+                g_jit_order.pop_front();
+            }
+        }
+        Loaded L;
+        L.mods.assign(prog.codes.size(), nullptr);
+        for (size_t c = 0; c < prog.codes.size(); ++c) {
+            hipError_t e = hipModuleLoadData(&L.mods[c], prog.codes[c].data());
+            if (e != hipSuccess) {
+                for (size_t k = 0; k < c; ++k) (void)hipModuleUnload(L.mods[k]);
+                return hip_fail(e, "hipModuleLoadData");
+            }
+        }
+        L.fns.assign(prog.names.size(), nullptr);
+        for (size_t i = 0; i < prog.names.size(); ++i) {
+            hipError_t e = hipModuleGetFunction(&L.fns[i], L.mods[prog.code_of_kernel[i]], prog.names[i].c_str());
+            if (e != hipSuccess) {
+                for (hipModule_t m : L.mods) (void)hipModuleUnload(m);
+                return hip_fail(e, "hipModuleGetFunction");
+            }
+        }
+        it = g_jit_fns.emplace(key, std::move(L)).first;
+        g_jit_order.push_back(key);
     }
-    *out = &it->second;
+    *out = &it->second.fns;
     return QF_OK;
 }
 
