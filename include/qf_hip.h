@@ -443,8 +443,8 @@ typedef struct {
     uint32_t graph_nodes;          /* kernel launches in the captured per-wave graph */
     uint32_t graph_replays;        /* replays of that graph (one per wave) */
     uint32_t grouped_instances;    /* instances run through the grouped graph */
-    uint32_t tile_qubits, low_qubits, reg_bits;  /* autotuned forward tile shape */
-    uint32_t reserved;
+    uint32_t tile_qubits, low_qubits, reg_bits;  /* autotuned forward tile shape (fusion width = tile_qubits) */
+    uint32_t reverse_tile_qubits;                /* autotuned reverse-sweep tile (two states per tile) */
     double tune_ms;                /* tuned forward time of the widest instance */
     double alg_bytes;              /* algorithmic HBM bytes of the grouped instances' passes: per instance
                                       forward passes x 2 x 2^n x s, costate 2 x 2^n x s, reverse passes

@@ -254,9 +254,17 @@ std::map<std::string, qfrt::TuneResult>& tune_cache() {
     return c;
 }
 
+std::map<std::string, uint32_t>& reverse_tile_cache() {
+    static std::map<std::string, uint32_t> c;
+    return c;
+}
+
 // Times a plan (built from `gates`) on a scratch state; returns ms per run.
+// two_state: the reverse sweep's form (psi, the costate and the overlap
+// partial table).
 qf_status time_plan(qf_backend* be, const std::vector<qf::GateIn>& gates, uint32_t qubits, int precision,
-                    const qf::PlanOptions& po, const std::vector<double>& row, uint32_t nslots, double* ms) {
+                    const qf::PlanOptions& po, const std::vector<double>& row, uint32_t nslots, double* ms,
+                    bool two_state = false) {
     qf::Plan plan;
     std::string e = qf::build_plan(gates, qubits, precision, po, plan);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
@@ -271,6 +279,17 @@ qf_status time_plan(qf_backend* be, const std::vector<qf::GateIn>& gates, uint32
         (void)hipFree(d);
         return s;
     }
+    void* lam = nullptr;
+    void* part = nullptr;
+    const uint32_t M = plan.passes.empty() ? qubits : plan.passes[0].tile_bits;
+    const size_t part_bytes = (size_t(1) << (qubits - std::min(M, qubits))) * std::max<uint32_t>(nslots, 1) * 8;
+    if (two_state && ((s = device_alloc(&lam, (size_t(1) << qubits) * amp_bytes(precision))) ||
+                      (s = device_alloc(&part, part_bytes)))) {
+        if (lam) (void)hipFree(lam);
+        (void)hipFree(d);
+        (void)hipFree(cs);
+        return s;
+    }
     std::vector<const qf::TileOp*> bases;
     hipEvent_t a = nullptr, b = nullptr;
     auto cleanup = [&]() {
@@ -279,6 +298,8 @@ qf_status time_plan(qf_backend* be, const std::vector<qf::GateIn>& gates, uint32
         if (b) (void)hipEventDestroy(b);
         (void)hipFree(d);
         (void)hipFree(cs);
+        if (lam) (void)hipFree(lam);
+        if (part) (void)hipFree(part);
     };
     if (!row.empty() && hipMemcpy(cs, row.data(), row.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         cleanup();
@@ -291,6 +312,8 @@ qf_status time_plan(qf_backend* be, const std::vector<qf::GateIn>& gates, uint32
     LaunchCtx ctx;
     ctx.cs = static_cast<const double*>(cs);
     ctx.nslots = nslots;
+    ctx.lam = lam;
+    ctx.partial = static_cast<double*>(part);
     // The state content does not matter for timing; one warm run, then two timed runs.
     if ((s = launch_plan(be, d, precision, qubits, 1, plan, bases[0], ctx))) {
         cleanup();
@@ -528,7 +551,44 @@ qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t co
     S.low_qubits = tune.low;
     S.reg_bits = tune.reg;
     S.tune_ms = tune.ms;
-    const uint32_t Mf = tune.tile, RBf = tune.reg, Mr = tune.tile ? tune.tile - 1 : 0, RBr = 3;
+    const uint32_t Mf = tune.tile, RBf = tune.reg, RBr = 3;
+    // The reverse sweep's tile (two states in LDS): the forward tile or one
+    // qubit less, timed on the widest instance.
+    uint32_t Mr = Mf ? Mf - 1 : 0;
+    if (want_grad && Mf) {
+        const PCirc& pc = pcs[widest];
+        std::vector<double> row(std::max<size_t>(pc.slot_param.size() * 2, 2));
+        fill_row(pc, inst[widest].params, -1, 0.0, row.data());
+        const std::string key = "r:" + std::to_string(inst[widest].qubits) + ":" + std::to_string(precision) + ":" +
+                                std::to_string(inst[widest].nops) + ":" + std::to_string(Mf);
+        auto it = reverse_tile_cache().find(key);
+        if (it != reverse_tile_cache().end()) {
+            Mr = it->second;
+        } else {
+            double best = -1.0;
+            for (uint32_t cand : {Mf - 1, Mf}) {
+                if (cand < RBr + 6 || cand - RBr > 8 || cand > inst[widest].qubits ||
+                    2 * (size_t(1) << cand) * amp_bytes(precision) > 64 * 1024)
+                    continue;
+                qf::PlanOptions po;
+                po.fuse = 1;
+                po.kernel = 2;
+                po.tile_qubits = cand;
+                po.low_qubits = tune.low;
+                po.reg_bits = RBr;
+                double ms = 0.0;
+                if ((s = time_plan(be, param_gates(pc, nullptr, true), inst[widest].qubits, precision, po, row,
+                                   uint32_t(pc.slot_param.size()), &ms, true)))
+                    return s;
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    Mr = cand;
+                }
+            }
+            reverse_tile_cache()[key] = Mr;
+        }
+    }
+    S.reverse_tile_qubits = Mr;
 
     // 3. plans (on host threads: one forward and one reverse plan per
     // instance); instances too narrow for the shared register tile run on
