@@ -105,7 +105,7 @@ public:
         }
         const uint32_t W = NT_ >= 64 ? NT_ / 64 : 1;
         const char* R = dbl_ ? "double" : "float";
-        o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << ") void " << name_
+        o_ << "extern \"C\" __global__ __launch_bounds__(" << NT_ << "@LB@) void " << name_
            << "(C* __restrict__ s, const double* __restrict__ cs, unsigned nslots"
            << (ns_ == 2 ? ", C* __restrict__ lam, double* __restrict__ partial" : "") << ") {\n";
         relabel_ = !p_.store_phys.empty();
@@ -185,6 +185,7 @@ public:
             o_ << "  }\n";
         }
         cur_ = 0;
+        flush_records();
         if (!rec_slots.empty()) {
             // The tile's overlap for each recorded rotation: its waves' sums
             // added in wave order, into the block's row of the partial table.
@@ -200,6 +201,15 @@ public:
         }
         o_ << "}\n";
         src = o_.str();
+        // 512-thread workgroups whose shared arrays let two of them share a
+        // CU (160 KiB of LDS) ask for four waves per SIMD, i.e. at most 128
+        // VGPRs: the two-state passes otherwise land just above and halve
+        // their resident waves.
+        const size_t lds = size_t(phases > 1 || relabel_ ? ns_ * (1u << (half_ ? M_ - 1 : M_)) : 0) *
+                               (dbl_ ? 16 : 8) +
+                           rec_slots.size() * W * 8;
+        const std::string lb = (NT_ == 512 && 2 * lds <= 160 * 1024) ? ", 4" : "";
+        src.replace(src.find("@LB@"), 4, lb);
         return true;
     }
 
@@ -252,6 +262,13 @@ private:
         bool on = false, real = true;
     };
     std::vector<Acc> acc_;
+    // Recorded overlaps waiting for their wave sums: a thread's share of
+    // records rec0_, rec0_ + 1, ... (variable names), summed over the wave
+    // together once kRecBatch of them are pending.
+    static constexpr size_t kRecBatch = 4;
+    std::vector<std::string> pend_;
+    uint32_t rec0_ = 0;
+    uint32_t recw_ = 1;  // waves per workgroup (rows of the shared table)
 
     static bool fail(std::string* err, const std::string& m) {
         if (err) *err = m;
@@ -1160,8 +1177,70 @@ private:
         if (acc_[0].on)
             o_ << "  " << v << " *= double(" << acc_[0].r << ") * double(" << acc_[0].r << ")"
                << (acc_[0].real ? "" : " + double(" + acc_[0].i + ") * double(" + acc_[0].i + ")") << ";\n";
-        o_ << "  " << v << " = qf_wave_sum(" << v << ");\n";
-        o_ << "  if ((t & 63u) == 63u) qred[" << r * W << "u + (t >> 6)] = " << v << ";\n";
+        // Pin the share here: otherwise the compiler sinks its arithmetic to
+        // the batch's wave sum and keeps the pre-rotation amplitudes alive
+        // until then (twice the registers).
+        o_ << "  QF_PIN(" << v << ");\n";
+        if (pend_.empty()) rec0_ = r;
+        recw_ = W;
+        pend_.push_back(v);
+        if (pend_.size() == std::min<size_t>(kRecBatch, std::min<uint32_t>(NT_, 64))) flush_records();
+    }
+
+    // The wave sums of the pending records, all at once.  Transpose steps on
+    // lane bits 0 and 1 (each lane keeps one half of the list and adds its
+    // partner's copy of that half) leave lane l with record l & (B - 1) of
+    // the batch summed over its quad; adding the partners
+    // across the higher lane bits (row rotations by 4 and 8 lanes, swizzles
+    // across 16 and a permute across 32) completes every record's sum in
+    // every lane of its class, and lanes 0..B-1 write them to the shared
+    // table (row = record, column = wave).  Lane bits the workgroup does not
+    // have are skipped; the order of the additions is fixed.
+    void flush_records() {
+        if (pend_.empty()) return;
+        const size_t n = pend_.size();
+        size_t B = 1;
+        while (B < n) B <<= 1;
+        const uint32_t lanes = std::min<uint32_t>(NT_, 64);
+        const std::string p = "rb" + std::to_string(rec0_);
+        std::vector<std::string> x;
+        o_ << "  {\n    const unsigned ln = t & 63u;\n";
+        for (size_t k = 0; k < B; ++k) {
+            x.push_back(p + "_" + std::to_string(k));
+            o_ << "    double " << x.back() << " = " << (k < n ? pend_[k] : std::string("0.0")) << ";\n";
+        }
+        uint32_t j = 0;
+        for (; (size_t(1) << j) < B; ++j) {
+            const char* ctrl = j == 0 ? "0xB1" : "0x4E";  // lane ^ 1, lane ^ 2
+            const std::string h = p + "h" + std::to_string(j);
+            o_ << "    const bool " << h << " = (ln >> " << j << ") & 1u;\n";
+            std::vector<std::string> y;
+            for (size_t k = 0; k + 1 < x.size(); k += 2) {
+                y.push_back(p + "_s" + std::to_string(j) + "_" + std::to_string(k / 2));
+                o_ << "    const double " << y.back() << " = (" << h << " ? " << x[k + 1] << " : " << x[k]
+                   << ") + qf_dpp<" << ctrl << ", 0xf>(" << h << " ? " << x[k] << " : " << x[k + 1] << ");\n";
+            }
+            x = y;
+        }
+        std::string v = x[0];
+        for (; (1u << j) < lanes; ++j) {
+            const uint32_t m = 1u << j;
+            const std::string w = p + "_x" + std::to_string(m);
+            std::string partner;
+            if (m == 1) partner = "qf_dpp<0xB1, 0xf>(" + v + ")";
+            else if (m == 2) partner = "qf_dpp<0x4E, 0xf>(" + v + ")";
+            else if (m < 16 && lanes >= 16) partner = "qf_dpp<" + std::to_string(0x120 + m) + ", 0xf>(" + v + ")";
+            else if (m < 32) partner = "qf_swz<" + std::to_string((m << 10) | 0x1F) + ">(" + v + ")";
+            else partner = "qf_bperm32(" + v + ", ln)";
+            // (Row rotations by 4 then 8 add the four lanes of a class in a row.)
+            o_ << "    const double " << w << " = " << v << " + " << partner << ";\n";
+            v = w;
+        }
+        // Lane l (l < B) holds record l of the batch.
+        const std::string rec = B == 1 ? std::string("0u") : "(ln & " + std::to_string(B - 1) + "u)";
+        o_ << "    if (ln < " << B << "u && " << rec << " < " << n << "u) qred[(" << rec0_ << "u + " << rec << ") * "
+           << recw_ << "u + (t >> 6)] = " << v << ";\n  }\n";
+        pend_.clear();
     }
 
     void rot(const TileOp& op) {
@@ -1185,23 +1264,31 @@ private:
             acc_mul(acc_[0], c, sz);
             return;
         }
-        // The larger of |c|, |s| goes into the thread's scale accumulator and
-        // the matrix keeps a unit pair and the ratio: one FMA per output
-        // component.  Which form applies is one branch per rotation, uniform
-        // over the workgroup (one instance's angles).
+        // Direct form: [[c, -s], [s, c]] (Ry) and [[c, -is], [-is, c]] (Rx)
+        // with the half-angle values from the table, a multiply and an FMA
+        // per output component, no division and no branch.
         const uint32_t i = uint32_t(op.q0);
         acc_flush_bit(i);
-        const std::string w = "w" + std::to_string(opn_), f = "f" + std::to_string(opn_),
-                          u = "u" + std::to_string(opn_);
-        o_ << "  const bool " << w << " = __builtin_fabs(" << c << ") >= __builtin_fabs(" << s << ");\n";
-        o_ << "  const R " << f << " = " << w << " ? " << c << " : " << s << ";\n";
-        o_ << "  const R " << u << " = " << w << " ? " << s << " / " << c << " : " << c << " / " << s << ";\n";
-        acc_mul(acc_[0], f, "");
-        o_ << "  if (" << w << ") {\n";
-        each_state([&] { rot_xy_regs(op, i, u, true); });
-        o_ << "  } else {\n";
-        each_state([&] { rot_xy_regs(op, i, u, false); });
-        o_ << "  }\n";
+        each_state([&] { rot_xy_direct(op, i, c, s); });
+    }
+    void rot_xy_direct(const TileOp& op, uint32_t i, const std::string& c, const std::string& s) {
+        for (uint32_t k = 0; k < NR_; ++k) {
+            if ((k >> i) & 1) continue;
+            const uint32_t k1 = k | (1u << i);
+            const std::string rA = rv(k), iA = iv(k), rB = rv(k1), iB = iv(k1);
+            o_ << "  { const R a0r = " << rA << ", a0i = " << iA << ", a1r = " << rB << ", a1i = " << iB << ";\n";
+            if (op.q1 == 0) {
+                o_ << "    " << rA << " = " << c << " * a0r + " << s << " * a1i; " << iA << " = " << c << " * a0i - " << s
+                   << " * a1r;\n";
+                o_ << "    " << rB << " = " << c << " * a1r + " << s << " * a0i; " << iB << " = " << c << " * a1i - " << s
+                   << " * a0r; }\n";
+            } else {
+                o_ << "    " << rA << " = " << c << " * a0r - " << s << " * a1r; " << iA << " = " << c << " * a0i - " << s
+                   << " * a1i;\n";
+                o_ << "    " << rB << " = " << s << " * a0r + " << c << " * a1r; " << iB << " = " << s << " * a0i + " << c
+                   << " * a1i; }\n";
+            }
+        }
     }
     // Rx = c [[1, -it], [-it, 1]] = s [[u, -i], [-i, u]] and
     // Ry = c [[1, -t], [t, 1]] = s [[u, -1], [1, u]] (t = s / c, u = c / s),
@@ -1473,6 +1560,20 @@ std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, s
             "  const long long b = __builtin_bit_cast(long long, v);\n"
             "  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xf, false);\n"
             "  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xf, false);\n"
+            "  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));\n"
+            "}\n"
+            "#define QF_PIN(v) asm volatile(\"\" : \"+v\"(v))\n"
+            "template <int PAT> __device__ inline double qf_swz(double v) {\n"
+            "  const long long b = __builtin_bit_cast(long long, v);\n"
+            "  const int lo = __builtin_amdgcn_ds_swizzle(int(b), PAT);\n"
+            "  const int hi = __builtin_amdgcn_ds_swizzle(int(b >> 32), PAT);\n"
+            "  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));\n"
+            "}\n"
+            "__device__ inline double qf_bperm32(double v, unsigned ln) {\n"
+            "  const long long b = __builtin_bit_cast(long long, v);\n"
+            "  const int a = int((ln ^ 32u) << 2);\n"
+            "  const int lo = __builtin_amdgcn_ds_bpermute(a, int(b));\n"
+            "  const int hi = __builtin_amdgcn_ds_bpermute(a, int(b >> 32));\n"
             "  return __builtin_bit_cast(double, (long long)(unsigned)lo | ((long long)hi << 32));\n"
             "}\n"
             "__device__ inline double qf_wave_sum(double v) {\n"

@@ -58,6 +58,33 @@ static inline double qf_wave_sum(double v) {
 }
 """
 
+# Lane exchanges of the batched record sums: DPP quad permutes and row
+# rotations (0x121..0x12f: lane i of a row reads lane i + n mod 16; the code
+# only uses rotations whose sums do not depend on the direction), swizzles
+# across lanes (bit mode: lane ^ xor mask within 32) and a permute across 32.
+PRELUDE += r"""
+static inline double qf_lane_read(double v, unsigned src) {
+    qf_shfl[qf_tid.x] = v;
+    qf_bar->arrive_and_wait();
+    const double r = qf_shfl[(qf_tid.x & ~63u) | src];
+    qf_bar->arrive_and_wait();
+    return r;
+}
+template <int CTRL, int ROWS> static inline double qf_dpp(double v) {
+    const unsigned ln = qf_tid.x & 63u;
+    unsigned src = ln;
+    if (CTRL < 0x100) src = (ln & ~3u) | ((unsigned(CTRL) >> (2 * (ln & 3u))) & 3u);
+    else if (CTRL > 0x120 && CTRL < 0x130) src = (ln & ~15u) | ((ln + unsigned(CTRL - 0x120)) & 15u);
+    return qf_lane_read(v, src);
+}
+template <int PAT> static inline double qf_swz(double v) {
+    const unsigned ln = qf_tid.x & 63u;
+    return qf_lane_read(v, (ln & ~31u) | ((ln & 31u) ^ ((unsigned(PAT) >> 10) & 31u)));
+}
+static inline double qf_bperm32(double v, unsigned ln) { return qf_lane_read(v, ln ^ 32u); }
+#define QF_PIN(v) ((void)0)
+"""
+
 LAUNCHER = r"""
 typedef void (*qf_kfn)(C*, const double*, unsigned);
 typedef void (*qf_kfn2)(C*, const double*, unsigned, C*, double*);
