@@ -450,13 +450,21 @@ typedef struct {
                                       forward passes x 2 x 2^n x s, costate 2 x 2^n x s, reverse passes
                                       x 4 x 2^n x s (s = 16 complex128, 8 complex64) */
     double device_ms;              /* device time of the graph replays (HIP events), summed over waves */
+    uint32_t qaoa_instances;       /* instances in QAOA form, run on the QAOA tile engine (12 <= n <= 30) */
+    uint32_t qaoa_forward_passes;  /* most forward passes of a QAOA-form instance */
 } qf_uneven_stats;
 
 /* Energies <obs_i> and adjoint gradients of `count` parameterized circuits of
  * different widths.  grads is the concatenation of every instance's [nparams].
  * Each instance is validated like qf_gradient.  stats may be NULL.  Instances
  * are packed into device-memory waves of at most 80% of the free memory, or of
- * the environment variable QF_UNEVEN_WAVE_BYTES bytes when that is smaller. */
+ * the environment variable QF_UNEVEN_WAVE_BYTES bytes when that is smaller.
+ * Instances in QAOA form (12 to 30 qubits; a Hadamard on every qubit, then
+ * layers of cost rotations CNOT(u, v) Rz(v) CNOT(u, v) followed by an X
+ * rotation on every qubit, every rotation one parameter occurrence; an
+ * observable of real Z and Z Z words and a constant) run on a tile engine of
+ * their own with the graph as run-time data; QF_QAOA_ENGINE=0 sends them
+ * through the general engine. */
 qf_status qf_uneven_adjoint_gradient(qf_backend* backend, const qf_uneven_instance* instances, uint32_t count,
                                      int32_t precision, double* energies, double* grads, qf_uneven_stats* stats);
 

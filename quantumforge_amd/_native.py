@@ -72,7 +72,8 @@ class UnevenStats(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("graph_nodes", C.c_uint32), ("graph_replays", C.c_uint32),
                 ("grouped_instances", C.c_uint32), ("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32),
                 ("reg_bits", C.c_uint32), ("reverse_tile_qubits", C.c_uint32), ("tune_ms", C.c_double),
-                ("alg_bytes", C.c_double), ("device_ms", C.c_double)]
+                ("alg_bytes", C.c_double), ("device_ms", C.c_double), ("qaoa_instances", C.c_uint32),
+                ("qaoa_forward_passes", C.c_uint32)]
 
 
 P = C.c_void_p

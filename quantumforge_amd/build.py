@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqfhip.so")
 ARCH = os.environ.get("QF_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_plan.cpp", "qf_jit.cpp", "qf_shots.hip",
+SOURCES = ["qf_runtime.hip", "qf_gradients.hip", "qf_uneven.hip", "qf_qaoa.hip", "qf_plan.cpp", "qf_jit.cpp", "qf_shots.hip",
            "qf_density.hip", "qf_lanczos.hip"]
 
 HIPCC_FLAGS = [

@@ -23,6 +23,7 @@
 #include <thread>
 #include <vector>
 
+#include "qf_qaoa.h"
 #include "qf_regtile.h"
 #include "qf_runtime.h"
 
@@ -527,13 +528,33 @@ qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t co
         I[k].terms = sorted_terms(t);
         I[k].grad_off = goff;
         goff += q.nparams;
-        if (q.qubits > inst[widest].qubits) widest = k;
     }
     if (want_grad && goff && !grads) return fail(QF_INVALID_ARGUMENT, "null gradient output");
     if ((s = activate(be))) return s;
     void* sp = nullptr;
     qf_backend_stream(be, &sp);
     hipStream_t st = static_cast<hipStream_t>(sp);
+
+    // QAOA-form instances on their own engine; the rest below.
+    std::vector<char> general(count, 1);
+    {
+        std::vector<qfq::Form> forms(count);
+        std::vector<uint32_t> qids;
+        std::vector<uint64_t> goffs(count);
+        const char* sw = std::getenv("QF_QAOA_ENGINE");
+        const bool on = !(sw && std::strcmp(sw, "0") == 0);
+        for (uint32_t k = 0; k < count; ++k) {
+            goffs[k] = I[k].grad_off;
+            if (on && qfq::match(inst[k], pcs[k], forms[k])) {
+                qids.push_back(k);
+                general[k] = 0;
+            }
+        }
+        if ((s = qfq::run(be, inst, qids, pcs, forms, goffs, precision, want_grad, energies, grads, S))) return s;
+        if (qids.size() == count) return QF_OK;
+    }
+    for (uint32_t k = 0; k < count; ++k)
+        if (general[k] && (!general[widest] || inst[k].qubits > inst[widest].qubits)) widest = k;
 
     // 2. autotune the forward tile shape on the widest instance
     qfrt::TuneResult tune;
@@ -595,6 +616,7 @@ qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t co
     // their own
     std::vector<uint32_t> grouped, single;
     for (uint32_t k = 0; k < count; ++k) {
+        if (!general[k]) continue;
         if (Mf == 0 || inst[k].qubits < Mf) single.push_back(k);
         else grouped.push_back(k);
     }
@@ -739,7 +761,7 @@ qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t co
             wave_bytes.push_back(f);
         }
     }
-    S.waves = uint32_t(waves.size());
+    S.waves += uint32_t(waves.size());
 
     // 6. wave buffers sized for the largest wave
     uint64_t max_amps = 0, max_part = 0, max_epart = 0;
@@ -789,7 +811,7 @@ qf_status uneven_run(qf_backend* be, const qf_uneven_instance* inst, uint32_t co
                                       static_cast<double*>(d_grads.p), static_cast<const uint64_t*>(B.total_amps),
                                       &exec, &nodes);
     if (s) return s;
-    S.graph_nodes = nodes;
+    S.graph_nodes = std::max(S.graph_nodes, nodes);
 
     for (uint32_t k : grouped) {
         const double sweep = 2.0 * double(uint64_t(1) << inst[k].qubits) * double(ab);

@@ -223,8 +223,9 @@ def test_torch_autograd_vqe_energy(backend):
         assert np.max(np.abs(params.grad.numpy()[s] - want)) < 1e-10
 
 
-def test_uneven_qaoa_batch_vs_oracle(backend):
+def test_uneven_qaoa_batch_vs_oracle(backend, monkeypatch):
     from quantumforge_amd.workloads import qaoa_occurrence_angles, qaoa_parameterized, qaoa_uneven_instances
+    monkeypatch.setenv("QF_QAOA_ENGINE", "0")  # the general engine (the QAOA engine has its own tests)
     specs = qaoa_uneven_instances(7, seed=11, p=2, qubit_choices=(8, 12, 14, 16))
     instances, meta = [], []
     for s in specs:
@@ -279,6 +280,7 @@ def test_uneven_qaoa_multi_wave_at_config_shape(backend, monkeypatch):
         instances.append((pc, prob.num_qubits, ang, maxcut_cost_hamiltonian(prob)))
     # One 18-qubit instance needs 2 x 2^18 x 16 bytes of psi and lambda plus its partials.
     monkeypatch.setenv("QF_UNEVEN_WAVE_BYTES", str(12 << 20))
+    monkeypatch.setenv("QF_QAOA_ENGINE", "0")  # the general engine (the QAOA engine has its own tests)
     e, g, stats = G.uneven_adjoint_gradient(backend, instances)
     print(f"[uneven multi-wave] stats {stats}", flush=True)
     assert stats["waves"] >= 3 and stats["graph_replays"] == stats["waves"]
