@@ -291,6 +291,9 @@ def uneven_bench(be, count: int, seed: int, check_instances: int) -> dict:
     t0 = time.perf_counter()
     e, g, stats = G.uneven_adjoint_gradient(be, insts)
     dt = time.perf_counter() - t0
+    # The engine keeps its device arena for the next call (a caching
+    # allocator); the rest of the bench gets the memory back.
+    be.release_cached()
     occ = sum(pc.parameter_count for pc, _, _, _ in insts)
     log(f"uneven: {count} instances, {occ} rotation occurrences, {dt:.2f} s, stats {stats}")
     dev_s = stats["device_ms"] * 1e-3

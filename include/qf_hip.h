@@ -218,6 +218,11 @@ qf_status qf_backend_create(int32_t ordinal, qf_backend** out);
 qf_status qf_backend_destroy(qf_backend* backend);
 /* cuCtxSynchronize, src/cuda.rs:1983; reports deferred device errors. */
 qf_status qf_backend_synchronize(qf_backend* backend);
+/* No reference counterpart: frees the device arena the uneven-batch engine
+ * keeps between calls (its psi, lambda and partial buffers), as a caching
+ * allocator's trim.  A device allocation that fails releases every backend's
+ * arena and tries once more by itself. */
+qf_status qf_backend_release_cached(qf_backend* backend);
 /* CudaBackend::device_ordinal. */
 qf_status qf_backend_device_ordinal(const qf_backend* backend, int32_t* out);
 /* Planner options used by qf_state_run_circuit when no plan is passed. */

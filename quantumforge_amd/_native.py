@@ -87,6 +87,7 @@ SIGNATURES = {
     "qf_backend_create": (None, [i32, PP]),
     "qf_backend_destroy": (None, [P]),
     "qf_backend_synchronize": (None, [P]),
+    "qf_backend_release_cached": (None, [P]),
     "qf_backend_device_ordinal": (None, [P, C.POINTER(i32)]),
     "qf_backend_set_plan_options": (None, [P, C.POINTER(PlanOptions)]),
     "qf_backend_stream": (None, [P, PP]),

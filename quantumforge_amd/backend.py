@@ -150,6 +150,11 @@ class HipBackend:
     def synchronize(self) -> None:
         check(lib().qf_backend_synchronize(self._h))
 
+    def release_cached(self) -> None:
+        """Frees the device arena the uneven-batch engine keeps between calls
+        (qf_backend_release_cached)."""
+        check(lib().qf_backend_release_cached(self._h))
+
     def set_profiling(self, on: bool) -> None:
         """HIP-event timing of each phase of batched gradient calls."""
         check(lib().qf_backend_set_profiling(self._h, int(bool(on))))

@@ -435,15 +435,24 @@ __device__ __forceinline__ uint32_t find_entry(const E* __restrict__ e, uint32_t
     return lo;
 }
 
-// One pass of every instance in the launch table: forward (NS == 1) or its
-// reverse (NS == 2: psi and lambda, overlap records).
-template <class R, int NS>
-__global__ __launch_bounds__(kQT, 2) void k_qaoa_pass(cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+// Pass shapes: the schedule makes every pass one of three (the cost layer
+// after sweep A in phase 3, or in phase 0 without it).
+enum Shape : int { SH_DB = 0, SH_ADB = 1, SH_B = 2 };
+constexpr int kShapes = 3;
+
+// One pass of shape SH of every instance in the launch table: forward
+// (NS == 1; COST: the instance's last pass, with the costate and the energy)
+// or its reverse (NS == 2: psi and lambda, overlap records).  Each shape is
+// its own straight-line kernel; the reverse form is held to 128 VGPRs, so two
+// workgroups (16 waves) share a CU.
+template <class R, int NS, int SH, bool COST>
+__global__ __launch_bounds__(kQT, NS == 2 ? 4 : 2) void k_qaoa_pass(cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
                                                       const unsigned char* __restrict__ launch,
                                                       const QDesc* __restrict__ descs, const QTerm* __restrict__ terms,
                                                       const double* __restrict__ cs,
                                                       const double* __restrict__ coef, double* __restrict__ partial,
                                                       dd* __restrict__ epart) {
+    constexpr bool A = SH == SH_ADB, D = SH != SH_B;
     __shared__ cx<R> xb[1 << kQM];
     __shared__ double qred[NS == 2 ? kRows * kQW : 1];
     __shared__ dd ered[kQW];
@@ -457,172 +466,144 @@ __global__ __launch_bounds__(kQT, 2) void k_qaoa_pass(cx<R>* __restrict__ psi, c
     // One workgroup per tile; the grid covers the largest wave's launch and
     // the workgroups past this wave's tiles leave at once.
     const uint64_t b = blockIdx.x;
-    if (b < total) {
-        const QEntry& en = E[find_entry(E, ne, b)];
-        const QDesc& d = descs[en.desc];
-        const uint32_t lb = uint32_t(b - en.block_start);
-        const uint32_t fl = d.flags;
-        uint32_t bb = 0;
-        for (uint32_t i = 0; i < d.nout; ++i) bb |= ((lb >> i) & 1u) << d.outpos[i];
-        const uint32_t g0 = bb | thread_bits<0>(d, t);
-        const double* csi = cs + d.cs_off;
-        if (t < 24) {
-            const uint32_t sl = t < 12 ? d.slotA[t] : d.slotB[t - 12];
-            rxc[t] = make_double2(csi[2 * sl], csi[2 * sl + 1]);
-        }
-        __syncthreads();
-        Regs<R> s[NS];
-        cx<R>* sp = psi + d.state_off;
-        cx<R>* lp = lam + d.state_off;
-        if (NS == 2 || (fl & F_LOAD)) {
+    if (b >= total) return;
+    const QEntry& en = E[find_entry(E, ne, b)];
+    const QDesc& d = descs[en.desc];
+    const uint32_t lb = uint32_t(b - en.block_start);
+    const uint32_t fl = d.flags;
+    uint32_t bb = 0;
+    for (uint32_t i = 0; i < d.nout; ++i) bb |= ((lb >> i) & 1u) << d.outpos[i];
+    const uint32_t g0 = bb | thread_bits<0>(d, t);
+    const double* csi = cs + d.cs_off;
+    if (t < 24) {
+        const uint32_t sl = t < 12 ? d.slotA[t] : d.slotB[t - 12];
+        rxc[t] = make_double2(csi[2 * sl], csi[2 * sl + 1]);
+    }
+    __syncthreads();
+    Regs<R> s[NS];
+    cx<R>* sp = psi + d.state_off;
+    cx<R>* lp = lam + d.state_off;
+    if (NS == 2 || (fl & F_LOAD)) {
 #pragma unroll
-            for (int st = 0; st < NS; ++st) {
-                const cx<R>* src = (st ? lp : sp) + g0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const cx<R> v = src[k];
-                    s[st].re[k] = v.re;
-                    s[st].im[k] = v.im;
-                }
-            }
-        } else {
-            // |+>^n: the Hadamard layer on |0..0>.
-            const R a = R(1.0 / std::sqrt(double(1ull << d.n)));
+        for (int st = 0; st < NS; ++st) {
+            const cx<R>* src = (st ? lp : sp) + g0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                s[0].re[k] = a;
-                s[0].im[k] = R(0);
+                const cx<R> v = src[k];
+                s[st].re[k] = v.re;
+                s[st].im[k] = v.im;
             }
         }
-        if constexpr (NS == 1) {
-            int ph = 0;
-            if (fl & F_SWA) {
-                rx_group<R, 1, 0>(s, rxc, qred, 0, t);
-                exchange<R, 1, 0, 1>(s, xb, t);
-                rx_group<R, 1, 1>(s, rxc, qred, 0, t);
-                exchange<R, 1, 1, 2>(s, xb, t);
-                rx_group<R, 1, 2>(s, rxc, qred, 0, t);
-                exchange<R, 1, 2, 3>(s, xb, t);
-                rx_group<R, 1, 3>(s, rxc, qred, 0, t);
-                ph = 3;
-            }
-            if (fl & F_DIAG) {
-                if (ph == 3) diag_layer<R, 1, 3>(s, d, terms, csi, qred, bb, t);
-                else diag_layer<R, 1, 0>(s, d, terms, csi, qred, bb, t);
-            }
-            if (fl & F_SWB) {
-                if (ph == 3) {
-                    rx_group<R, 1, 3>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 3, 2>(s, xb, t);
-                    rx_group<R, 1, 2>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 2, 1>(s, xb, t);
-                    rx_group<R, 1, 1>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 1, 0>(s, xb, t);
-                    rx_group<R, 1, 0>(s, rxc, qred, 12, t);
-                } else {
-                    rx_group<R, 1, 0>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 0, 1>(s, xb, t);
-                    rx_group<R, 1, 1>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 1, 2>(s, xb, t);
-                    rx_group<R, 1, 2>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 2, 3>(s, xb, t);
-                    rx_group<R, 1, 3>(s, rxc, qred, 12, t);
-                    exchange<R, 1, 3, 0>(s, xb, t);
-                }
-                ph = 0;
-            }
-            if (ph == 3) exchange<R, 1, 3, 0>(s, xb, t);
-            if (fl & F_COST) {
-                Regs<R> l;
-                double en_t = costate<R>(s[0], l, d, terms, coef + d.coef_off, g0);
-                if (fl & F_LAMBDA) {
+    } else {
+        // |+>^n: the Hadamard layer on |0..0>.
+        const R a = R(1.0 / std::sqrt(double(1ull << d.n)));
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) lp[g0 + k] = cx<R>{l.re[k], l.im[k]};
-                }
-                // The tile's energy partial: wave sums, then the waves in order.
-                dd acc{en_t, 0.0};
-                for (int m = 32; m >= 1; m >>= 1) acc = qf::dd_add(acc, dd{__shfl_xor(acc.s, m, 64), __shfl_xor(acc.c, m, 64)});
-                if ((t & 63u) == 0) ered[t >> 6] = acc;
-                __syncthreads();
-                if (t == 0) {
-                    dd tot = ered[0];
-                    for (int w = 1; w < kQW; ++w) tot = qf::dd_add(tot, ered[w]);
-                    epart[d.epart_off + lb] = tot;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
+        for (int k = 0; k < 8; ++k) {
+            s[0].re[k] = a;
+            s[0].im[k] = R(0);
+        }
+    }
+    if constexpr (NS == 1) {
+        if constexpr (A) {
+            rx_group<R, 1, 0>(s, rxc, qred, 0, t);
+            exchange<R, 1, 0, 1>(s, xb, t);
+            rx_group<R, 1, 1>(s, rxc, qred, 0, t);
+            exchange<R, 1, 1, 2>(s, xb, t);
+            rx_group<R, 1, 2>(s, rxc, qred, 0, t);
+            exchange<R, 1, 2, 3>(s, xb, t);
+            rx_group<R, 1, 3>(s, rxc, qred, 0, t);
+            diag_layer<R, 1, 3>(s, d, terms, csi, qred, bb, t);
+            rx_group<R, 1, 3>(s, rxc, qred, 12, t);
+            exchange<R, 1, 3, 2>(s, xb, t);
+            rx_group<R, 1, 2>(s, rxc, qred, 12, t);
+            exchange<R, 1, 2, 1>(s, xb, t);
+            rx_group<R, 1, 1>(s, rxc, qred, 12, t);
+            exchange<R, 1, 1, 0>(s, xb, t);
+            rx_group<R, 1, 0>(s, rxc, qred, 12, t);
         } else {
-            // The forward pass backwards.
-            int ph = 0;
-            const bool A = fl & F_SWA;
-            if (fl & F_SWB) {
-                if (A) {  // forward ran 3, 2, 1, 0
-                    rx_group<R, 2, 0>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 0, 1>(s, xb, t);
-                    rx_group<R, 2, 1>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 1, 2>(s, xb, t);
-                    rx_group<R, 2, 2>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 2, 3>(s, xb, t);
-                    rx_group<R, 2, 3>(s, rxc, qred, 12, t);
-                    ph = 3;
-                } else {  // forward ran 0, 1, 2, 3 and back to 0
-                    exchange<R, 2, 0, 3>(s, xb, t);
-                    rx_group<R, 2, 3>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 3, 2>(s, xb, t);
-                    rx_group<R, 2, 2>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 2, 1>(s, xb, t);
-                    rx_group<R, 2, 1>(s, rxc, qred, 12, t);
-                    exchange<R, 2, 1, 0>(s, xb, t);
-                    rx_group<R, 2, 0>(s, rxc, qred, 12, t);
-                }
-            }
-            if (fl & F_DIAG) {
-                if (A) {
-                    if (ph == 0) exchange<R, 2, 0, 3>(s, xb, t);
-                    ph = 3;
-                    diag_layer<R, 2, 3>(s, d, terms, csi, qred, bb, t);
-                } else {
-                    diag_layer<R, 2, 0>(s, d, terms, csi, qred, bb, t);
-                }
-            }
-            if (A) {
-                if (ph == 0) exchange<R, 2, 0, 3>(s, xb, t);
-                rx_group<R, 2, 3>(s, rxc, qred, 0, t);
-                exchange<R, 2, 3, 2>(s, xb, t);
-                rx_group<R, 2, 2>(s, rxc, qred, 0, t);
-                exchange<R, 2, 2, 1>(s, xb, t);
-                rx_group<R, 2, 1>(s, rxc, qred, 0, t);
-                exchange<R, 2, 1, 0>(s, xb, t);
-                rx_group<R, 2, 0>(s, rxc, qred, 0, t);
-                ph = 0;
-            }
-            if (ph == 3) exchange<R, 2, 3, 0>(s, xb, t);
+            if constexpr (D) diag_layer<R, 1, 0>(s, d, terms, csi, qred, bb, t);
+            rx_group<R, 1, 0>(s, rxc, qred, 12, t);
+            exchange<R, 1, 0, 1>(s, xb, t);
+            rx_group<R, 1, 1>(s, rxc, qred, 12, t);
+            exchange<R, 1, 1, 2>(s, xb, t);
+            rx_group<R, 1, 2>(s, rxc, qred, 12, t);
+            exchange<R, 1, 2, 3>(s, xb, t);
+            rx_group<R, 1, 3>(s, rxc, qred, 12, t);
+            exchange<R, 1, 3, 0>(s, xb, t);
+        }
+        if constexpr (COST) {
+            Regs<R> l;
+            const double en_t = costate<R>(s[0], l, d, terms, coef + d.coef_off, g0);
+            if (fl & F_LAMBDA) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
-                lp[g0 + k] = cx<R>{s[1].re[k], s[1].im[k]};
+                for (int k = 0; k < 8; ++k) lp[g0 + k] = cx<R>{l.re[k], l.im[k]};
             }
-            // The tile's overlap of every recorded rotation: its waves' sums
-            // in wave order, into the [slot][tile] partial table.
+            // The tile's energy partial: wave sums, then the waves in order.
+            dd acc{en_t, 0.0};
+            for (int m = 32; m >= 1; m >>= 1)
+                acc = qf::dd_add(acc, dd{__shfl_xor(acc.s, m, 64), __shfl_xor(acc.c, m, 64)});
+            if ((t & 63u) == 0) ered[t >> 6] = acc;
             __syncthreads();
-            const uint32_t nblk = 1u << d.logb;
-            const uint32_t rows = 24 + ((fl & F_DIAG) ? d.t_n : 0);
-            for (uint32_t r = t; r < rows; r += kQT) {
-                uint32_t slot;
-                if (r < 12) {
-                    if (!A) continue;
-                    slot = d.slotA[r];
-                } else if (r < 24) {
-                    if (!(fl & F_SWB)) continue;
-                    slot = d.slotB[r - 12];
-                } else {
-                    slot = terms[d.t_off + (r - 24)].slot;
-                }
-                double q = 0.0;
-                for (int w = 0; w < kQW; ++w) q += qred[r * kQW + w];
-                partial[d.part_off + uint64_t(slot) * nblk + lb] = q;
+            if (t == 0) {
+                dd tot = ered[0];
+                for (int w = 1; w < kQW; ++w) tot = qf::dd_add(tot, ered[w]);
+                epart[d.epart_off + lb] = tot;
             }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
+    } else {
+        // The forward pass backwards.
+        if constexpr (A) {  // forward: A on 0..3, D in phase 3, B on 3..0
+            rx_group<R, 2, 0>(s, rxc, qred, 12, t);
+            exchange<R, 2, 0, 1>(s, xb, t);
+            rx_group<R, 2, 1>(s, rxc, qred, 12, t);
+            exchange<R, 2, 1, 2>(s, xb, t);
+            rx_group<R, 2, 2>(s, rxc, qred, 12, t);
+            exchange<R, 2, 2, 3>(s, xb, t);
+            rx_group<R, 2, 3>(s, rxc, qred, 12, t);
+            diag_layer<R, 2, 3>(s, d, terms, csi, qred, bb, t);
+            rx_group<R, 2, 3>(s, rxc, qred, 0, t);
+            exchange<R, 2, 3, 2>(s, xb, t);
+            rx_group<R, 2, 2>(s, rxc, qred, 0, t);
+            exchange<R, 2, 2, 1>(s, xb, t);
+            rx_group<R, 2, 1>(s, rxc, qred, 0, t);
+            exchange<R, 2, 1, 0>(s, xb, t);
+            rx_group<R, 2, 0>(s, rxc, qred, 0, t);
+        } else {  // forward: D in phase 0 (SH_DB), B on 0..3 and back to 0
+            exchange<R, 2, 0, 3>(s, xb, t);
+            rx_group<R, 2, 3>(s, rxc, qred, 12, t);
+            exchange<R, 2, 3, 2>(s, xb, t);
+            rx_group<R, 2, 2>(s, rxc, qred, 12, t);
+            exchange<R, 2, 2, 1>(s, xb, t);
+            rx_group<R, 2, 1>(s, rxc, qred, 12, t);
+            exchange<R, 2, 1, 0>(s, xb, t);
+            rx_group<R, 2, 0>(s, rxc, qred, 12, t);
+            if constexpr (D) diag_layer<R, 2, 0>(s, d, terms, csi, qred, bb, t);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
+            lp[g0 + k] = cx<R>{s[1].re[k], s[1].im[k]};
+        }
+        // The tile's overlap of every recorded rotation: its waves' sums in
+        // wave order, into the [slot][tile] partial table.
+        __syncthreads();
+        const uint32_t nblk = 1u << d.logb;
+        const uint32_t rows = 24 + (D ? d.t_n : 0);
+        for (uint32_t r = t; r < rows; r += kQT) {
+            uint32_t slot;
+            if (r < 12) {
+                if (!A) continue;
+                slot = d.slotA[r];
+            } else if (r < 24) {
+                slot = d.slotB[r - 12];
+            } else {
+                slot = terms[d.t_off + (r - 24)].slot;
+            }
+            double q = 0.0;
+            for (int w = 0; w < kQW; ++w) q += qred[r * kQW + w];
+            partial[d.part_off + uint64_t(slot) * nblk + lb] = q;
         }
     }
 }
@@ -710,6 +691,16 @@ int tile_bit_of(const uint32_t hi[9], uint32_t q) {
     return -1;
 }
 
+// The pass shape of a pass's flags (-1: none of the three).  Qubits 0..2 sit
+// in every tile, so a pass with a cost layer always rotates after it.
+int shape_of(uint32_t fl) {
+    const bool a = fl & F_SWA, dg = fl & F_DIAG, b = fl & F_SWB;
+    if (a && dg && b) return SH_ADB;
+    if (!a && dg && b) return SH_DB;
+    if (!a && !dg && b) return SH_B;
+    return -1;
+}
+
 // Rotations of cost layer l at pass l * k; X rotation (l, q) at the first pass
 // from there whose tile holds q: before the next cost layer, which the tile
 // sets guarantee.  A pass with a cost layer rotates the previous layer's
@@ -753,13 +744,17 @@ bool schedule(const Form& f, std::vector<PassPlan>& out) {
                 P.rot_b[b] = true;
             }
         }
-    for (int j = 0; j <= last; ++j) {
-        PassPlan& P = out[size_t(j)];
+    // A pass whose tile holds no rotation still owed (three sets on a narrow
+    // instance) does nothing and is dropped.
+    std::vector<PassPlan> kept;
+    for (const PassPlan& P : out)
+        if (P.layer_a >= 0 || P.diag >= 0 || P.layer_b >= 0) kept.push_back(P);
+    out.swap(kept);
+    for (size_t j = 0; j < out.size(); ++j) {
+        PassPlan& P = out[j];
         P.flags = (j > 0 ? F_LOAD : 0u) | (P.layer_a >= 0 ? F_SWA : 0u) | (P.diag >= 0 ? F_DIAG : 0u) |
-                  (P.layer_b >= 0 ? F_SWB : 0u) | (j == last ? F_COST : 0u);
-        // Sweep B after a cost layer without sweep A, or without a cost layer
-        // at all, runs from phase 0; an empty pass (no rotations) still has
-        // the costate or the cost layer.
+                  (P.layer_b >= 0 ? F_SWB : 0u) | (j + 1 == out.size() ? F_COST : 0u);
+        if (shape_of(P.flags) < 0) return false;
     }
     return true;
 }
@@ -847,20 +842,53 @@ struct Buffers {
     double* energies;
 };
 
+// Launch slots of a wave: forward pass j, shape, last pass or not
+// (F x kShapes x 2), then reverse pass j (from each instance's last), shape.
+struct Slot {
+    bool fwd;
+    uint32_t j;
+    int shape;
+    bool cost;
+};
+std::vector<Slot> launch_slots(uint32_t F, uint32_t Rv) {
+    std::vector<Slot> v;
+    for (uint32_t j = 0; j < F; ++j)
+        for (int sh = 0; sh < kShapes; ++sh)
+            for (int c = 0; c < 2; ++c) v.push_back({true, j, sh, c == 1});
+    for (uint32_t j = 0; j < Rv; ++j)
+        for (int sh = 0; sh < kShapes; ++sh) v.push_back({false, j, sh, false});
+    return v;
+}
+
+template <class R, int NS, int SH, bool COST>
+void launch_pass(hipStream_t st, dim3 g, const unsigned char* L, const Buffers& B) {
+    hipLaunchKernelGGL((k_qaoa_pass<R, NS, SH, COST>), g, dim3(kQT), 0, st, static_cast<cx<R>*>(B.psi),
+                       static_cast<cx<R>*>(B.lam), L, B.descs, B.terms, B.cs, B.coef, B.part, B.epart);
+}
+
 template <class R>
-qf_status capture(hipStream_t st, uint32_t F, uint32_t Rv, const std::vector<uint64_t>& grid, const Buffers& B,
-                  hipGraphExec_t* exec, uint32_t* nodes) {
+void launch_slot(hipStream_t st, const Slot& sl, dim3 g, const unsigned char* L, const Buffers& B) {
+    if (sl.fwd) {
+        if (sl.shape == SH_DB) sl.cost ? launch_pass<R, 1, SH_DB, true>(st, g, L, B) : launch_pass<R, 1, SH_DB, false>(st, g, L, B);
+        else if (sl.shape == SH_ADB) sl.cost ? launch_pass<R, 1, SH_ADB, true>(st, g, L, B) : launch_pass<R, 1, SH_ADB, false>(st, g, L, B);
+        else sl.cost ? launch_pass<R, 1, SH_B, true>(st, g, L, B) : launch_pass<R, 1, SH_B, false>(st, g, L, B);
+    } else {
+        if (sl.shape == SH_DB) launch_pass<R, 2, SH_DB, false>(st, g, L, B);
+        else if (sl.shape == SH_ADB) launch_pass<R, 2, SH_ADB, false>(st, g, L, B);
+        else launch_pass<R, 2, SH_B, false>(st, g, L, B);
+    }
+}
+
+template <class R>
+qf_status capture(hipStream_t st, const std::vector<Slot>& slots, const std::vector<uint64_t>& grid,
+                  const Buffers& B, hipGraphExec_t* exec, uint32_t* nodes) {
     hipGraph_t graph = nullptr;
+    uint32_t launched = 0;
     QF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (uint32_t j = 0; j < F + Rv; ++j) {
-        const unsigned char* L = B.launch + size_t(j) * B.stride;
-        const dim3 g(unsigned(std::max<uint64_t>(grid[j], 1)));
-        if (j < F)
-            hipLaunchKernelGGL((k_qaoa_pass<R, 1>), g, dim3(kQT), 0, st, static_cast<cx<R>*>(B.psi),
-                               static_cast<cx<R>*>(B.lam), L, B.descs, B.terms, B.cs, B.coef, B.part, B.epart);
-        else
-            hipLaunchKernelGGL((k_qaoa_pass<R, 2>), g, dim3(kQT), 0, st, static_cast<cx<R>*>(B.psi),
-                               static_cast<cx<R>*>(B.lam), L, B.descs, B.terms, B.cs, B.coef, B.part, B.epart);
+    for (size_t k = 0; k < slots.size(); ++k) {
+        if (grid[k] == 0) continue;  // no instance of this shape at this pass in any wave
+        launch_slot<R>(st, slots[k], dim3(unsigned(grid[k])), B.launch + k * B.stride, B);
+        ++launched;
     }
     hipLaunchKernelGGL(k_qaoa_finish, dim3(4096), dim3(256), 0, st, B.fin, B.nfin, B.slots, B.part, B.epart, B.grads,
                        B.energies);
@@ -872,7 +900,7 @@ qf_status capture(hipStream_t st, uint32_t F, uint32_t Rv, const std::vector<uin
     hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ie != hipSuccess) return qfrt::hip_fail(ie, "hipGraphInstantiate");
-    *nodes = F + Rv + 1;
+    *nodes = launched + 1;
     return QF_OK;
 }
 
@@ -981,7 +1009,8 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return X[a].n > X[b].n; });
     size_t freeb = 0, totalb = 0;
     QF_HIP_TRY(hipMemGetInfo(&freeb, &totalb));
-    uint64_t budget = uint64_t(double(freeb) * 0.8);
+    // The backend's arena from an earlier call is reused, so it counts as free.
+    uint64_t budget = uint64_t(double(freeb + be->arena_bytes) * 0.8);
     if (const char* cap = std::getenv("QF_UNEVEN_WAVE_BYTES")) {
         const unsigned long long v = std::strtoull(cap, nullptr, 10);
         if (v > 0 && v < budget) budget = v;
@@ -1034,13 +1063,26 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
         max_slots = std::max(max_slots, nsl);
     }
     const size_t stride = (sizeof(QHeader) + max_inst * sizeof(QEntry) + 255) / 256 * 256;
-    Dev d_psi, d_lam, d_part, d_epart, d_launch, d_desc, d_terms, d_cs, d_coef, d_fin, d_nfin, d_slots, d_grads,
-        d_energy;
-    if ((s = qfrt::device_alloc(&d_psi.p, max_amps * ab))) return s;
-    if (want_grad && (s = qfrt::device_alloc(&d_lam.p, max_amps * ab))) return s;
-    if ((s = qfrt::device_alloc(&d_part.p, std::max<uint64_t>(max_part, 1) * 8))) return s;
-    if ((s = qfrt::device_alloc(&d_epart.p, max_epart * sizeof(dd)))) return s;
-    if ((s = qfrt::device_alloc(&d_launch.p, size_t(F + Rv) * stride))) return s;
+    // psi, lambda, the partial tables and the energy partials live in the
+    // backend's arena, kept for the next call (freeing a wave's worth of
+    // device memory costs the host most of a second).
+    Dev d_launch, d_desc, d_terms, d_cs, d_coef, d_fin, d_nfin, d_slots, d_grads, d_energy;
+    struct {
+        void* p;
+    } d_psi{}, d_lam{}, d_part{}, d_epart{};
+    {
+        auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
+        const uint64_t b_psi = up(max_amps * ab), b_lam = want_grad ? up(max_amps * ab) : 0,
+                       b_part = up(std::max<uint64_t>(max_part, 1) * 8), b_ep = up(max_epart * sizeof(dd));
+        void* base = nullptr;
+        if ((s = qfrt::arena(be, b_psi + b_lam + b_part + b_ep, &base))) return s;
+        unsigned char* a = static_cast<unsigned char*>(base);
+        d_psi.p = a;
+        d_lam.p = want_grad ? a + b_psi : nullptr;
+        d_part.p = a + b_psi + b_lam;
+        d_epart.p = a + b_psi + b_lam + b_part;
+    }
+    if ((s = qfrt::device_alloc(&d_launch.p, launch_slots(F, Rv).size() * stride))) return s;
     if ((s = qfrt::device_alloc(&d_desc.p, max_desc * sizeof(QDesc)))) return s;
     if ((s = qfrt::device_alloc(&d_terms.p, max_terms * sizeof(QTerm)))) return s;
     if ((s = qfrt::device_alloc(&d_cs.p, max_cs * 8))) return s;
@@ -1057,18 +1099,29 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
               static_cast<const QFin*>(d_fin.p), static_cast<const uint32_t*>(d_nfin.p),
               static_cast<const uint32_t*>(d_slots.p), static_cast<double*>(d_grads.p),
               static_cast<double*>(d_energy.p)};
-    // Workgroups of launch j: the most tiles it has in any wave.
-    std::vector<uint64_t> grid(F + Rv, 0);
+    // Instance i's pass for launch slot sl, or -1.
+    const std::vector<Slot> slots_l = launch_slots(F, Rv);
+    auto pass_of = [&](const Inst& I, const Slot& sl) -> int {
+        const uint32_t np = uint32_t(I.passes.size());
+        if (sl.j >= np) return -1;
+        const uint32_t j = sl.fwd ? sl.j : np - 1 - sl.j;
+        const PassPlan& P = I.passes[j];
+        if (shape_of(P.flags) != sl.shape) return -1;
+        if (sl.fwd && bool(P.flags & F_COST) != sl.cost) return -1;
+        return int(j);
+    };
+    // Workgroups of each slot: the most tiles it has in any wave.
+    std::vector<uint64_t> grid(slots_l.size(), 0);
     for (const auto& w : waves)
-        for (uint32_t j = 0; j < F + Rv; ++j) {
+        for (size_t k = 0; k < slots_l.size(); ++k) {
             uint64_t blocks = 0;
             for (uint32_t i : w)
-                if ((j < F ? j : j - F) < X[i].passes.size()) blocks += nblk_of(X[i].n);
-            grid[j] = std::max(grid[j], blocks);
+                if (pass_of(X[i], slots_l[k]) >= 0) blocks += nblk_of(X[i].n);
+            grid[k] = std::max(grid[k], blocks);
         }
     hipGraphExec_t exec = nullptr;
     uint32_t nodes = 0;
-    if ((s = capture<R>(st, F, Rv, grid, B, &exec, &nodes))) return s;
+    if ((s = capture<R>(st, slots_l, grid, B, &exec, &nodes))) return s;
     struct ExecGuard {
         hipGraphExec_t e;
         ~ExecGuard() {
@@ -1086,7 +1139,7 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
         }
     } evg{ev0, ev1};
 
-    std::vector<unsigned char> launch(size_t(F + Rv) * stride);
+    std::vector<unsigned char> launch(slots_l.size() * stride);
     std::vector<QDesc> descs;
     std::vector<QTerm> terms;
     std::vector<double> csbuf, coef;
@@ -1191,20 +1244,18 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
             if (want_grad) part_off += uint64_t(I.nslots + 1) * nb;
             ep_off += nb;
         }
-        // Launch tables: forward pass j, then reverse pass j (the instance's
-        // passes from its last).
-        for (uint32_t j = 0; j < F + Rv; ++j) {
-            QHeader* h = reinterpret_cast<QHeader*>(launch.data() + size_t(j) * stride);
-            QEntry* E = reinterpret_cast<QEntry*>(launch.data() + size_t(j) * stride + sizeof(QHeader));
+        // Launch tables, one per slot.
+        for (size_t k = 0; k < slots_l.size(); ++k) {
+            QHeader* h = reinterpret_cast<QHeader*>(launch.data() + k * stride);
+            QEntry* E = reinterpret_cast<QEntry*>(launch.data() + k * stride + sizeof(QHeader));
             uint64_t blocks = 0;
             uint32_t ne = 0;
             for (size_t wi = 0; wi < w.size(); ++wi) {
                 const Inst& I = X[w[wi]];
-                const uint32_t np = uint32_t(I.passes.size());
-                const uint32_t r = j < F ? j : j - F;
-                if (r >= np) continue;
+                const int j = pass_of(I, slots_l[k]);
+                if (j < 0) continue;
                 E[ne].block_start = blocks;
-                E[ne].desc = desc0[wi] + (j < F ? r : np - 1 - r);
+                E[ne].desc = desc0[wi] + uint32_t(j);
                 ++ne;
                 blocks += nblk_of(I.n);
             }

@@ -42,6 +42,11 @@ struct qf_backend {
     // Measurement record of the last batched gradient / expectation call.
     int32_t profile = 0;
     qf_gradient_profile grad_prof{};
+    // Device arena of the batch engines (QAOA waves): kept for the next call,
+    // released by qf_backend_release_cached, at destroy, or when a device
+    // allocation fails.
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
 };
 
 struct qf_state {
@@ -113,6 +118,9 @@ qf_status lower_ops(const qf_op* ops, size_t count, uint32_t qubits, std::vector
 qf_status activate(qf_backend* be);
 qf_status device_alloc(void** out, size_t bytes);
 qf_status ensure(qf_backend* be, void** buf, size_t* cap, size_t bytes);
+// The backend's arena of at least `bytes` (kept between calls; grown by a
+// free and a new allocation).
+qf_status arena(qf_backend* be, size_t bytes, void** out);
 qf_status launch_check(const char* what);
 size_t amp_bytes(int precision);
 unsigned grid_for(uint64_t work);

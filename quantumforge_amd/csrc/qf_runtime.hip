@@ -153,9 +153,68 @@ namespace qfrt {
 
 qf_backend* state_backend(const qf_state* st) { return st->be; }
 
+// Backends holding an arena, so that a failed allocation can release them.
+std::mutex g_arena_mu;
+std::vector<qf_backend*> g_arena_owners;
+
+void arena_free(qf_backend* be) {
+    if (!be->arena) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(be->ordinal);
+    (void)hipStreamSynchronize(be->stream);
+    (void)hipFree(be->arena);
+    (void)hipSetDevice(cur);
+    be->arena = nullptr;
+    be->arena_bytes = 0;
+}
+
+// Releases every backend's arena; true when one was held.
+bool release_arenas() {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    const bool any = !g_arena_owners.empty();
+    for (qf_backend* be : g_arena_owners) arena_free(be);
+    g_arena_owners.clear();
+    return any;
+}
+
+qf_status arena(qf_backend* be, size_t bytes, void** out) {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    if (be->arena && be->arena_bytes >= bytes) {
+        *out = be->arena;
+        return QF_OK;
+    }
+    arena_free(be);
+    g_arena_owners.erase(std::remove(g_arena_owners.begin(), g_arena_owners.end(), be), g_arena_owners.end());
+    hipError_t e = hipMalloc(&be->arena, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        be->arena = nullptr;
+        // Other backends' arenas may hold the memory.
+        for (qf_backend* o : g_arena_owners) arena_free(o);
+        g_arena_owners.clear();
+        e = hipMalloc(&be->arena, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            be->arena = nullptr;
+            return fail(QF_OUT_OF_MEMORY, "could not reserve " + std::to_string(bytes) + " bytes of device memory");
+        }
+    }
+    be->arena_bytes = bytes;
+    g_arena_owners.push_back(be);
+    *out = be->arena;
+    return QF_OK;
+}
+
 void backend_release(qf_backend* be) {
     if (!be) return;
     if (be->refs.fetch_sub(1) == 1) {
+        {
+            std::lock_guard<std::mutex> lk(g_arena_mu);
+            arena_free(be);
+            g_arena_owners.erase(std::remove(g_arena_owners.begin(), g_arena_owners.end(), be),
+                                 g_arena_owners.end());
+        }
         (void)hipSetDevice(be->ordinal);
         (void)hipStreamSynchronize(be->stream);
         if (be->scratch) (void)hipFree(be->scratch);
@@ -209,6 +268,12 @@ qf_status device_alloc(void** out, size_t bytes) {
     hipError_t e = hipMalloc(out, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        // A batch engine's arena kept from an earlier call may hold the memory.
+        if (release_arenas()) {
+            e = hipMalloc(out, bytes);
+            if (e == hipSuccess) return QF_OK;
+            (void)hipGetLastError();
+        }
         *out = nullptr;
         return fail(QF_OUT_OF_MEMORY, "could not reserve " + std::to_string(bytes) +
                                           " bytes of device memory for the dense quantum state");
@@ -823,6 +888,14 @@ extern "C" qf_status qf_backend_create(int32_t ordinal, qf_backend** out) {
 extern "C" qf_status qf_backend_destroy(qf_backend* be) {
     if (!be) return fail(QF_INVALID_ARGUMENT, "null backend");
     backend_release(be);
+    return QF_OK;
+}
+
+extern "C" qf_status qf_backend_release_cached(qf_backend* be) {
+    if (!be) return fail(QF_INVALID_ARGUMENT, "null backend");
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    arena_free(be);
+    g_arena_owners.erase(std::remove(g_arena_owners.begin(), g_arena_owners.end(), be), g_arena_owners.end());
     return QF_OK;
 }
 

@@ -85,9 +85,9 @@ def test_three_tile_sets_vs_oracle(backend, monkeypatch):
     e, g, stats = G.uneven_adjoint_gradient(backend, insts)
     print(f"[qaoa engine] stats {stats}", flush=True)
     assert stats["qaoa_instances"] == len(insts)
-    # a cost layer every second pass: 2p - 1 passes carry the p cost layers,
-    # one or two more finish the last mixer (p = 3)
-    assert stats["qaoa_forward_passes"] in (6, 7)
+    # a cost layer every second pass (p = 3): at most 2p + 1 passes, fewer
+    # when a narrow instance's tile sets repeat qubits
+    assert 2 * 3 - 1 <= stats["qaoa_forward_passes"] <= 2 * 3 + 1
     check_vs_oracle(insts, e, g)
 
 
