@@ -32,7 +32,7 @@ static std::barrier<>* qf_bar;
 #define blockIdx qf_bid
 #define gridDim qf_grid
 #define __global__
-#define __launch_bounds__(n)
+#define __launch_bounds__(...)
 #define __shared__ static
 #define __syncthreads() qf_bar->arrive_and_wait()
 // Lane exchange of the wave reductions: every work item of the workgroup
@@ -109,7 +109,7 @@ extern "C" void qf_emu_launch(void* fn, C* s, const double* cs, unsigned nslots,
 }
 """
 
-KERNEL_RE = re.compile(r'extern "C" __global__ __launch_bounds__\((\d+)\) void (\w+)\(')
+KERNEL_RE = re.compile(r'extern "C" __global__ __launch_bounds__\((\d+)(?:, \d+)?\) void (\w+)\(')
 
 
 class EmulatedProgram:
