@@ -509,15 +509,36 @@ __global__ __launch_bounds__(qf::kThreads) void k_gen_overlap(const cx<R>* __res
 
 struct DevBuf {
     void* p = nullptr;
+    bool owned = true;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
     }
     qf_status alloc(size_t bytes) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
+        if (p && owned) (void)hipFree(p);
+        p = nullptr;
+        owned = true;
         return device_alloc(&p, bytes ? bytes : 16);
+    }
+};
+
+// Buffers of one call carved from the backend's arena (kept between calls:
+// an optimizer loop pays for its device allocations and frees once).
+struct ArenaCarve {
+    std::vector<std::pair<DevBuf*, size_t>> v;
+    void add(DevBuf& b, size_t bytes) { v.push_back({&b, ((bytes ? bytes : 16) + 255) / 256 * 256}); }
+    qf_status commit(qf_backend* be) {
+        size_t total = 0;
+        for (const auto& x : v) total += x.second;
+        void* base = nullptr;
+        qf_status s = arena(be, total, &base);
+        if (s) return s;
+        size_t off = 0;
+        for (auto& x : v) {
+            x.first->p = static_cast<unsigned char*>(base) + off;
+            x.first->owned = false;
+            off += x.second;
+        }
+        return QF_OK;
     }
 };
 
@@ -570,9 +591,11 @@ uint32_t reduce_blocks(uint64_t dim) {
 }
 
 // Largest wave of instances that fits `per_inst` bytes each in the free device memory.
-uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_t log_blocks_max) {
+uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_t log_blocks_max,
+                   size_t reusable = 0) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = size_t(8) << 30;
+    freeb += reusable;  // the backend's arena, reused by this call
     const double budget = double(freeb) * 0.8;
     uint64_t n = per_inst ? uint64_t(budget / double(per_inst)) : want;
     uint32_t ylim = 65535u >> (log_blocks_max > 16 ? std::min<uint32_t>(log_blocks_max - 16, 16) : 0);
@@ -586,7 +609,7 @@ uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_
 // the reverse sweep (grads [cnt][nparams]).  Waves of instances sized to the
 // free device memory.
 qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_t chunk_opt, double* energies,
-                    double* grads) {
+                    double* grads, double* stage = nullptr) {
     qf_backend* be = R.be;
     hipStream_t st = nullptr;
     {
@@ -607,18 +630,9 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
                               eblk * 32 + 64;
     const uint32_t fwd_M = R.fwd.passes.empty() ? n : R.fwd.passes[0].tile_bits;
     const uint32_t logmax = n - std::min(std::min(fwd_M, rev_M), n);
-    const uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax);
+    const uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax, be->arena_bytes);
     if (wave == 0) return fail(QF_OUT_OF_MEMORY, "device memory cannot hold one parameter set's states");
 
-    DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms, wterms;
-    qf_status s;
-    if ((s = psi.alloc(size_t(wave) * dim * ab))) return s;
-    if ((s = lam.alloc(want_grad ? size_t(wave) * dim * ab : 16))) return s;
-    if ((s = cs.alloc(size_t(wave) * nslots * 16))) return s;
-    if (want_grad && (s = part.alloc(size_t(wave) * nblk * nslots * 8))) return s;
-    if ((s = epart.alloc(size_t(wave) * eblk * 2 * sizeof(qf::dd)))) return s;
-    if ((s = eout.alloc(size_t(wave) * 2 * sizeof(double)))) return s;
-    if (want_grad && (s = gout.alloc(size_t(wave) * R.pc.nparams * sizeof(double)))) return s;
     // parameter -> occurrences in reverse circuit order (CSR)
     std::vector<uint32_t> ps(R.pc.nparams + 1, 0), pl(nslots);
     for (uint32_t k = 0; k < nslots; ++k) ps[R.pc.slot_param[k] + 1]++;
@@ -627,17 +641,30 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         std::vector<uint32_t> fill(ps.begin(), ps.end() - 1);
         for (uint32_t k = nslots; k-- > 0;) pl[fill[R.pc.slot_param[k]]++] = k;
     }
-    if ((s = pstart.alloc(ps.size() * 4))) return s;
-    if ((s = pslots.alloc(std::max<size_t>(pl.size(), 1) * 4))) return s;
     const std::vector<qf::PauliDev> tsorted = sorted_terms(R.terms);
     uint32_t ngroups = 0;
     const std::vector<unsigned char> tpack = pack_terms(tsorted, &ngroups);
-    if ((s = terms.alloc(tpack.size()))) return s;
     const std::vector<unsigned char> wpack = pack_pauli_windows(tsorted);
+    DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms, wterms;
+    qf_status s;
+    {
+        ArenaCarve ac;
+        ac.add(psi, size_t(wave) * dim * ab);
+        ac.add(lam, want_grad ? size_t(wave) * dim * ab : 16);
+        ac.add(cs, size_t(wave) * nslots * 16);
+        ac.add(part, want_grad ? size_t(wave) * nblk * nslots * 8 : 16);
+        ac.add(epart, size_t(wave) * eblk * 2 * sizeof(qf::dd));
+        ac.add(eout, size_t(wave) * 2 * sizeof(double));
+        ac.add(gout, want_grad ? size_t(wave) * R.pc.nparams * sizeof(double) : 16);
+        ac.add(pstart, ps.size() * 4);
+        ac.add(pslots, std::max<size_t>(pl.size(), 1) * 4);
+        ac.add(terms, tpack.size());
+        ac.add(wterms, wpack.size());
+        if ((s = ac.commit(be))) return s;
+    }
     uint32_t wcoef = 0;
     if (!wpack.empty()) {
         wcoef = reinterpret_cast<const PWHeader*>(wpack.data())->ncoef;
-        if ((s = wterms.alloc(wpack.size()))) return s;
         QF_HIP_TRY(hipMemcpyAsync(wterms.p, wpack.data(), wpack.size(), hipMemcpyHostToDevice, st));
     }
     QF_HIP_TRY(hipMemcpyAsync(pstart.p, ps.data(), ps.size() * 4, hipMemcpyHostToDevice, st));
@@ -741,14 +768,19 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
                                    static_cast<const uint32_t*>(pstart.p), static_cast<const uint32_t*>(pslots.p),
                                    R.pc.nparams, cnt, static_cast<double*>(gout.p));
                 if ((s = launch_check("k_grad_finish"))) return s;
-                QF_HIP_TRY(hipMemcpyAsync(grads + size_t(off) * R.pc.nparams, gout.p, work * sizeof(double),
-                                          hipMemcpyDeviceToHost, st));
+                // Into pinned staging when the caller has one (one DMA), else
+                // straight to the caller's memory.
+                QF_HIP_TRY(hipMemcpyAsync((stage ? stage : grads) + size_t(off) * R.pc.nparams, gout.p,
+                                          work * sizeof(double), hipMemcpyDeviceToHost, st));
             }
         }
         QF_HIP_TRY(mark(4));
         QF_HIP_TRY(hipMemcpyAsync(eo.data(), eout.p, size_t(cnt) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         QF_HIP_TRY(hipStreamSynchronize(st));
         for (uint32_t b = 0; b < cnt; ++b) energies[off + b] = eo[2 * b];
+        if (stage && want_grad && R.pc.nparams)
+            std::memcpy(grads + size_t(off) * R.pc.nparams, stage + size_t(off) * R.pc.nparams,
+                        size_t(cnt) * R.pc.nparams * sizeof(double));
         if (be->profile) {
             // ev 0..1 forward, 1..2 costate, 2..3 energy finish, 3..4 reverse + gradient finish
             float t[4] = {0, 0, 0, 0};
@@ -766,11 +798,24 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
 qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t chunk_opt, double* energies,
                        double* grads) {
     const uint32_t nslots = uint32_t(R.pc.slot_param.size());
-    std::vector<double> table(std::max<size_t>(size_t(sets) * nslots * 2, 1));
+    // The angle table and the gradient staging in the backend's pinned buffer.
+    const size_t tdoubles = std::max<size_t>(size_t(sets) * nslots * 2, 1);
+    const size_t gdoubles = grads ? size_t(sets) * R.pc.nparams : 0;
+    void* hp = nullptr;
+    std::vector<double> fallback;
+    double* table = nullptr;
+    double* stage = nullptr;
+    if (pinned_host(R.be, (tdoubles + gdoubles) * sizeof(double), &hp) == QF_OK) {
+        table = static_cast<double*>(hp);
+        stage = gdoubles ? table + tdoubles : nullptr;
+    } else {
+        fallback.assign(tdoubles, 0.0);
+        table = fallback.data();
+    }
     parallel_rows(sets, [&](uint64_t b) {
-        fill_row(R.pc, params + size_t(b) * R.pc.nparams, -1, 0.0, table.data() + size_t(b) * nslots * 2);
+        fill_row(R.pc, params + size_t(b) * R.pc.nparams, -1, 0.0, table + size_t(b) * nslots * 2);
     });
-    return table_run(R, table.data(), sets, chunk_opt, energies, grads);
+    return table_run(R, table, sets, chunk_opt, energies, grads, stage);
 }
 
 // Shift rule (fd = false, delta = pi/2) or central differences (fd = true,

@@ -47,6 +47,10 @@ struct qf_backend {
     // allocation fails.
     void* arena = nullptr;
     size_t arena_bytes = 0;
+    // Pinned host staging of the batch engines' tables and results (DMA at
+    // full link rate; pageable copies are staged by the runtime in chunks).
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
 };
 
 struct qf_state {
@@ -121,6 +125,8 @@ qf_status ensure(qf_backend* be, void** buf, size_t* cap, size_t bytes);
 // The backend's arena of at least `bytes` (kept between calls; grown by a
 // free and a new allocation).
 qf_status arena(qf_backend* be, size_t bytes, void** out);
+// The backend's pinned host buffer of at least `bytes` (kept between calls).
+qf_status pinned_host(qf_backend* be, size_t bytes, void** out);
 qf_status launch_check(const char* what);
 size_t amp_bytes(int precision);
 unsigned grid_for(uint64_t work);

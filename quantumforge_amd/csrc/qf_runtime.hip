@@ -206,6 +206,27 @@ qf_status arena(qf_backend* be, size_t bytes, void** out) {
     return QF_OK;
 }
 
+qf_status pinned_host(qf_backend* be, size_t bytes, void** out) {
+    if (be->pinned && be->pinned_bytes >= bytes) {
+        *out = be->pinned;
+        return QF_OK;
+    }
+    if (be->pinned) {
+        HIP_TRY(hipStreamSynchronize(be->stream));
+        (void)hipHostFree(be->pinned);
+        be->pinned = nullptr;
+        be->pinned_bytes = 0;
+    }
+    if (hipHostMalloc(&be->pinned, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        be->pinned = nullptr;
+        return fail(QF_OUT_OF_MEMORY, "could not allocate " + std::to_string(bytes) + " bytes of pinned host memory");
+    }
+    be->pinned_bytes = bytes;
+    *out = be->pinned;
+    return QF_OK;
+}
+
 void backend_release(qf_backend* be) {
     if (!be) return;
     if (be->refs.fetch_sub(1) == 1) {
@@ -223,6 +244,7 @@ void backend_release(qf_backend* be) {
         if (be->d_dense) (void)hipFree(be->d_dense);
         if (be->meas_a) (void)hipFree(be->meas_a);
         if (be->meas_b) (void)hipFree(be->meas_b);
+        if (be->pinned) (void)hipHostFree(be->pinned);
         (void)hipStreamDestroy(be->stream);
         delete be;
     }
