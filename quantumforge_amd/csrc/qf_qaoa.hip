@@ -224,6 +224,32 @@ __device__ __forceinline__ void exchange(Regs<R> (&s)[NS], cx<R>* xb, uint32_t t
     }
 }
 
+// Stores the thread's 8 amplitudes (phase 0: tile bits 0..2 in registers, one
+// contiguous run) so that every store instruction writes whole runs: the
+// wave's threads pass their registers through the wave's own 512 slots of
+// the LDS tile, and lane l then writes amplitude l % 8 of the wave's thread
+// 8k + l / 8 (k = 0..7).  Stored straight from the registers, each
+// instruction writes 16 bytes into each of 64 runs, and the L2 wrote the
+// lines of the two-state reverse passes back 1.4 to 1.9 times over.  The LDS tile
+// must be free (every exchange ends with a barrier).
+template <class R>
+__device__ __forceinline__ void store_runs(cx<R>* __restrict__ dst, const Regs<R>& s, uint32_t g0, cx<R>* xb,
+                                           uint32_t t) {
+    const uint32_t ln = t & 63u, e = ln & 7u, hi = ln >> 3;
+    cx<R>* wb = xb + (t & ~63u) * 8u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) wb[ln * 8u + (uint32_t(k) ^ e)] = cx<R>{s.re[k], s.im[k]};
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t sl = 8u * uint32_t(k) + hi;
+        const cx<R> v = wb[sl * 8u + (e ^ hi)];
+        const uint32_t ga = uint32_t(__shfl(int(g0), int(sl), 64));
+        dst[ga + e] = v;
+    }
+    __syncthreads();
+}
+
 // X rotation exp(-i theta/2 X) (s = sin(theta/2); the reverse sweep passes
 // -s) on register bit J of every state.
 template <class R, int NS, int J>
@@ -534,10 +560,7 @@ __global__ __launch_bounds__(kQT, NS == 2 ? 4 : 2) void k_qaoa_pass(cx<R>* __res
         if constexpr (COST) {
             Regs<R> l;
             const double en_t = costate<R>(s[0], l, d, terms, coef + d.coef_off, g0);
-            if (fl & F_LAMBDA) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) lp[g0 + k] = cx<R>{l.re[k], l.im[k]};
-            }
+            if (fl & F_LAMBDA) store_runs<R>(lp, l, g0, xb, t);
             // The tile's energy partial: wave sums, then the waves in order.
             dd acc{en_t, 0.0};
             for (int m = 32; m >= 1; m >>= 1)
@@ -550,8 +573,7 @@ __global__ __launch_bounds__(kQT, NS == 2 ? 4 : 2) void k_qaoa_pass(cx<R>* __res
                 epart[d.epart_off + lb] = tot;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
+        store_runs<R>(sp, s[0], g0, xb, t);
     } else {
         // The forward pass backwards.
         if constexpr (A) {  // forward: A on 0..3, D in phase 3, B on 3..0
@@ -581,11 +603,8 @@ __global__ __launch_bounds__(kQT, NS == 2 ? 4 : 2) void k_qaoa_pass(cx<R>* __res
             rx_group<R, 2, 0>(s, rxc, qred, 12, t);
             if constexpr (D) diag_layer<R, 2, 0>(s, d, terms, csi, qred, bb, t);
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            sp[g0 + k] = cx<R>{s[0].re[k], s[0].im[k]};
-            lp[g0 + k] = cx<R>{s[1].re[k], s[1].im[k]};
-        }
+        store_runs<R>(sp, s[0], g0, xb, t);
+        store_runs<R>(lp, s[1], g0, xb, t);
         // The tile's overlap of every recorded rotation: its waves' sums in
         // wave order, into the [slot][tile] partial table.
         __syncthreads();
@@ -1015,21 +1034,65 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
         const unsigned long long v = std::strtoull(cap, nullptr, 10);
         if (v > 0 && v < budget) budget = v;
     }
+    // First fit, widest first; false (too_big = the instance) when an
+    // instance alone exceeds cap.
+    uint32_t too_big = 0;
+    auto pack = [&](uint64_t cap, std::vector<std::vector<uint32_t>>& W, std::vector<uint64_t>& WB) {
+        W.clear();
+        WB.clear();
+        for (uint32_t i : order) {
+            const uint64_t f = footprint(X[i]);
+            if (f > cap) {
+                too_big = X[i].k;
+                return false;
+            }
+            bool placed = false;
+            for (size_t w = 0; w < W.size() && !placed; ++w)
+                if (WB[w] + f <= cap) {
+                    W[w].push_back(i);
+                    WB[w] += f;
+                    placed = true;
+                }
+            if (!placed) {
+                W.push_back({i});
+                WB.push_back(f);
+            }
+        }
+        return true;
+    };
+    auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    // Arena bytes of a wave set: psi, lambda, partial table and energy
+    // partials, each sized for its largest wave.
+    auto arena_need = [&](const std::vector<std::vector<uint32_t>>& W) {
+        uint64_t a_max = 0, p_max = 0, e_max = 0;
+        for (const auto& w : W) {
+            uint64_t a = 0, p = 0, e = 0;
+            for (uint32_t i : w) {
+                a += uint64_t(1) << X[i].n;
+                if (want_grad) p += uint64_t(X[i].nslots + 1) * nblk_of(X[i].n);
+                e += nblk_of(X[i].n);
+            }
+            a_max = std::max(a_max, a);
+            p_max = std::max(p_max, p);
+            e_max = std::max(e_max, e);
+        }
+        return up(a_max * ab) * (want_grad ? 2 : 1) + up(std::max<uint64_t>(p_max, 1) * 8) + up(e_max * sizeof(dd));
+    };
     std::vector<std::vector<uint32_t>> waves;
     std::vector<uint64_t> wave_bytes;
-    for (uint32_t i : order) {
-        const uint64_t f = footprint(X[i]);
-        if (f > budget) return fail(QF_OUT_OF_MEMORY, "instance " + std::to_string(X[i].k) + " does not fit in device memory");
-        bool placed = false;
-        for (size_t w = 0; w < waves.size() && !placed; ++w)
-            if (wave_bytes[w] + f <= budget) {
-                waves[w].push_back(i);
-                wave_bytes[w] += f;
-                placed = true;
-            }
-        if (!placed) {
-            waves.push_back({i});
-            wave_bytes.push_back(f);
+    if (!pack(budget, waves, wave_bytes))
+        return fail(QF_OUT_OF_MEMORY, "instance " + std::to_string(too_big) + " does not fit in device memory");
+    // An arena kept from an earlier call is used as it is when the batch
+    // packs into as few waves within it: growing it means freeing most of
+    // HBM and allocating it again, and such an allocation right after the
+    // free has taken 6 s.
+    if (be->arena_bytes > 0 && arena_need(waves) > be->arena_bytes) {
+        std::vector<std::vector<uint32_t>> w2;
+        std::vector<uint64_t> b2;
+        if (pack(std::min<uint64_t>(be->arena_bytes, budget), w2, b2) && w2.size() <= waves.size() &&
+            arena_need(w2) <= be->arena_bytes) {
+            waves.swap(w2);
+            wave_bytes.swap(b2);
         }
     }
     // Buffers sized for the largest wave.
@@ -1071,7 +1134,6 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
         void* p;
     } d_psi{}, d_lam{}, d_part{}, d_epart{};
     {
-        auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
         const uint64_t b_psi = up(max_amps * ab), b_lam = want_grad ? up(max_amps * ab) : 0,
                        b_part = up(std::max<uint64_t>(max_part, 1) * 8), b_ep = up(max_epart * sizeof(dd));
         void* base = nullptr;
