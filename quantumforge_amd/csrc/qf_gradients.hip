@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -177,11 +178,12 @@ void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double 
 
 
 // Runs f(b) for b in [0, count) on up to 16 host threads (the angle tables
-// of large batches: one cos and one sin per occurrence and row).
+// of large batches: one cos and one sin per occurrence and row), about one
+// thread per 8192 occurrences (row_work occurrences a row).
 template <class F>
-void parallel_rows(uint64_t count, F f) {
+void parallel_rows(uint64_t count, F f, uint64_t row_work = 32) {
     const uint64_t hw = std::max<uint64_t>(1, std::min<uint64_t>(16, std::thread::hardware_concurrency()));
-    const uint64_t nt = std::min<uint64_t>(hw, count / 256 + 1);
+    const uint64_t nt = std::min<uint64_t>({hw, count * std::max<uint64_t>(row_work, 1) / 8192 + 1, count});
     if (nt <= 1) {
         for (uint64_t b = 0; b < count; ++b) f(b);
         return;
@@ -608,8 +610,12 @@ uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_
 // [cnt][nslots][2] host doubles): forward, energy, and with grads != nullptr
 // the reverse sweep (grads [cnt][nparams]).  Waves of instances sized to the
 // free device memory.
+// Fills the angle-table rows [a, b) of a call; table_run calls it for each
+// chunk of sets just before the chunk's rows are needed.
+using RowFill = std::function<void(uint32_t, uint32_t)>;
+
 qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_t chunk_opt, double* energies,
-                    double* grads, double* stage = nullptr) {
+                    double* grads, double* stage = nullptr, const RowFill* fill = nullptr) {
     qf_backend* be = R.be;
     hipStream_t st = nullptr;
     {
@@ -704,7 +710,7 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         const uint32_t cnt = std::min(wave, cnt_total - off);
         ++P.waves;
         // Synchronous copy: the previous wave finished at its download below.
-        if (nslots)
+        if (nslots && !fill)
             QF_HIP_TRY(hipMemcpy(cs.p, table + size_t(off) * nslots * 2, size_t(cnt) * nslots * 16,
                                  hipMemcpyHostToDevice));
         if (R.precision == QF_C128)
@@ -716,8 +722,38 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         fctx.cs = static_cast<const double*>(cs.p);
         fctx.nslots = nslots;
         QF_HIP_TRY(mark(0));
-        if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.fwd, bases[0], fctx))) return s;
-        P.forward_kernels |= R.fwd.kernels_run;
+        if (fill && nslots) {
+            // The host fills the angle rows of the next chunk of sets while
+            // the GPU runs the first forward pass over this one (and, before
+            // the first chunk, the zero fill); the other passes then run
+            // over the whole wave.
+            const uint32_t K = cnt >= 1024 && R.fwd.passes.size() > 1 ? 4 : 1;
+            for (uint32_t j = 0; j < K; ++j) {
+                const uint32_t a = uint32_t(uint64_t(cnt) * j / K), b = uint32_t(uint64_t(cnt) * (j + 1) / K);
+                if (j == 0) (*fill)(off + a, off + b);
+                double* csj = static_cast<double*>(cs.p) + size_t(a) * nslots * 2;
+                QF_HIP_TRY(hipMemcpyAsync(csj, table + size_t(off + a) * nslots * 2, size_t(b - a) * nslots * 16,
+                                          hipMemcpyHostToDevice, st));
+                LaunchCtx c1 = fctx;
+                c1.cs = csj;
+                if (K > 1) c1.pass_end = 1;
+                if ((s = launch_plan(be, static_cast<char*>(psi.p) + size_t(a) * dim * ab, R.precision, n, b - a,
+                                     R.fwd, bases[0], c1)))
+                    return s;
+                P.forward_kernels |= R.fwd.kernels_run;
+                if (j + 1 < K) (*fill)(off + b, off + uint32_t(uint64_t(cnt) * (j + 2) / K));
+            }
+            if (K > 1) {
+                LaunchCtx c2 = fctx;
+                c2.pass_begin = 1;
+                if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.fwd, bases[0], c2))) return s;
+                P.forward_kernels |= R.fwd.kernels_run;
+            }
+        } else {
+            if (fill) (*fill)(off, off + cnt);
+            if ((s = launch_plan(be, psi.p, R.precision, n, cnt, R.fwd, bases[0], fctx))) return s;
+            P.forward_kernels |= R.fwd.kernels_run;
+        }
         QF_HIP_TRY(mark(1));
         void* lam_out = want_grad ? lam.p : nullptr;
         if (!wpack.empty()) {
@@ -812,10 +848,15 @@ qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t
         fallback.assign(tdoubles, 0.0);
         table = fallback.data();
     }
-    parallel_rows(sets, [&](uint64_t b) {
-        fill_row(R.pc, params + size_t(b) * R.pc.nparams, -1, 0.0, table + size_t(b) * nslots * 2);
-    });
-    return table_run(R, table, sets, chunk_opt, energies, grads, stage);
+    const RowFill rows = [&](uint32_t a, uint32_t b) {
+        parallel_rows(
+            b - a,
+            [&](uint64_t i) {
+                fill_row(R.pc, params + size_t(a + i) * R.pc.nparams, -1, 0.0, table + size_t(a + i) * nslots * 2);
+            },
+            nslots);
+    };
+    return table_run(R, table, sets, chunk_opt, energies, grads, stage, &rows);
 }
 
 // Shift rule (fd = false, delta = pi/2) or central differences (fd = true,
@@ -839,11 +880,14 @@ qf_status shift_sets(GradRun& R, const double* params, uint32_t sets, uint32_t c
         const uint32_t gs = std::min(per_group, sets - g0);
         const uint32_t cnt = gs * nslots * 2;
         table.assign(size_t(cnt) * nslots * 2, 0.0);
-        parallel_rows(uint64_t(gs) * nslots * 2, [&](uint64_t row) {
-            const uint64_t b = row / (uint64_t(nslots) * 2), k = (row / 2) % nslots;
-            fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), row % 2 == 0 ? delta : -delta,
-                     table.data() + row * nslots * 2);
-        });
+        parallel_rows(
+            uint64_t(gs) * nslots * 2,
+            [&](uint64_t row) {
+                const uint64_t b = row / (uint64_t(nslots) * 2), k = (row / 2) % nslots;
+                fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), row % 2 == 0 ? delta : -delta,
+                         table.data() + row * nslots * 2);
+            },
+            nslots);
         eg.assign(cnt, 0.0);
         if ((s = table_run(R, table.data(), cnt, chunk_opt, eg.data(), nullptr))) return s;
         for (uint32_t b = 0; b < gs; ++b) {

@@ -103,6 +103,8 @@ struct LaunchCtx {
     double* partial = nullptr;
     // Dense-block passes: the plan's uploaded operand blob (dense_blob).
     const unsigned char* dense = nullptr;
+    // Passes [pass_begin, pass_end) of the plan are launched.
+    size_t pass_begin = 0, pass_end = SIZE_MAX;
 };
 
 qf_status fail(qf_status st, const std::string& msg);

@@ -702,8 +702,13 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
     const qf::TileOp* cur = d_ops;
     uint32_t kinds = 0;
     size_t dense_off = 0;
-    for (size_t i = 0; i < plan.passes.size(); ++i) {
+    for (size_t i = 0; i < std::min(plan.passes.size(), ctx.pass_end); ++i) {
         const qf::Pass& p = plan.passes[i];
+        if (i < ctx.pass_begin) {
+            if (p.kind == qf::Pass::DENSE) dense_off += dense_pass_bytes(p, precision);
+            if (cur) cur += p.ops.size();
+            continue;
+        }
         qf_status s;
         const int kern = prog ? prog->kernel_of_pass[i] : -1;
         if (p.kind == qf::Pass::DENSE) {

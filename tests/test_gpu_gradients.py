@@ -159,6 +159,24 @@ def test_large_batch_runs_the_compiled_adjoint_sweep(backend, precision, tol):
         assert np.max(np.abs(g[s] - want)) < tol, s
 
 
+def test_first_forward_pass_in_chunks_of_sets(backend):
+    # 1030 sets of 14 qubits: the first forward pass runs over four chunks of
+    # sets (the host fills the next chunk's angle rows meanwhile), the other
+    # passes over all sets.  Sets on both sides of every chunk boundary
+    # (257, 515, 772) against the oracle.
+    n, sets = 14, 1030
+    pc = hea_circuit(n, 2)
+    H = heisenberg_hamiltonian(n, seed=8)
+    params = uniform_parameters(pc.parameter_count, sets, seed=23)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    t = H.to_terms()
+    for s in (0, 256, 257, 514, 515, 771, 772, 1029):
+        want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        assert abs(e[s] - want_e) < 1e-11, s
+        assert np.max(np.abs(g[s] - want)) < 1e-10, s
+
+
 def test_adjoint_at_18_qubits_runs_compiled(backend):
     # From 18 qubits every gradient plan is compiled, one set included.
     n = 18
