@@ -311,14 +311,14 @@ def test_uneven_qaoa_multi_wave_at_config_shape(backend, monkeypatch):
         assert np.max(np.abs(g[k] - want_g)) < 1e-10, k
 
 
-def test_windowed_pauli_tables_on_general_sums(backend):
+@pytest.mark.parametrize("n,sets", [(9, 5), (12, 5), (14, 9)])
+def test_windowed_pauli_tables_on_general_sums(backend, n, sets):
     """The costate kernel's windowed coefficient tables on a Pauli sum with
     X, Y and Z mixed in one word, complex coefficients, several words per x
     mask, z masks wider than one table window (per-term signs) and a constant
     term: energies and adjoint gradients against the oracle (PauliSum
-    expectation, src/analytic.rs:2751-2778)."""
-    n = 12
-    rng = np.random.default_rng(5)
+    expectation, src/analytic.rs:2751-2778), at three widths."""
+    rng = np.random.default_rng(5 + n)
     H = PauliSum(n)
     H.add(0.7, "I" * n)
     for _ in range(40):
@@ -331,8 +331,10 @@ def test_windowed_pauli_tables_on_general_sums(backend):
     H.add(-0.4, "Y" + "Z" * (n - 2) + "Z")
     H.add(0.25, "Z" * n)
     pc = hea_circuit(n, 2)
-    params = uniform_parameters(pc.parameter_count, 5, seed=3)
+    params = uniform_parameters(pc.parameter_count, sets, seed=3)
     e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
+    prof = backend.last_gradient_profile()
+    assert prof["costate_kernel"] == "windowed"
     t = H.to_terms()
     for s in range(params.shape[0]):
         want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
