@@ -615,7 +615,7 @@ uint32_t wave_size(uint64_t per_inst, uint32_t want, uint32_t cap_chunk, uint32_
 using RowFill = std::function<void(uint32_t, uint32_t)>;
 
 qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_t chunk_opt, double* energies,
-                    double* grads, double* stage = nullptr, const RowFill* fill = nullptr) {
+                    double* grads, double* stage = nullptr, const RowFill* fill = nullptr, double* estage = nullptr) {
     qf_backend* be = R.be;
     hipStream_t st = nullptr;
     {
@@ -705,7 +705,10 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
         for (auto& x : ev.e) QF_HIP_TRY(hipEventCreate(&x));
     auto mark = [&](int k) { return be->profile ? hipEventRecord(ev.e[k], st) : hipSuccess; };
 
-    std::vector<double> eo(size_t(wave) * 2);
+    // The energies come back through pinned staging when the caller has
+    // some (a copy to pageable memory waits on a bounce buffer).
+    std::vector<double> eo_pageable(estage ? 0 : size_t(wave) * 2);
+    double* eo = estage ? estage : eo_pageable.data();
     for (uint32_t off = 0; off < cnt_total; off += wave) {
         const uint32_t cnt = std::min(wave, cnt_total - off);
         ++P.waves;
@@ -811,12 +814,23 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
             }
         }
         QF_HIP_TRY(mark(4));
-        QF_HIP_TRY(hipMemcpyAsync(eo.data(), eout.p, size_t(cnt) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        QF_HIP_TRY(hipMemcpyAsync(eo, eout.p, size_t(cnt) * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         QF_HIP_TRY(hipStreamSynchronize(st));
         for (uint32_t b = 0; b < cnt; ++b) energies[off + b] = eo[2 * b];
-        if (stage && want_grad && R.pc.nparams)
-            std::memcpy(grads + size_t(off) * R.pc.nparams, stage + size_t(off) * R.pc.nparams,
-                        size_t(cnt) * R.pc.nparams * sizeof(double));
+        if (stage && want_grad && R.pc.nparams) {
+            // Out of the pinned staging in 1 MiB pieces on host threads.
+            const size_t bytes = size_t(cnt) * R.pc.nparams * sizeof(double);
+            char* dst = reinterpret_cast<char*>(grads + size_t(off) * R.pc.nparams);
+            const char* src = reinterpret_cast<const char*>(stage + size_t(off) * R.pc.nparams);
+            constexpr size_t piece = size_t(1) << 20;
+            parallel_rows(
+                (bytes + piece - 1) / piece,
+                [&](uint64_t k) {
+                    const size_t a = size_t(k) * piece;
+                    std::memcpy(dst + a, src + a, std::min(piece, bytes - a));
+                },
+                8192);
+        }
         if (be->profile) {
             // ev 0..1 forward, 1..2 costate, 2..3 energy finish, 3..4 reverse + gradient finish
             float t[4] = {0, 0, 0, 0};
@@ -837,13 +851,17 @@ qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t
     // The angle table and the gradient staging in the backend's pinned buffer.
     const size_t tdoubles = std::max<size_t>(size_t(sets) * nslots * 2, 1);
     const size_t gdoubles = grads ? size_t(sets) * R.pc.nparams : 0;
+    // Energies of one memory wave (at most all sets).
+    const size_t edoubles = size_t(sets) * 2;
     void* hp = nullptr;
     std::vector<double> fallback;
     double* table = nullptr;
     double* stage = nullptr;
-    if (pinned_host(R.be, (tdoubles + gdoubles) * sizeof(double), &hp) == QF_OK) {
+    double* estage = nullptr;
+    if (pinned_host(R.be, (tdoubles + gdoubles + edoubles) * sizeof(double), &hp) == QF_OK) {
         table = static_cast<double*>(hp);
         stage = gdoubles ? table + tdoubles : nullptr;
+        estage = table + tdoubles + gdoubles;
     } else {
         fallback.assign(tdoubles, 0.0);
         table = fallback.data();
@@ -856,7 +874,7 @@ qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t
             },
             nslots);
     };
-    return table_run(R, table, sets, chunk_opt, energies, grads, stage, &rows);
+    return table_run(R, table, sets, chunk_opt, energies, grads, stage, &rows, estage);
 }
 
 // Shift rule (fd = false, delta = pi/2) or central differences (fd = true,
