@@ -25,6 +25,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -120,6 +122,63 @@ __global__ __launch_bounds__(qf::kThreads) void k_tfim(const double* __restrict_
         const int differ = __popcll((a ^ (a >> 1)) & mask);
         double r = double(2 * differ - int(n - 1)) * v[a];
         for (uint32_t i = 0; i < n; ++i) r -= v[a ^ (uint64_t(1) << (n - 1 - i))];
+        w[a] = vprev ? r - beta * vprev[a] : r;
+    }
+}
+
+// The same operator in two passes for chains of kTfimSplit sites or more,
+// whose high-bit flip partners miss the caches in the one-pass kernel.
+// k_tfim_high stages 2^(kTfimHigh + kTfimRun) entries in LDS (runs of
+// 2^kTfimRun contiguous entries, every combination of the top kTfimHigh bits)
+// and writes the diagonal term minus the partners across those bits;
+// k_tfim_low stages 2^kTfimLow contiguous entries and subtracts the partners
+// across the bits between (read through the XCD's L2: each XCD walks its own
+// contiguous share of the vector), then those across the low kTfimLow bits
+// from LDS, then beta * vprev.  The subtractions run in the one-pass kernel's
+// order (bits n-1 down to 0), so w is the same to the bit.
+constexpr uint32_t kTfimRun = 4, kTfimHigh = 9, kTfimLow = 13, kTfimSplit = 14;
+// 1024-thread workgroups: two share a CU beside their 64 KiB tiles, 32 waves.
+constexpr uint32_t kTfimThreads = 1024;
+
+__global__ __launch_bounds__(kTfimThreads) void k_tfim_high(const double* __restrict__ v, double* __restrict__ w,
+                                                            uint32_t n) {
+#pragma clang fp contract(off)
+    constexpr uint32_t T = kTfimHigh + kTfimRun, E = 1u << T;
+    __shared__ double xs[E];
+    const uint64_t mid = blockIdx.x;  // index bits kTfimRun .. n - kTfimHigh - 1
+    const uint32_t hs = n - kTfimHigh;
+    auto gidx = [&](uint32_t e) {
+        return (uint64_t(e >> kTfimRun) << hs) | (mid << kTfimRun) | uint64_t(e & ((1u << kTfimRun) - 1));
+    };
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) xs[e] = v[gidx(e)];
+    __syncthreads();
+    const uint64_t mask = (uint64_t(1) << (n - 1)) - 1;
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+        const uint64_t a = gidx(e);
+        const int differ = __popcll((a ^ (a >> 1)) & mask);
+        double r = double(2 * differ - int(n - 1)) * xs[e];
+        for (uint32_t i = 0; i < kTfimHigh; ++i) r -= xs[e ^ (1u << (T - 1 - i))];
+        w[a] = r;
+    }
+}
+
+// lo = min(kTfimLow, n - kTfimHigh) staged bits; the grid (2^(n - lo)
+// workgroups) is a multiple of 8.
+__global__ __launch_bounds__(kTfimThreads) void k_tfim_low(const double* __restrict__ v, double* __restrict__ w,
+                                                           uint32_t n, uint32_t lo, const double* __restrict__ vprev,
+                                                           double beta) {
+#pragma clang fp contract(off)
+    __shared__ double xs[1u << kTfimLow];
+    const uint32_t E = 1u << lo;
+    const uint64_t per = gridDim.x >> 3;
+    const uint64_t base = (uint64_t(blockIdx.x & 7) * per + (blockIdx.x >> 3)) << lo;
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) xs[e] = v[base + e];
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+        const uint64_t a = base + e;
+        double r = w[a];
+        for (uint32_t b = n - kTfimHigh; b-- > lo;) r -= v[a ^ (uint64_t(1) << b)];
+        for (uint32_t b = lo; b-- > 0;) r -= xs[e ^ (1u << b)];
         w[a] = vprev ? r - beta * vprev[a] : r;
     }
 }
@@ -384,8 +443,21 @@ extern "C" qf_status qf_tfim_lanczos(qf_backend* be, uint32_t sites, double* low
     qf_status s = activate(be);
     if (s) return s;
     const uint64_t dim = uint64_t(1) << sites;
+    // QF_TFIM_PASSES=1 keeps the one-pass kernel at every size (the two
+    // forms give the same vectors to the bit).
+    const char* passes = std::getenv("QF_TFIM_PASSES");
+    const bool split = sites >= kTfimSplit && !(passes && std::strcmp(passes, "1") == 0);
+    const uint32_t lo = std::min(kTfimLow, sites - kTfimHigh);
     auto op = [&](const double* v, double* w, const double* vp, double b) {
-        hipLaunchKernelGGL(k_tfim, dim3(grid_for(dim)), dim3(qf::kThreads), 0, be->stream, v, w, sites, dim, vp, b);
+        if (split) {
+            hipLaunchKernelGGL(k_tfim_high, dim3(unsigned(dim >> (kTfimHigh + kTfimRun))), dim3(kTfimThreads), 0,
+                               be->stream, v, w, sites);
+            hipLaunchKernelGGL(k_tfim_low, dim3(unsigned(dim >> lo)), dim3(kTfimThreads), 0, be->stream, v, w, sites,
+                               lo, vp, b);
+        } else {
+            hipLaunchKernelGGL(k_tfim, dim3(grid_for(dim)), dim3(qf::kThreads), 0, be->stream, v, w, sites, dim, vp,
+                               b);
+        }
     };
     const size_t m = size_t(std::min<uint64_t>(sites >= 24 ? 30 : 100, dim));
     return lanczos(be, dim, m, true, 0.0, op, lowest, alpha, beta, size);

@@ -61,8 +61,10 @@ def test_random_sparse_row_lengths(backend, density, d):
     assert abs(e - want) < 1e-9 * max(1.0, abs(want))
 
 
-@pytest.mark.parametrize("n", [2, 3, 5, 8, 11, 14])
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 11, 13, 14, 16])
 def test_tfim_lanczos_matches_oracle(backend, n):
+    # From 14 sites the operator runs as two passes (high-bit partners from an
+    # LDS tile, then the rest), in the one-pass kernel's subtraction order.
     e, a, b, m = backend.tfim_lanczos(n, details=True)
     want, wa, wb, wm = O.tfim_lanczos(n)
     assert m == wm
@@ -85,6 +87,17 @@ def test_tfim_lanczos_at_size(backend):
     # and below every single-step bound.
     e24, a24, b24, m24 = backend.tfim_lanczos(24, details=True)
     assert m24 == 30 and e24 < a24[0]
+
+
+def test_tfim_two_pass_operator_equals_one_pass_at_24_sites(backend, monkeypatch):
+    # 24 sites: the two-pass operator's middle bits (13..14) come through L2;
+    # the Lanczos coefficients equal the one-pass kernel's to the bit.
+    e2, a2, b2, m2 = backend.tfim_lanczos(24, details=True)
+    monkeypatch.setenv("QF_TFIM_PASSES", "1")
+    e1, a1, b1, m1 = backend.tfim_lanczos(24, details=True)
+    assert m1 == m2 == 30
+    assert e1 == e2
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
 
 
 def test_lanczos_errors(backend):
