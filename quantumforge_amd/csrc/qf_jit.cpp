@@ -115,8 +115,13 @@ public:
         if (!rec_slots.empty()) o_ << "  __shared__ double qred[" << rec_slots.size() * W << "];\n";
         o_ << "  typedef " << R << " R;\n";
         o_ << "  const unsigned t = threadIdx.x;\n";
-        o_ << "  const unsigned long long b = (unsigned long long)blockIdx.x + (unsigned long long)blockIdx.y * "
-              "gridDim.x;\n";
+        // Workgroup x of a row runs on XCD x mod 8 (round-robin dispatch);
+        // each XCD takes a contiguous run of the row's tiles, so tiles that
+        // differ in the lowest block bits (the neighbouring runs in memory)
+        // are fetched by one XCD close together in time.
+        o_ << "  const unsigned xg = gridDim.x >= 8u && (gridDim.x & 7u) == 0u ? gridDim.x >> 3 : 0u;\n";
+        o_ << "  const unsigned long long b = (unsigned long long)(xg ? (blockIdx.x & 7u) * xg + (blockIdx.x >> 3) : "
+              "blockIdx.x) + (unsigned long long)blockIdx.y * gridDim.x;\n";
         o_ << "  const unsigned long long inst = b >> " << LOGB_ << ";\n";
         o_ << "  const unsigned long long bl = b & " << mask64(LOGB_) << "ull;\n";
         o_ << "  const unsigned long long base = (inst << " << n_ << ")" << block_scatter() << ";\n";
