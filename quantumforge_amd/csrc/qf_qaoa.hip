@@ -489,9 +489,14 @@ __global__ __launch_bounds__(kQT, NS == 2 ? 4 : 2) void k_qaoa_pass(cx<R>* __res
     const uint64_t total = h->total_blocks;
     const uint32_t ne = h->nentries;
     const uint32_t t = threadIdx.x;
-    // One workgroup per tile; the grid covers the largest wave's launch and
-    // the workgroups past this wave's tiles leave at once.
-    const uint64_t b = blockIdx.x;
+    // One workgroup per tile; the grid (a multiple of 8) covers the largest
+    // wave's launch and the workgroups past this wave's tiles leave at once.
+    // Workgroup x runs on XCD x mod 8 and takes tile (x mod 8) * per + x / 8
+    // (per = this launch's tiles / 8, rounded up), so each XCD walks a
+    // contiguous run of tiles and every XCD gets an eighth of them.
+    const uint64_t per = (total + 7) / 8, m = blockIdx.x >> 3;
+    if (m >= per) return;
+    const uint64_t b = uint64_t(blockIdx.x & 7u) * per + m;
     if (b >= total) return;
     const QEntry& en = E[find_entry(E, ne, b)];
     const QDesc& d = descs[en.desc];
@@ -1179,7 +1184,7 @@ qf_status run_t(qf_backend* be, const qf_uneven_instance* inst, const std::vecto
             uint64_t blocks = 0;
             for (uint32_t i : w)
                 if (pass_of(X[i], slots_l[k]) >= 0) blocks += nblk_of(X[i].n);
-            grid[k] = std::max(grid[k], blocks);
+            grid[k] = std::max(grid[k], (blocks + 7) / 8 * 8);
         }
     hipGraphExec_t exec = nullptr;
     uint32_t nodes = 0;
