@@ -183,6 +183,21 @@ std::vector<Node> lower(const std::vector<GateIn>& gates, uint32_t qubits) {
     return live;
 }
 
+// Arithmetic of a node per amplitude, in half-instruction units, for
+// balancing passes: run-time X/Y rotations and literal 1-qubit matrices are
+// a multiply-add per output component, Z rotations and other diagonals a
+// per-register factor, CNOT/SWAP renamings or selects.
+int node_cost(const Node& n) {
+    switch (n.kind) {
+        case N_ROT: return n.axis == 2 ? 4 : 8;
+        case N_MAT1: return 6;
+        case N_MAT2: return 32;
+        case N_CNOT:
+        case N_SWAP: return 1;
+        default: return 2;
+    }
+}
+
 struct Scheduler {
     const std::vector<Node>& nodes;
     uint32_t qubits;
@@ -197,6 +212,9 @@ struct Scheduler {
     // gate counted in blocks of `block` gates, then most pending gates.
     int tie = 0;
     int block = 1;
+    // When positive, a pass that is committed (simulate with an order) stops
+    // taking nodes once their node_cost sum would exceed cap.
+    int cap = 0;
 
     Scheduler(const std::vector<Node>& n, uint32_t nq, int m = 0)
         : nodes(n), qubits(nq), queue(nq), head(nq, 0), mode(m) {
@@ -225,7 +243,7 @@ struct Scheduler {
     // (updated).  Records the order in `out` when given.  Returns the weighted
     // count (non-diagonal nodes weigh 2, diagonal 1).
     int simulate(std::vector<size_t>& h, const std::vector<char>& in, std::vector<int>* out) const {
-        int score = 0;
+        int score = 0, cost = 0;
         std::deque<int> work;
         for (uint32_t q = 0; q < qubits; ++q)
             if (h[q] < queue[q].size()) work.push_back(queue[q][h[q]]);
@@ -235,6 +253,8 @@ struct Scheduler {
             if (!at_head(h, id)) continue;
             const Node& n = nodes[id];
             if (!fits(n, in)) continue;
+            if (out && cap > 0 && cost > 0 && cost + node_cost(n) > cap) break;
+            cost += node_cost(n);
             for (int k = 0; k < n.nq; ++k) ++h[n.q[k]];
             score += n.diag() ? 1 : 2;
             if (out) out->push_back(id);
@@ -825,7 +845,8 @@ std::string restore_layout(uint32_t qubits, uint32_t low, uint32_t M, uint32_t R
 
 namespace {
 std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_t M, uint32_t low, uint32_t RB,
-                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out);
+                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out, int cap = 0,
+                        int* max_cost = nullptr);
 }  // namespace
 
 uint32_t default_tile_bits(int precision) { return precision == 0 ? 11 : 12; }
@@ -896,6 +917,7 @@ std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int prec
     k.val(opt.reg_bits);
     k.val(opt.relabel);
     k.val(opt.compile);
+    k.val(opt.balance);
     k.val(relabel_enabled() ? 1 : 0);
     mix_gates(k, gates);
     return k.hex();
@@ -1258,9 +1280,11 @@ std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubit
     std::string first_err;
     bool have = false;
     Plan best;
+    int best_max = 0;  // node_cost of the chosen plan's heaviest pass
     for (const auto& pol : policies) {
         Plan trial = base;
-        std::string e = plan_passes(nodes, qubits, M, low, RB, regk, opt, pol.first, pol.second, trial);
+        int mx = 0;
+        std::string e = plan_passes(nodes, qubits, M, low, RB, regk, opt, pol.first, pol.second, trial, 0, &mx);
         if (!e.empty()) {
             if (first_err.empty()) first_err = e;
             continue;
@@ -1268,10 +1292,31 @@ std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubit
         if (!have || trial.passes.size() < best.passes.size() ||
             (trial.passes.size() == best.passes.size() && trial.stats.phases < best.stats.phases)) {
             best = std::move(trial);
+            best_max = mx;
             have = true;
         }
     }
     if (!have) return first_err;
+    if (out.compiled && opt.balance && best.passes.size() >= 2) {
+        // Caps at a share of the arithmetic per pass: a capped plan with no
+        // more passes and a lighter heaviest pass replaces the greedy one.
+        int total = 0;
+        for (const Node& n : nodes) total += node_cost(n);
+        const size_t P = best.passes.size();
+        for (int pct : {100, 106, 112, 120, 130, 140, 150, 160, 175, 190}) {
+            const int cap = int((int64_t(total) * pct + int64_t(P) * 100 - 1) / (int64_t(P) * 100));
+            for (const auto& pol : policies) {
+                Plan trial = base;
+                int mx = 0;
+                if (!plan_passes(nodes, qubits, M, low, RB, regk, opt, pol.first, pol.second, trial, cap, &mx).empty())
+                    continue;
+                if (trial.passes.size() <= P && mx < best_max) {
+                    best = std::move(trial);
+                    best_max = mx;
+                }
+            }
+        }
+    }
     out = std::move(best);
     return "";
 }
@@ -1282,10 +1327,12 @@ namespace {
 
 // The pass loop of build_plan under one tie-break policy of the set choice.
 std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_t M, uint32_t low, uint32_t RB,
-                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out) {
+                        bool regk, const PlanOptions& opt, int tie, int block, Plan& out, int cap, int* max_cost) {
     Scheduler sch(nodes, qubits, regk ? 1 : 0);
     sch.tie = tie;
     sch.block = block > 0 ? block : 1;
+    sch.cap = cap;
+    if (max_cost) *max_cost = 0;
     std::vector<uint32_t> all_q(qubits);
     for (uint32_t q = 0; q < qubits; ++q) all_q[q] = q;
     // Layout: phys[q] = the index bit that holds logical qubit q.  Compiled
@@ -1328,6 +1375,11 @@ std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_
         sch.simulate(sch.head, in, &order);
         if (order.empty()) return "planner made no progress";
         done += order.size();
+        if (max_cost) {
+            int c = 0;
+            for (int id : order) c += node_cost(nodes[size_t(id)]);
+            *max_cost = std::max(*max_cost, c);
+        }
         Pass p{};
         p.kind = Pass::TILE;
         p.tile_bits = M;

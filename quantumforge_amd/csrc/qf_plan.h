@@ -124,6 +124,11 @@ struct PlanOptions {
     // 1: compiled passes below kCompileMinQubits as well (a large batch of
     // small states, where one launch still streams much memory).
     int compile = 0;
+    // 1: among compiled plans with the fewest passes, prefer the one whose
+    // heaviest pass carries the least arithmetic (passes capped at a share
+    // of the circuit's arithmetic, so that work moves from issue-bound
+    // passes into streaming-bound ones).
+    int balance = 0;
 };
 
 // Circuits of at least this many qubits run compiled passes by default
