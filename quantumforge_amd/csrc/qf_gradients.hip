@@ -573,7 +573,6 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     // sets: complex128 12 qubits (4 reverse passes: 54.4 ms; 11 qubits, 6
     // passes: 57.3 ms), complex64 11 qubits (39.8 ms; 12: 41.0 ms, 13: 47.7 ms).
     po.tile_qubits = tile ? tile : compiled ? (R.precision == QF_C128 ? 12u : 11u) : qf::default_tile_bits(R.precision);
-    po.balance = compiled ? 1 : 0;
     std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
@@ -1259,10 +1258,8 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
     qf::PlanOptions po = plan_options_from(options);
     po.fuse = 1;
     const bool compiled = po.kernel == 3 || (po.kernel == 0 && (po.compile || qubits >= qf::kCompileMinQubits));
-    // A reverse plan relabels only when it runs compiled (its adjoint form);
-    // compiled gradient plans balance their passes, as the gradient engine's.
+    // A reverse plan relabels only when it runs compiled (its adjoint form).
     if (reverse && !compiled) po.relabel = 0;
-    po.balance = compiled ? 1 : 0;
     qf_plan* p = new qf_plan;
     const int precision = options ? options->precision : QF_C128;
     if (precision != QF_C128 && precision != QF_C64) {

@@ -1303,7 +1303,7 @@ std::string build_plan_uncached(const std::vector<GateIn>& gates, uint32_t qubit
         int total = 0;
         for (const Node& n : nodes) total += node_cost(n);
         const size_t P = best.passes.size();
-        for (int pct : {100, 106, 112, 120, 130, 140, 150, 160, 175, 190}) {
+        for (int pct : {120, 140, 160}) {
             const int cap = int((int64_t(total) * pct + int64_t(P) * 100 - 1) / (int64_t(P) * 100));
             for (const auto& pol : policies) {
                 Plan trial = base;

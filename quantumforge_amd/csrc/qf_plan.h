@@ -128,7 +128,7 @@ struct PlanOptions {
     // heaviest pass carries the least arithmetic (passes capped at a share
     // of the circuit's arithmetic, so that work moves from issue-bound
     // passes into streaming-bound ones).
-    int balance = 0;
+    int balance = 1;
 };
 
 // Circuits of at least this many qubits run compiled passes by default
