@@ -638,7 +638,7 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
                               eblk * 32 + 64;
     const uint32_t fwd_M = R.fwd.passes.empty() ? n : R.fwd.passes[0].tile_bits;
     const uint32_t logmax = n - std::min(std::min(fwd_M, rev_M), n);
-    const uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax, be->arena_bytes);
+    uint32_t wave = wave_size(per_inst, cnt_total, chunk_opt, logmax, be->arena_bytes);
     if (wave == 0) return fail(QF_OUT_OF_MEMORY, "device memory cannot hold one parameter set's states");
 
     // parameter -> occurrences in reverse circuit order (CSR)
@@ -653,6 +653,26 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
     uint32_t ngroups = 0;
     const std::vector<unsigned char> tpack = pack_terms(tsorted, &ngroups);
     const std::vector<unsigned char> wpack = pack_pauli_windows(tsorted);
+    // Arena bytes of a wave of w sets (the carve below, 256-byte aligned).
+    auto carve_bytes = [&](uint64_t w) {
+        auto a = [](uint64_t b) { return ((b ? b : 16) + 255) / 256 * 256; };
+        return a(w * dim * ab) + a(want_grad ? w * dim * ab : 16) + a(w * nslots * 16) +
+               a(want_grad ? w * nblk * nslots * 8 : 16) + a(w * eblk * 2 * sizeof(qf::dd)) + a(w * 2 * 8) +
+               a(want_grad ? w * R.pc.nparams * 8 : 16) + a(ps.size() * 4) + a(std::max<size_t>(pl.size(), 1) * 4) +
+               a(tpack.size()) + a(wpack.size());
+    };
+    // An arena kept from an earlier call is used as it is when the call
+    // needs no more waves within it: growing it means freeing most of HBM
+    // and allocating it again, which has taken seconds.
+    if (be->arena_bytes > 0 && carve_bytes(wave) > be->arena_bytes) {
+        uint64_t lo = 0, hi = wave;  // largest w with carve_bytes(w) <= arena_bytes
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) / 2;
+            if (carve_bytes(mid) <= be->arena_bytes) lo = mid;
+            else hi = mid - 1;
+        }
+        if (lo > 0 && (cnt_total + lo - 1) / lo == (cnt_total + wave - 1) / wave) wave = uint32_t(lo);
+    }
     DevBuf psi, lam, cs, part, epart, eout, gout, pstart, pslots, terms, wterms;
     qf_status s;
     {

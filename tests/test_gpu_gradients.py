@@ -177,6 +177,26 @@ def test_first_forward_pass_in_chunks_of_sets(backend):
         assert np.max(np.abs(g[s] - want)) < 1e-10, s
 
 
+def test_kept_arena_runs_a_larger_call_in_as_many_waves(backend):
+    # 100 sets leave an arena sized for 100; then 150 sets in chunks of 128
+    # need 2 memory waves either way, so they run in the kept arena as waves
+    # of 100 and 50 sets.  Sets on both sides of the wave boundary against
+    # the oracle.
+    n = 10
+    pc = hea_circuit(n, 2)
+    H = heisenberg_hamiltonian(n, seed=4)
+    backend.release_cached()
+    G.batch_adjoint_gradient(backend, pc, n, uniform_parameters(pc.parameter_count, 100, seed=31), H)
+    params = uniform_parameters(pc.parameter_count, 150, seed=32)
+    e, g = G.batch_adjoint_gradient(backend, pc, n, params, H, chunk=128)
+    assert backend.last_gradient_profile()["waves"] == 2
+    t = H.to_terms()
+    for s in (0, 99, 100, 149):
+        want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
+        assert abs(e[s] - O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)) < 1e-11, s
+        assert np.max(np.abs(g[s] - want)) < 1e-10, s
+
+
 def test_adjoint_at_18_qubits_runs_compiled(backend):
     # From 18 qubits every gradient plan is compiled, one set included.
     n = 18
