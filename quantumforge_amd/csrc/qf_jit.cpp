@@ -65,6 +65,12 @@ struct Bit {
     int idx;  // register index, thread bit, or global qubit
 };
 
+std::string hex64(unsigned long long v) {
+    char b[24];
+    std::snprintf(b, sizeof b, "%016llx", v);
+    return b;
+}
+
 class PassGen {
     using zc = std::complex<double>;
 
@@ -134,6 +140,10 @@ public:
         acc_[0].r = "ar";
         acc_[0].i = "ai";
         o_ << "  R ar = R(1), ai = R(0);\n";
+        // Phase (in 2^-64 turns) of the literal diagonal factors that depend
+        // on tile-index bits only: the same for the whole tile, so it is
+        // added up in scalar integer registers and applied once at the end.
+        o_ << "  unsigned long long uph = 0ull;\n";
         for (uint32_t b = 0; b < RB_; ++b)
             for (int v = 0; v < 2; ++v) {
                 Acc& a = acc_[1 + 2 * b + v];
@@ -177,6 +187,12 @@ public:
         }
         if (g_ != zc(1.0, 0.0)) acc_mul_lit(acc_[0], g_.real(), g_.imag());
         g_ = 1.0;
+        if (uph_) {
+            o_ << "  { const double th = double((long long)uph) * 0x1.921fb54442d18p-62;  // 2 pi 2^-64\n"
+                  "    const R upr = R(cos(th)), upi = R(sin(th));\n";
+            acc_mul(acc_[0], "upr", "upi");
+            o_ << "  }\n";
+        }
         acc_flush_all();
         // A relabeling pass first moves the tile into the store layout, whose
         // lanes 0..low-1 carry the tile bits stored to index bits 0..low-1.
@@ -241,6 +257,7 @@ private:
     std::vector<int> var_;
     zc g_{1.0, 0.0};              // the true amplitudes are g_ times the tile's (take_scale)
     bool relabel_ = false;        // the store permutes the tile's bits (Pass::store_phys)
+    bool uph_ = false;            // some diagonal factor went into the tile phase uph
     bool half_ = false;           // exchanges through a half-tile buffer
     // Half-buffer exchanges in emission order: the round bit, and for an
     // exchange whose two layouts share no thread bit, an intermediate layout
@@ -1087,6 +1104,31 @@ private:
                 acc_mul(a, fn + "r", fn + "i");
             }
         };
+        if (nreg == 0 && b0.kind == Bit::BLOCK && (!two || b1.kind == Bit::BLOCK)) {
+            // Tile-index bits only, unit-modulus values: a phase of the whole
+            // tile, added up in fixed point (2^-64 turns) from the values'
+            // own phases and applied once before the store.
+            const int nv = two ? 4 : 2;
+            bool unit = true;
+            unsigned long long K[4] = {0, 0, 0, 0};
+            for (int x = 0; x < nv; ++x) {
+                const double re = d(x, 0), im = d(x, 1);
+                if (std::fabs(re * re + im * im - 1.0) > 1e-14) unit = false;
+                const long double t = (long double)std::atan2(im, re) / (2.0L * 3.14159265358979323846264338327950288L);
+                K[x] = (unsigned long long)(long long)std::llroundl(t * 0x1p63L) << 1;
+            }
+            if (unit) {
+                auto k = [&](int x) { return "0x" + hex64(K[x]) + "ull"; };
+                o_ << "  uph += " << pfx << "c0 ? ";
+                if (two)
+                    o_ << "(" << pfx << "c1 ? " << k(3) << " : " << k(1) << ") : (" << pfx << "c1 ? " << k(2) << " : "
+                       << k(0) << ");\n";
+                else
+                    o_ << k(1) << " : " << k(0) << ";\n";
+                uph_ = true;
+                return;
+            }
+        }
         if (nreg == 0) {
             factor(acc_[0], -1);
             return;

@@ -21,6 +21,7 @@ import numpy as np
 
 PRELUDE = r"""
 #include <barrier>
+#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
