@@ -751,10 +751,17 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
             // The host fills the angle rows of the next chunk of sets while
             // the GPU runs the first forward pass over this one (and, before
             // the first chunk, the zero fill); the other passes then run
-            // over the whole wave.
+            // over the whole wave.  Chunks of 1/16, 2/16, 4/16 and 9/16 of
+            // the sets: only the first, small one waits for the host, and
+            // each later fill is shorter than the GPU's run of the chunk
+            // before it.
             const uint32_t K = cnt >= 1024 && R.fwd.passes.size() > 1 ? 4 : 1;
+            static const uint32_t edge16[5] = {0, 1, 3, 7, 16};
+            auto edge = [&](uint32_t j) {
+                return K == 1 ? (j ? cnt : 0u) : uint32_t(uint64_t(cnt) * edge16[j] / 16);
+            };
             for (uint32_t j = 0; j < K; ++j) {
-                const uint32_t a = uint32_t(uint64_t(cnt) * j / K), b = uint32_t(uint64_t(cnt) * (j + 1) / K);
+                const uint32_t a = edge(j), b = edge(j + 1);
                 if (j == 0) (*fill)(off + a, off + b);
                 double* csj = static_cast<double*>(cs.p) + size_t(a) * nslots * 2;
                 QF_HIP_TRY(hipMemcpyAsync(csj, table + size_t(off + a) * nslots * 2, size_t(b - a) * nslots * 16,
@@ -766,7 +773,7 @@ qf_status table_run(GradRun& R, const double* table, uint32_t cnt_total, uint32_
                                      R.fwd, bases[0], c1)))
                     return s;
                 P.forward_kernels |= R.fwd.kernels_run;
-                if (j + 1 < K) (*fill)(off + b, off + uint32_t(uint64_t(cnt) * (j + 2) / K));
+                if (j + 1 < K) (*fill)(off + b, off + edge(j + 2));
             }
             if (K > 1) {
                 LaunchCtx c2 = fctx;

@@ -161,16 +161,16 @@ def test_large_batch_runs_the_compiled_adjoint_sweep(backend, precision, tol):
 
 def test_first_forward_pass_in_chunks_of_sets(backend):
     # 1030 sets of 14 qubits: the first forward pass runs over four chunks of
-    # sets (the host fills the next chunk's angle rows meanwhile), the other
-    # passes over all sets.  Sets on both sides of every chunk boundary
-    # (257, 515, 772) against the oracle.
+    # sets, 1/16, 2/16, 4/16 and 9/16 of them (the host fills the next chunk's
+    # angle rows meanwhile), the other passes over all sets.  Sets on both
+    # sides of every chunk boundary (64, 193, 450) against the oracle.
     n, sets = 14, 1030
     pc = hea_circuit(n, 2)
     H = heisenberg_hamiltonian(n, seed=8)
     params = uniform_parameters(pc.parameter_count, sets, seed=23)
     e, g = G.batch_adjoint_gradient(backend, pc, n, params, H)
     t = H.to_terms()
-    for s in (0, 256, 257, 514, 515, 771, 772, 1029):
+    for s in (0, 63, 64, 192, 193, 449, 450, 1029):
         want_e = O.pc_expectation(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
         want = O.pc_adjoint_gradient(pc.to_pops(), pc.parameter_count, 0, params[s], [], n, t)
         assert abs(e[s] - want_e) < 1e-11, s
