@@ -166,13 +166,29 @@ std::vector<qf::GateIn> param_gates(const PCirc& pc, const double* inputs, bool 
 
 // Angle table row: (cos(theta/2), sin(theta/2)) per rotation occurrence with
 // theta = params[param] (+ delta on occurrence `shift_slot`), computed on the
-// host as Gate::matrix does (src/lib.rs:2303-2331).
-void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row) {
+// host as Gate::matrix does (src/lib.rs:2303-2331).  scaled (the rows of a
+// plan with Plan::scaled_rotations): X and Y occurrences hold instead
+// (f, u) with f the larger of c = cos, s = sin in magnitude and u the other
+// over f, the lowest mantissa bit of u set when f = s; the rotation is then
+// f [[1, -u], [u, 1]] (Ry, f = c) or f [[u, -1], [1, u]] (f = s), and Rx
+// likewise with -i u, -i.
+void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row, bool scaled) {
     for (size_t s = 0; s < pc.slot_param.size(); ++s) {
         double th = params[pc.slot_param[s]];
         if (int64_t(s) == shift_slot) th = th + delta;
-        row[2 * s] = host_cos(th * 0.5);
-        row[2 * s + 1] = host_sin(th * 0.5);
+        const double c = host_cos(th * 0.5), sn = host_sin(th * 0.5);
+        if (!scaled || pc.ops[pc.slot_op[s]].gate == 2) {
+            row[2 * s] = c;
+            row[2 * s + 1] = sn;
+            continue;
+        }
+        const bool by_s = std::fabs(sn) > std::fabs(c);
+        uint64_t ub;
+        const double u = by_s ? c / sn : sn / c;
+        std::memcpy(&ub, &u, 8);
+        ub = (ub & ~uint64_t(1)) | uint64_t(by_s);
+        row[2 * s] = by_s ? sn : c;
+        std::memcpy(&row[2 * s + 1], &ub, 8);
     }
 }
 
@@ -561,6 +577,13 @@ struct GradRun {
     uint32_t fwd_tile = 0, rev_tile = 0;
 };
 
+// Compiled gradient plans read X/Y rotations as (scale, ratio) rows unless
+// QF_SCALED_ROTATIONS=0 (the direct (cos, sin) form).
+bool scaled_rotations_on() {
+    const char* e = std::getenv("QF_SCALED_ROTATIONS");
+    return !(e && e[0] == '0');
+}
+
 qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_rev, uint64_t sets) {
     qf::PlanOptions po;
     po.fuse = 1;
@@ -573,6 +596,7 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     // sets: complex128 12 qubits (4 reverse passes: 54.4 ms; 11 qubits, 6
     // passes: 57.3 ms), complex64 11 qubits (39.8 ms; 12: 41.0 ms, 13: 47.7 ms).
     po.tile_qubits = tile ? tile : compiled ? (R.precision == QF_C128 ? 12u : 11u) : qf::default_tile_bits(R.precision);
+    po.scaled_rotations = compiled && scaled_rotations_on() ? 1 : 0;
     std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
@@ -584,6 +608,14 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
         po.relabel = compiled ? 1 : 0;
         e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
         if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
+        // One angle table serves both plans: both read (scale, ratio) rows
+        // or neither does.
+        if (R.rev.scaled_rotations != R.fwd.scaled_rotations && R.fwd.runtime_rotations) {
+            po.scaled_rotations = 0;
+            e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
+            if (e.empty()) e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
+            if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
+        }
     }
     return QF_OK;
 }
@@ -899,7 +931,8 @@ qf_status adjoint_sets(GradRun& R, const double* params, uint32_t sets, uint32_t
         parallel_rows(
             b - a,
             [&](uint64_t i) {
-                fill_row(R.pc, params + size_t(a + i) * R.pc.nparams, -1, 0.0, table + size_t(a + i) * nslots * 2);
+                fill_row(R.pc, params + size_t(a + i) * R.pc.nparams, -1, 0.0, table + size_t(a + i) * nslots * 2,
+                         R.fwd.scaled_rotations != 0);
             },
             nslots);
     };
@@ -932,7 +965,7 @@ qf_status shift_sets(GradRun& R, const double* params, uint32_t sets, uint32_t c
             [&](uint64_t row) {
                 const uint64_t b = row / (uint64_t(nslots) * 2), k = (row / 2) % nslots;
                 fill_row(R.pc, params + size_t(g0 + b) * R.pc.nparams, int64_t(k), row % 2 == 0 ? delta : -delta,
-                         table.data() + row * nslots * 2);
+                         table.data() + row * nslots * 2, R.fwd.scaled_rotations != 0);
             },
             nslots);
         eg.assign(cnt, 0.0);
@@ -1240,7 +1273,7 @@ extern "C" qf_status qf_state_run_parameterized(qf_state* st, const qf_pop* ops,
     if ((s = plan_for(R, inputs, 0, false, 1))) return s;
     const uint32_t nslots = uint32_t(R.pc.slot_param.size());
     std::vector<double> row(std::max<size_t>(size_t(nslots) * 2, 2));
-    fill_row(R.pc, params, -1, 0.0, row.data());
+    fill_row(R.pc, params, -1, 0.0, row.data(), R.fwd.scaled_rotations != 0);
     void* sp = nullptr;
     qf_backend_stream(be, &sp);
     hipStream_t strm = static_cast<hipStream_t>(sp);
@@ -1285,8 +1318,10 @@ extern "C" qf_status qf_plan_build_parameterized(const qf_pop* ops, size_t nops,
     qf::PlanOptions po = plan_options_from(options);
     po.fuse = 1;
     const bool compiled = po.kernel == 3 || (po.kernel == 0 && (po.compile || qubits >= qf::kCompileMinQubits));
-    // A reverse plan relabels only when it runs compiled (its adjoint form).
+    // A reverse plan relabels only when it runs compiled (its adjoint form);
+    // compiled plans take the gradient engine's (scale, ratio) rows.
     if (reverse && !compiled) po.relabel = 0;
+    po.scaled_rotations = compiled && scaled_rotations_on() ? 1 : 0;
     qf_plan* p = new qf_plan;
     const int precision = options ? options->precision : QF_C128;
     if (precision != QF_C128 && precision != QF_C64) {

@@ -78,8 +78,8 @@ public:
     // ns = 2: the adjoint form, which carries a second state (the costate
     // lambda) through the same ops and records Im <lambda| G |psi> for the
     // TF_RECORD rotations before applying them.
-    PassGen(const Pass& p, uint32_t n, int precision, const std::string& name, int ns = 1)
-        : p_(p), n_(n), dbl_(precision == 0), name_(name), ns_(ns) {}
+    PassGen(const Pass& p, uint32_t n, int precision, const std::string& name, int ns = 1, bool scaled = false)
+        : p_(p), n_(n), dbl_(precision == 0), name_(name), ns_(ns), scaled_(scaled) {}
 
     bool run(std::string* err) {
         if (p_.kind != Pass::TILE || p_.reg_bits == 0 || p_.ops.empty() || p_.ops[0].kind != TOP_PHASE)
@@ -162,10 +162,12 @@ public:
                    << reg_name(st, 'i', k) << " = v.im; }\n";
             o_ << "  }\n";
         }
+        if (scaled_) scaled_flags();
         bool lds_used = false;
         uint32_t rec_index = 0;
         for (size_t i = 1; i < p_.ops.size(); ++i) {
             const TileOp& op = p_.ops[i];
+            cur_op_ = i;
             ++opn_;
             switch (op.kind) {
                 case TOP_PHASE: exchange(layout_of(op), lds_used); lds_used = true; break;
@@ -248,6 +250,14 @@ private:
     bool dbl_;
     std::string name_;
     int ns_ = 1;                  // states carried (2: psi and the costate lambda)
+    bool scaled_ = false;         // X/Y rotation rows hold (scale, ratio)
+    // Scaled rotations of the pass: op index -> position in pass order (its
+    // form flag is bit pos % 32 of word fw<pos / 32>), and the variables of
+    // the ratios already loaded.
+    std::map<size_t, uint32_t> sc_pos_;
+    std::vector<size_t> sc_ops_;
+    std::map<size_t, std::string> sc_u_;
+    size_t cur_op_ = 0;
     int cur_ = 0;                 // the state rv()/iv() name
     uint32_t M_ = 0, RB_ = 0, NR_ = 0, NT_ = 0, LOGB_ = 0;
     std::vector<int> tile_of_q_;
@@ -1291,6 +1301,10 @@ private:
     }
 
     void rot(const TileOp& op) {
+        if (op.q1 != 2 && scaled_) {
+            rot_xy_scaled(op);
+            return;
+        }
         std::string c, s;
         load_cs(op, c, s);
         if (op.q1 == 2) {  // exp(-i theta Z / 2): (c - is) on bit 0, (c + is) on bit 1
@@ -1311,12 +1325,66 @@ private:
             acc_mul(acc_[0], c, sz);
             return;
         }
+        const uint32_t i = uint32_t(op.q0);
+        acc_flush_bit(i);
         // Direct form: [[c, -s], [s, c]] (Ry) and [[c, -is], [-is, c]] (Rx)
         // with the half-angle values from the table, a multiply and an FMA
         // per output component, no division and no branch.
+        each_state([&] { rot_xy_direct(op, i, c, s); });
+    }
+    // The row of a scaled X/Y rotation holds f, the larger of c, s in
+    // magnitude, and u, the other over f, whose lowest mantissa bit is 0 for
+    // f = c and 1 for f = s (the host's choice per instance).  The flags of
+    // the pass's rotations are read once at the start into scalar words, so
+    // each rotation branches on a bit of a word (uniform over the workgroup,
+    // no wait on a load); each output component is one FMA (the forms of
+    // rot_xy_regs), the ratio of the next rotation is loaded before this
+    // one's branch, and f goes into the thread's scale accumulator after it.
+    // For the inverse rotation (s -> -s) both forms take -u, and the f = s
+    // form also -f.
+    void scaled_flags() {
+        for (size_t i = 1; i < p_.ops.size(); ++i) {
+            const TileOp& op = p_.ops[i];
+            if (op.kind == TOP_ROT && op.q1 != 2) {
+                sc_pos_[i] = uint32_t(sc_ops_.size());
+                sc_ops_.push_back(i);
+            }
+        }
+        for (size_t w = 0; w * 32 < sc_ops_.size(); ++w) {
+            o_ << "  const unsigned fw" << w << " = __builtin_amdgcn_readfirstlane(int(0u";
+            for (size_t j = w * 32; j < std::min(sc_ops_.size(), w * 32 + 32); ++j)
+                o_ << "\n      | ((unsigned(__builtin_bit_cast(unsigned long long, csr["
+                   << 2 * p_.ops[sc_ops_[j]].slot + 1 << "])) & 1u) << " << j % 32 << ")";
+            o_ << "));\n";
+        }
+    }
+    const std::string& scaled_ratio(size_t i) {
+        auto it = sc_u_.find(i);
+        if (it != sc_u_.end()) return it->second;
+        const TileOp& op = p_.ops[i];
+        const std::string u = "u" + std::to_string(sc_pos_[i]) + "_" + std::to_string(opn_);
+        o_ << "  const R " << u << " = " << ((op.flags & TF_NEGATE) ? "-" : "") << "R(csr[" << 2 * op.slot + 1
+           << "]);\n";
+        return sc_u_[i] = u;
+    }
+    void rot_xy_scaled(const TileOp& op) {
         const uint32_t i = uint32_t(op.q0);
         acc_flush_bit(i);
-        each_state([&] { rot_xy_direct(op, i, c, s); });
+        const uint32_t pos = sc_pos_.at(cur_op_);
+        const std::string u = scaled_ratio(cur_op_);
+        if (pos + 1 < sc_ops_.size()) scaled_ratio(sc_ops_[pos + 1]);
+        const std::string w = "((fw" + std::to_string(pos / 32) + " >> " + std::to_string(pos % 32) + ") & 1u)";
+        each_state([&] {
+            o_ << "  if (" << w << ") {\n";
+            rot_xy_regs(op, i, u, false);
+            o_ << "  } else {\n";
+            rot_xy_regs(op, i, u, true);
+            o_ << "  }\n";
+        });
+        const std::string f = "f" + std::to_string(opn_), x = "R(csr[" + std::to_string(2 * op.slot) + "])";
+        o_ << "  const R " << f << " = " << ((op.flags & TF_NEGATE) ? "(" + w + " ? -" + x + " : " + x + ")" : x)
+           << ";\n";
+        acc_mul(acc_[0], f, "");
     }
     void rot_xy_direct(const TileOp& op, uint32_t i, const std::string& c, const std::string& s) {
         for (uint32_t k = 0; k < NR_; ++k) {
@@ -1527,6 +1595,7 @@ std::string plan_fingerprint(const Plan& plan, int precision, int ns) {
     val(plan.qubits);
     val(precision);
     val(ns);
+    val(plan.scaled_rotations);
     const char* half = std::getenv("QF_JIT_HALF_LDS");
     val(half && half[0] == '0' ? 0 : 1);
     for (const auto& o : jit_compile_options()) {
@@ -1566,8 +1635,8 @@ std::vector<std::string> jit_compile_options() {
 }
 
 std::string jit_pass_source(const Pass& p, uint32_t qubits, int precision, const std::string& name,
-                            std::string* err, int ns) {
-    PassGen g(p, qubits, precision, name, ns);
+                            std::string* err, int ns, bool scaled) {
+    PassGen g(p, qubits, precision, name, ns, scaled);
     if (!g.run(err)) return "";
     return g.src;
 }
@@ -1585,7 +1654,8 @@ std::string jit_plan_source(const Plan& plan, int precision, JitProgram* prog, s
         if (p.kind != Pass::TILE || p.reg_bits == 0) continue;
         std::string e;
         std::string body =
-            jit_pass_source(p, plan.qubits, precision, "qf_pass_@" + std::to_string(bodies.size()), &e, ns);
+            jit_pass_source(p, plan.qubits, precision, "qf_pass_@" + std::to_string(bodies.size()), &e, ns,
+                            plan.scaled_rotations != 0);
         if (body.empty()) {
             if (err) *err = "pass " + std::to_string(i) + ": " + e;
             return "";

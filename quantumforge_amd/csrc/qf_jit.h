@@ -29,8 +29,10 @@ namespace qf {
 // a register-kernel tile pass the compiler handles.
 // ns = 2: the adjoint form (kernel arguments add the costate lambda and the
 // partial table; TF_RECORD rotations record Im <lambda| G |psi> per tile).
+// scaled: run-time X/Y rotations read (scale, ratio) angle rows
+// (Plan::scaled_rotations).
 std::string jit_pass_source(const Pass& p, uint32_t qubits, int precision, const std::string& name,
-                            std::string* err, int ns = 1);
+                            std::string* err, int ns = 1, bool scaled = false);
 
 // The compiled kernels of one plan at one precision.
 struct JitProgram {

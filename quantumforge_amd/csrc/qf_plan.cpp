@@ -640,7 +640,8 @@ std::string describe_plan(const Plan& plan) {
         s += "]";
     };
     s += "{\"qubits\":" + std::to_string(plan.qubits) + ",\"compiled\":" + std::to_string(plan.compiled) +
-         ",\"relabeled\":" + std::to_string(plan.relabeled) + ",\"passes\":[";
+         ",\"relabeled\":" + std::to_string(plan.relabeled) +
+         ",\"scaled_rotations\":" + std::to_string(plan.scaled_rotations) + ",\"passes\":[";
     for (size_t i = 0; i < plan.passes.size(); ++i) {
         const Pass& p = plan.passes[i];
         if (i) s += ",";
@@ -918,6 +919,7 @@ std::string plan_key(const std::vector<GateIn>& gates, uint32_t qubits, int prec
     k.val(opt.relabel);
     k.val(opt.compile);
     k.val(opt.balance);
+    k.val(opt.scaled_rotations);
     k.val(relabel_enabled() ? 1 : 0);
     mix_gates(k, gates);
     return k.hex();
@@ -1423,6 +1425,13 @@ std::string plan_passes(const std::vector<Node>& nodes, uint32_t qubits, uint32_
     }
     for (const Pass& p : out.passes)
         if (!p.store_phys.empty()) ++out.relabeled;
+    if (opt.scaled_rotations && out.compiled && out.runtime_rotations) {
+        bool all = true;
+        for (const Pass& p : out.passes)
+            for (const TileOp& o : p.ops)
+                if (o.kind == TOP_ROT && o.q1 != 2 && (p.kind != Pass::TILE || p.reg_bits == 0)) all = false;
+        out.scaled_rotations = all ? 1 : 0;
+    }
     out.stats.passes = out.passes.size();
     return "";
 }

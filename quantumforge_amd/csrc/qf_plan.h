@@ -129,6 +129,11 @@ struct PlanOptions {
     // of the circuit's arithmetic, so that work moves from issue-bound
     // passes into streaming-bound ones).
     int balance = 1;
+    // 1: the run-time X and Y rotations of a compiled plan read angle rows of
+    // (scale, ratio) instead of (cos, sin) (PassGen and fill_row): one FMA per
+    // output component.  Set only where every such rotation lands in a
+    // compiled pass (Plan::scaled_rotations).
+    int scaled_rotations = 0;
 };
 
 // Circuits of at least this many qubits run compiled passes by default
@@ -182,6 +187,7 @@ struct Plan {
     mutable uint32_t kernels_run = 0;
     int compiled = 0;          // 1: register passes run as compiled kernels (2: explicitly requested)
     int relabeled = 0;         // passes with a store permutation (Pass::store_phys): compiled only
+    int scaled_rotations = 0;  // X/Y rotations read (scale, ratio) rows: compiled only
     // Compiled programs per form (0: one state, 1: adjoint, psi and lambda)
     // and precision, attached on first launch (jit_state: 0 not tried, 1
     // compiled, -1 the compile failed and the register kernel runs).

@@ -161,7 +161,8 @@ qf_status validate_values(const PCirc& pc, const double* inputs, size_t inputs_l
                           size_t params_len);
 qf_status validate_ops(PCirc& pc, uint32_t qubits);
 std::vector<qf::GateIn> param_gates(const PCirc& pc, const double* inputs, bool reverse);
-void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row);
+void fill_row(const PCirc& pc, const double* params, int64_t shift_slot, double delta, double* row,
+              bool scaled = false);
 std::vector<qf::PauliDev> sorted_terms(const std::vector<qf::PauliDev>& t);
 void adjoint_gate_id(int gate, double theta, int* g, double* th);
 qf_status pauli_terms(const qf_pauli_term* terms, size_t count, uint32_t qubits, std::vector<qf::PauliDev>& out);

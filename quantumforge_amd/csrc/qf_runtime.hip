@@ -497,8 +497,9 @@ constexpr size_t kLoadedPrograms = 256;
 
 // The plan's compiled program for `precision` (compiled on first use), or
 // nullptr when the plan runs the register kernel.  A failed compile is an
-// error when compiled passes were requested explicitly; otherwise it is
-// reported once and the register kernel runs the same plan.
+// error when compiled passes were requested explicitly or the plan runs only
+// compiled (relabeled passes, rotations read as (scale, ratio)); otherwise it
+// is reported once and the register kernel runs the same plan.
 qf_status plan_program(const qf::Plan& plan, int precision, int ns, const qf::JitProgram** out) {
     *out = nullptr;
     if (!plan.compiled) return QF_OK;
@@ -509,7 +510,7 @@ qf_status plan_program(const qf::Plan& plan, int precision, int ns, const qf::Ji
         plan.jit[f][pr] = qf::jit_compile_plan(plan, pr, &err, ns);
         plan.jit_state[f][pr] = plan.jit[f][pr] ? 1 : -1;
         if (!plan.jit[f][pr]) {
-            if (plan.compiled == 2 || plan.relabeled) {
+            if (plan.compiled == 2 || plan.relabeled || plan.scaled_rotations) {
                 plan.jit_state[f][pr] = 0;
                 return fail(QF_INVALID_ARGUMENT, "pass compiler: " + err);
             }
@@ -699,6 +700,8 @@ qf_status launch_plan(qf_backend* be, void* d, int precision, uint32_t qubits, u
     }
     if (plan.relabeled && !prog)
         return fail(QF_INVALID_ARGUMENT, "a plan that relabels qubits between passes runs only as compiled kernels");
+    if (plan.scaled_rotations && !prog)
+        return fail(QF_INVALID_ARGUMENT, "a plan whose rotations read (scale, ratio) rows runs only as compiled kernels");
     const qf::TileOp* cur = d_ops;
     uint32_t kinds = 0;
     size_t dense_off = 0;

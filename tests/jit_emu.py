@@ -84,6 +84,7 @@ template <int PAT> static inline double qf_swz(double v) {
 }
 static inline double qf_bperm32(double v, unsigned ln) { return qf_lane_read(v, ln ^ 32u); }
 #define QF_PIN(v) ((void)0)
+#define __builtin_amdgcn_readfirstlane(x) (x)
 """
 
 LAUNCHER = r"""

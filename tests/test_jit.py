@@ -204,10 +204,36 @@ def mixed_rotations(n: int) -> ParameterizedCircuit:
     return pc
 
 
+def angle_rows(pops, params, scaled: bool) -> np.ndarray:
+    """One instance's angle-table row: (cos, sin) of each rotation
+    occurrence's half angle; in a plan with scaled rotations an X or Y
+    occurrence holds (f, u) instead, f the larger of the two in magnitude,
+    u the other over f, the lowest mantissa bit of u set when f = sin."""
+    out = []
+    for r in pops:
+        if r["kind"] != 1:
+            continue
+        th = float(params[int(r["index"])])
+        c, s = np.cos(th / 2), np.sin(th / 2)
+        if scaled and int(r["gate"]) != 2:
+            by_s = abs(s) > abs(c)
+            f, u = (s, c / s) if by_s else (c, s / c)
+            ub = (np.array([u]).view(np.uint64) & ~np.uint64(1)) | np.uint64(by_s)
+            out += [f, float(ub.view(np.float64)[0])]
+        else:
+            out += [c, s]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("scaled", [True, False])
 @pytest.mark.parametrize("case", ["hea", "qaoa", "mixed"])
-def test_generated_kernels_read_rotation_angles_per_instance(case):
+def test_generated_kernels_read_rotation_angles_per_instance(case, scaled, monkeypatch):
     # Rotations (X/Y on register bits, Z and ZZ on any qubit) read
-    # (cos, sin) of their half angle from the instance's table row.
+    # (cos, sin) of their half angle from the instance's table row, or
+    # (scale, ratio) for X/Y in the scaled form (both forms of the
+    # workgroup-uniform branch: angles of pi, -pi, 0 and pi/2 among them).
+    if not scaled:
+        monkeypatch.setenv("QF_SCALED_ROTATIONS", "0")
     if case == "hea":
         n, pc, M, RB = 12, hea_circuit(12, 3), 9, 3
     elif case == "qaoa":
@@ -217,10 +243,12 @@ def test_generated_kernels_read_rotation_angles_per_instance(case):
         n, pc, M, RB = 11, mixed_rotations(11), 9, 3
     P = pc.parameter_count
     plan = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB)
+    assert plan.describe()["scaled_rotations"] == int(scaled)
     pops = pc.to_pops()
     slots = [int(r["index"]) for r in pops if r["kind"] == 1]
-    sets = uniform_parameters(P, 2, seed=7)
-    cs = np.concatenate([np.stack([np.cos(s[slots] / 2), np.sin(s[slots] / 2)], axis=1).ravel() for s in sets])
+    sets = [np.array(x) for x in uniform_parameters(P, 2, seed=7)]
+    sets[0][:4] = [np.pi, -np.pi, 0.0, np.pi / 2]
+    cs = np.concatenate([angle_rows(pops, x, scaled) for x in sets])
     a0 = np.zeros(2 << n, complex)
     a0[0] = a0[1 << n] = 1
     out = emulate(plan, a0, n, count=2, cs=cs, nslots=len(slots)).reshape(2, -1)
@@ -236,27 +264,35 @@ def with_fixed_gates(pc: ParameterizedCircuit, n: int) -> ParameterizedCircuit:
     return pc
 
 
-@pytest.mark.parametrize("n,M,RB,precision,circ", [(11, 9, 3, "complex128", "hea"), (12, 10, 3, "complex128", "hea"),
-                                                     (12, 10, 4, "complex64", "hea"),
-                                                     (11, 9, 3, "complex128", "mixed")])
-def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision, circ):
+@pytest.mark.parametrize("n,M,RB,precision,circ,scaled", [(11, 9, 3, "complex128", "hea", True),
+                                                            (12, 10, 3, "complex128", "hea", True),
+                                                            (12, 10, 4, "complex64", "hea", True),
+                                                            (11, 9, 3, "complex128", "mixed", True),
+                                                            (11, 9, 3, "complex128", "mixed", False)])
+def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision, circ, scaled, monkeypatch):
     # The compiled reverse sweep of the adjoint gradient: the forward plan's
     # kernels make psi, lambda = H psi, then the reverse plan's kernels in
     # their two-state form carry psi and lambda back and record
     # Im <lambda| G |psi> per tile for every rotation.  Summing the partial
     # table per parameter gives the oracle's adjoint gradient, and psi
-    # returns to |0>.
+    # returns to |0>.  Scaled: X/Y rotations in the (scale, ratio) form,
+    # their scales carried in the thread accumulator and weighed into the
+    # records.
+    if not scaled:
+        monkeypatch.setenv("QF_SCALED_ROTATIONS", "0")
     pc = hea_circuit(n, 2) if circ == "hea" else with_fixed_gates(mixed_rotations(n), n)
     if circ == "hea":
         with_fixed_gates(pc, n)
     P = pc.parameter_count
     params = np.array(uniform_parameters(P, 1, seed=n + M)[0])
+    params[:3] = [np.pi, -np.pi / 2, 3.0]
     H = heisenberg_hamiltonian(n, seed=3)
     pops = pc.to_pops()
     slots = [int(r["index"]) for r in pops if r["kind"] == 1]
-    cs = np.stack([np.cos(params[slots] / 2), np.sin(params[slots] / 2)], axis=1).ravel()
+    cs = angle_rows(pops, params, scaled)
     fwd = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
     rev = Plan.parameterized(pc, n, reverse=True, kernel=3, tile_qubits=M, reg_bits=RB, precision=precision)
+    assert fwd.describe()["scaled_rotations"] == rev.describe()["scaled_rotations"] == int(scaled)
     src = rev.source(precision, adjoint=True)
     assert "qf_wave_sum" in src and "partial[b * nslots" in src
     a0 = np.zeros(1 << n, complex)
