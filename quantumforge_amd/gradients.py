@@ -177,8 +177,10 @@ def batch_gradient(backend, circuit: ParameterizedCircuit, qubits: int, paramete
         raise QuantumError(8, f"expected {circuit.parameter_count} circuit parameters, got {params.shape[1]}")
     x = np.ascontiguousarray(inputs, dtype=np.float64)
     t = _terms(observable, qubits)
-    energies = np.zeros(sets, dtype=np.float64)
-    grads = np.zeros((sets, max(circuit.parameter_count, 1)), dtype=np.float64)
+    # Every element is written by the library (all strategies), so the
+    # outputs skip the zero fill (8 MiB for 4096 sets x 256 parameters).
+    energies = np.empty(sets, dtype=np.float64)
+    grads = np.empty((sets, max(circuit.parameter_count, 1)), dtype=np.float64)
     o = _options(GradientOptions(strategy, finite_difference_step, 0, precision, tile_qubits, chunk))
     check(lib().qf_batch_gradient(backend._h, ptr(ops), len(ops), circuit.parameter_count, circuit.input_count,
                                   ptr(x), qubits, ptr(params), sets, ptr(t), len(t), C.byref(o), ptr(energies),
@@ -203,7 +205,7 @@ def batch_expectation(backend, circuit: ParameterizedCircuit, qubits: int, param
     params = np.ascontiguousarray(np.atleast_2d(parameters), dtype=np.float64)
     x = np.ascontiguousarray(inputs, dtype=np.float64)
     t = _terms(observable, qubits)
-    energies = np.zeros(params.shape[0], dtype=np.float64)
+    energies = np.empty(params.shape[0], dtype=np.float64)
     o = _options(GradientOptions(GradientStrategy.Adjoint, 1e-3, 0, precision, tile_qubits, chunk))
     check(lib().qf_batch_expectation(backend._h, ptr(ops), len(ops), circuit.parameter_count, circuit.input_count,
                                      ptr(x), qubits, ptr(params), params.shape[0], ptr(t), len(t), C.byref(o),
