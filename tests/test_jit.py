@@ -225,8 +225,23 @@ def angle_rows(pops, params, scaled: bool) -> np.ndarray:
     return np.array(out)
 
 
+def deep_xy(n: int, layers: int = 5) -> ParameterizedCircuit:
+    # X and Y rotations on every qubit per layer and a CNOT ladder: with the
+    # whole state in one tile, 50 scaled rotations in one pass (two words of
+    # form flags).
+    pc = ParameterizedCircuit(layers * n)
+    k = 0
+    for layer in range(layers):
+        for q in range(n):
+            pc.rotation(RotationAxis.X if (q + layer) % 2 else RotationAxis.Y, q, k)
+            k += 1
+        for q in range(n - 1):
+            pc.cnot(q, q + 1)
+    return pc
+
+
 @pytest.mark.parametrize("scaled", [True, False])
-@pytest.mark.parametrize("case", ["hea", "qaoa", "mixed"])
+@pytest.mark.parametrize("case", ["hea", "qaoa", "mixed", "deep"])
 def test_generated_kernels_read_rotation_angles_per_instance(case, scaled, monkeypatch):
     # Rotations (X/Y on register bits, Z and ZZ on any qubit) read
     # (cos, sin) of their half angle from the instance's table row, or
@@ -239,11 +254,16 @@ def test_generated_kernels_read_rotation_angles_per_instance(case, scaled, monke
     elif case == "qaoa":
         n, M, RB = 12, 10, 4
         pc = qaoa_parameterized(n, MaxCutProblem.random_3_regular(n, 5).edges, 2)
-    else:
+    elif case == "mixed":
         n, pc, M, RB = 11, mixed_rotations(11), 9, 3
+    else:
+        n, pc, M, RB = 10, deep_xy(10), 10, 3
     P = pc.parameter_count
     plan = Plan.parameterized(pc, n, kernel=3, tile_qubits=M, reg_bits=RB)
     assert plan.describe()["scaled_rotations"] == int(scaled)
+    if case == "deep":
+        assert len(plan.describe()["passes"]) == 1
+        assert ("fw1" in plan.source()) == scaled
     pops = pc.to_pops()
     slots = [int(r["index"]) for r in pops if r["kind"] == 1]
     sets = [np.array(x) for x in uniform_parameters(P, 2, seed=7)]
@@ -268,7 +288,8 @@ def with_fixed_gates(pc: ParameterizedCircuit, n: int) -> ParameterizedCircuit:
                                                             (12, 10, 3, "complex128", "hea", True),
                                                             (12, 10, 4, "complex64", "hea", True),
                                                             (11, 9, 3, "complex128", "mixed", True),
-                                                            (11, 9, 3, "complex128", "mixed", False)])
+                                                            (11, 9, 3, "complex128", "mixed", False),
+                                                            (10, 10, 3, "complex128", "deep", True)])
 def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision, circ, scaled, monkeypatch):
     # The compiled reverse sweep of the adjoint gradient: the forward plan's
     # kernels make psi, lambda = H psi, then the reverse plan's kernels in
@@ -280,7 +301,10 @@ def test_adjoint_form_gives_the_oracle_gradient(n, M, RB, precision, circ, scale
     # records.
     if not scaled:
         monkeypatch.setenv("QF_SCALED_ROTATIONS", "0")
-    pc = hea_circuit(n, 2) if circ == "hea" else with_fixed_gates(mixed_rotations(n), n)
+    if circ == "deep":
+        pc = deep_xy(n)
+    else:
+        pc = hea_circuit(n, 2) if circ == "hea" else with_fixed_gates(mixed_rotations(n), n)
     if circ == "hea":
         with_fixed_gates(pc, n)
     P = pc.parameter_count
