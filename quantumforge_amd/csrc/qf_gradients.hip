@@ -593,18 +593,27 @@ qf_status plan_for(GradRun& R, const double* inputs, uint32_t tile, bool need_re
     if ((sets << R.qubits) >= (uint64_t(1) << 22)) po.compile = 1;
     const bool compiled = po.compile || R.qubits >= qf::kCompileMinQubits;
     // Tiles of compiled gradient plans, measured on the 16-qubit VQE over 4096
-    // sets: complex128 12 qubits (4 reverse passes: 54.4 ms; 11 qubits, 6
-    // passes: 57.3 ms), complex64 11 qubits (39.8 ms; 12: 41.0 ms, 13: 47.7 ms).
-    po.tile_qubits = tile ? tile : compiled ? (R.precision == QF_C128 ? 12u : 11u) : qf::default_tile_bits(R.precision);
+    // sets (forward / reverse): complex128 12 / 12 (51.8 ms; 12 / 11: 54.0 ms,
+    // 13 / 12: 54.4 ms; 11 / 11 took 57.3 ms with direct-form rotations),
+    // complex64 12 / 11 (38.0-38.5 ms; 11 / 11: 39.0-39.3 ms, 13 / 12: 40.3
+    // ms, 12 / 12 and 13 / 13 were slower than 11 / 11); complex64 states of
+    // fewer than 12 qubits keep 11 / 11.
+    const bool c64_split = compiled && !tile && R.precision != QF_C128 && R.qubits >= 12;
+    po.tile_qubits = tile ? tile
+                     : compiled ? (R.precision == QF_C128 || c64_split ? 12u : 11u)
+                                : qf::default_tile_bits(R.precision);
     po.scaled_rotations = compiled && scaled_rotations_on() ? 1 : 0;
     std::string e = qf::build_plan(param_gates(R.pc, inputs, false), R.qubits, R.precision, po, R.fwd);
     if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
     if (need_rev) {
         // A reverse plan that runs compiled (its adjoint form) may relabel
         // and keeps the forward tile (its half-buffer exchanges fit both
-        // states); one that runs through the interpreting adjoint kernel may
-        // not relabel and takes a tile one qubit smaller (two states in LDS).
-        if (!compiled || tile) po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
+        // states), except complex64 from 12 qubits (one qubit smaller, as
+        // measured above); one that runs through the interpreting adjoint
+        // kernel may not relabel and takes a tile one qubit smaller (two
+        // states in LDS).
+        if (!compiled || tile || c64_split)
+            po.tile_qubits = po.tile_qubits > 2 ? po.tile_qubits - 1 : po.tile_qubits;
         po.relabel = compiled ? 1 : 0;
         e = qf::build_plan(param_gates(R.pc, inputs, true), R.qubits, R.precision, po, R.rev);
         if (!e.empty()) return fail(QF_INVALID_ARGUMENT, e);
