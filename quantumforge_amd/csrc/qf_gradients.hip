@@ -381,34 +381,55 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_windows(const cx<R>* __r
     cx<R>* __restrict__ l = lam ? lam + inst * dim : nullptr;
     const uint32_t ngroups = h->ngroups;
     qf::dd er{0, 0}, ei{0, 0};
+    // Two amplitudes per thread and step, i and i + stride (the thread's
+    // next one, so its additions keep their order): each group's window and
+    // term records are read once for both, and the gathers of 4 groups for
+    // both amplitudes are in flight together.
     const uint64_t stride = uint64_t(eblk) * blockDim.x;
-    for (uint64_t i = chunk * blockDim.x + threadIdx.x; i < dim; i += stride) {
-        double ar = 0.0, ai = 0.0;
-        for (uint32_t g0 = 0; g0 < ngroups; g0 += 8) {
-            cx<R> v[8];
+    for (uint64_t i = chunk * blockDim.x + threadIdx.x; i < dim; i += 2 * stride) {
+        const uint64_t j = i + stride;
+        const bool two = j < dim;
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        for (uint32_t g0 = 0; g0 < ngroups; g0 += 4) {
+            cx<R> v[4], w2[4];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u)
-                if (g0 + u < ngroups) v[u] = p[i ^ groups[g0 + u].x];
+            for (uint32_t u = 0; u < 4; ++u)
+                if (g0 + u < ngroups) {
+                    const uint64_t x = groups[g0 + u].x;
+                    v[u] = p[i ^ x];
+                    if (two) w2[u] = p[j ^ x];
+                }
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
+            for (uint32_t u = 0; u < 4; ++u) {
                 if (g0 + u >= ngroups) break;
                 const PWGroup G = groups[g0 + u];
-                double cr = 0.0, ci = 0.0;
+                double cr = 0.0, ci = 0.0, dr = 0.0, di = 0.0;
                 for (uint32_t w = G.w0; w < G.w1; ++w) {
                     const PWWin W = wins[w];
                     const double2 c = coef_s[W.off + (uint32_t(i >> W.lo) & W.mask)];
+                    const double2 d = coef_s[W.off + (uint32_t(j >> W.lo) & W.mask)];
                     cr += c.x;
                     ci += c.y;
+                    dr += d.x;
+                    di += d.y;
                 }
                 for (uint32_t t = G.t0; t < G.t1; ++t) {
                     const qf::PauliDev& T = wide[t];
                     const double sg = (__popcll((i ^ G.x) & T.z) & 1) ? -1.0 : 1.0;
+                    const double sh = (__popcll((j ^ G.x) & T.z) & 1) ? -1.0 : 1.0;
                     cr += sg * T.cr;
                     ci += sg * T.ci;
+                    dr += sh * T.cr;
+                    di += sh * T.ci;
                 }
                 const double pr = v[u].re, pi = v[u].im;
                 ar += cr * pr - ci * pi;
                 ai += cr * pi + ci * pr;
+                if (two) {
+                    const double qr = w2[u].re, qi = w2[u].im;
+                    br += dr * qr - di * qi;
+                    bi += dr * qi + di * qr;
+                }
             }
         }
         if (l) l[i] = cx<R>{R(ar), R(ai)};
@@ -416,6 +437,13 @@ __global__ __launch_bounds__(qf::kThreads) void k_pauli_windows(const cx<R>* __r
         const double vr = v.re, vi = v.im;
         qf::neumaier(er, vr * ar + vi * ai);  // Re(conj(psi) lambda)
         qf::neumaier(ei, vr * ai - vi * ar);  // Im(conj(psi) lambda)
+        if (two) {
+            if (l) l[j] = cx<R>{R(br), R(bi)};
+            const cx<R> q = p[j];
+            const double qr = q.re, qi = q.im;
+            qf::neumaier(er, qr * br + qi * bi);
+            qf::neumaier(ei, qr * bi - qi * br);
+        }
     }
     qf::block_reduce(er, ei);
     if (threadIdx.x == 0) {
